@@ -1,0 +1,182 @@
+"""Seeded synthetic k-space generator for the benchmark workloads.
+
+Every case is what `IsmrmrdFile::read_kspace_with_mask`
+(io/ismrmrd/reader.rs:225-468) would hand the strategies for a scanner file
+of that shape, plus the noise scans of `read_calibration`
+(io/ismrmrd/reader.rs:167-183):
+
+* phantom: 10 random ellipses (centres U[-0.6,0.6]^2 of the recon FOV,
+  semi-axes U[0.05,0.45], angle U[0,pi), amplitudes U[0.1,1]), additive;
+* coil sensitivities: Gaussian magnitudes (sigma 0.4 FOV) centred on a ring
+  of radius 0.6 FOV at angles 2 pi c / nc, times exp(i(a0 + a1 x + a2 y + a3 xy))
+  with a_i ~ U[-pi/4, pi/4];
+* k-space: centred forward FFT fftshift(fft2(ifftshift(img * S))) -- the exact
+  inverse of the reference's centred IFFT for even sizes (recon/fft.rs:19-29);
+* noise: A w with A = CN(0,1)/sqrt(nc) (nc x nc) and w ~ CN(0, sigma^2 I); the
+  same A colours the n_noise noise-scan samples per coil;
+* readout oversampling: the phantom lies inside the central 1/os of x.
+
+Masks are integer rules (bit-exact) except the CS variable-density mask,
+which is a seeded draw whose SHA-256 the tests pin.
+"""
+import hashlib
+
+import numpy as np
+
+SEED_BASE = 20261203
+
+
+def _ellipses(rng, ny, nx, os_x, n=10):
+    y = (np.arange(ny) - ny // 2) / (ny / 2)
+    x = (np.arange(nx) - nx // 2) / (nx / 2) * os_x
+    X, Y = np.meshgrid(x, y)
+    img = np.zeros((ny, nx))
+    for _ in range(n):
+        cy, cx = rng.uniform(-0.6, 0.6, 2)
+        ay, ax = rng.uniform(0.05, 0.45, 2)
+        th = rng.uniform(0, np.pi)
+        amp = rng.uniform(0.1, 1.0)
+        c, s = np.cos(th), np.sin(th)
+        u = (X - cx) * c + (Y - cy) * s
+        v = -(X - cx) * s + (Y - cy) * c
+        img += amp * ((u / ax) ** 2 + (v / ay) ** 2 <= 1.0)
+    return img
+
+
+def coil_maps(rng, nc, ny, nx, os_x=1):
+    y = (np.arange(ny) - ny // 2) / (ny / 2)
+    x = (np.arange(nx) - nx // 2) / (nx / 2) * os_x
+    X, Y = np.meshgrid(x, y)
+    maps = np.empty((nc, ny, nx), dtype=np.complex128)
+    for c in range(nc):
+        ang = 2 * np.pi * c / nc
+        cy, cx = 0.6 * np.sin(ang), 0.6 * np.cos(ang)
+        mag = np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (2 * 0.4 ** 2))
+        a = rng.uniform(-np.pi / 4, np.pi / 4, 4)
+        maps[c] = mag * np.exp(1j * (a[0] + a[1] * X + a[2] * Y + a[3] * X * Y))
+    return maps
+
+
+def centred_fft2(img):
+    return np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(img, axes=(-2, -1))), axes=(-2, -1))
+
+
+def grappa_mask(ny, r, acs):
+    """Rows sampled iff ky % r == (ny/2) % r, plus ACS rows [ny/2-acs/2, ny/2+acs/2)."""
+    ky = np.arange(ny)
+    rows = (ky % r) == ((ny // 2) % r)
+    rows[ny // 2 - acs // 2: ny // 2 + acs // 2] = True
+    return rows
+
+
+def cs_mask(rng, ny, nx, frac=1.0 / 6.0, centre=24, power=4.0):
+    """Variable-density random ky-kx mask: p(k) ~ (1 - |k|/k_max)^power scaled so the
+    sampled fraction is `frac`, with a fully sampled centre x centre block."""
+    ky = (np.arange(ny) - ny // 2) / (ny / 2)
+    kx = (np.arange(nx) - nx // 2) / (nx / 2)
+    KX, KY = np.meshgrid(kx, ky)
+    r = np.sqrt(KX ** 2 + KY ** 2) / np.sqrt(2.0)
+    pdf = (1.0 - r) ** power
+    cen = np.zeros((ny, nx), dtype=bool)
+    cen[ny // 2 - centre // 2: ny // 2 + centre // 2, nx // 2 - centre // 2: nx // 2 + centre // 2] = True
+    target = frac * ny * nx - cen.sum()
+    lo, hi = 0.0, 1e6
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if np.minimum(mid * pdf[~cen], 1.0).sum() < target:
+            lo = mid
+        else:
+            hi = mid
+    p = np.minimum(hi * pdf, 1.0)
+    m = rng.random((ny, nx)) < p
+    m |= cen
+    return m
+
+
+def mask_sha256(mask):
+    return hashlib.sha256(np.ascontiguousarray(mask, dtype=np.uint8).tobytes()).hexdigest()
+
+
+def make_case(nc, nz, ny, nx, os_x=1, seed=SEED_BASE, kind="full", r=4, acs=24,
+              n_noise=256, noise_level=0.01, three_d=False, cs_frac=1.0 / 6.0, cs_centre=24,
+              ellipses=10):
+    """Build one synthetic acquisition.
+
+    kind: "full" (fully sampled), "grappa" (ky-only regular R with ACS rows,
+    also used for SENSE), "cs" (variable-density ky-kx mask per slice).
+    Returns a dict with kspace [nc,nz,ny,nx] c64, mask [nz,ny,nx] u8, noise
+    [nc,n_noise] c64, hdr (oks_header fields) and truth images [nz,ny,nx].
+    """
+    rng = np.random.default_rng(seed)
+    maps = coil_maps(rng, nc, ny, nx, os_x)
+    A = (rng.standard_normal((nc, nc)) + 1j * rng.standard_normal((nc, nc))) / np.sqrt(2 * nc)
+    kspace = np.empty((nc, nz, ny, nx), dtype=np.complex64)
+    mask = np.zeros((nz, ny, nx), dtype=np.uint8)
+    truth = np.empty((nz, ny, nx), dtype=np.float32)
+    sigma = None
+    if three_d:
+        # 3-D: one ellipse-stack volume, centred 3-D FFT along (kz, ky, kx).
+        vol = np.stack([_ellipses(np.random.default_rng(seed + 1000 + z // max(1, nz // 8)), ny, nx, os_x,
+                                  ellipses) for z in range(nz)])
+        zc = (np.arange(nz) - nz // 2) / (nz / 2)
+        vol *= (np.abs(zc) <= 0.8)[:, None, None]
+        truth[:] = vol
+        peak = 0.0
+        for c in range(nc):
+            coil = vol * maps[c][None]
+            peak = max(peak, float(np.abs(coil).max()))
+            k = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(coil)))
+            kspace[c] = k
+        sigma = noise_level * peak * np.sqrt(nz * ny * nx)
+    else:
+        peak = 0.0
+        for z in range(nz):
+            img = _ellipses(rng, ny, nx, os_x, ellipses)
+            truth[z] = img
+            coil = img[None] * maps
+            peak = max(peak, float(np.abs(coil).max()))
+            kspace[:, z] = centred_fft2(coil)
+        sigma = noise_level * peak * np.sqrt(ny * nx)
+    # coloured complex noise A w, w ~ CN(0, sigma^2)
+    for z in range(nz):
+        w = (rng.standard_normal((nc, ny * nx)) + 1j * rng.standard_normal((nc, ny * nx))) * (sigma / np.sqrt(2))
+        kspace[:, z] += (A @ w).reshape(nc, ny, nx).astype(np.complex64)
+    wn = (rng.standard_normal((nc, n_noise)) + 1j * rng.standard_normal((nc, n_noise))) * (sigma / np.sqrt(2))
+    noise = (A @ wn).astype(np.complex64)
+
+    if kind == "full":
+        mask[:] = 1
+    elif kind == "grappa":
+        rows = grappa_mask(ny, r, acs)
+        mask[:] = rows[None, :, None]
+    elif kind == "cs":
+        for z in range(nz):
+            mask[z] = cs_mask(rng, ny, nx, cs_frac, cs_centre)
+    else:
+        raise ValueError(kind)
+    kspace *= mask[None].astype(np.complex64)
+    kspace = np.ascontiguousarray(kspace)
+    hdr = dict(encoded_x=nx, encoded_y=ny, encoded_z=(nz if three_d else 1),
+               recon_x=nx // os_x, recon_y=ny, recon_z=(nz if three_d else 1),
+               is_3d_encoding=int(three_d))
+    return dict(kspace=kspace, mask=mask, noise=np.ascontiguousarray(noise), hdr=hdr, truth=truth,
+                seed=seed)
+
+
+# The benchmark configurations (sizes as the workload descriptions give them).
+CONFIGS = {
+    "c1_ifft_rss": dict(nc=15, nz=16, ny=368, nx=640, os_x=2, kind="full", seed=SEED_BASE + 0),
+    "c2_grappa": dict(nc=32, nz=8, ny=384, nx=384, kind="grappa", r=4, acs=24, seed=SEED_BASE + 1),
+    "c3_cs": dict(nc=8, nz=32, ny=320, nx=320, kind="cs", seed=SEED_BASE + 2),
+    "c4_sense3d": dict(nc=32, nz=160, ny=256, nx=256, kind="grappa", r=3, acs=32, three_d=True,
+                       seed=SEED_BASE + 3),
+}
+
+# Served mixed stream request shapes (ny, nx, nc, os_x).
+SERVE_SHAPES = [(256, 256, 8, 1), (320, 320, 16, 1), (368, 640, 15, 2), (384, 384, 32, 1)]
+
+
+def make_config(name, **over):
+    p = dict(CONFIGS[name])
+    p.update(over)
+    return make_case(**p)
