@@ -1,0 +1,97 @@
+"""ctypes binding of liboks_hip.so (the C ABI declared in include/oks.h).
+
+There is no fallback: if the library is missing or no HIP device is present
+the calls raise.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboks_hip.so")
+
+_P = C.c_void_p
+_I32 = C.c_int32
+_I64 = C.c_int64
+_lib = None
+
+
+class OksError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"oks status {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OksError(-1, f"{LIB_PATH} is not built; run python -m openkspace_amd.build")
+    L = C.CDLL(LIB_PATH)
+    L.oks_ctx_create.argtypes = [C.c_int, C.POINTER(_P)]
+    L.oks_ctx_destroy.argtypes = [_P]
+    L.oks_ctx_destroy.restype = None
+    L.oks_last_error.argtypes = [_P]
+    L.oks_last_error.restype = C.c_char_p
+    L.oks_ctx_set_stream.argtypes = [_P, _P]
+    L.oks_ctx_stream.argtypes = [_P]
+    L.oks_ctx_stream.restype = _P
+    L.oks_ctx_synchronize.argtypes = [_P]
+    L.oks_ctx_take_flags.argtypes = [_P, C.POINTER(C.c_uint32)]
+    L.oks_output_shape.argtypes = [_P, _I32, _P]
+    L.oks_output_shape.restype = None
+    L.oks_detect_pattern.argtypes = [_P, _I64, _I64, _I64, _P]
+    for name in ("ifft_rss", "grappa", "sense", "cs"):
+        for suf in ("", "_dev"):
+            fn = getattr(L, f"oks_recon_{name}{suf}", None)
+            if fn is not None:
+                fn.argtypes = [_P, _P, _P, _P]
+    L.oks_fft_centered_dev.argtypes = [_P, _P, _P, _I32, _P, _I32, _I32, _I32]
+    L.oks_rss_dev.argtypes = [_P, _P, _I64, _I64, _P]
+    L.oks_prewhiten_calibrate_dev.argtypes = [_P, _P, _I32, _I64, _P]
+    L.oks_prewhiten_apply_dev.argtypes = [_P, _P, _P, _I32, _I64]
+    _lib = L
+    return L
+
+
+def check(ctx, st):
+    if st != 0:
+        msg = lib().oks_last_error(ctx)
+        raise OksError(st, msg.decode() if msg else "")
+    return st
+
+
+class Context:
+    """Owns one oks_ctx (a HIP stream, FFT plans and workspaces on one device)."""
+
+    def __init__(self, device=0):
+        self.ptr = _P()
+        st = lib().oks_ctx_create(device, C.byref(self.ptr))
+        if st != 0:
+            raise OksError(st, "oks_ctx_create failed")
+
+    def close(self):
+        if self.ptr:
+            lib().oks_ctx_destroy(self.ptr)
+            self.ptr = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_ptr):
+        check(self.ptr, lib().oks_ctx_set_stream(self.ptr, stream_ptr))
+
+    @property
+    def stream(self):
+        return lib().oks_ctx_stream(self.ptr)
+
+    def synchronize(self):
+        check(self.ptr, lib().oks_ctx_synchronize(self.ptr))
+
+    def take_flags(self):
+        f = C.c_uint32(0)
+        check(self.ptr, lib().oks_ctx_take_flags(self.ptr, C.byref(f)))
+        return int(f.value)

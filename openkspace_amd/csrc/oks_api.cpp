@@ -1,0 +1,227 @@
+// oks_api.cpp -- C ABI of the MI355X backend: context management, host-side
+// index logic (pattern detection, partial-Fourier gate, crop windows) and the
+// ReconStrategy compositions that drive the HIP kernels.
+//
+// No C++ exception crosses the ABI: every entry point catches and maps to a
+// status code, with the message available from oks_last_error().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/oks.h"
+#include "oks_ctx.hpp"
+#include "oks_host.hpp"
+#include "oks_kernels.hpp"
+
+using namespace oks;
+
+// ------------------------------------------------------------ context
+oks::FftPlan oks_ctx::plan(int n) {
+    FftPlan P{};
+    P.n = n;
+    int m = n, s = 0;
+    auto push = [&](int r) {
+        if (s >= kMaxStages) fail(OKS_E_UNSUPPORTED, "FFT length has too many factors");
+        P.rad[s++] = r;
+        m /= r;
+    };
+    while (m % 8 == 0) push(8);
+    while (m % 4 == 0) push(4);
+    while (m % 2 == 0) push(2);
+    while (m % 3 == 0) push(3);
+    while (m % 5 == 0) push(5);
+    for (int p = 7; m > 1; p += 2)
+        while (m % p == 0) {
+            if (p > 257) fail(OKS_E_UNSUPPORTED, "FFT length has a prime factor > 257");
+            push(p);
+        }
+    P.nst = s;
+    auto it = tw.find(n);
+    if (it == tw.end()) {
+        std::vector<float2> h(std::max(n, 1));
+        const long double two_pi = 6.283185307179586476925286766559L;
+        for (int k = 0; k < n; ++k) {
+            long double a = two_pi * (long double)k / (long double)n;
+            h[k] = make_float2((float)cosl(a), (float)sinl(a));
+        }
+        float2* d = nullptr;
+        OKS_HIP(hipMalloc(&d, sizeof(float2) * h.size()));
+        OKS_HIP(hipMemcpy(d, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice));
+        it = tw.emplace(n, d).first;
+    }
+    P.tw = it->second;
+    return P;
+}
+
+void* oks_ctx::buf(const std::string& name, size_t bytes) {
+    DevBuf& b = ws[name];
+    if (bytes > b.cap) {
+        if (b.p) {
+            OKS_HIP(hipStreamSynchronize(stream));
+            OKS_HIP(hipFree(b.p));
+            b.p = nullptr;
+            b.cap = 0;
+        }
+        size_t cap = std::max<size_t>(bytes, 256);
+        OKS_HIP(hipMalloc(&b.p, cap));
+        b.cap = cap;
+    }
+    return b.p;
+}
+
+void oks_ctx::release() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& kv : tw) (void)hipFree(kv.second);
+    tw.clear();
+    for (auto& kv : ws)
+        if (kv.second.p) (void)hipFree(kv.second.p);
+    ws.clear();
+    if (d_status) (void)hipFree(d_status);
+    d_status = nullptr;
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+}
+
+namespace {
+
+template <class F>
+int guarded(oks_ctx* ctx, F f) {
+    if (!ctx) return OKS_E_INVALID_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    try {
+        ctx->err.clear();
+        OKS_HIP(hipSetDevice(ctx->device));
+        return f();
+    } catch (const Status& s) {
+        ctx->err = s.what();
+        return s.code;
+    } catch (const std::bad_alloc&) {
+        ctx->err = "host allocation failed";
+        return OKS_E_OOM;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return OKS_E_INCONSISTENT;
+    } catch (...) {
+        ctx->err = "unknown error";
+        return OKS_E_INCONSISTENT;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int oks_ctx_create(int device, oks_ctx** out) {
+    if (!out) return OKS_E_INVALID_ARG;
+    *out = nullptr;
+    oks_ctx* c = new (std::nothrow) oks_ctx();
+    if (!c) return OKS_E_OOM;
+    c->device = device;
+    int st = guarded(c, [&] {
+        int n = 0;
+        OKS_HIP(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) fail(OKS_E_DEVICE, "no such HIP device");
+        OKS_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->own_stream = true;
+        OKS_HIP(hipMalloc(&c->d_status, 8 * sizeof(int)));
+        OKS_HIP(hipMemsetAsync(c->d_status, 0, 8 * sizeof(int), c->stream));
+        OKS_HIP(hipStreamSynchronize(c->stream));
+        return OKS_OK;
+    });
+    if (st != OKS_OK) {
+        c->release();
+        delete c;
+        return st;
+    }
+    *out = c;
+    return OKS_OK;
+}
+
+void oks_ctx_destroy(oks_ctx* ctx) {
+    if (!ctx) return;
+    {
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        (void)hipSetDevice(ctx->device);
+        ctx->release();
+    }
+    delete ctx;
+}
+
+const char* oks_last_error(const oks_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int oks_ctx_set_stream(oks_ctx* ctx, void* stream) {
+    return guarded(ctx, [&] {
+        OKS_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->own_stream && ctx->stream) OKS_HIP(hipStreamDestroy(ctx->stream));
+        if (stream) {
+            ctx->stream = (hipStream_t)stream;
+            ctx->own_stream = false;
+        } else {
+            OKS_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+            ctx->own_stream = true;
+        }
+        return OKS_OK;
+    });
+}
+
+void* oks_ctx_stream(oks_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int oks_ctx_synchronize(oks_ctx* ctx) {
+    return guarded(ctx, [&] {
+        OKS_HIP(hipStreamSynchronize(ctx->stream));
+        return OKS_OK;
+    });
+}
+
+int oks_ctx_take_flags(oks_ctx* ctx, uint32_t* flags) {
+    return guarded(ctx, [&] {
+        int h[8];
+        OKS_HIP(hipMemcpyAsync(h, ctx->d_status, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        OKS_HIP(hipStreamSynchronize(ctx->stream));
+        OKS_HIP(hipMemsetAsync(ctx->d_status, 0, sizeof(h), ctx->stream));
+        uint32_t f = ctx->pending_flags;
+        if (h[0]) f |= OKS_F_PREWHITEN_SKIPPED;
+        if (h[1]) f |= OKS_F_SENSE_PIXEL_FAILED;
+        if (h[2]) f |= OKS_F_GRAPPA_CALIB_FAILED;
+        ctx->pending_flags = 0;
+        if (flags) *flags = f;
+        return OKS_OK;
+    });
+}
+
+void oks_ifft_rss_default(oks_ifft_rss_desc* d) {
+    if (d) *d = oks_ifft_rss_desc{1, 1, 1, 1, OKS_FFT_AUTO, 1};
+}
+void oks_grappa_default(oks_grappa_desc* d) {
+    if (d) *d = oks_grappa_desc{1, 1, 1, 4, 5, 1e-3f, OKS_FFT_AUTO, 1};
+}
+void oks_sense_default(oks_sense_desc* d) {
+    if (d) *d = oks_sense_desc{1, 1, 1, OKS_MAPS_WALSH, 3, 6, 5, 0.02f, 30, 1e-4f, 0, OKS_FFT_AUTO, 1};
+}
+void oks_cs_default(oks_cs_desc* d) {
+    if (d) *d = oks_cs_desc{1, 1, 1, 60, 0.01f, OKS_FFT_AUTO, 1};
+}
+
+void oks_output_shape(const oks_kspace* k, int32_t crop, int64_t out_shape[3]) {
+    if (!k || !out_shape) return;
+    int64_t t[3];
+    if (crop && crop_target(k->nz, k->ny, k->nx, k->hdr.recon_z, k->hdr.recon_y, k->hdr.recon_x, t)) {
+        out_shape[0] = std::max<int64_t>(std::min(t[0], k->nz), 1);
+        out_shape[1] = std::max<int64_t>(std::min(t[1], k->ny), 1);
+        out_shape[2] = std::max<int64_t>(std::min(t[2], k->nx), 1);
+    } else {
+        out_shape[0] = k->nz;
+        out_shape[1] = k->ny;
+        out_shape[2] = k->nx;
+    }
+}
+
+int oks_detect_pattern(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx, oks_pattern* out) {
+    if (!mask || !out) return OKS_E_INVALID_ARG;
+    return detect_pattern(mask, nz, ny, nx, out) ? OKS_OK : OKS_E_UNSUPPORTED;
+}
+
+}  // extern "C"

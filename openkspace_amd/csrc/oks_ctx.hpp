@@ -1,0 +1,55 @@
+// oks_ctx.hpp -- backend context: stream, FFT plan cache, device workspace.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/oks.h"
+#include "oks_fft.hpp"
+
+namespace oks {
+
+struct Status : std::runtime_error {
+    int code;
+    Status(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        const int code = (e == hipErrorOutOfMemory) ? OKS_E_OOM : OKS_E_DEVICE;
+        throw Status(code, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+#define OKS_HIP(x) ::oks::hip_check((x), #x)
+
+inline void fail(int code, const std::string& m) { throw Status(code, m); }
+
+// Grow-only device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+}  // namespace oks
+
+struct oks_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    std::recursive_mutex mu;                       // one ctx = one logical owner; calls serialise
+    std::map<int, float2*> tw;                     // n -> twiddle table on device
+    std::map<std::string, oks::DevBuf> ws;         // named workspaces
+    int* d_status = nullptr;                       // device-side flags (whitener) [8]
+    uint32_t pending_flags = 0;
+
+    oks::FftPlan plan(int n);
+    void* buf(const std::string& name, size_t bytes);
+    void release();
+};

@@ -1,0 +1,188 @@
+// oks_fft.hpp -- LDS-resident mixed-radix FFT for CDNA4 (gfx950).
+//
+// One transform is owned by a group of G lanes (a single 64-lane wavefront in
+// every kernel here, so stages synchronise with a wavefront barrier only).
+// Each stage is a Stockham autosort step: butterfly j reads x[j + r n/R],
+// applies the twiddle w_{Ns R}^{r (j mod Ns)}, runs an R-point DFT and writes
+// y[(j - j mod Ns) R + j mod Ns + r Ns]; after all stages the result is in
+// natural order.  Radices 2, 3, 4, 5, 8 have hand-written butterflies; any
+// other prime (23 for the 368-line ky axis) uses a per-output DFT stage that
+// spreads the n outputs over the lanes instead of the n/R butterflies.
+//
+// Twiddles come from a per-size table tw[k] = exp(+2 pi i k / n) built in
+// double precision on the host; the forward direction conjugates on the fly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace oks {
+
+constexpr int kMaxStages = 12;
+
+struct FftPlan {
+    int n;
+    int nst;
+    int rad[kMaxStages];
+    const float2* tw;  // n entries, exp(+2 pi i k / n)
+};
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by s*i, s = +1 (inverse) or -1 (forward)
+template <bool INV>
+__device__ __forceinline__ float2 mul_si(float2 a) {
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+template <bool INV>
+__device__ __forceinline__ float2 twid(const float2* tw, int idx) {
+    float2 w = tw[idx];
+    if (!INV) w.y = -w.y;
+    return w;
+}
+
+// Wavefront-scope LDS ordering between Stockham stages.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft(float2* v);
+
+template <> __device__ __forceinline__ void dft<2, true>(float2* v) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b); v[1] = csub(a, b);
+}
+template <> __device__ __forceinline__ void dft<2, false>(float2* v) { dft<2, true>(v); }
+
+template <bool INV>
+__device__ __forceinline__ void dft4(float2& v0, float2& v1, float2& v2, float2& v3) {
+    float2 t0 = cadd(v0, v2), t1 = csub(v0, v2), t2 = cadd(v1, v3), t3 = mul_si<INV>(csub(v1, v3));
+    v0 = cadd(t0, t2); v2 = csub(t0, t2);
+    v1 = cadd(t1, t3); v3 = csub(t1, t3);
+}
+template <> __device__ __forceinline__ void dft<4, true>(float2* v) { dft4<true>(v[0], v[1], v[2], v[3]); }
+template <> __device__ __forceinline__ void dft<4, false>(float2* v) { dft4<false>(v[0], v[1], v[2], v[3]); }
+
+template <bool INV>
+__device__ __forceinline__ void dft8(float2* v) {
+    // even/odd split: E = DFT4(v0,v2,v4,v6), O = DFT4(v1,v3,v5,v7)
+    float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<INV>(e0, e1, e2, e3);
+    dft4<INV>(o0, o1, o2, o3);
+    const float c = 0.70710678118654752440f;
+    const float s = INV ? 1.f : -1.f;
+    float2 w1o1 = cmul(o1, make_float2(c, s * c));
+    float2 w2o2 = mul_si<INV>(o2);
+    float2 w3o3 = cmul(o3, make_float2(-c, s * c));
+    v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+    v[1] = cadd(e1, w1o1); v[5] = csub(e1, w1o1);
+    v[2] = cadd(e2, w2o2); v[6] = csub(e2, w2o2);
+    v[3] = cadd(e3, w3o3); v[7] = csub(e3, w3o3);
+}
+template <> __device__ __forceinline__ void dft<8, true>(float2* v) { dft8<true>(v); }
+template <> __device__ __forceinline__ void dft<8, false>(float2* v) { dft8<false>(v); }
+
+template <bool INV>
+__device__ __forceinline__ void dft3(float2* v) {
+    const float h = 0.86602540378443864676f;
+    float2 t = cadd(v[1], v[2]);
+    float2 u = mul_si<INV>(cscale(csub(v[1], v[2]), h));
+    float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
+    v[0] = cadd(v[0], t);
+    v[1] = cadd(m, u);
+    v[2] = csub(m, u);
+}
+template <> __device__ __forceinline__ void dft<3, true>(float2* v) { dft3<true>(v); }
+template <> __device__ __forceinline__ void dft<3, false>(float2* v) { dft3<false>(v); }
+
+template <bool INV>
+__device__ __forceinline__ void dft5(float2* v) {
+    const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+    const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+    float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    float2 u1 = csub(v[1], v[4]), u2 = csub(v[2], v[3]);
+    float2 a = make_float2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+    float2 b = make_float2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+    float2 p = mul_si<INV>(make_float2(s1 * u1.x + s2 * u2.x, s1 * u1.y + s2 * u2.y));
+    float2 q = mul_si<INV>(make_float2(s2 * u1.x - s1 * u2.x, s2 * u1.y - s1 * u2.y));
+    v[0] = cadd(v[0], cadd(t1, t2));
+    v[1] = cadd(a, p); v[4] = csub(a, p);
+    v[2] = cadd(b, q); v[3] = csub(b, q);
+}
+template <> __device__ __forceinline__ void dft<5, true>(float2* v) { dft5<true>(v); }
+template <> __device__ __forceinline__ void dft<5, false>(float2* v) { dft5<false>(v); }
+
+template <int R, bool INV>
+__device__ __forceinline__ void stage(const float2* __restrict__ in, float2* __restrict__ out, int n,
+                                      int Ns, int tws, const float2* __restrict__ tw, int t, int G) {
+    const int nR = n / R;
+    for (int j = t; j < nR; j += G) {
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = in[j + r * nR];
+        const int k = j % Ns;
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twid<INV>(tw, r * k * tws));
+        }
+        dft<R, INV>(v);
+        const int o = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) out[o + r * Ns] = v[r];
+    }
+}
+
+// Generic radix R (any prime): one lane per output element.
+template <bool INV>
+__device__ __forceinline__ void stage_generic(const float2* __restrict__ in, float2* __restrict__ out,
+                                              int n, int R, int Ns, int tws,
+                                              const float2* __restrict__ tw, int t, int G) {
+    const int nR = n / R;
+    for (int i = t; i < n; i += G) {
+        const int j = i / R, rr = i - j * R;
+        const int k = j % Ns;
+        // sum_q in[j + q nR] * w_n^{q (k tws + rr nR)}
+        const int step = (k * tws + rr * nR) % n;
+        float2 acc = in[j];
+        int e = step;
+        for (int q = 1; q < R; ++q) {
+            acc = cadd(acc, cmul(in[j + q * nR], twid<INV>(tw, e)));
+            e += step;
+            if (e >= n) e -= n;
+        }
+        out[(j - k) * R + k + rr * Ns] = acc;
+    }
+}
+
+// Run all stages on a[0..n) with scratch b[0..n); returns the buffer holding
+// the natural-order result.  Lanes t in [0, G) of one wavefront.
+template <bool INV>
+__device__ float2* fft_wave(float2* a, float2* b, const FftPlan& P, int t, int G) {
+    int Ns = 1;
+    const int n = P.n;
+    for (int s = 0; s < P.nst; ++s) {
+        const int R = P.rad[s];
+        const int tws = n / (Ns * R);
+        switch (R) {
+            case 2: stage<2, INV>(a, b, n, Ns, tws, P.tw, t, G); break;
+            case 3: stage<3, INV>(a, b, n, Ns, tws, P.tw, t, G); break;
+            case 4: stage<4, INV>(a, b, n, Ns, tws, P.tw, t, G); break;
+            case 5: stage<5, INV>(a, b, n, Ns, tws, P.tw, t, G); break;
+            case 8: stage<8, INV>(a, b, n, Ns, tws, P.tw, t, G); break;
+            default: stage_generic<INV>(a, b, n, R, Ns, tws, P.tw, t, G); break;
+        }
+        wave_sync();
+        float2* tmp = a; a = b; b = tmp;
+        Ns *= R;
+    }
+    return a;
+}
+
+}  // namespace oks

@@ -1,0 +1,384 @@
+// oks_kernels.hip -- CDNA4 kernels of the centred-FFT / pre-whitening / RSS
+// path (ifft-rss) and the generic row/column transform passes the other
+// strategies reuse.
+//
+// Layouts are the reference's C-contiguous ones: k-space [nc, nz, ny, nx]
+// complex64, image [nz, ty, tx] float32.
+#include <hip/hip_runtime.h>
+
+#include "oks_fft.hpp"
+#include "oks_kernels.hpp"
+
+namespace oks {
+
+#define OKS_ATTR(x)                         \
+    do {                                    \
+        hipError_t e_ = (x);                \
+        if (e_ != hipSuccess) return e_;    \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Noise whitener, FP64 (recon/prewhiten.rs:48-111).
+// Psi = (1/max(N-1,1)) sum_s n_s n_s^H, L = chol(Psi), W = L^-1, all in double
+// in LDS; W is stored as complex64.  If Psi is not positive definite (the
+// reference returns None and skips pre-whitening) W = I and *status = 1.
+// One workgroup of 256 threads, nc <= 32.
+__global__ void __launch_bounds__(256) k_whitener(const float2* __restrict__ noise, int nc, int ns,
+                                                  float2* __restrict__ W, int* __restrict__ status) {
+    __shared__ double2 psi[32 * 32];
+    __shared__ double2 L[32 * 32];
+    __shared__ double2 Li[32 * 32];
+    __shared__ int bad;
+    const int t = threadIdx.x;
+    const double denom = fmax((double)ns - 1.0, 1.0);
+    for (int e = t; e < nc * nc; e += blockDim.x) {
+        const int i = e / nc, j = e % nc;
+        double re = 0.0, im = 0.0;
+        if (j <= i) {
+            const float2* a = noise + (size_t)i * ns;
+            const float2* b = noise + (size_t)j * ns;
+            for (int s = 0; s < ns; ++s) {
+                const double ar = a[s].x, ai = a[s].y, br = b[s].x, bi = b[s].y;
+                re += ar * br + ai * bi;  // a * conj(b)
+                im += ai * br - ar * bi;
+            }
+        }
+        psi[e] = make_double2(re / denom, im / denom);
+        L[e] = make_double2(0.0, 0.0);
+        Li[e] = make_double2(0.0, 0.0);
+    }
+    if (t == 0) bad = 0;
+    __syncthreads();
+    // Row-wise Cholesky (recon/prewhiten.rs:148-179) on the lower triangle.
+    for (int i = 0; i < nc; ++i) {
+        if (t == 0) {
+            double dr = psi[i * nc + i].x;
+            for (int k = 0; k < i; ++k) {
+                double2 l = L[i * nc + k];
+                dr -= l.x * l.x + l.y * l.y;
+            }
+            if (!(dr > 0.0) || !isfinite(dr)) bad = 1;
+            L[i * nc + i] = make_double2(sqrt(fmax(dr, 0.0)), 0.0);
+        }
+        __syncthreads();
+        if (bad) break;
+        const double inv = 1.0 / L[i * nc + i].x;
+        for (int j = i + 1 + t; j < nc; j += blockDim.x) {
+            double sr = psi[j * nc + i].x, si = psi[j * nc + i].y;
+            for (int k = 0; k < i; ++k) {
+                const double2 a = L[j * nc + k], b = L[i * nc + k];  // a * conj(b)
+                sr -= a.x * b.x + a.y * b.y;
+                si -= a.y * b.x - a.x * b.y;
+            }
+            L[j * nc + i] = make_double2(sr * inv, si * inv);
+        }
+        __syncthreads();
+    }
+    if (!bad) {
+        // W = L^-1 by forward substitution, one column per thread.
+        for (int col = t; col < nc; col += blockDim.x) {
+            for (int i = col; i < nc; ++i) {
+                double sr = (i == col) ? 1.0 : 0.0, si = 0.0;
+                for (int j = col; j < i; ++j) {
+                    const double2 a = L[i * nc + j], b = Li[j * nc + col];
+                    sr -= a.x * b.x - a.y * b.y;
+                    si -= a.x * b.y + a.y * b.x;
+                }
+                const double d = L[i * nc + i].x;
+                Li[i * nc + col] = make_double2(sr / d, si / d);
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = t; e < nc * nc; e += blockDim.x) {
+        const int i = e / nc, j = e % nc;
+        double2 v = bad ? make_double2(i == j ? 1.0 : 0.0, 0.0) : Li[e];
+        W[e] = make_float2((float)v.x, (float)v.y);
+    }
+    if (t == 0 && bad) *status = 1;  // sticky until the host takes the flags
+}
+
+// ---------------------------------------------------------------------------
+// Row pass of ifft-rss.  One workgroup per (z, y) row: every coil's readout
+// is loaded once from HBM, whitened in registers (y = W s,
+// recon/prewhiten.rs:128-141), written to LDS at its ifftshift position, then
+// one wavefront per coil runs the kx IFFT and stores the fftshifted, x-cropped
+// line (recon/fft.rs:19-29 + recon/strategy.rs:227-253).  With os_mode the
+// reference's oversampling-removal chain (recon/oversampling.rs:122-147) and
+// its re-centring into the nx grid (io/ismrmrd/reader.rs:392-399) run on the
+// line first.
+template <int NCMAX>
+__global__ void __launch_bounds__(512) k_rows_prewhiten_ifft(RowPwArgs a) {
+    extern __shared__ float2 lds[];
+    const int nx = a.nx, nc = a.nc;
+    const int WPB = blockDim.x >> 6;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float2* A = lds;                       // nc * nx
+    float2* B = lds + (size_t)nc * nx;     // WPB * nx scratch
+    const size_t row = blockIdx.x;
+    const size_t plane = (size_t)a.rows * nx;
+    const float2* __restrict__ src = a.in + row * nx;
+    const int half = nx / 2;
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
+        float2 s[NCMAX];
+#pragma unroll
+        for (int j = 0; j < NCMAX; ++j)
+            if (j < nc) s[j] = src[j * plane + x];
+        const int pos = (x + half) % nx;
+        if (a.W) {
+#pragma unroll
+            for (int i = 0; i < NCMAX; ++i) {
+                if (i < nc) {
+                    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+                    for (int j = 0; j <= i; ++j) acc = cadd(acc, cmul(a.W[i * nc + j], s[j]));
+                    A[(size_t)i * nx + pos] = acc;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < NCMAX; ++i)
+                if (i < nc) A[(size_t)i * nx + pos] = s[i];
+        }
+    }
+    __syncthreads();
+    const int tx = a.tx, x0 = a.x0;
+    for (int c = w; c < nc; c += WPB) {
+        float2* buf = A + (size_t)c * nx;
+        float2* scr = B + (size_t)w * nx;
+        float2* res = fft_wave<true>(buf, scr, a.p_enc, lane, 64);
+        float sc = a.scale;
+        if (a.os_mode) {
+            // res holds b = IFFT_nx(ifftshift(line)); img[x] = b[(x+nx/2)%nx] / nx
+            const int rec = a.rec, crop0 = (nx - rec) / 2, rh = (rec + 1) / 2;
+            float2* q = (res == buf) ? scr : buf;
+            const float inv_nx = 1.0f / (float)nx;
+            for (int i = lane; i < rec; i += 64) {
+                const int m = (i + rh) % rec;  // ifftshift of the cropped line
+                q[i] = cscale(res[(m + crop0 + half) % nx], inv_nx);
+            }
+            wave_sync();
+            float2* other = (q == buf) ? scr : buf;
+            float2* S = fft_wave<false>(q, other, a.p_rec, lane, 64);
+            float2* K = (S == buf) ? scr : buf;
+            const int dst = half - rec / 2;
+            const int nh = (nx + 1) / 2;
+            for (int i = lane; i < nx; i += 64) {
+                const int xk = (i + nh) % nx;  // ifftshift of the re-centred grid line
+                const int m = xk - dst;
+                K[i] = (m >= 0 && m < rec) ? S[(m + rec / 2) % rec] : make_float2(0.f, 0.f);
+            }
+            wave_sync();
+            float2* other2 = (K == buf) ? scr : buf;
+            res = fft_wave<true>(K, other2, a.p_enc, lane, 64);
+        }
+        float2* dstp = a.out + (size_t)c * a.rows * tx + row * tx;
+        for (int x = lane; x < tx; x += 64) dstp[x] = cscale(res[(x0 + x + half) % nx], sc);
+        wave_sync();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Generic row pass: centred 1-D transform along the contiguous axis of
+// [rows, n]; optional input mask multiply (mask row index = row % mask_rows),
+// output crop window [x0, x0+tx), scale.  One wavefront per row.
+template <bool INV>
+__global__ void __launch_bounds__(256) k_rows(RowArgs a) {
+    extern __shared__ float2 lds[];
+    const int n = a.p.n;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const size_t row = (size_t)blockIdx.x * (blockDim.x >> 6) + w;
+    if (row >= (size_t)a.rows) return;
+    float2* A = lds + (size_t)w * 2 * n;
+    float2* B = A + n;
+    const float2* __restrict__ src = a.in + row * n;
+    const uint8_t* m = a.mask ? a.mask + (row % a.mask_rows) * n : nullptr;
+    const int half = a.centred ? n / 2 : 0;
+    for (int x = lane; x < n; x += 64) {
+        float2 v = src[x];
+        if (m && !m[x]) v = make_float2(0.f, 0.f);
+        A[(x + half) % n] = v;
+    }
+    wave_sync();
+    float2* res = fft_wave<INV>(A, B, a.p, lane, 64);
+    float2* dst = a.out + row * a.tx;
+    for (int x = lane; x < a.tx; x += 64) dst[x] = cscale(res[(a.x0 + x + half) % n], a.scale);
+}
+
+// ---------------------------------------------------------------------------
+// Column pass along the middle axis of [C, Z, n, X] (X contiguous).  One
+// workgroup per (z, strip of SX columns); coils are looped inside so that the
+// RSS epilogue (recon/coil.rs:24-27, coil order 0..C-1) accumulates in LDS
+// and the image is written once.  mode 0 writes the transformed complex data
+// (crop window [y0, y0+ty) along n); mode 1 writes sqrt(sum_c |.|^2) * scale.
+template <bool INV>
+__global__ void __launch_bounds__(512) k_cols(ColArgs a) {
+    extern __shared__ float2 lds[];
+    const int n = a.p.n, SX = a.sx;
+    const int WPB = blockDim.x >> 6;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float2* A = lds;                        // SX * n
+    float2* B = lds + (size_t)SX * n;       // SX * n
+    float* acc = (float*)(lds + (size_t)2 * SX * n);  // SX * ty (mode 1)
+    const int strips = (a.X + SX - 1) / SX;
+    const int zi = blockIdx.x / strips;      // output z
+    const int z = a.z0 + zi;                 // input z
+    const int xs = (blockIdx.x % strips) * SX;
+    const int half = a.centred ? n / 2 : 0;
+    const int ty = a.ty, y0 = a.y0;
+    if (a.mode == 1)
+        for (int i = threadIdx.x; i < SX * ty; i += blockDim.x) acc[i] = 0.f;
+    for (int c = 0; c < a.C; ++c) {
+        const float2* __restrict__ src = a.in + ((size_t)c * a.Z + z) * n * a.X;
+        for (int i = threadIdx.x; i < n * SX; i += blockDim.x) {
+            const int y = i / SX, col = i - y * SX;
+            float2 v = make_float2(0.f, 0.f);
+            if (xs + col < a.X) v = src[(size_t)y * a.X + xs + col];
+            A[(size_t)col * n + (y + half) % n] = v;
+        }
+        __syncthreads();
+        for (int col = w; col < SX; col += WPB) {
+            float2* res = fft_wave<INV>(A + (size_t)col * n, B + (size_t)col * n, a.p, lane, 64);
+            if (a.mode == 1) {
+                float* ac = acc + (size_t)col * ty;
+                for (int yo = lane; yo < ty; yo += 64) {
+                    const float2 v = res[(y0 + yo + half) % n];
+                    ac[yo] += v.x * v.x + v.y * v.y;
+                }
+            } else if (res != A + (size_t)col * n) {
+                float2* d = A + (size_t)col * n;
+                for (int y = lane; y < n; y += 64) d[y] = res[y];
+            }
+        }
+        __syncthreads();
+        if (a.mode == 0) {
+            float2* dst = a.out + ((size_t)c * a.Zo + zi) * ty * a.X;
+            for (int i = threadIdx.x; i < ty * SX; i += blockDim.x) {
+                const int yo = i / SX, col = i - yo * SX;
+                if (xs + col < a.X)
+                    dst[(size_t)yo * a.X + xs + col] = cscale(A[(size_t)col * n + (y0 + yo + half) % n], a.scale);
+            }
+            __syncthreads();
+        }
+    }
+    if (a.mode == 1) {
+        __syncthreads();
+        float* dst = a.out_r + (size_t)zi * ty * a.X;
+        for (int i = threadIdx.x; i < ty * SX; i += blockDim.x) {
+            const int yo = i / SX, col = i - yo * SX;
+            if (xs + col < a.X) dst[(size_t)yo * a.X + xs + col] = sqrtf(acc[(size_t)col * ty + yo]) * a.scale;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stand-alone pre-whitening y = W s over the sample columns of [nc, M]
+// (recon/prewhiten.rs:128-141), in place.
+template <int NCMAX>
+__global__ void __launch_bounds__(256) k_prewhiten(float2* __restrict__ d, const float2* __restrict__ W,
+                                                   int nc, long long M) {
+    for (long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x; x < M;
+         x += (long long)gridDim.x * blockDim.x) {
+        float2 s[NCMAX];
+#pragma unroll
+        for (int j = 0; j < NCMAX; ++j)
+            if (j < nc) s[j] = d[j * M + x];
+#pragma unroll
+        for (int i = 0; i < NCMAX; ++i) {
+            if (i < nc) {
+                float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+                for (int j = 0; j <= i; ++j) acc = cadd(acc, cmul(W[i * nc + j], s[j]));
+                d[i * M + x] = acc;
+            }
+        }
+    }
+}
+
+// RSS over the leading axis of [C, M] complex (recon/coil.rs:24-27).
+__global__ void k_rss(const float2* __restrict__ in, int C, long long M, float* __restrict__ out, float scale) {
+    for (long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x; x < M;
+         x += (long long)gridDim.x * blockDim.x) {
+        float s = 0.f;
+        for (int c = 0; c < C; ++c) {
+            const float2 v = in[c * M + x];
+            s += v.x * v.x + v.y * v.y;
+        }
+        out[x] = sqrtf(s) * scale;
+    }
+}
+
+// ------------------------------------------------------------- launchers
+hipError_t launch_whitener(const float2* noise, int nc, int ns, float2* W, int* status, hipStream_t s) {
+    hipLaunchKernelGGL(k_whitener, dim3(1), dim3(256), 0, s, noise, nc, ns, W, status);
+    return hipGetLastError();
+}
+
+size_t rows_prewhiten_lds(int nc, int nx, int os_mode) {
+    (void)os_mode;
+    return ((size_t)nc * nx + (size_t)8 * nx) * sizeof(float2);
+}
+
+hipError_t launch_rows_prewhiten_ifft(const RowPwArgs& a, hipStream_t s) {
+    const size_t lds = rows_prewhiten_lds(a.nc, a.nx, a.os_mode);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    if (a.nc <= 16) {
+        OKS_ATTR(hipFuncSetAttribute((const void*)k_rows_prewhiten_ifft<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_rows_prewhiten_ifft<16>, dim3((unsigned)a.rows), dim3(512), lds, s, a);
+    } else if (a.nc <= 32) {
+        OKS_ATTR(hipFuncSetAttribute((const void*)k_rows_prewhiten_ifft<32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_rows_prewhiten_ifft<32>, dim3((unsigned)a.rows), dim3(512), lds, s, a);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_rows(const RowArgs& a, bool inverse, hipStream_t s) {
+    const int wpb = 4;
+    const size_t lds = (size_t)wpb * 2 * a.p.n * sizeof(float2);
+    const unsigned grid = (unsigned)((a.rows + wpb - 1) / wpb);
+    if (inverse) {
+        OKS_ATTR(hipFuncSetAttribute((const void*)k_rows<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_rows<true>, dim3(grid), dim3(64 * wpb), lds, s, a);
+    } else {
+        OKS_ATTR(hipFuncSetAttribute((const void*)k_rows<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_rows<false>, dim3(grid), dim3(64 * wpb), lds, s, a);
+    }
+    return hipGetLastError();
+}
+
+size_t cols_lds(int sx, int n, int ty, int mode) {
+    return (size_t)2 * sx * n * sizeof(float2) + (mode == 1 ? (size_t)sx * ty * sizeof(float) : 0);
+}
+
+hipError_t launch_cols(const ColArgs& a, bool inverse, hipStream_t s) {
+    const size_t lds = cols_lds(a.sx, a.p.n, a.ty, a.mode);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    const int strips = (a.X + a.sx - 1) / a.sx;
+    const unsigned grid = (unsigned)(a.Zo * strips);
+    if (inverse) {
+        OKS_ATTR(hipFuncSetAttribute((const void*)k_cols<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_cols<true>, dim3(grid), dim3(512), lds, s, a);
+    } else {
+        OKS_ATTR(hipFuncSetAttribute((const void*)k_cols<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_cols<false>, dim3(grid), dim3(512), lds, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_prewhiten(float2* d, const float2* W, int nc, long long M, hipStream_t s) {
+    const unsigned grid = (unsigned)std::min<long long>((M + 255) / 256, 4096);
+    if (nc <= 16) hipLaunchKernelGGL(k_prewhiten<16>, dim3(grid), dim3(256), 0, s, d, W, nc, M);
+    else if (nc <= 32) hipLaunchKernelGGL(k_prewhiten<32>, dim3(grid), dim3(256), 0, s, d, W, nc, M);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_rss(const float2* in, int C, long long M, float* out, float scale, hipStream_t s) {
+    const unsigned grid = (unsigned)std::min<long long>((M + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_rss, dim3(grid), dim3(256), 0, s, in, C, M, out, scale);
+    return hipGetLastError();
+}
+
+}  // namespace oks

@@ -1,0 +1,63 @@
+// oks_kernels.hpp -- argument blocks and launchers of the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "oks_fft.hpp"
+
+namespace oks {
+
+// Fused pre-whitening + centred kx IFFT (+ oversampling chain) + x crop.
+struct RowPwArgs {
+    const float2* in;   // [nc, rows, nx]
+    float2* out;        // [nc, rows, tx]
+    const float2* W;    // [nc, nc] lower triangular, or nullptr
+    int nc;
+    long long rows;     // nz * ny
+    int nx, tx, x0;     // output = centred columns [x0, x0 + tx)
+    float scale;
+    int os_mode, rec;   // reference oversampling removal to `rec` samples
+    FftPlan p_enc;      // n = nx, inverse
+    FftPlan p_rec;      // n = rec, forward (os_mode)
+};
+
+// Generic centred transform along the contiguous axis.
+struct RowArgs {
+    const float2* in;   // [rows, n]
+    float2* out;        // [rows, tx]
+    const uint8_t* mask;  // optional [mask_rows, n] (row % mask_rows), 0 -> zero input
+    long long rows, mask_rows;
+    int tx, x0;
+    int centred;
+    float scale;
+    FftPlan p;
+};
+
+// Transform along the middle axis of [C, Z, n, X]; the grid covers output
+// slices z in [z0, z0 + Zo).
+struct ColArgs {
+    const float2* in;
+    float2* out;        // mode 0: [C, Zo, ty, X]
+    float* out_r;       // mode 1: [Zo, ty, X]
+    int C, Z, X, sx;
+    int z0, Zo;
+    int mode;           // 0 complex, 1 RSS over C
+    int y0, ty;         // centred output window along n
+    int centred;
+    float scale;
+    FftPlan p;
+};
+
+hipError_t launch_whitener(const float2* noise, int nc, int ns, float2* W, int* status, hipStream_t s);
+size_t rows_prewhiten_lds(int nc, int nx, int os_mode);
+hipError_t launch_rows_prewhiten_ifft(const RowPwArgs& a, hipStream_t s);
+hipError_t launch_rows(const RowArgs& a, bool inverse, hipStream_t s);
+size_t cols_lds(int sx, int n, int ty, int mode);
+hipError_t launch_cols(const ColArgs& a, bool inverse, hipStream_t s);
+hipError_t launch_prewhiten(float2* d, const float2* W, int nc, long long M, hipStream_t s);
+hipError_t launch_rss(const float2* in, int C, long long M, float* out, float scale, hipStream_t s);
+
+}  // namespace oks

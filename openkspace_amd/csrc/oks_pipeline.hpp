@@ -1,0 +1,26 @@
+// oks_pipeline.hpp -- building blocks shared by the strategy compositions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/oks.h"
+#include "oks_ctx.hpp"
+#include "oks_host.hpp"
+
+namespace oks {
+
+const float2* whitener(oks_ctx* ctx, const float2* noise, int64_t nc, int64_t nn, uint32_t* flags, const char* slot);
+void col_pass(oks_ctx* ctx, const float2* in, float2* out_c, float* out_r, int C, int Z, int z0, int Zo, int n,
+              int X, int mode, int y0, int ty, float scale, bool inverse, bool centred);
+void row_pass(oks_ctx* ctx, const float2* in, float2* out, long long rows, int n, int tx, int x0, float scale,
+              bool inverse, bool centred, const uint8_t* mask, long long mask_rows);
+void fft_axis(oks_ctx* ctx, float2* d, const int64_t* shape, int ndim, int axis, bool inverse, float scale,
+              bool centred);
+bool three_d_of(int32_t mode, const oks_kspace* k);
+void output_windows(const oks_kspace* k, int crop, Win w[3], uint32_t* flags);
+void ifft_rss_grid(oks_ctx* ctx, const float2* k, int64_t nc, int64_t nz, int64_t ny, int64_t nx, const float2* W,
+                   int os_mode, int rec, bool three_d, const Win w[3], float* out);
+
+}  // namespace oks

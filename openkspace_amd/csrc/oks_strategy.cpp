@@ -1,0 +1,305 @@
+// oks_strategy.cpp -- ReconStrategy compositions on the GPU and the
+// function-level device entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/oks.h"
+#include "oks_ctx.hpp"
+#include "oks_host.hpp"
+#include "oks_kernels.hpp"
+#include "oks_pipeline.hpp"
+
+namespace oks {
+
+// Noise whitener W = chol(Psi)^-1 on the device (FP64), or nullptr where the
+// reference would return None before computing anything
+// (recon/prewhiten.rs:49-66).  A non-PD covariance leaves W = I on the device
+// and raises the sticky OKS_F_PREWHITEN_SKIPPED flag.
+const float2* whitener(oks_ctx* ctx, const float2* noise, int64_t nc, int64_t nn, uint32_t* flags,
+                       const char* slot) {
+    if (!noise || nn <= 0 || nc <= 0 || nn < nc) {
+        *flags |= OKS_F_PREWHITEN_SKIPPED;
+        return nullptr;
+    }
+    if (nc > 32) fail(OKS_E_UNSUPPORTED, "pre-whitening supports at most 32 channels");
+    float2* W = (float2*)ctx->buf(slot, sizeof(float2) * nc * nc);
+    OKS_HIP(launch_whitener(noise, (int)nc, (int)nn, W, ctx->d_status + 0, ctx->stream));
+    return W;
+}
+
+static int pick_sx(int n, int ty, int mode, int X) {
+    int sx = 16;
+    while (sx > 1 && cols_lds(sx, n, ty, mode) > 160 * 1024) sx /= 2;
+    while (sx > 1 && sx / 2 >= X) sx /= 2;
+    return sx;
+}
+
+void col_pass(oks_ctx* ctx, const float2* in, float2* out_c, float* out_r, int C, int Z, int z0, int Zo, int n,
+              int X, int mode, int y0, int ty, float scale, bool inverse, bool centred) {
+    ColArgs a{};
+    a.in = in;
+    a.out = out_c;
+    a.out_r = out_r;
+    a.C = C;
+    a.Z = Z;
+    a.X = X;
+    a.z0 = z0;
+    a.Zo = Zo;
+    a.mode = mode;
+    a.y0 = y0;
+    a.ty = ty;
+    a.centred = centred;
+    a.scale = scale;
+    a.p = ctx->plan(n);
+    a.sx = pick_sx(n, ty, mode, X);
+    if (cols_lds(a.sx, n, ty, mode) > 160 * 1024) fail(OKS_E_UNSUPPORTED, "transform length too large for LDS");
+    OKS_HIP(launch_cols(a, inverse, ctx->stream));
+}
+
+void row_pass(oks_ctx* ctx, const float2* in, float2* out, long long rows, int n, int tx, int x0, float scale,
+              bool inverse, bool centred, const uint8_t* mask, long long mask_rows) {
+    RowArgs a{};
+    a.in = in;
+    a.out = out;
+    a.mask = mask;
+    a.mask_rows = mask_rows > 0 ? mask_rows : 1;
+    a.rows = rows;
+    a.tx = tx;
+    a.x0 = x0;
+    a.centred = centred;
+    a.scale = scale;
+    a.p = ctx->plan(n);
+    if ((size_t)4 * 2 * n * sizeof(float2) > 160 * 1024) fail(OKS_E_UNSUPPORTED, "row length too large for LDS");
+    OKS_HIP(launch_rows(a, inverse, ctx->stream));
+}
+
+// In-place centred (or plain) transform along one axis of an N-d tensor.
+void fft_axis(oks_ctx* ctx, float2* d, const int64_t* shape, int ndim, int axis, bool inverse, float scale,
+              bool centred) {
+    int64_t outer = 1, inner = 1;
+    for (int i = 0; i < axis; ++i) outer *= shape[i];
+    for (int i = axis + 1; i < ndim; ++i) inner *= shape[i];
+    const int n = (int)shape[axis];
+    if (n < 2) {
+        if (scale != 1.0f) fail(OKS_E_INVALID_ARG, "scale on a length-1 axis");
+        return;
+    }
+    if (inner == 1)
+        row_pass(ctx, d, d, outer, n, n, 0, scale, inverse, centred, nullptr, 0);
+    else
+        col_pass(ctx, d, d, nullptr, 1, (int)outer, 0, (int)outer, n, (int)inner, 0, 0, n, scale, inverse, centred);
+}
+
+static void check_kspace(const oks_kspace* k, bool need_noise_ok = true) {
+    (void)need_noise_ok;
+    if (!k || !k->kspace || !k->mask) fail(OKS_E_INVALID_ARG, "null k-space or mask");
+    if (k->nc <= 0 || k->nz <= 0 || k->ny <= 0 || k->nx <= 0) fail(OKS_E_INVALID_ARG, "empty k-space");
+    if (k->nc > 32) fail(OKS_E_UNSUPPORTED, "at most 32 channels");
+    if (k->ny > 4096 || k->nx > 4096 || k->nz > 65535) fail(OKS_E_UNSUPPORTED, "grid too large");
+}
+
+bool three_d_of(int32_t mode, const oks_kspace* k) {
+    if (mode == OKS_FFT_2D) return false;
+    if (mode == OKS_FFT_3D) return true;
+    return k->hdr.is_3d_encoding != 0;
+}
+
+void output_windows(const oks_kspace* k, int crop, Win w[3], uint32_t* flags) {
+    int64_t t[3] = {k->nz, k->ny, k->nx};
+    if (crop && crop_target(k->nz, k->ny, k->nx, k->hdr.recon_z, k->hdr.recon_y, k->hdr.recon_x, t))
+        *flags |= OKS_F_CROPPED;
+    w[0] = crop_win(k->nz, t[0]);
+    w[1] = crop_win(k->ny, t[1]);
+    w[2] = crop_win(k->nx, t[2]);
+}
+
+// IFFT + RSS + crop of an (already pre-processed) grid [nc, nz, ny, nx], with
+// the optional fused whitening W and oversampling chain in the row pass.
+void ifft_rss_grid(oks_ctx* ctx, const float2* k, int64_t nc, int64_t nz, int64_t ny, int64_t nx, const float2* W,
+                   int os_mode, int rec, bool three_d, const Win w[3], float* out) {
+    const int64_t rows = nz * ny;
+    const int tx = (int)w[2].len;
+    float2* tmp = (float2*)ctx->buf("rss_tmp", sizeof(float2) * nc * rows * tx);
+    RowPwArgs a{};
+    a.in = k;
+    a.out = tmp;
+    a.W = W;
+    a.nc = (int)nc;
+    a.rows = rows;
+    a.nx = (int)nx;
+    a.tx = tx;
+    a.x0 = (int)w[2].off;
+    a.scale = 1.0f / (float)nx;
+    a.os_mode = os_mode;
+    a.rec = rec;
+    a.p_enc = ctx->plan((int)nx);
+    if (os_mode) a.p_rec = ctx->plan(rec);
+    if (rows_prewhiten_lds((int)nc, (int)nx, os_mode) > 160 * 1024)
+        fail(OKS_E_UNSUPPORTED, "readout too long for the fused row pass");
+    OKS_HIP(launch_rows_prewhiten_ifft(a, ctx->stream));
+    if (!three_d) {
+        col_pass(ctx, tmp, nullptr, out, (int)nc, (int)nz, (int)w[0].off, (int)w[0].len, (int)ny, tx, 1,
+                 (int)w[1].off, (int)w[1].len, 1.0f / (float)ny, true, true);
+    } else {
+        const int ty = (int)w[1].len;
+        float2* tmp2 = (float2*)ctx->buf("rss_tmp2", sizeof(float2) * nc * nz * ty * tx);
+        col_pass(ctx, tmp, tmp2, nullptr, 1, (int)(nc * nz), 0, (int)(nc * nz), (int)ny, tx, 0, (int)w[1].off, ty,
+                 1.0f / (float)ny, true, true);
+        col_pass(ctx, tmp2, nullptr, out, (int)nc, 1, 0, 1, (int)nz, ty * tx, 1, (int)w[0].off, (int)w[0].len,
+                 (nz > 1) ? 1.0f / (float)nz : 1.0f, true, true);
+    }
+}
+
+// IfftRss::reconstruct (recon/strategy.rs:113-223) on device-resident inputs.
+void recon_ifft_rss_dev(oks_ctx* ctx, const oks_ifft_rss_desc* d, const oks_kspace* k, oks_image* img) {
+    check_kspace(k);
+    if (!d || !img || !img->data) fail(OKS_E_INVALID_ARG, "null descriptor or image");
+    uint32_t flags = 0;
+    const bool td = three_d_of(d->fft_mode, k);
+    if (!td && d->partial_fourier && pf_detect(k->mask, k->nz, k->ny, k->nx, k->ny / 2))
+        fail(OKS_E_UNSUPPORTED, "partial-Fourier acquisition: homodyne reconstruction is not on this path");
+    const float2* W = d->prewhiten ? whitener(ctx, (const float2*)k->noise, k->nc, k->n_noise, &flags, "W") : nullptr;
+    int os_mode = 0, rec = 0;
+    if (d->remove_oversampling) {
+        if (os_valid(k->hdr.encoded_x, k->hdr.recon_x) && k->nx == k->hdr.encoded_x) {
+            os_mode = 1;
+            rec = k->hdr.recon_x;
+        } else {
+            flags |= OKS_F_OS_SKIPPED;
+        }
+    }
+    Win w[3];
+    output_windows(k, d->crop_to_recon_matrix, w, &flags);
+    ifft_rss_grid(ctx, (const float2*)k->kspace, k->nc, k->nz, k->ny, k->nx, W, os_mode, rec, td, w, img->data);
+    img->nz = w[0].len;
+    img->ny = w[1].len;
+    img->nx = w[2].len;
+    img->flags = flags;
+}
+
+}  // namespace oks
+
+using namespace oks;
+
+namespace {
+
+template <class F>
+int guarded(oks_ctx* ctx, F f) {
+    if (!ctx) return OKS_E_INVALID_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    try {
+        ctx->err.clear();
+        OKS_HIP(hipSetDevice(ctx->device));
+        f();
+        return OKS_OK;
+    } catch (const Status& s) {
+        ctx->err = s.what();
+        return s.code;
+    } catch (const std::bad_alloc&) {
+        ctx->err = "host allocation failed";
+        return OKS_E_OOM;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return OKS_E_INCONSISTENT;
+    } catch (...) {
+        ctx->err = "unknown error";
+        return OKS_E_INCONSISTENT;
+    }
+}
+
+// Host-buffer wrapper: upload, run the device composition, download, sync.
+template <class Desc, class Fn>
+int host_call(oks_ctx* ctx, const Desc* d, const oks_kspace* k, oks_image* img, Fn dev) {
+    return guarded(ctx, [&] {
+        if (!k || !k->kspace || !k->mask || !img || !img->data) fail(OKS_E_INVALID_ARG, "null argument");
+        const size_t nk = (size_t)k->nc * k->nz * k->ny * k->nx;
+        const size_t nimg = (size_t)k->nz * k->ny * k->nx;
+        float2* dk = (float2*)ctx->buf("host_k", nk * sizeof(float2));
+        OKS_HIP(hipMemcpyAsync(dk, k->kspace, nk * sizeof(float2), hipMemcpyHostToDevice, ctx->stream));
+        float2* dn = nullptr;
+        if (k->noise && k->n_noise > 0) {
+            const size_t nn = (size_t)k->nc * k->n_noise;
+            dn = (float2*)ctx->buf("host_noise", nn * sizeof(float2));
+            OKS_HIP(hipMemcpyAsync(dn, k->noise, nn * sizeof(float2), hipMemcpyHostToDevice, ctx->stream));
+        }
+        float* dimg = (float*)ctx->buf("host_img", nimg * sizeof(float));
+        float* dg = nullptr;
+        if (img->gfactor) dg = (float*)ctx->buf("host_g", nimg * sizeof(float));
+        oks_kspace kd = *k;
+        kd.kspace = (const oks_c32*)dk;
+        kd.noise = (const oks_c32*)dn;
+        oks_image id = *img;
+        id.data = dimg;
+        id.gfactor = dg;
+        dev(ctx, d, &kd, &id);
+        const size_t nout = (size_t)id.nz * id.ny * id.nx;
+        OKS_HIP(hipMemcpyAsync(img->data, dimg, nout * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        if (dg) OKS_HIP(hipMemcpyAsync(img->gfactor, dg, nout * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        int h[8];
+        OKS_HIP(hipMemcpyAsync(h, ctx->d_status, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        OKS_HIP(hipStreamSynchronize(ctx->stream));
+        OKS_HIP(hipMemsetAsync(ctx->d_status, 0, sizeof(h), ctx->stream));
+        uint32_t f = id.flags;
+        if (h[0]) f |= OKS_F_PREWHITEN_SKIPPED;
+        if (h[1]) f |= OKS_F_SENSE_PIXEL_FAILED;
+        if (h[2]) f |= OKS_F_GRAPPA_CALIB_FAILED;
+        img->nz = id.nz;
+        img->ny = id.ny;
+        img->nx = id.nx;
+        img->flags = f;
+    });
+}
+
+}  // namespace
+
+extern "C" {
+
+int oks_recon_ifft_rss(oks_ctx* ctx, const oks_ifft_rss_desc* d, const oks_kspace* k, oks_image* img) {
+    return host_call(ctx, d, k, img, recon_ifft_rss_dev);
+}
+
+int oks_recon_ifft_rss_dev(oks_ctx* ctx, const oks_ifft_rss_desc* d, const oks_kspace* k, oks_image* img) {
+    return guarded(ctx, [&] { recon_ifft_rss_dev(ctx, d, k, img); });
+}
+
+int oks_fft_centered_dev(oks_ctx* ctx, oks_c32* data, const int64_t* shape, int32_t ndim, const int32_t* axes,
+                         int32_t naxes, int32_t inverse, int32_t norm) {
+    return guarded(ctx, [&] {
+        if (!data || !shape || !axes || ndim < 1 || ndim > 8 || naxes < 1 || naxes > ndim)
+            fail(OKS_E_INVALID_ARG, "bad tensor description");
+        for (int i = 0; i < naxes; ++i) {
+            if (axes[i] < 0 || axes[i] >= ndim) fail(OKS_E_INVALID_ARG, "axis out of range");
+            const int64_t n = shape[axes[i]];
+            const float s = norm == 1 ? 1.0f / (float)n : norm == 2 ? 1.0f / std::sqrt((float)n) : 1.0f;
+            if (n >= 2) fft_axis(ctx, (float2*)data, shape, ndim, axes[i], inverse != 0, s, true);
+        }
+    });
+}
+
+int oks_rss_dev(oks_ctx* ctx, const oks_c32* in, int64_t nc, int64_t inner, float* out) {
+    return guarded(ctx, [&] {
+        if (!in || !out || nc < 1 || inner < 0) fail(OKS_E_INVALID_ARG, "bad RSS arguments");
+        OKS_HIP(launch_rss((const float2*)in, (int)nc, inner, out, 1.0f, ctx->stream));
+    });
+}
+
+int oks_prewhiten_calibrate_dev(oks_ctx* ctx, const oks_c32* noise, int32_t nc, int64_t ns, oks_c32* W) {
+    return guarded(ctx, [&] {
+        if (!noise || !W || nc < 1 || nc > 32 || ns < 1) fail(OKS_E_INVALID_ARG, "bad whitener arguments");
+        if (ns < nc) fail(OKS_E_INCONSISTENT, "fewer noise samples than channels");
+        OKS_HIP(launch_whitener((const float2*)noise, nc, (int)ns, (float2*)W, ctx->d_status + 0, ctx->stream));
+    });
+}
+
+int oks_prewhiten_apply_dev(oks_ctx* ctx, const oks_c32* W, oks_c32* data, int32_t nc, int64_t ns) {
+    return guarded(ctx, [&] {
+        if (!W || !data || nc < 1 || nc > 32 || ns < 0) fail(OKS_E_INVALID_ARG, "bad whitening arguments");
+        OKS_HIP(launch_prewhiten((float2*)data, (const float2*)W, nc, ns, ctx->stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,128 @@
+"""GPU parity of the centred FFT passes and the ifft-rss strategy against the oracle.
+
+Bar: rel-L2 <= 1e-5 against the FP32 restatement (FP32 path); the distance to
+the FP64 twin is printed beside it.  Index windows (crop shapes) are exact.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from openkspace_amd import abi, synth
+from tests.gpu_util import fft_centered, recon_dev, rel
+
+pytestmark = pytest.mark.gpu
+TOL32 = 1e-5
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from openkspace_amd._lib import Context
+    c = Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("name", ["ifft2", "ifft2odd", "ifft3", "ifft1", "ifft2big"])
+def test_centred_ifft_golden(ctx, golden, name):
+    x = golden[f"{name}_in"]
+    axes = tuple(int(a) for a in golden[f"{name}_axes"])
+    got = fft_centered(ctx, x, axes, inverse=True, norm=1)
+    err = rel(got, golden[f"{name}_out"])
+    print(f"{name}: rel vs numpy FP64 = {err:.2e}")
+    assert err < 2e-6
+
+
+@pytest.mark.parametrize("n", [8, 16, 23, 46, 160, 256, 320, 368, 384, 640])
+@pytest.mark.parametrize("inverse", [True, False])
+def test_centred_1d_every_radix(ctx, n, inverse):
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n))).astype(np.complex64)
+    got = fft_centered(ctx, x, (1,), inverse=inverse, norm=0)
+    f = np.fft.ifft if inverse else np.fft.fft
+    want = np.roll(f(np.roll(x.astype(np.complex128), -((n + 1) // 2), axis=1), axis=1), -(n // 2), axis=1)
+    if inverse:
+        want = want * n
+    err = rel(got, want)
+    assert err < 2e-6, err
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_strided_axis(ctx, axis):
+    rng = np.random.default_rng(3)
+    x = (rng.standard_normal((5, 24, 40)) + 1j * rng.standard_normal((5, 24, 40))).astype(np.complex64)
+    got = fft_centered(ctx, x, (axis,), inverse=True, norm=1)
+    want = O.ifft_centered(x, (axis,), "f64")
+    assert rel(got, want) < 2e-6
+
+
+def _case(**kw):
+    p = dict(nc=6, nz=3, ny=48, nx=64, os_x=2, kind="full", seed=21, ellipses=6)
+    p.update(kw)
+    return synth.make_case(**p)
+
+
+@pytest.mark.parametrize("remove_os", [0, 1])
+@pytest.mark.parametrize("prewhiten", [0, 1])
+def test_ifft_rss_small(ctx, remove_os, prewhiten):
+    case = _case()
+    d = abi.IfftRssDesc.default(remove_oversampling=remove_os, prewhiten=prewhiten)
+    st, ref32, rflags, _ = O.recon("ifft_rss", d, case, "f32")
+    _, ref64, _, _ = O.recon("ifft_rss", d, case, "f64")
+    got, flags, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", d, case)
+    assert st == 0 and got.shape == ref32.shape
+    assert flags == rflags
+    e32, e64 = rel(got, ref32), rel(got, ref64)
+    print(f"os={remove_os} pw={prewhiten}: rel vs f32 {e32:.2e}, vs f64 {e64:.2e} (f32 oracle vs f64 {rel(ref32, ref64):.2e})")
+    assert e32 < TOL32
+
+
+def test_ifft_rss_3d_and_zcrop(ctx):
+    case = synth.make_case(nc=4, nz=10, ny=32, nx=40, os_x=2, kind="full", three_d=True, seed=5, ellipses=4)
+    case["hdr"]["recon_z"] = 6
+    d = abi.IfftRssDesc.default()
+    st, ref32, rflags, _ = O.recon("ifft_rss", d, case, "f32")
+    got, flags, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", d, case)
+    assert st == 0 and got.shape == ref32.shape == (6, 32, 20)
+    assert rel(got, ref32) < TOL32
+
+
+def test_ifft_rss_partial_fourier_unsupported(ctx):
+    case = _case(ny=16)
+    from openkspace_amd._lib import OksError
+    with pytest.raises(OksError) as e:
+        recon_dev(ctx, "oks_recon_ifft_rss_dev", abi.IfftRssDesc.default(), case)
+    assert e.value.code == abi.OKS_E_UNSUPPORTED
+
+
+def test_ifft_rss_prewhiten_skipped(ctx):
+    case = _case()
+    case["noise"] = np.ascontiguousarray(case["noise"][:, :3])
+    d = abi.IfftRssDesc.default()
+    _, ref, rflags, _ = O.recon("ifft_rss", d, case, "f32")
+    got, flags, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", d, case)
+    assert flags & abi.F_PREWHITEN_SKIPPED and rflags & abi.F_PREWHITEN_SKIPPED
+    assert rel(got, ref) < TOL32
+
+
+@pytest.mark.parametrize("remove_os", [0, 1])
+def test_ifft_rss_c1_full_size(ctx, remove_os):
+    case = synth.make_config("c1_ifft_rss")
+    d = abi.IfftRssDesc.default(remove_oversampling=remove_os)
+    O.set_threads(8)
+    st, ref32, _, _ = O.recon("ifft_rss", d, case, "f32")
+    _, ref64, _, _ = O.recon("ifft_rss", d, case, "f64")
+    O.set_threads(1)
+    got, flags, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", d, case)
+    assert got.shape == (16, 368, 320)
+    e32, e64 = rel(got, ref32), rel(got, ref64)
+    print(f"C1 os={remove_os}: rel vs f32 {e32:.2e}, vs f64 {e64:.2e}")
+    assert e32 < TOL32
+
+
+def test_host_buffer_entry_point(ctx):
+    from openkspace_amd.strategy import IfftRss
+    case = _case()
+    vol = IfftRss(ctx=ctx, remove_oversampling=0).reconstruct(case)
+    _, ref, rflags, _ = O.recon("ifft_rss", abi.IfftRssDesc.default(remove_oversampling=0), case, "f32")
+    assert vol.strategy == "ifft-rss" and vol.flags == rflags
+    assert rel(vol.data, ref) < TOL32
