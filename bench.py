@@ -36,24 +36,6 @@ def _args():
     return p.parse_args()
 
 
-class DevCase:
-    """Device-resident copy of a synthetic case (torch tensors as plumbing)."""
-
-    def __init__(self, case):
-        import torch
-        from openkspace_amd import abi
-        self.case = case
-        k = case["kspace"]
-        self.k = torch.from_numpy(k.view(np.float32).copy()).cuda()
-        self.noise = torch.from_numpy(case["noise"].view(np.float32).copy()).cuda()
-        self.mask = np.ascontiguousarray(case["mask"], dtype=np.uint8)
-        nc, nz, ny, nx = k.shape
-        self.out = torch.zeros(nz * ny * nx, dtype=torch.float32, device="cuda")
-        self.ks = abi.KSpace(self.k.data_ptr(), self.mask.ctypes.data, nc, nz, ny, nx,
-                             self.noise.data_ptr(), case["noise"].shape[1], abi.Header(**case["hdr"]))
-        self.img = abi.Image(self.out.data_ptr(), None, 0, 0, 0, 0)
-
-
 def workload_c1():
     from openkspace_amd import abi, synth
     case = synth.make_config("c1_ifft_rss")
@@ -105,33 +87,29 @@ def main():
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
-    import torch
-    from openkspace_amd._lib import Context, check, lib
+    from openkspace_amd._lib import Context, Event
+    from openkspace_amd.device import DevCase
 
     wl = WORKLOADS[a.workload]()
     ctx = Context(0)
-    stream = torch.cuda.current_stream()
-    ctx.set_stream(stream.cuda_stream)
-    dc = DevCase(wl["case"])
-    fn = getattr(lib(), wl["fn"])
-    torch.cuda.synchronize()
+    dc = DevCase(ctx, wl["case"])
+    ctx.synchronize()
 
     def step():
-        check(ctx.ptr, fn(ctx.ptr, C.byref(wl["desc"]), C.byref(dc.ks), C.byref(dc.img)))
+        dc.run(wl["fn"], wl["desc"])
 
     for _ in range(a.warmup):
         step()
-    torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
-    torch.cuda.synchronize()
+    ctx.synchronize()
+    evs = [(Event(ctx), Event(ctx)) for _ in range(a.steps)]
     t0 = time.perf_counter()
     for s, e in evs:
-        s.record(stream)
+        s.record()
         step()
-        e.record(stream)
-    torch.cuda.synchronize()
+        e.record()
+    ctx.synchronize()
     wall = time.perf_counter() - t0
-    lat = np.array([s.elapsed_time(e) for s, e in evs])  # ms, device time per step
+    lat = np.array([s.elapsed_ms(e) for s, e in evs])  # device time per step, ms
     slices = wl["slices"] * a.steps
     value = slices / wall
     out = {
@@ -150,8 +128,7 @@ def main():
     if not a.no_check:
         import oracle as O
         O.set_threads(min(16, os.cpu_count() or 1))
-        nz_, ny_, nx_ = dc.img.nz, dc.img.ny, dc.img.nx
-        got = dc.out[: nz_ * ny_ * nx_].cpu().numpy().reshape(nz_, ny_, nx_)
+        got, _ = dc.result()
         _, r32, _, _ = O.recon(wl["oracle"], wl["desc"], wl["case"], "f32")
         _, r64, _, _ = O.recon(wl["oracle"], wl["desc"], wl["case"], "f64")
         out["rel_l2"] = {"vs_fp32_oracle": float(np.linalg.norm(got - r32) / np.linalg.norm(r32)),

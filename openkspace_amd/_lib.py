@@ -46,6 +46,14 @@ def lib():
             fn = getattr(L, f"oks_recon_{name}{suf}", None)
             if fn is not None:
                 fn.argtypes = [_P, _P, _P, _P]
+    L.oks_device_malloc.argtypes = [_P, C.c_size_t, C.POINTER(_P)]
+    L.oks_device_free.argtypes = [_P, _P]
+    L.oks_memcpy.argtypes = [_P, _P, _P, C.c_size_t, _I32]
+    L.oks_memset.argtypes = [_P, _P, C.c_int, C.c_size_t]
+    L.oks_event_create.argtypes = [_P, C.POINTER(_P)]
+    L.oks_event_record.argtypes = [_P, _P]
+    L.oks_event_elapsed_ms.argtypes = [_P, _P, _P, C.POINTER(C.c_float)]
+    L.oks_event_destroy.argtypes = [_P, _P]
     L.oks_fft_centered_dev.argtypes = [_P, _P, _P, _I32, _P, _I32, _I32, _I32]
     L.oks_rss_dev.argtypes = [_P, _P, _I64, _I64, _P]
     L.oks_prewhiten_calibrate_dev.argtypes = [_P, _P, _I32, _I64, _P]
@@ -95,3 +103,77 @@ class Context:
         f = C.c_uint32(0)
         check(self.ptr, lib().oks_ctx_take_flags(self.ptr, C.byref(f)))
         return int(f.value)
+
+
+class DeviceArray:
+    """A device allocation owned through the backend's C ABI (no other HIP stack)."""
+
+    def __init__(self, ctx, shape, dtype):
+        import numpy as np
+        self.ctx = ctx
+        self.shape = tuple(int(v) for v in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = _P()
+        check(ctx.ptr, lib().oks_device_malloc(ctx.ptr, max(self.nbytes, 1), C.byref(self.ptr)))
+
+    @classmethod
+    def from_host(cls, ctx, arr):
+        import numpy as np
+        arr = np.ascontiguousarray(arr)
+        d = cls(ctx, arr.shape, arr.dtype)
+        d.upload(arr)
+        return d
+
+    @property
+    def addr(self):
+        return self.ptr.value
+
+    def upload(self, arr):
+        import numpy as np
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert arr.nbytes == self.nbytes
+        check(self.ctx.ptr, lib().oks_memcpy(self.ctx.ptr, self.ptr, arr.ctypes.data, self.nbytes, 0))
+
+    def download(self, shape=None):
+        import numpy as np
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(self.ctx.ptr, lib().oks_memcpy(self.ctx.ptr, out.ctypes.data, self.ptr, self.nbytes, 1))
+        return out if shape is None else out.reshape(-1)[: int(np.prod(shape))].reshape(shape)
+
+    def zero(self):
+        check(self.ctx.ptr, lib().oks_memset(self.ctx.ptr, self.ptr, 0, self.nbytes))
+
+    def free(self):
+        if self.ptr:
+            lib().oks_device_free(self.ctx.ptr, self.ptr)
+            self.ptr = _P()
+
+    def __del__(self):
+        try:
+            if self.ctx.ptr:
+                self.free()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.ptr = _P()
+        check(ctx.ptr, lib().oks_event_create(ctx.ptr, C.byref(self.ptr)))
+
+    def record(self):
+        check(self.ctx.ptr, lib().oks_event_record(self.ctx.ptr, self.ptr))
+
+    def elapsed_ms(self, stop):
+        ms = C.c_float(0)
+        check(self.ctx.ptr, lib().oks_event_elapsed_ms(self.ctx.ptr, self.ptr, stop.ptr, C.byref(ms)))
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            if self.ctx.ptr and self.ptr:
+                lib().oks_event_destroy(self.ctx.ptr, self.ptr)
+        except Exception:
+            pass

@@ -219,6 +219,74 @@ void oks_output_shape(const oks_kspace* k, int32_t crop, int64_t out_shape[3]) {
     }
 }
 
+// ---- device memory and timing helpers (hosts without their own HIP plumbing)
+int oks_device_malloc(oks_ctx* ctx, size_t bytes, void** ptr) {
+    return guarded(ctx, [&] {
+        if (!ptr) fail(OKS_E_INVALID_ARG, "null out pointer");
+        *ptr = nullptr;
+        OKS_HIP(hipMalloc(ptr, std::max<size_t>(bytes, 1)));
+        return OKS_OK;
+    });
+}
+
+int oks_device_free(oks_ctx* ctx, void* ptr) {
+    return guarded(ctx, [&] {
+        OKS_HIP(hipStreamSynchronize(ctx->stream));
+        if (ptr) OKS_HIP(hipFree(ptr));
+        return OKS_OK;
+    });
+}
+
+int oks_memcpy(oks_ctx* ctx, void* dst, const void* src, size_t bytes, int32_t kind) {
+    return guarded(ctx, [&] {
+        if ((!dst || !src) && bytes) fail(OKS_E_INVALID_ARG, "null copy pointer");
+        const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                                : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+        OKS_HIP(hipMemcpyAsync(dst, src, bytes, k, ctx->stream));
+        OKS_HIP(hipStreamSynchronize(ctx->stream));
+        return OKS_OK;
+    });
+}
+
+int oks_memset(oks_ctx* ctx, void* dst, int value, size_t bytes) {
+    return guarded(ctx, [&] {
+        OKS_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream));
+        return OKS_OK;
+    });
+}
+
+int oks_event_create(oks_ctx* ctx, void** ev) {
+    return guarded(ctx, [&] {
+        if (!ev) fail(OKS_E_INVALID_ARG, "null out pointer");
+        hipEvent_t e;
+        OKS_HIP(hipEventCreate(&e));
+        *ev = (void*)e;
+        return OKS_OK;
+    });
+}
+
+int oks_event_record(oks_ctx* ctx, void* ev) {
+    return guarded(ctx, [&] {
+        OKS_HIP(hipEventRecord((hipEvent_t)ev, ctx->stream));
+        return OKS_OK;
+    });
+}
+
+int oks_event_elapsed_ms(oks_ctx* ctx, void* start, void* stop, float* ms) {
+    return guarded(ctx, [&] {
+        OKS_HIP(hipEventSynchronize((hipEvent_t)stop));
+        OKS_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+        return OKS_OK;
+    });
+}
+
+int oks_event_destroy(oks_ctx* ctx, void* ev) {
+    return guarded(ctx, [&] {
+        if (ev) OKS_HIP(hipEventDestroy((hipEvent_t)ev));
+        return OKS_OK;
+    });
+}
+
 int oks_detect_pattern(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx, oks_pattern* out) {
     if (!mask || !out) return OKS_E_INVALID_ARG;
     return detect_pattern(mask, nz, ny, nx, out) ? OKS_OK : OKS_E_UNSUPPORTED;

@@ -1,0 +1,70 @@
+"""Device-resident inputs for the benchmark and the GPU tests: buffers are
+allocated through the backend's own C ABI (oks_device_malloc / oks_memcpy)."""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from ._lib import DeviceArray, check, lib
+
+
+def fft_centered(ctx, x, axes, inverse=True, norm=1):
+    """oks_fft_centered_dev on a host array (copied in and out)."""
+    x = np.ascontiguousarray(x, dtype=np.complex64)
+    d = DeviceArray.from_host(ctx, x)
+    shape = np.array(x.shape, dtype=np.int64)
+    ax = np.array(axes, dtype=np.int32)
+    check(ctx.ptr, lib().oks_fft_centered_dev(ctx.ptr, d.addr, shape.ctypes.data, x.ndim,
+                                              ax.ctypes.data, len(axes), int(inverse), norm))
+    ctx.synchronize()
+    out = d.download()
+    d.free()
+    return out
+
+
+class DevCase:
+    """Device-resident k-space / noise / image buffers for one synthetic case."""
+
+    def __init__(self, ctx, case, gfactor=False):
+        self.ctx = ctx
+        self.case = case
+        self.k = DeviceArray.from_host(ctx, np.ascontiguousarray(case["kspace"], dtype=np.complex64))
+        self.noise = (DeviceArray.from_host(ctx, np.ascontiguousarray(case["noise"], dtype=np.complex64))
+                      if case.get("noise") is not None else None)
+        self.mask = np.ascontiguousarray(case["mask"], dtype=np.uint8)
+        nc, nz, ny, nx = case["kspace"].shape
+        self.out = DeviceArray(ctx, (nz * ny * nx,), np.float32)
+        self.g = DeviceArray(ctx, (nz * ny * nx,), np.float32) if gfactor else None
+        self.ks = abi.KSpace(self.k.addr, self.mask.ctypes.data, nc, nz, ny, nx,
+                             self.noise.addr if self.noise is not None else None,
+                             case["noise"].shape[1] if self.noise is not None else 0,
+                             abi.Header(**case["hdr"]))
+        self.img = abi.Image(self.out.addr, self.g.addr if self.g is not None else None, 0, 0, 0, 0)
+
+    def run(self, fn_name, desc):
+        check(self.ctx.ptr, getattr(lib(), fn_name)(self.ctx.ptr, C.byref(desc), C.byref(self.ks),
+                                                    C.byref(self.img)))
+
+    def result(self):
+        self.ctx.synchronize()
+        shape = (self.img.nz, self.img.ny, self.img.nx)
+        o = self.out.download(shape)
+        g = self.g.download(shape) if self.g is not None else None
+        return o, g
+
+    def free(self):
+        for b in (self.k, self.noise, self.out, self.g):
+            if b is not None:
+                b.free()
+
+
+def recon_dev(ctx, fn_name, desc, case):
+    """Run a strategy with device-resident k-space/noise/image (mask on host)."""
+    dc = DevCase(ctx, case, gfactor=bool(getattr(desc, "compute_gfactor", 0)))
+    try:
+        dc.run(fn_name, desc)
+        o, g = dc.result()
+        flags = dc.img.flags | ctx.take_flags()
+        return o, flags, g
+    finally:
+        dc.free()
