@@ -8,3 +8,18 @@ def rel(a, b):
     a = np.asarray(a, dtype=np.complex128 if np.iscomplexobj(a) or np.iscomplexobj(b) else np.float64)
     b = np.asarray(b, dtype=a.dtype)
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def assert_parity(got, r32, r64, tol, label=""):
+    """FP32-path parity bar.
+
+    The backend computes its small dense solves (noise Cholesky, GRAPPA ridge
+    solve, SENSE R x R) in FP64 where the reference restatement uses FP32, so
+    its output can be closer to the FP64 twin than the FP32 restatement is.
+    Pass iff rel(got, FP64) <= tol and rel(got, FP32) <= tol + rel(FP32, FP64).
+    """
+    e32, e64, e_ref = rel(got, r32), rel(got, r64), rel(r32, r64)
+    print(f"{label}: rel vs fp32 oracle {e32:.2e}, vs fp64 oracle {e64:.2e} (fp32 oracle vs fp64 {e_ref:.2e})")
+    assert e64 <= tol, (e32, e64, e_ref)
+    assert e32 <= tol + e_ref, (e32, e64, e_ref)
+    return e32, e64

@@ -1,14 +1,17 @@
 """GPU parity of the centred FFT passes and the ifft-rss strategy against the oracle.
 
-Bar: rel-L2 <= 1e-5 against the FP32 restatement (FP32 path); the distance to
-the FP64 twin is printed beside it.  Index windows (crop shapes) are exact.
+Bar (FP32 path): rel-L2 <= 1e-5 against the FP64 twin, and <= 1e-5 plus the
+FP32 restatement's own distance to FP64 against the FP32 restatement (the
+backend's noise whitener is FP64, the restatement's FP32).  Without
+pre-whitening the plain 1e-5 bar against the FP32 restatement applies.
+Index windows (crop shapes) and fallback flags are exact.
 """
 import numpy as np
 import pytest
 
 import oracle as O
 from openkspace_amd import abi, synth
-from tests.gpu_util import fft_centered, recon_dev, rel
+from tests.gpu_util import assert_parity, fft_centered, recon_dev, rel
 
 pytestmark = pytest.mark.gpu
 TOL32 = 1e-5
@@ -71,9 +74,9 @@ def test_ifft_rss_small(ctx, remove_os, prewhiten):
     got, flags, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", d, case)
     assert st == 0 and got.shape == ref32.shape
     assert flags == rflags
-    e32, e64 = rel(got, ref32), rel(got, ref64)
-    print(f"os={remove_os} pw={prewhiten}: rel vs f32 {e32:.2e}, vs f64 {e64:.2e} (f32 oracle vs f64 {rel(ref32, ref64):.2e})")
-    assert e32 < TOL32
+    e32, _ = assert_parity(got, ref32, ref64, TOL32, f"os={remove_os} pw={prewhiten}")
+    if not prewhiten:
+        assert e32 < TOL32
 
 
 def test_ifft_rss_3d_and_zcrop(ctx):
@@ -81,9 +84,10 @@ def test_ifft_rss_3d_and_zcrop(ctx):
     case["hdr"]["recon_z"] = 6
     d = abi.IfftRssDesc.default()
     st, ref32, rflags, _ = O.recon("ifft_rss", d, case, "f32")
+    _, ref64, _, _ = O.recon("ifft_rss", d, case, "f64")
     got, flags, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", d, case)
     assert st == 0 and got.shape == ref32.shape == (6, 32, 20)
-    assert rel(got, ref32) < TOL32
+    assert_parity(got, ref32, ref64, TOL32, "3d")
 
 
 def test_ifft_rss_partial_fourier_unsupported(ctx):
@@ -114,15 +118,15 @@ def test_ifft_rss_c1_full_size(ctx, remove_os):
     O.set_threads(1)
     got, flags, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", d, case)
     assert got.shape == (16, 368, 320)
-    e32, e64 = rel(got, ref32), rel(got, ref64)
-    print(f"C1 os={remove_os}: rel vs f32 {e32:.2e}, vs f64 {e64:.2e}")
-    assert e32 < TOL32
+    assert_parity(got, ref32, ref64, TOL32, f"C1 os={remove_os}")
 
 
 def test_host_buffer_entry_point(ctx):
     from openkspace_amd.strategy import IfftRss
     case = _case()
     vol = IfftRss(ctx=ctx, remove_oversampling=0).reconstruct(case)
-    _, ref, rflags, _ = O.recon("ifft_rss", abi.IfftRssDesc.default(remove_oversampling=0), case, "f32")
+    d = abi.IfftRssDesc.default(remove_oversampling=0)
+    _, ref, rflags, _ = O.recon("ifft_rss", d, case, "f32")
+    _, ref64, _, _ = O.recon("ifft_rss", d, case, "f64")
     assert vol.strategy == "ifft-rss" and vol.flags == rflags
-    assert rel(vol.data, ref) < TOL32
+    assert_parity(vol.data, ref, ref64, TOL32, "host entry")
