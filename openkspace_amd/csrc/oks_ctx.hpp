@@ -48,6 +48,7 @@ struct oks_ctx {
     std::map<std::string, oks::DevBuf> ws;         // named workspaces
     int* d_status = nullptr;                       // device-side flags (whitener) [8]
     uint32_t pending_flags = 0;
+    int zchunk = 4;                                // slices per row/column chunk (ifft-rss)
 
     oks::FftPlan plan(int n);
     void* buf(const std::string& name, size_t bytes);

@@ -186,3 +186,216 @@ __device__ float2* fft_wave(float2* a, float2* b, const FftPlan& P, int t, int G
 }
 
 }  // namespace oks
+
+// ---------------------------------------------------------------------------
+// Compile-time-length transform.  The radix sequence (8s, then 4, 2, 3, 5,
+// then primes) and every index division are constants, and each stage is in
+// place: a wavefront reads all of its butterfly inputs into registers, syncs,
+// and writes the outputs back into the same LDS line, so no scratch line is
+// needed and LDS per transform is n * 8 bytes.
+namespace oks {
+
+constexpr int cfft_spf(int m) {
+    for (int p = 7; p * p <= m; p += 2)
+        if (m % p == 0) return p;
+    return m;
+}
+// Primes without a hand-written butterfly come first (their stage then needs
+// no inter-stage twiddles), then 8s, 4, 2, 3, 5.
+constexpr int cfft_big_prime(int m) {
+    int r = m;
+    while (r % 2 == 0) r /= 2;
+    while (r % 3 == 0) r /= 3;
+    while (r % 5 == 0) r /= 5;
+    return r > 1 ? cfft_spf(r) : 0;
+}
+constexpr int cfft_next_radix(int m) {
+    return cfft_big_prime(m) ? cfft_big_prime(m)
+           : (m % 8 == 0) ? 8 : (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 3 == 0) ? 3 : 5;
+}
+constexpr int cfft_radix(int n, int s) {
+    int m = n;
+    for (int i = 0; i < s; ++i) m /= cfft_next_radix(m);
+    return cfft_next_radix(m);
+}
+constexpr int cfft_nst(int n) {
+    int m = n, s = 0;
+    while (m > 1) {
+        m /= cfft_next_radix(m);
+        ++s;
+    }
+    return s;
+}
+constexpr bool cfft_coded(int r) { return r == 2 || r == 3 || r == 4 || r == 5 || r == 8; }
+
+template <int N, bool INV>
+struct CFft {
+    static constexpr int NST = cfft_nst(N);
+
+    template <int S, int NS>
+    __device__ __forceinline__ static void stage(float2* __restrict__ x, const float2* __restrict__ tw, int lane) {
+        constexpr int R = cfft_radix(N, S);
+        constexpr int NR = N / R;
+        constexpr int TWS = N / (NS * R);
+        if constexpr (cfft_coded(R)) {
+            constexpr int BPL = (NR + 63) / 64;
+            float2 v[BPL][R];
+#pragma unroll
+            for (int b = 0; b < BPL; ++b) {
+                const int j = lane + 64 * b;
+                if (j < NR) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) v[b][r] = x[j + r * NR];
+                }
+            }
+            wave_sync();
+#pragma unroll
+            for (int b = 0; b < BPL; ++b) {
+                const int j = lane + 64 * b;
+                if (j < NR) {
+                    const int k = j % NS;
+                    if constexpr (NS > 1) {
+#pragma unroll
+                        for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], twid<INV>(tw, r * k * TWS));
+                    }
+                    dft<R, INV>(v[b]);
+                    const int o = (j - k) * R + k;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) x[o + r * NS] = v[b][r];
+                }
+            }
+            wave_sync();
+        } else if constexpr (NS == 1 && 64 % NR == 0) {
+            // First stage, odd prime R: pair x_q with x_{R-q}.  y[p] and
+            // y[R-p] share t_q = x_q + x_{R-q} and u_q = x_q - x_{R-q}:
+            //   y[p]   = x_0 + sum t_q cos(2 pi q p/R) + i s sum u_q sin(2 pi q p/R)
+            //   y[R-p] = x_0 + sum t_q cos(2 pi q p/R) - i s sum u_q sin(2 pi q p/R)
+            // 64/NR lanes share one butterfly j and split its output pairs.
+            constexpr int H = (R - 1) / 2;
+            constexpr int LPJ = 64 / NR;
+            constexpr int PPL = (H + 1 + LPJ - 1) / LPJ;  // output pairs per lane
+            const int j = lane % NR, p0 = lane / NR;
+            float2 x0 = x[j];
+            float2 t[H], u[H];
+#pragma unroll
+            for (int q = 1; q <= H; ++q) {
+                const float2 a = x[j + q * NR], b = x[j + (R - q) * NR];
+                t[q - 1] = cadd(a, b);
+                u[q - 1] = csub(a, b);
+            }
+            float2 ya[PPL], yb[PPL];
+#pragma unroll
+            for (int m = 0; m < PPL; ++m) {
+                const int p = p0 + m * LPJ;
+                float2 A = x0, Bs = make_float2(0.f, 0.f);
+                if (p <= H) {
+                    int e = 0;
+#pragma unroll
+                    for (int q = 1; q <= H; ++q) {
+                        e += p;
+                        if (e >= R) e -= R;
+                        const float2 w = tw[e * NR];  // (cos, sin)(2 pi e / R)
+                        A.x += t[q - 1].x * w.x;
+                        A.y += t[q - 1].y * w.x;
+                        Bs.x += u[q - 1].x * w.y;
+                        Bs.y += u[q - 1].y * w.y;
+                    }
+                }
+                const float2 iB = mul_si<INV>(Bs);
+                ya[m] = cadd(A, iB);
+                yb[m] = csub(A, iB);
+            }
+            wave_sync();
+#pragma unroll
+            for (int m = 0; m < PPL; ++m) {
+                const int p = p0 + m * LPJ;
+                if (p == 0) {
+                    x[j * R] = ya[m];
+                } else if (p <= H) {
+                    x[j * R + p] = ya[m];
+                    x[j * R + R - p] = yb[m];
+                }
+            }
+            wave_sync();
+        } else {
+            constexpr int OPL = (N + 63) / 64;
+            float2 acc[OPL];
+#pragma unroll
+            for (int m = 0; m < OPL; ++m) {
+                const int i = lane + 64 * m;
+                if (i < N) {
+                    const int j = i / R, rr = i - (i / R) * R;
+                    const int k = j % NS;
+                    const int step = (k * TWS + rr * NR) % N;
+                    float2 a = x[j];
+                    int e = step;
+#pragma unroll 2
+                    for (int q = 1; q < R; ++q) {
+                        a = cadd(a, cmul(x[j + q * NR], twid<INV>(tw, e)));
+                        e += step;
+                        if (e >= N) e -= N;
+                    }
+                    acc[m] = a;
+                }
+            }
+            wave_sync();
+#pragma unroll
+            for (int m = 0; m < OPL; ++m) {
+                const int i = lane + 64 * m;
+                if (i < N) {
+                    const int j = i / R, rr = i - (i / R) * R;
+                    const int k = j % NS;
+                    x[(j - k) * R + k + rr * NS] = acc[m];
+                }
+            }
+            wave_sync();
+        }
+        if constexpr (S + 1 < NST) stage<S + 1, NS * R>(x, tw, lane);
+    }
+
+    __device__ __forceinline__ static void run(float2* x, const float2* tw, int lane) {
+        if constexpr (NST > 0) stage<0, 1>(x, tw, lane);
+    }
+};
+
+// Policies the kernels are written against: a fixed length transforms in
+// place; the runtime-length fallback ping-pongs with a scratch line.
+// `tw` is the workgroup's LDS copy of the twiddle table for fixed lengths
+// (see stage_twiddles) and the global table P.tw for the runtime path.
+template <int N, bool INV>
+struct FixedFft {
+    static constexpr int n = N;
+    static constexpr bool scratch = false;
+    static constexpr bool lds_tw = true;
+    __device__ __forceinline__ static float2* run(float2* x, float2*, const FftPlan&, const float2* tw, int lane) {
+        CFft<N, INV>::run(x, tw, lane);
+        return x;
+    }
+};
+template <bool INV>
+struct DynFft {
+    static constexpr int n = 0;
+    static constexpr bool scratch = true;
+    static constexpr bool lds_tw = false;
+    __device__ __forceinline__ static float2* run(float2* x, float2* s, const FftPlan& P, const float2*, int lane) {
+        return fft_wave<INV>(x, s, P, lane, 64);
+    }
+};
+
+// Copy the twiddle table into LDS (fixed lengths); returns the table to use.
+template <class F>
+__device__ __forceinline__ const float2* stage_twiddles(float2* lds_tw, const FftPlan& P) {
+    if constexpr (F::lds_tw) {
+        for (int i = threadIdx.x; i < F::n; i += blockDim.x) lds_tw[i] = P.tw[i];
+        return lds_tw;
+    } else {
+        return P.tw;
+    }
+}
+// float2 slots the LDS twiddle copy occupies.
+template <class F>
+constexpr int tw_slots() {
+    return F::lds_tw ? ((F::n + 1) & ~1) : 0;
+}
+
+}  // namespace oks

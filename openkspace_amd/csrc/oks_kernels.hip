@@ -103,28 +103,18 @@ __global__ void __launch_bounds__(256) k_whitener(const float2* __restrict__ noi
 // is loaded once from HBM, whitened in registers (y = W s,
 // recon/prewhiten.rs:128-141), written to LDS at its ifftshift position, then
 // one wavefront per coil runs the kx IFFT and stores the fftshifted, x-cropped
-// line (recon/fft.rs:19-29 + recon/strategy.rs:227-253).  With os_mode the
-// reference's oversampling-removal chain (recon/oversampling.rs:122-147) and
-// its re-centring into the nx grid (io/ismrmrd/reader.rs:392-399) run on the
-// line first.
+// line (recon/fft.rs:19-29 + recon/strategy.rs:227-253).
 template <int NCMAX>
-__global__ void __launch_bounds__(512) k_rows_prewhiten_ifft(RowPwArgs a) {
-    extern __shared__ float2 lds[];
-    const int nx = a.nx, nc = a.nc;
-    const int WPB = blockDim.x >> 6;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    float2* A = lds;                       // nc * nx
-    float2* B = lds + (size_t)nc * nx;     // WPB * nx scratch
-    const size_t row = blockIdx.x;
-    const size_t plane = (size_t)a.rows * nx;
-    const float2* __restrict__ src = a.in + row * nx;
-    const int half = nx / 2;
-    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
+__device__ __forceinline__ void load_whitened(const RowPwArgs& a, float2* A, int n, size_t row) {
+    const int nc = a.nc;
+    const float2* __restrict__ src = a.in + row * n;
+    const int half = n / 2;
+    for (int x = threadIdx.x; x < n; x += blockDim.x) {
         float2 s[NCMAX];
 #pragma unroll
         for (int j = 0; j < NCMAX; ++j)
-            if (j < nc) s[j] = src[j * plane + x];
-        const int pos = (x + half) % nx;
+            if (j < nc) s[j] = src[(size_t)j * a.in_plane + x];
+        const int pos = (x + half) % n;
         if (a.W) {
 #pragma unroll
             for (int i = 0; i < NCMAX; ++i) {
@@ -132,48 +122,83 @@ __global__ void __launch_bounds__(512) k_rows_prewhiten_ifft(RowPwArgs a) {
                     float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
                     for (int j = 0; j <= i; ++j) acc = cadd(acc, cmul(a.W[i * nc + j], s[j]));
-                    A[(size_t)i * nx + pos] = acc;
+                    A[(size_t)i * n + pos] = acc;
                 }
             }
         } else {
 #pragma unroll
             for (int i = 0; i < NCMAX; ++i)
-                if (i < nc) A[(size_t)i * nx + pos] = s[i];
+                if (i < nc) A[(size_t)i * n + pos] = s[i];
         }
     }
+}
+
+template <int NCMAX, class F>
+__global__ void __launch_bounds__(512) k_rows_pw(RowPwArgs a) {
+    extern __shared__ float2 lds[];
+    const int n = F::n ? F::n : a.nx;
+    const int nc = a.nc;
+    const int WPB = blockDim.x >> 6;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const float2* tw = stage_twiddles<F>(lds, a.p_enc);
+    float2* A = lds + tw_slots<F>();                          // nc * n
+    float2* B = F::scratch ? A + (size_t)nc * n : nullptr;    // WPB * n
+    const size_t row = blockIdx.x;
+    load_whitened<NCMAX>(a, A, n, row);
     __syncthreads();
+    const int tx = a.tx, x0 = a.x0, half = n / 2;
+    for (int c = w; c < nc; c += WPB) {
+        float2* res = F::run(A + (size_t)c * n, B ? B + (size_t)w * n : nullptr, a.p_enc, tw, lane);
+        float2* dstp = a.out + (size_t)c * a.out_plane + row * tx;
+        for (int x = lane; x < tx; x += 64) dstp[x] = cscale(res[(x0 + x + half) % n], a.scale);
+        wave_sync();
+    }
+}
+
+// Same row pass with the reference's oversampling-removal chain
+// (recon/oversampling.rs:122-147) and its re-centring into the nx grid
+// (io/ismrmrd/reader.rs:392-399) run on every line first.
+template <int NCMAX>
+__global__ void __launch_bounds__(512) k_rows_pw_os(RowPwArgs a) {
+    extern __shared__ float2 lds[];
+    const int nx = a.nx, nc = a.nc;
+    const int WPB = blockDim.x >> 6;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float2* A = lds;                       // nc * nx
+    float2* B = lds + (size_t)nc * nx;     // WPB * nx scratch
+    const size_t row = blockIdx.x;
+    load_whitened<NCMAX>(a, A, nx, row);
+    __syncthreads();
+    const int half = nx / 2;
     const int tx = a.tx, x0 = a.x0;
     for (int c = w; c < nc; c += WPB) {
         float2* buf = A + (size_t)c * nx;
         float2* scr = B + (size_t)w * nx;
         float2* res = fft_wave<true>(buf, scr, a.p_enc, lane, 64);
-        float sc = a.scale;
-        if (a.os_mode) {
-            // res holds b = IFFT_nx(ifftshift(line)); img[x] = b[(x+nx/2)%nx] / nx
-            const int rec = a.rec, crop0 = (nx - rec) / 2, rh = (rec + 1) / 2;
-            float2* q = (res == buf) ? scr : buf;
-            const float inv_nx = 1.0f / (float)nx;
-            for (int i = lane; i < rec; i += 64) {
-                const int m = (i + rh) % rec;  // ifftshift of the cropped line
-                q[i] = cscale(res[(m + crop0 + half) % nx], inv_nx);
-            }
-            wave_sync();
-            float2* other = (q == buf) ? scr : buf;
-            float2* S = fft_wave<false>(q, other, a.p_rec, lane, 64);
-            float2* K = (S == buf) ? scr : buf;
-            const int dst = half - rec / 2;
-            const int nh = (nx + 1) / 2;
-            for (int i = lane; i < nx; i += 64) {
-                const int xk = (i + nh) % nx;  // ifftshift of the re-centred grid line
-                const int m = xk - dst;
-                K[i] = (m >= 0 && m < rec) ? S[(m + rec / 2) % rec] : make_float2(0.f, 0.f);
-            }
-            wave_sync();
-            float2* other2 = (K == buf) ? scr : buf;
-            res = fft_wave<true>(K, other2, a.p_enc, lane, 64);
+        // res holds b = IFFT_nx(ifftshift(line)); img[x] = b[(x+nx/2)%nx] / nx
+        const int rec = a.rec, crop0 = (nx - rec) / 2, rh = (rec + 1) / 2;
+        float2* q = (res == buf) ? scr : buf;
+        const float inv_nx = 1.0f / (float)nx;
+        for (int i = lane; i < rec; i += 64) {
+            const int m = (i + rh) % rec;  // ifftshift of the cropped line
+            q[i] = cscale(res[(m + crop0 + half) % nx], inv_nx);
         }
-        float2* dstp = a.out + (size_t)c * a.rows * tx + row * tx;
-        for (int x = lane; x < tx; x += 64) dstp[x] = cscale(res[(x0 + x + half) % nx], sc);
+        wave_sync();
+        float2* other = (q == buf) ? scr : buf;
+        float2* S = fft_wave<false>(q, other, a.p_rec, lane, 64);
+        float2* K = (S == buf) ? scr : buf;
+        const int dst = half - rec / 2;
+        const int nh = (nx + 1) / 2;
+        for (int i = lane; i < nx; i += 64) {
+            const int xk = (i + nh) % nx;  // ifftshift of the re-centred grid line
+            const int m = xk - dst;
+            K[i] = (m >= 0 && m < rec) ? S[(m + rec / 2) % rec] : make_float2(0.f, 0.f);
+        }
+        wave_sync();
+        float2* other2 = (K == buf) ? scr : buf;
+        res = fft_wave<true>(K, other2, a.p_enc, lane, 64);
+        float2* dstp = a.out + (size_t)c * a.out_plane + row * tx;
+        for (int x = lane; x < tx; x += 64) dstp[x] = cscale(res[(x0 + x + half) % nx], a.scale);
         wave_sync();
     }
 }
@@ -182,15 +207,17 @@ __global__ void __launch_bounds__(512) k_rows_prewhiten_ifft(RowPwArgs a) {
 // Generic row pass: centred 1-D transform along the contiguous axis of
 // [rows, n]; optional input mask multiply (mask row index = row % mask_rows),
 // output crop window [x0, x0+tx), scale.  One wavefront per row.
-template <bool INV>
+template <class F>
 __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
     extern __shared__ float2 lds[];
-    const int n = a.p.n;
+    const int n = F::n ? F::n : a.p.n;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const float2* tw = stage_twiddles<F>(lds, a.p);
+    __syncthreads();
     const size_t row = (size_t)blockIdx.x * (blockDim.x >> 6) + w;
     if (row >= (size_t)a.rows) return;
-    float2* A = lds + (size_t)w * 2 * n;
-    float2* B = A + n;
+    float2* A = lds + tw_slots<F>() + (size_t)w * (F::scratch ? 2 : 1) * n;
+    float2* B = F::scratch ? A + n : nullptr;
     const float2* __restrict__ src = a.in + row * n;
     const uint8_t* m = a.mask ? a.mask + (row % a.mask_rows) * n : nullptr;
     const int half = a.centred ? n / 2 : 0;
@@ -200,55 +227,64 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
         A[(x + half) % n] = v;
     }
     wave_sync();
-    float2* res = fft_wave<INV>(A, B, a.p, lane, 64);
+    float2* res = F::run(A, B, a.p, tw, lane);
     float2* dst = a.out + row * a.tx;
     for (int x = lane; x < a.tx; x += 64) dst[x] = cscale(res[(a.x0 + x + half) % n], a.scale);
 }
 
 // ---------------------------------------------------------------------------
 // Column pass along the middle axis of [C, Z, n, X] (X contiguous).  One
-// workgroup per (z, strip of SX columns); coils are looped inside so that the
-// RSS epilogue (recon/coil.rs:24-27, coil order 0..C-1) accumulates in LDS
-// and the image is written once.  mode 0 writes the transformed complex data
-// (crop window [y0, y0+ty) along n); mode 1 writes sqrt(sum_c |.|^2) * scale.
-template <bool INV>
+// workgroup per (coil group, z, strip of SX columns); the group's coils are
+// looped inside so the RSS epilogue (recon/coil.rs:24-27, coils ascending)
+// accumulates in LDS.  mode 0 writes the transformed complex data (window
+// [y0, y0+ty) along n); mode 1 writes sqrt(sum_c |.|^2) * scale, or with
+// G > 1 coil groups the per-group partial sums for k_rss_finalize.
+template <class F>
 __global__ void __launch_bounds__(512) k_cols(ColArgs a) {
     extern __shared__ float2 lds[];
-    const int n = a.p.n, SX = a.sx;
+    const int n = F::n ? F::n : a.p.n;
+    const int SX = a.sx;
     const int WPB = blockDim.x >> 6;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    float2* A = lds;                        // SX * n
-    float2* B = lds + (size_t)SX * n;       // SX * n
-    float* acc = (float*)(lds + (size_t)2 * SX * n);  // SX * ty (mode 1)
+    const int ls = n + 1;                                        // padded line stride (bank spread)
+    const float2* tw = stage_twiddles<F>(lds, a.p);
+    float2* A = lds + tw_slots<F>();                             // SX * ls
+    float2* B = F::scratch ? A + (size_t)SX * ls : nullptr;      // SX * ls
+    float* acc = (float*)(A + (size_t)(F::scratch ? 2 : 1) * SX * ls);  // SX * ty (mode 1)
     const int strips = (a.X + SX - 1) / SX;
-    const int zi = blockIdx.x / strips;      // output z
+    const int per_g = a.Zo * strips;
+    const int g = blockIdx.x / per_g;
+    const int rem = blockIdx.x - g * per_g;
+    const int zi = rem / strips;             // output z
     const int z = a.z0 + zi;                 // input z
-    const int xs = (blockIdx.x % strips) * SX;
+    const int xs = (rem - zi * strips) * SX;
+    const int c0 = g * a.C / a.G, c1 = (g + 1) * a.C / a.G;
     const int half = a.centred ? n / 2 : 0;
     const int ty = a.ty, y0 = a.y0;
     if (a.mode == 1)
         for (int i = threadIdx.x; i < SX * ty; i += blockDim.x) acc[i] = 0.f;
-    for (int c = 0; c < a.C; ++c) {
+    for (int c = c0; c < c1; ++c) {
         const float2* __restrict__ src = a.in + ((size_t)c * a.Z + z) * n * a.X;
         for (int i = threadIdx.x; i < n * SX; i += blockDim.x) {
             const int y = i / SX, col = i - y * SX;
             float2 v = make_float2(0.f, 0.f);
             if (xs + col < a.X) v = src[(size_t)y * a.X + xs + col];
-            A[(size_t)col * n + (y + half) % n] = v;
+            A[(size_t)col * ls + (y + half) % n] = v;
         }
         __syncthreads();
         for (int col = w; col < SX; col += WPB) {
-            float2* res = fft_wave<INV>(A + (size_t)col * n, B + (size_t)col * n, a.p, lane, 64);
+            float2* line = A + (size_t)col * ls;
+            float2* res = F::run(line, B ? B + (size_t)col * ls : nullptr, a.p, tw, lane);
             if (a.mode == 1) {
                 float* ac = acc + (size_t)col * ty;
                 for (int yo = lane; yo < ty; yo += 64) {
                     const float2 v = res[(y0 + yo + half) % n];
                     ac[yo] += v.x * v.x + v.y * v.y;
                 }
-            } else if (res != A + (size_t)col * n) {
-                float2* d = A + (size_t)col * n;
-                for (int y = lane; y < n; y += 64) d[y] = res[y];
+            } else if (res != line) {
+                for (int y = lane; y < n; y += 64) line[y] = res[y];
             }
+            wave_sync();
         }
         __syncthreads();
         if (a.mode == 0) {
@@ -256,18 +292,37 @@ __global__ void __launch_bounds__(512) k_cols(ColArgs a) {
             for (int i = threadIdx.x; i < ty * SX; i += blockDim.x) {
                 const int yo = i / SX, col = i - yo * SX;
                 if (xs + col < a.X)
-                    dst[(size_t)yo * a.X + xs + col] = cscale(A[(size_t)col * n + (y0 + yo + half) % n], a.scale);
+                    dst[(size_t)yo * a.X + xs + col] = cscale(A[(size_t)col * ls + (y0 + yo + half) % n], a.scale);
             }
             __syncthreads();
         }
     }
     if (a.mode == 1) {
         __syncthreads();
-        float* dst = a.out_r + (size_t)zi * ty * a.X;
-        for (int i = threadIdx.x; i < ty * SX; i += blockDim.x) {
-            const int yo = i / SX, col = i - yo * SX;
-            if (xs + col < a.X) dst[(size_t)yo * a.X + xs + col] = sqrtf(acc[(size_t)col * ty + yo]) * a.scale;
+        if (a.G == 1) {
+            float* dst = a.out_r + (size_t)zi * ty * a.X;
+            for (int i = threadIdx.x; i < ty * SX; i += blockDim.x) {
+                const int yo = i / SX, col = i - yo * SX;
+                if (xs + col < a.X) dst[(size_t)yo * a.X + xs + col] = sqrtf(acc[(size_t)col * ty + yo]) * a.scale;
+            }
+        } else {
+            float* dst = a.part + ((size_t)g * a.Zo + zi) * ty * a.X;
+            for (int i = threadIdx.x; i < ty * SX; i += blockDim.x) {
+                const int yo = i / SX, col = i - yo * SX;
+                if (xs + col < a.X) dst[(size_t)yo * a.X + xs + col] = acc[(size_t)col * ty + yo];
+            }
         }
+    }
+}
+
+// out = sqrt(sum_g part[g]) * scale, groups summed in ascending order.
+__global__ void k_rss_finalize(const float* __restrict__ part, int G, long long M, float* __restrict__ out,
+                               float scale) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < M;
+         i += (long long)gridDim.x * blockDim.x) {
+        float s = part[i];
+        for (int g = 1; g < G; ++g) s += part[(size_t)g * M + i];
+        out[i] = sqrtf(s) * scale;
     }
 }
 
@@ -314,56 +369,120 @@ hipError_t launch_whitener(const float2* noise, int nc, int ns, float2* W, int* 
     return hipGetLastError();
 }
 
+#define OKS_FIXED_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(160) X(256) X(320) X(368) X(384) X(512) X(640)
+
+static bool is_fixed(int n) {
+    switch (n) {
+#define OKS_CASE(N) case N:
+        OKS_FIXED_SIZES(OKS_CASE)
+#undef OKS_CASE
+        return true;
+        default: return false;
+    }
+}
+
+static size_t tw_lds_slots(int n) { return is_fixed(n) ? (size_t)((n + 1) & ~1) : 0; }
+
 size_t rows_prewhiten_lds(int nc, int nx, int os_mode) {
-    (void)os_mode;
-    return ((size_t)nc * nx + (size_t)8 * nx) * sizeof(float2);
+    if (os_mode) return ((size_t)nc * nx + (size_t)8 * nx) * sizeof(float2);
+    const bool scratch = !is_fixed(nx);
+    return (tw_lds_slots(nx) + (size_t)nc * nx + (scratch ? (size_t)8 * nx : 0)) * sizeof(float2);
+}
+
+template <int NCMAX, class F>
+static hipError_t go_rows_pw(const RowPwArgs& a, size_t lds, hipStream_t s) {
+    OKS_ATTR(hipFuncSetAttribute((const void*)k_rows_pw<NCMAX, F>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_rows_pw<NCMAX, F>), dim3((unsigned)a.rows), dim3(512), lds, s, a);
+    return hipGetLastError();
+}
+
+template <int NCMAX>
+static hipError_t go_rows_pw_n(const RowPwArgs& a, size_t lds, hipStream_t s) {
+    if (a.os_mode) {
+        OKS_ATTR(hipFuncSetAttribute((const void*)k_rows_pw_os<NCMAX>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_rows_pw_os<NCMAX>, dim3((unsigned)a.rows), dim3(512), lds, s, a);
+        return hipGetLastError();
+    }
+    switch (a.nx) {
+#define OKS_CASE(N) \
+    case N: return go_rows_pw<NCMAX, FixedFft<N, true>>(a, lds, s);
+        OKS_FIXED_SIZES(OKS_CASE)
+#undef OKS_CASE
+        default: return go_rows_pw<NCMAX, DynFft<true>>(a, lds, s);
+    }
 }
 
 hipError_t launch_rows_prewhiten_ifft(const RowPwArgs& a, hipStream_t s) {
     const size_t lds = rows_prewhiten_lds(a.nc, a.nx, a.os_mode);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
-    if (a.nc <= 16) {
-        OKS_ATTR(hipFuncSetAttribute((const void*)k_rows_prewhiten_ifft<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_rows_prewhiten_ifft<16>, dim3((unsigned)a.rows), dim3(512), lds, s, a);
-    } else if (a.nc <= 32) {
-        OKS_ATTR(hipFuncSetAttribute((const void*)k_rows_prewhiten_ifft<32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_rows_prewhiten_ifft<32>, dim3((unsigned)a.rows), dim3(512), lds, s, a);
-    } else {
-        return hipErrorInvalidValue;
-    }
+    if (a.nc <= 16) return go_rows_pw_n<16>(a, lds, s);
+    if (a.nc <= 32) return go_rows_pw_n<32>(a, lds, s);
+    return hipErrorInvalidValue;
+}
+
+size_t rows_lds(int n) { return (tw_lds_slots(n) + (size_t)4 * (is_fixed(n) ? 1 : 2) * n) * sizeof(float2); }
+
+template <class F>
+static hipError_t go_rows(const RowArgs& a, hipStream_t s) {
+    const int wpb = 4;
+    const size_t lds = rows_lds(a.p.n);
+    const unsigned grid = (unsigned)((a.rows + wpb - 1) / wpb);
+    OKS_ATTR(hipFuncSetAttribute((const void*)k_rows<F>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_rows<F>, dim3(grid), dim3(64 * wpb), lds, s, a);
     return hipGetLastError();
+}
+
+template <bool INV>
+static hipError_t go_rows_dir(const RowArgs& a, hipStream_t s) {
+    switch (a.p.n) {
+#define OKS_CASE(N) \
+    case N: return go_rows<FixedFft<N, INV>>(a, s);
+        OKS_FIXED_SIZES(OKS_CASE)
+#undef OKS_CASE
+        default: return go_rows<DynFft<INV>>(a, s);
+    }
 }
 
 hipError_t launch_rows(const RowArgs& a, bool inverse, hipStream_t s) {
-    const int wpb = 4;
-    const size_t lds = (size_t)wpb * 2 * a.p.n * sizeof(float2);
-    const unsigned grid = (unsigned)((a.rows + wpb - 1) / wpb);
-    if (inverse) {
-        OKS_ATTR(hipFuncSetAttribute((const void*)k_rows<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_rows<true>, dim3(grid), dim3(64 * wpb), lds, s, a);
-    } else {
-        OKS_ATTR(hipFuncSetAttribute((const void*)k_rows<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_rows<false>, dim3(grid), dim3(64 * wpb), lds, s, a);
-    }
-    return hipGetLastError();
+    return inverse ? go_rows_dir<true>(a, s) : go_rows_dir<false>(a, s);
 }
 
 size_t cols_lds(int sx, int n, int ty, int mode) {
-    return (size_t)2 * sx * n * sizeof(float2) + (mode == 1 ? (size_t)sx * ty * sizeof(float) : 0);
+    return (tw_lds_slots(n) + (size_t)(is_fixed(n) ? 1 : 2) * sx * (n + 1)) * sizeof(float2) +
+           (mode == 1 ? (size_t)sx * ty * sizeof(float) : 0);
+}
+
+template <class F>
+static hipError_t go_cols(const ColArgs& a, hipStream_t s) {
+    const size_t lds = cols_lds(a.sx, a.p.n, a.ty, a.mode);
+    const int strips = (a.X + a.sx - 1) / a.sx;
+    const unsigned grid = (unsigned)(a.G * a.Zo * strips);
+    OKS_ATTR(hipFuncSetAttribute((const void*)k_cols<F>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_cols<F>, dim3(grid), dim3(512), lds, s, a);
+    return hipGetLastError();
+}
+
+template <bool INV>
+static hipError_t go_cols_dir(const ColArgs& a, hipStream_t s) {
+    switch (a.p.n) {
+#define OKS_CASE(N) \
+    case N: return go_cols<FixedFft<N, INV>>(a, s);
+        OKS_FIXED_SIZES(OKS_CASE)
+#undef OKS_CASE
+        default: return go_cols<DynFft<INV>>(a, s);
+    }
 }
 
 hipError_t launch_cols(const ColArgs& a, bool inverse, hipStream_t s) {
-    const size_t lds = cols_lds(a.sx, a.p.n, a.ty, a.mode);
-    if (lds > 160 * 1024) return hipErrorInvalidValue;
-    const int strips = (a.X + a.sx - 1) / a.sx;
-    const unsigned grid = (unsigned)(a.Zo * strips);
-    if (inverse) {
-        OKS_ATTR(hipFuncSetAttribute((const void*)k_cols<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_cols<true>, dim3(grid), dim3(512), lds, s, a);
-    } else {
-        OKS_ATTR(hipFuncSetAttribute((const void*)k_cols<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_cols<false>, dim3(grid), dim3(512), lds, s, a);
-    }
+    if (cols_lds(a.sx, a.p.n, a.ty, a.mode) > 160 * 1024) return hipErrorInvalidValue;
+    if (a.G < 1 || a.G > a.C) return hipErrorInvalidValue;
+    if (a.mode == 1 && a.G > 1 && !a.part) return hipErrorInvalidValue;
+    return inverse ? go_cols_dir<true>(a, s) : go_cols_dir<false>(a, s);
+}
+
+hipError_t launch_rss_finalize(const float* part, int G, long long M, float* out, float scale, hipStream_t s) {
+    const unsigned grid = (unsigned)std::min<long long>((M + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_rss_finalize, dim3(grid), dim3(256), 0, s, part, G, M, out, scale);
     return hipGetLastError();
 }
 

@@ -12,11 +12,12 @@ namespace oks {
 
 // Fused pre-whitening + centred kx IFFT (+ oversampling chain) + x crop.
 struct RowPwArgs {
-    const float2* in;   // [nc, rows, nx]
-    float2* out;        // [nc, rows, tx]
+    const float2* in;   // first row of the chunk, coil 0; coil c at in + c * in_plane
+    float2* out;        // [nc, rows, tx] with coil stride out_plane
     const float2* W;    // [nc, nc] lower triangular, or nullptr
     int nc;
-    long long rows;     // nz * ny
+    long long rows;     // rows in this launch
+    long long in_plane, out_plane;
     int nx, tx, x0;     // output = centred columns [x0, x0 + tx)
     float scale;
     int os_mode, rec;   // reference oversampling removal to `rec` samples
@@ -44,6 +45,8 @@ struct ColArgs {
     float* out_r;       // mode 1: [Zo, ty, X]
     int C, Z, X, sx;
     int z0, Zo;
+    int G;              // coil groups (one workgroup per group); mode 1 with G > 1 writes `part`
+    float* part;        // [G, Zo, ty, X] partial sums of |.|^2
     int mode;           // 0 complex, 1 RSS over C
     int y0, ty;         // centred output window along n
     int centred;
@@ -57,6 +60,8 @@ hipError_t launch_rows_prewhiten_ifft(const RowPwArgs& a, hipStream_t s);
 hipError_t launch_rows(const RowArgs& a, bool inverse, hipStream_t s);
 size_t cols_lds(int sx, int n, int ty, int mode);
 hipError_t launch_cols(const ColArgs& a, bool inverse, hipStream_t s);
+size_t rows_lds(int n);
+hipError_t launch_rss_finalize(const float* part, int G, long long M, float* out, float scale, hipStream_t s);
 hipError_t launch_prewhiten(float2* d, const float2* W, int nc, long long M, hipStream_t s);
 hipError_t launch_rss(const float2* in, int C, long long M, float* out, float scale, hipStream_t s);
 

@@ -13,7 +13,7 @@ namespace oks {
 
 const float2* whitener(oks_ctx* ctx, const float2* noise, int64_t nc, int64_t nn, uint32_t* flags, const char* slot);
 void col_pass(oks_ctx* ctx, const float2* in, float2* out_c, float* out_r, int C, int Z, int z0, int Zo, int n,
-              int X, int mode, int y0, int ty, float scale, bool inverse, bool centred);
+              int X, int mode, int y0, int ty, float scale, bool inverse, bool centred, int G, const char* slot);
 void row_pass(oks_ctx* ctx, const float2* in, float2* out, long long rows, int n, int tx, int x0, float scale,
               bool inverse, bool centred, const uint8_t* mask, long long mask_rows);
 void fft_axis(oks_ctx* ctx, float2* d, const int64_t* shape, int ndim, int axis, bool inverse, float scale,

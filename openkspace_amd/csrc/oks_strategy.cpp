@@ -39,7 +39,7 @@ static int pick_sx(int n, int ty, int mode, int X) {
 }
 
 void col_pass(oks_ctx* ctx, const float2* in, float2* out_c, float* out_r, int C, int Z, int z0, int Zo, int n,
-              int X, int mode, int y0, int ty, float scale, bool inverse, bool centred) {
+              int X, int mode, int y0, int ty, float scale, bool inverse, bool centred, int G, const char* slot) {
     ColArgs a{};
     a.in = in;
     a.out = out_c;
@@ -57,7 +57,11 @@ void col_pass(oks_ctx* ctx, const float2* in, float2* out_c, float* out_r, int C
     a.p = ctx->plan(n);
     a.sx = pick_sx(n, ty, mode, X);
     if (cols_lds(a.sx, n, ty, mode) > 160 * 1024) fail(OKS_E_UNSUPPORTED, "transform length too large for LDS");
+    a.G = std::max(1, std::min(G, C));
+    const long long M = (long long)Zo * ty * X;
+    if (mode == 1 && a.G > 1) a.part = (float*)ctx->buf(slot ? slot : "rss_part", sizeof(float) * a.G * M);
     OKS_HIP(launch_cols(a, inverse, ctx->stream));
+    if (mode == 1 && a.G > 1) OKS_HIP(launch_rss_finalize(a.part, a.G, M, out_r, scale, ctx->stream));
 }
 
 void row_pass(oks_ctx* ctx, const float2* in, float2* out, long long rows, int n, int tx, int x0, float scale,
@@ -73,7 +77,7 @@ void row_pass(oks_ctx* ctx, const float2* in, float2* out, long long rows, int n
     a.centred = centred;
     a.scale = scale;
     a.p = ctx->plan(n);
-    if ((size_t)4 * 2 * n * sizeof(float2) > 160 * 1024) fail(OKS_E_UNSUPPORTED, "row length too large for LDS");
+    if (rows_lds(n) > 160 * 1024) fail(OKS_E_UNSUPPORTED, "row length too large for LDS");
     OKS_HIP(launch_rows(a, inverse, ctx->stream));
 }
 
@@ -91,7 +95,8 @@ void fft_axis(oks_ctx* ctx, float2* d, const int64_t* shape, int ndim, int axis,
     if (inner == 1)
         row_pass(ctx, d, d, outer, n, n, 0, scale, inverse, centred, nullptr, 0);
     else
-        col_pass(ctx, d, d, nullptr, 1, (int)outer, 0, (int)outer, n, (int)inner, 0, 0, n, scale, inverse, centred);
+        col_pass(ctx, d, d, nullptr, 1, (int)outer, 0, (int)outer, n, (int)inner, 0, 0, n, scale, inverse, centred, 1,
+                 nullptr);
 }
 
 static void check_kspace(const oks_kspace* k, bool need_noise_ok = true) {
@@ -117,40 +122,67 @@ void output_windows(const oks_kspace* k, int crop, Win w[3], uint32_t* flags) {
     w[2] = crop_win(k->nx, t[2]);
 }
 
+// Coil groups for an RSS column pass: enough workgroups to fill 256 CUs
+// twice, at least 3 coils per group.
+static int rss_groups(int nc, long long wgs_per_group) {
+    int G = 1;
+    while (G < nc / 3 && wgs_per_group * G < 1024) ++G;
+    return G;
+}
+
 // IFFT + RSS + crop of an (already pre-processed) grid [nc, nz, ny, nx], with
 // the optional fused whitening W and oversampling chain in the row pass.
+// 2-D grids are processed in chunks of slices so the row-pass intermediate of
+// one chunk is still cache-resident when the column pass reads it.
 void ifft_rss_grid(oks_ctx* ctx, const float2* k, int64_t nc, int64_t nz, int64_t ny, int64_t nx, const float2* W,
                    int os_mode, int rec, bool three_d, const Win w[3], float* out) {
-    const int64_t rows = nz * ny;
-    const int tx = (int)w[2].len;
-    float2* tmp = (float2*)ctx->buf("rss_tmp", sizeof(float2) * nc * rows * tx);
+    const int tx = (int)w[2].len, ty = (int)w[1].len;
     RowPwArgs a{};
-    a.in = k;
-    a.out = tmp;
     a.W = W;
     a.nc = (int)nc;
-    a.rows = rows;
     a.nx = (int)nx;
     a.tx = tx;
     a.x0 = (int)w[2].off;
     a.scale = 1.0f / (float)nx;
     a.os_mode = os_mode;
     a.rec = rec;
+    a.in_plane = nz * ny * nx;
     a.p_enc = ctx->plan((int)nx);
     if (os_mode) a.p_rec = ctx->plan(rec);
     if (rows_prewhiten_lds((int)nc, (int)nx, os_mode) > 160 * 1024)
         fail(OKS_E_UNSUPPORTED, "readout too long for the fused row pass");
-    OKS_HIP(launch_rows_prewhiten_ifft(a, ctx->stream));
     if (!three_d) {
-        col_pass(ctx, tmp, nullptr, out, (int)nc, (int)nz, (int)w[0].off, (int)w[0].len, (int)ny, tx, 1,
-                 (int)w[1].off, (int)w[1].len, 1.0f / (float)ny, true, true);
+        // slices actually written: the z window of the crop
+        const int64_t zlo = w[0].off, zn = w[0].len;
+        const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(zn, ctx->zchunk > 0 ? ctx->zchunk : zn));
+        float2* tmp = (float2*)ctx->buf("rss_tmp", sizeof(float2) * nc * chunk * ny * tx);
+        const int strips = (tx + 15) / 16;
+        for (int64_t z = 0; z < zn; z += chunk) {
+            const int64_t cz = std::min(chunk, zn - z);
+            a.in = k + (zlo + z) * ny * nx;
+            a.out = tmp;
+            a.rows = cz * ny;
+            a.out_plane = cz * ny * tx;
+            OKS_HIP(launch_rows_prewhiten_ifft(a, ctx->stream));
+            const int G = rss_groups((int)nc, cz * strips);
+            col_pass(ctx, tmp, nullptr, out + z * ty * tx, (int)nc, (int)cz, 0, (int)cz, (int)ny, tx, 1,
+                     (int)w[1].off, ty, 1.0f / (float)ny, true, true, G, "rss_part");
+        }
     } else {
-        const int ty = (int)w[1].len;
+        const int64_t rows = nz * ny;
+        float2* tmp = (float2*)ctx->buf("rss_tmp", sizeof(float2) * nc * rows * tx);
+        a.in = k;
+        a.out = tmp;
+        a.rows = rows;
+        a.out_plane = rows * tx;
+        OKS_HIP(launch_rows_prewhiten_ifft(a, ctx->stream));
         float2* tmp2 = (float2*)ctx->buf("rss_tmp2", sizeof(float2) * nc * nz * ty * tx);
         col_pass(ctx, tmp, tmp2, nullptr, 1, (int)(nc * nz), 0, (int)(nc * nz), (int)ny, tx, 0, (int)w[1].off, ty,
-                 1.0f / (float)ny, true, true);
+                 1.0f / (float)ny, true, true, 1, nullptr);
+        const int strips = (ty * tx + 15) / 16;
+        const int G = rss_groups((int)nc, strips);
         col_pass(ctx, tmp2, nullptr, out, (int)nc, 1, 0, 1, (int)nz, ty * tx, 1, (int)w[0].off, (int)w[0].len,
-                 (nz > 1) ? 1.0f / (float)nz : 1.0f, true, true);
+                 (nz > 1) ? 1.0f / (float)nz : 1.0f, true, true, G, "rss_part");
     }
 }
 
