@@ -54,6 +54,8 @@ def lib():
     L.oks_event_record.argtypes = [_P, _P]
     L.oks_event_elapsed_ms.argtypes = [_P, _P, _P, C.POINTER(C.c_float)]
     L.oks_event_destroy.argtypes = [_P, _P]
+    L.oks_grappa_calibrate_dev.argtypes = [_P, _P] + [_I64] * 6 + [C.c_float, _P]
+    L.oks_grappa_synthesize_dev.argtypes = [_P, _P, _P] + [_I64] * 7
     L.oks_fft_centered_dev.argtypes = [_P, _P, _P, _I32, _P, _I32, _I32, _I32]
     L.oks_rss_dev.argtypes = [_P, _P, _I64, _I64, _P]
     L.oks_prewhiten_calibrate_dev.argtypes = [_P, _P, _I32, _I64, _P]

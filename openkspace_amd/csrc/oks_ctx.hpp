@@ -49,6 +49,7 @@ struct oks_ctx {
     int* d_status = nullptr;                       // device-side flags (whitener) [8]
     uint32_t pending_flags = 0;
     int zchunk = 4;                                // slices per row/column chunk (ifft-rss)
+    std::vector<uint8_t> h_acq;                    // host staging: acquired rows
 
     oks::FftPlan plan(int n);
     void* buf(const std::string& name, size_t bytes);

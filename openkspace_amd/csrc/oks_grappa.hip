@@ -1,0 +1,522 @@
+// oks_grappa.hip -- GRAPPA calibration and synthesis on CDNA4.
+//
+// * k_grappa_fills: the reference's data-derived sampling set (coil-0 row has a
+//   non-zero sample, recon/grappa.rs:338-344) and, per missing target row,
+//   the fill that writes it last in the reference's ascending-ky0 order
+//   (recon/grappa.rs:363-420), compacted per offset d.
+// * k_zgemm_sbf16: complex GEMM on MFMA (v_mfma_f32_16x16x32_bf16) with every
+//   operand split into bf16 hi + lo and three products per real product
+//   (hi*hi + hi*lo + lo*hi), FP32 accumulation.  Loaders gather the operands
+//   straight from the k-space grid (implicit im2col): the ACS Gram A^H A, the
+//   right-hand side A^H B (recon/grappa.rs:230-263) and the synthesis
+//   (recon/grappa.rs:386-418).
+// * FP64 ridge normal-equation solve (recon/grappa.rs:266-284): blocked
+//   Cholesky with explicitly inverted 64x64 diagonal blocks, forward and
+//   backward block substitution, on a small FP64 complex GEMM.
+#include <hip/hip_runtime.h>
+
+#include "oks_fft.hpp"
+#include "oks_grappa.hpp"
+
+namespace oks {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------------ fills
+// One workgroup per slice.  list[z][d-1][i] = {ty, ky0}, cnt[z][d-1].
+__global__ void __launch_bounds__(256) k_grappa_fills(GrappaFillArgs a) {
+    extern __shared__ unsigned char smem[];
+    unsigned char* sampled = smem;                    // ny
+    int* cnt = (int*)(smem + ((a.ny + 15) & ~15));   // r-1
+    const int z = blockIdx.x;
+    const int ny = a.ny, nx = a.nx, r = a.r;
+    if (threadIdx.x < r - 1) cnt[threadIdx.x] = 0;
+    const float2* row0 = a.grid + (size_t)z * ny * nx;  // coil 0 plane of slice z
+    for (int y = threadIdx.x >> 6; y < ny; y += blockDim.x >> 6) {
+        const int lane = threadIdx.x & 63;
+        int any = 0;
+        for (int x = lane; x < nx; x += 64) {
+            const float2 v = row0[(size_t)y * nx + x];
+            any |= (v.x != 0.f) | (v.y != 0.f);
+        }
+        any = __any(any);
+        if (lane == 0) sampled[y] = (unsigned char)(any != 0);
+    }
+    __syncthreads();
+    const int kc = a.kky / 2 - 1, span = (a.kky - 1) * r + 1;
+    for (int ty = threadIdx.x; ty < ny; ty += blockDim.x) {
+        if (sampled[ty]) continue;
+        for (int d = 1; d < r; ++d) {
+            const int ky0 = ty - kc * r - d;
+            if (ky0 < 0 || ky0 + span > ny) continue;
+            bool ok = true;
+            for (int q = 0; q < a.kky; ++q) ok = ok && sampled[ky0 + q * r];
+            if (!ok) continue;
+            const int i = atomicAdd(&cnt[d - 1], 1);
+            a.list[((size_t)z * (r - 1) + (d - 1)) * ny + i] = make_int2(ty, ky0);
+            break;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < r - 1) a.cnt[z * (r - 1) + threadIdx.x] = cnt[threadIdx.x];
+}
+
+// ----------------------------------------------------- split-bf16 GEMM
+__device__ __forceinline__ unsigned short bf16_bits_rne(float x) {
+    unsigned int u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(unsigned short h) { return __uint_as_float((unsigned int)h << 16); }
+
+constexpr int BK = 32;
+constexpr int BKP = BK + 8;  // padded LDS row (bf16 elements)
+
+// LDS operand tile: 4 planes (re_hi, re_lo, im_hi, im_lo) of [rows][BKP] bf16.
+struct Planes {
+    unsigned short* p;
+    int rows;
+    __device__ __forceinline__ void put(int row, int k, float2 v) const {
+        const unsigned short rh = bf16_bits_rne(v.x), ih = bf16_bits_rne(v.y);
+        const unsigned short rl = bf16_bits_rne(v.x - bf16_to_f(rh));
+        const unsigned short il = bf16_bits_rne(v.y - bf16_to_f(ih));
+        const int o = row * BKP + k;
+        const int ps = rows * BKP;
+        p[o] = rh;
+        p[ps + o] = rl;
+        p[2 * ps + o] = ih;
+        p[3 * ps + o] = il;
+    }
+    __device__ __forceinline__ bf16x8 get(int plane, int row, int k8) const {
+        return *(const bf16x8*)(p + plane * rows * BKP + row * BKP + k8);
+    }
+};
+
+__device__ __forceinline__ bf16x8 neg8(bf16x8 v) {
+    u32x4 u = __builtin_bit_cast(u32x4, v);
+    u ^= (u32x4){0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+    return __builtin_bit_cast(bf16x8, u);
+}
+
+#define MFMA(a, b, c) __builtin_amdgcn_mfma_f32_16x16x32_bf16((a), (b), (c), 0, 0, 0)
+
+// C[m, n] = sum_k A(m, k) B(k, n), complex.  BM = 64, BN = 32 * WN_TILES;
+// 4 wavefronts: BN = 64 -> 2 x 2 waves of 32 x 32; BN = 32 -> 4 x 1 waves of 16 x 32.
+template <class L, int BN>
+__global__ void __launch_bounds__(256) k_zgemm_sbf16(typename L::Args args) {
+    constexpr int BM = 64;
+    constexpr int WAVES_N = BN / 32;
+    constexpr int WAVES_M = 4 / WAVES_N;
+    constexpr int MS = BM / (16 * WAVES_M);  // 16-row subtiles per wave
+    __shared__ __attribute__((aligned(16))) unsigned short sA[4 * BM * BKP];
+    __shared__ __attribute__((aligned(16))) unsigned short sB[4 * BN * BKP];
+    L ld(args);
+    if (!ld.active()) return;
+    const Planes PA{sA, BM}, PB{sB, BN};
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wm = w / WAVES_N, wn = w % WAVES_N;
+    f32x4 cr[MS][2], ci[MS][2];
+#pragma unroll
+    for (int i = 0; i < MS; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            cr[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+            ci[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+        }
+    const int K = ld.K();
+    const int lr = lane & 15, lk = (lane >> 4) * 8;
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        ld.load_a(PA, k0);
+        ld.template load_b<BN>(PB, k0);
+        __syncthreads();
+#pragma unroll
+        for (int ns = 0; ns < 2; ++ns) {
+            const int nrow = wn * 32 + ns * 16 + lr;
+            const bf16x8 brh = PB.get(0, nrow, lk), brl = PB.get(1, nrow, lk);
+            const bf16x8 bih = PB.get(2, nrow, lk), bil = PB.get(3, nrow, lk);
+            const bf16x8 nbih = neg8(bih), nbil = neg8(bil);
+#pragma unroll
+            for (int ms = 0; ms < MS; ++ms) {
+                const int mrow = wm * MS * 16 + ms * 16 + lr;
+                const bf16x8 arh = PA.get(0, mrow, lk), arl = PA.get(1, mrow, lk);
+                const bf16x8 aih = PA.get(2, mrow, lk), ail = PA.get(3, mrow, lk);
+                f32x4 r = cr[ms][ns], i = ci[ms][ns];
+                r = MFMA(arh, brh, r);
+                r = MFMA(arh, brl, r);
+                r = MFMA(arl, brh, r);
+                r = MFMA(aih, nbih, r);
+                r = MFMA(aih, nbil, r);
+                r = MFMA(ail, nbih, r);
+                i = MFMA(arh, bih, i);
+                i = MFMA(arh, bil, i);
+                i = MFMA(arl, bih, i);
+                i = MFMA(aih, brh, i);
+                i = MFMA(aih, brl, i);
+                i = MFMA(ail, brh, i);
+                cr[ms][ns] = r;
+                ci[ms][ns] = i;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int ms = 0; ms < MS; ++ms)
+#pragma unroll
+        for (int ns = 0; ns < 2; ++ns)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int m = wm * MS * 16 + ms * 16 + (lane >> 4) * 4 + e;
+                const int n = wn * 32 + ns * 16 + lr;
+                ld.store(m, n, make_float2(cr[ms][ns][e], ci[ms][ns][e]));
+            }
+}
+
+// Gram / right-hand side of the ACS fit for slice z = blockIdx.z:
+//   A(m, k) = conj(P(k, m)), B(k, n) = P(k, n)  (Gram)   or  T(k, n)  (rhs)
+//   P(p, col) = grid[ch][z][acs0 + ky0 + a r][kx0 + b],  p = ky0 n_kx + kx0,
+//   col = ch kky kkx + a kkx + b;  T(p, (d-1) nc + ch) = grid[ch][z][acs0 + ky0 + kc r + d][kx0 + kx/2].
+struct CalibLoader {
+    struct Args {
+        const float2* grid;
+        float2* out;  // [nz][M][N]
+        int nc, nz, ny, nx, r, kky, kkx, acs0, n_kx, n_pos, rhs, M, N;
+        const int* skip;  // per slice, or null
+    };
+    Args a;
+    int m0, n0, z;
+    __device__ CalibLoader(const Args& x) : a(x) {
+        m0 = blockIdx.x * 64;
+        n0 = blockIdx.y * (a.rhs ? 32 : 64);
+        z = blockIdx.z;
+    }
+    __device__ bool active() const { return true; }
+    __device__ int K() const { return a.n_pos; }
+    __device__ float2 src(int p, int col) const {
+        const int ky0 = p / a.n_kx, kx0 = p - ky0 * a.n_kx;
+        const int kk = a.kky * a.kkx;
+        const int ch = col / kk, rem = col - ch * kk;
+        const int ai = rem / a.kkx, bi = rem - ai * a.kkx;
+        return a.grid[(((size_t)ch * a.nz + z) * a.ny + a.acs0 + ky0 + ai * a.r) * a.nx + kx0 + bi];
+    }
+    __device__ float2 tgt(int p, int t) const {
+        const int ky0 = p / a.n_kx, kx0 = p - ky0 * a.n_kx;
+        const int d = t / a.nc + 1, ch = t - (d - 1) * a.nc;
+        const int kc = a.kky / 2 - 1;
+        return a.grid[(((size_t)ch * a.nz + z) * a.ny + a.acs0 + ky0 + kc * a.r + d) * a.nx + kx0 + a.kkx / 2];
+    }
+    // k (position) fastest across threads: consecutive kx0 are contiguous.
+    __device__ void load_a(const Planes& P, int k0) const {
+        const int k = threadIdx.x & 31;
+        for (int m = threadIdx.x >> 5; m < 64; m += 8) {
+            float2 v = make_float2(0.f, 0.f);
+            if (m0 + m < a.M && k0 + k < a.n_pos) {
+                v = src(k0 + k, m0 + m);
+                v.y = -v.y;  // conj
+            }
+            P.put(m, k, v);
+        }
+    }
+    template <int BN>
+    __device__ void load_b(const Planes& P, int k0) const {
+        const int k = threadIdx.x & 31;
+        for (int n = threadIdx.x >> 5; n < BN; n += 8) {
+            float2 v = make_float2(0.f, 0.f);
+            if (n0 + n < a.N && k0 + k < a.n_pos) v = a.rhs ? tgt(k0 + k, n0 + n) : src(k0 + k, n0 + n);
+            P.put(n, k, v);
+        }
+    }
+    __device__ void store(int m, int n, float2 v) const {
+        m += m0;
+        n += n0;
+        if (m < a.M && n < a.N) a.out[((size_t)z * a.M + m) * a.N + n] = v;
+    }
+};
+
+// Synthesis of target row ty from fill (ky0, d) of slice z:
+//   out[kx0, ch] = sum_j S(kx0, j) w_d[ch, j],  S(kx0, j) = grid[c'][z][ky0 + a r][kx0 + b].
+// blockIdx.z = z (r-1) + (d-1), blockIdx.y = fill index, blockIdx.x = kx0 tile.
+struct ApplyLoader {
+    struct Args {
+        float2* grid;
+        const float2* w;     // [nz][r-1][nc][n_src] (slice stride wz; 0 = one kernel for all slices)
+        const int2* list;    // [nz][r-1][ny]
+        const int* cnt;      // [nz][r-1]
+        const int* skip;     // [nz] calibration failed -> leave slice undersampled
+        int nc, nz, ny, nx, r, kky, kkx, n_kx, n_src;
+        long long wz;
+    };
+    Args a;
+    int z, d, ty, ky0, m0;
+    bool act;
+    __device__ ApplyLoader(const Args& x) : a(x) {
+        const int zd = blockIdx.z;
+        z = zd / (a.r - 1);
+        d = zd - z * (a.r - 1) + 1;
+        const int f = blockIdx.y;
+        act = f < a.cnt[zd] && !(a.skip && a.skip[z]);
+        ty = ky0 = 0;
+        if (act) {
+            const int2 e = a.list[(size_t)zd * a.ny + f];
+            ty = e.x;
+            ky0 = e.y;
+        }
+        m0 = blockIdx.x * 64;
+    }
+    __device__ bool active() const { return act; }
+    __device__ int K() const { return a.n_src; }
+    // kx0 (m) fastest: contiguous samples of one source row.
+    __device__ void load_a(const Planes& P, int k0) const {
+        const int m = threadIdx.x & 63;
+        const int kk = a.kky * a.kkx;
+        for (int k = threadIdx.x >> 6; k < BK; k += 4) {
+            float2 v = make_float2(0.f, 0.f);
+            const int j = k0 + k;
+            if (m0 + m < a.n_kx && j < a.n_src) {
+                const int ch = j / kk, rem = j - ch * kk;
+                const int ai = rem / a.kkx, bi = rem - ai * a.kkx;
+                v = a.grid[(((size_t)ch * a.nz + z) * a.ny + ky0 + ai * a.r) * a.nx + m0 + m + bi];
+            }
+            P.put(m, k, v);
+        }
+    }
+    template <int BN>
+    __device__ void load_b(const Planes& P, int k0) const {
+        const int k = threadIdx.x & 31;
+        const float2* wd = a.w + (size_t)z * a.wz + (size_t)(d - 1) * a.nc * a.n_src;
+        for (int n = threadIdx.x >> 5; n < BN; n += 8) {
+            float2 v = make_float2(0.f, 0.f);
+            if (n < a.nc && k0 + k < a.n_src) v = wd[(size_t)n * a.n_src + k0 + k];
+            P.put(n, k, v);
+        }
+    }
+    __device__ void store(int m, int n, float2 v) const {
+        m += m0;
+        if (m < a.n_kx && n < a.nc)
+            a.grid[(((size_t)n * a.nz + z) * a.ny + ty) * a.nx + m + a.kkx / 2] = v;
+    }
+};
+
+// --------------------------------------------------------------- FP64
+__device__ __forceinline__ double2 zmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 zconj(double2 a) { return make_double2(a.x, -a.y); }
+
+// C = alpha op(A) op(B) + beta C, complex double, batched over blockIdx.z.
+// op 0 = N, 2 = conjugate transpose.  Tile 32 x 32, 256 threads x (2 x 2).
+__global__ void __launch_bounds__(256) k_zgemm64(Zgemm64Args g) {
+    __shared__ double2 As[16][33];
+    __shared__ double2 Bs[16][33];
+    const int b = blockIdx.z;
+    const double2* A = g.A + (size_t)b * g.sA;
+    const double2* B = g.B + (size_t)b * g.sB;
+    double2* C = g.C + (size_t)b * g.sC;
+    if (g.skip && g.skip[b]) return;
+    const int m0 = blockIdx.x * 32, n0 = blockIdx.y * 32;
+    const int tm = threadIdx.x / 16, tn = threadIdx.x % 16;
+    double2 acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) acc[i][j] = make_double2(0.0, 0.0);
+    for (int k0 = 0; k0 < g.K; k0 += 16) {
+        for (int e = threadIdx.x; e < 16 * 32; e += 256) {
+            const int kk = e / 32, mm = e % 32;
+            const int m = m0 + mm, k = k0 + kk;
+            double2 v = make_double2(0.0, 0.0);
+            if (m < g.M && k < g.K) v = g.opA ? zconj(A[(size_t)k * g.lda + m]) : A[(size_t)m * g.lda + k];
+            As[kk][mm] = v;
+            const int n = n0 + mm;
+            double2 u = make_double2(0.0, 0.0);
+            if (n < g.N && k < g.K) u = g.opB ? zconj(B[(size_t)n * g.ldb + k]) : B[(size_t)k * g.ldb + n];
+            Bs[kk][mm] = u;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < 16; ++kk) {
+            const double2 a0 = As[kk][tm * 2], a1 = As[kk][tm * 2 + 1];
+            const double2 b0 = Bs[kk][tn * 2], b1 = Bs[kk][tn * 2 + 1];
+            double2 p;
+            p = zmul(a0, b0); acc[0][0].x += p.x; acc[0][0].y += p.y;
+            p = zmul(a0, b1); acc[0][1].x += p.x; acc[0][1].y += p.y;
+            p = zmul(a1, b0); acc[1][0].x += p.x; acc[1][0].y += p.y;
+            p = zmul(a1, b1); acc[1][1].x += p.x; acc[1][1].y += p.y;
+        }
+        __syncthreads();
+    }
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            const int m = m0 + tm * 2 + i, n = n0 + tn * 2 + j;
+            if (m < g.M && n < g.N) {
+                double2* c = &C[(size_t)m * g.ldc + n];
+                double2 v = make_double2(g.alpha * acc[i][j].x, g.alpha * acc[i][j].y);
+                if (g.beta != 0.0) {
+                    v.x += g.beta * c->x;
+                    v.y += g.beta * c->y;
+                }
+                *c = v;
+            }
+        }
+}
+
+// Normal matrix in FP64 with the ridge lambda = ridge * mean_i Re G_ii
+// (recon/grappa.rs:266-275); also the FP64 right-hand side.  One workgroup per slice.
+__global__ void __launch_bounds__(256) k_normal64(Normal64Args a) {
+    __shared__ double red[256];
+    const int z = blockIdx.x;
+    const float2* G = a.G32 + (size_t)z * a.n * a.n;
+    double2* H = a.G64 + (size_t)z * a.n * a.n;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < a.n; i += blockDim.x) s += (double)G[(size_t)i * a.n + i].x;
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    const double mean = red[0] / fmax((double)a.n, 1.0);
+    const double lam = (double)a.ridge * fmax(mean, (double)1.1920929e-07f);
+    for (size_t e = threadIdx.x; e < (size_t)a.n * a.n; e += blockDim.x) {
+        const size_t i = e / a.n, j = e % a.n;
+        const float2 v = G[e];
+        H[e] = make_double2((double)v.x + (i == j ? lam : 0.0), (double)v.y);
+    }
+    const float2* R = a.R32 + (size_t)z * a.n * a.m;
+    double2* Rd = a.R64 + (size_t)z * a.n * a.m;
+    for (size_t e = threadIdx.x; e < (size_t)a.n * a.m; e += blockDim.x) Rd[e] = make_double2(R[e].x, R[e].y);
+}
+
+// Cholesky of the diagonal block G[k0:k0+kn, k0:k0+kn] (lower) and its
+// explicit inverse (recon/prewhiten.rs:148-210, in FP64).  A non-PD pivot
+// marks the slice failed (the reference leaves it undersampled).
+__global__ void __launch_bounds__(256) k_potrf64_diag(Potrf64Args a) {
+    __shared__ double2 L[64][65];
+    __shared__ double2 I[64][65];
+    __shared__ int bad;
+    const int z = blockIdx.x;
+    if (a.skip[z]) return;
+    const int n = a.n, k0 = a.k0, kn = a.kn;
+    const double2* G = a.G + (size_t)z * n * n;
+    double2* Lg = a.L + (size_t)z * n * n;
+    double2* D = a.D + ((size_t)z * a.nblk + a.kb) * 64 * 64;
+    for (int e = threadIdx.x; e < kn * kn; e += 256) {
+        const int i = e / kn, j = e % kn;
+        L[i][j] = (j <= i) ? G[(size_t)(k0 + i) * n + k0 + j] : make_double2(0.0, 0.0);
+        I[i][j] = make_double2(0.0, 0.0);
+    }
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    for (int j = 0; j < kn; ++j) {
+        if (threadIdx.x == 0) {
+            const double d = L[j][j].x;
+            if (!(d > 0.0) || !isfinite(d)) bad = 1;
+            L[j][j] = make_double2(sqrt(fmax(d, 1e-300)), 0.0);
+        }
+        __syncthreads();
+        const double inv = 1.0 / L[j][j].x;
+        for (int i = j + 1 + threadIdx.x; i < kn; i += 256) L[i][j] = make_double2(L[i][j].x * inv, L[i][j].y * inv);
+        __syncthreads();
+        const int t = kn - j - 1;
+        for (int e = threadIdx.x; e < t * t; e += 256) {
+            const int i = j + 1 + e / t, c = j + 1 + e % t;
+            if (c <= i) {
+                const double2 p = zmul(L[i][j], zconj(L[c][j]));
+                L[i][c].x -= p.x;
+                L[i][c].y -= p.y;
+            }
+        }
+        __syncthreads();
+    }
+    // inverse of the lower-triangular block, one column per thread
+    if (threadIdx.x < kn) {
+        const int c = threadIdx.x;
+        for (int i = c; i < kn; ++i) {
+            double2 s = make_double2(i == c ? 1.0 : 0.0, 0.0);
+            for (int j = c; j < i; ++j) {
+                const double2 p = zmul(L[i][j], I[j][c]);
+                s.x -= p.x;
+                s.y -= p.y;
+            }
+            const double dd = L[i][i].x;
+            I[i][c] = make_double2(s.x / dd, s.y / dd);
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int i = e / 64, j = e % 64;
+        D[e] = (i < kn && j < kn) ? I[i][j] : make_double2(0.0, 0.0);
+        if (i < kn && j < kn) Lg[(size_t)(k0 + i) * n + k0 + j] = (j <= i) ? L[i][j] : make_double2(0.0, 0.0);
+    }
+    if (threadIdx.x == 0 && bad) {
+        a.fail[z] = 1;
+        *a.status = 1;
+    }
+}
+
+// weights[z][d-1][ch][j] = X[z][j][(d-1) nc + ch]  (recon/grappa.rs:286-299)
+__global__ void k_weights_out(const double2* X, float2* w, int nz, int nc, int r, int n_src) {
+    const int n_tgt = nc * (r - 1);
+    const size_t total = (size_t)nz * n_tgt * n_src;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t z = e / ((size_t)n_tgt * n_src);
+        const size_t rem = e - z * n_tgt * n_src;
+        const int t = (int)(rem / n_src), j = (int)(rem % n_src);
+        const double2 v = X[(z * n_src + j) * n_tgt + t];
+        w[e] = make_float2((float)v.x, (float)v.y);
+    }
+}
+
+// ------------------------------------------------------------ launchers
+hipError_t launch_grappa_fills(const GrappaFillArgs& a, hipStream_t s) {
+    const size_t lds = ((a.ny + 15) & ~15) + 64;
+    hipLaunchKernelGGL(k_grappa_fills, dim3(a.nz), dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_calib_gemm(const CalibArgs& c, hipStream_t s) {
+    CalibLoader::Args a{c.grid, c.out, c.nc, c.nz, c.ny, c.nx, c.r, c.kky, c.kkx, c.acs0, c.n_kx, c.n_pos, c.rhs,
+                        c.M, c.N, nullptr};
+    if (c.rhs) {
+        dim3 grid((c.M + 63) / 64, (c.N + 31) / 32, c.nz);
+        hipLaunchKernelGGL((k_zgemm_sbf16<CalibLoader, 32>), grid, dim3(256), 0, s, a);
+    } else {
+        dim3 grid((c.M + 63) / 64, (c.N + 63) / 64, c.nz);
+        hipLaunchKernelGGL((k_zgemm_sbf16<CalibLoader, 64>), grid, dim3(256), 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_apply_gemm(const ApplyArgs& c, hipStream_t s) {
+    if (c.nc > 32) return hipErrorInvalidValue;
+    ApplyLoader::Args a{c.grid, c.w, c.list, c.cnt, c.skip, c.nc, c.nz, c.ny, c.nx, c.r, c.kky, c.kkx, c.n_kx, c.n_src,
+                        c.wz};
+    dim3 grid((c.n_kx + 63) / 64, c.ny, c.nz * (c.r - 1));
+    hipLaunchKernelGGL((k_zgemm_sbf16<ApplyLoader, 32>), grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_zgemm64(const Zgemm64Args& g, int batch, hipStream_t s) {
+    if (g.M <= 0 || g.N <= 0) return hipSuccess;
+    dim3 grid((g.M + 31) / 32, (g.N + 31) / 32, batch);
+    hipLaunchKernelGGL(k_zgemm64, grid, dim3(256), 0, s, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_normal64(const Normal64Args& a, int nz, hipStream_t s) {
+    hipLaunchKernelGGL(k_normal64, dim3(nz), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_potrf64_diag(const Potrf64Args& a, int nz, hipStream_t s) {
+    hipLaunchKernelGGL(k_potrf64_diag, dim3(nz), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_weights_out(const double2* X, float2* w, int nz, int nc, int r, int n_src, hipStream_t s) {
+    const size_t total = (size_t)nz * nc * (r - 1) * n_src;
+    const unsigned grid = (unsigned)std::min<size_t>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_weights_out, dim3(grid), dim3(256), 0, s, X, w, nz, nc, r, n_src);
+    return hipGetLastError();
+}
+
+}  // namespace oks

@@ -1,0 +1,69 @@
+// oks_grappa.hpp -- argument blocks and launchers of the GRAPPA kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace oks {
+
+struct GrappaFillArgs {
+    const float2* grid;  // [nc, nz, ny, nx]; coil 0 decides the sampled rows
+    int2* list;          // [nz][r-1][ny] {ty, ky0}
+    int* cnt;            // [nz][r-1]
+    int nz, ny, nx, r, kky;
+};
+
+struct CalibArgs {
+    const float2* grid;
+    float2* out;  // [nz][M][N]
+    int nc, nz, ny, nx, r, kky, kkx, acs0, n_kx, n_pos, rhs, M, N;
+};
+
+struct ApplyArgs {
+    float2* grid;
+    const float2* w;
+    const int2* list;
+    const int* cnt;
+    const int* skip;
+    int nc, nz, ny, nx, r, kky, kkx, n_kx, n_src;
+    long long wz;  // weight stride between slices (0: one kernel for every slice)
+};
+
+struct Zgemm64Args {
+    const double2* A;
+    const double2* B;
+    double2* C;
+    long long sA, sB, sC;  // batch strides (elements)
+    int M, N, K, lda, ldb, ldc;
+    int opA, opB;          // 0 = N, 2 = conjugate transpose
+    double alpha, beta;
+    const int* skip;       // per batch, or null
+};
+
+struct Normal64Args {
+    const float2* G32;
+    const float2* R32;
+    double2* G64;
+    double2* R64;
+    int n, m;
+    float ridge;
+};
+
+struct Potrf64Args {
+    const double2* G;
+    double2* L;
+    double2* D;  // [nz][nblk][64][64]
+    int* fail;   // [nz]
+    const int* skip;
+    int* status; // sticky flag word
+    int n, k0, kn, kb, nblk;
+};
+
+hipError_t launch_grappa_fills(const GrappaFillArgs& a, hipStream_t s);
+hipError_t launch_calib_gemm(const CalibArgs& a, hipStream_t s);
+hipError_t launch_apply_gemm(const ApplyArgs& a, hipStream_t s);
+hipError_t launch_zgemm64(const Zgemm64Args& g, int batch, hipStream_t s);
+hipError_t launch_normal64(const Normal64Args& a, int nz, hipStream_t s);
+hipError_t launch_potrf64_diag(const Potrf64Args& a, int nz, hipStream_t s);
+hipError_t launch_weights_out(const double2* X, float2* w, int nz, int nc, int r, int n_src, hipStream_t s);
+
+}  // namespace oks

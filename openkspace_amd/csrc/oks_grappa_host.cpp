@@ -1,0 +1,221 @@
+// oks_grappa_host.cpp -- GrappaRss on the GPU: calibration (MFMA Gram and
+// right-hand side, FP64 blocked ridge solve), synthesis (MFMA) and the
+// shared IFFT + RSS tail; plus the function-level GRAPPA entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/oks.h"
+#include "oks_abi_util.hpp"
+#include "oks_ctx.hpp"
+#include "oks_grappa.hpp"
+#include "oks_host.hpp"
+#include "oks_kernels.hpp"
+#include "oks_pipeline.hpp"
+
+namespace oks {
+
+// Geometry and checks of GrappaKernel::calibrate (recon/grappa.rs:163-217).
+struct GrGeom {
+    int n_src, n_tgt, n_ky_pos, n_kx_pos, n_pos;
+};
+
+static int grappa_geom(int64_t nc, int64_t ny_acs, int64_t nx, int64_t r, int64_t kky, int64_t kkx, GrGeom* g,
+                       std::string* why) {
+    if (r < 2) { *why = "acceleration must be >= 2"; return OKS_E_GRAPPA_BAD_CONFIG; }
+    if (kky < 2 || kky % 2 != 0) { *why = "kernel_ky must be even >= 2"; return OKS_E_GRAPPA_BAD_CONFIG; }
+    if (kkx == 0 || kkx % 2 == 0) { *why = "kernel_kx must be odd >= 1"; return OKS_E_GRAPPA_BAD_CONFIG; }
+    const int64_t span = (kky - 1) * r + 1;
+    if (ny_acs < span + r) {
+        *why = "ACS region too small (need " + std::to_string(span + r) + ", got " + std::to_string(ny_acs) + ")";
+        return OKS_E_GRAPPA_ACS_TOO_SMALL;
+    }
+    if (nx < kkx) {
+        *why = "ACS region too small (need " + std::to_string(kkx) + ", got " + std::to_string(nx) + ")";
+        return OKS_E_GRAPPA_ACS_TOO_SMALL;
+    }
+    const int64_t kc = kky / 2 - 1, max_off = kc * r + (r - 1);
+    int64_t n_ky = std::max<int64_t>(ny_acs - span, 0);
+    n_ky = std::max<int64_t>(std::min(n_ky, ny_acs - max_off - 1), 0);
+    const int64_t n_kx = std::max<int64_t>(nx - (kkx - 1), 0);
+    g->n_src = (int)(nc * kky * kkx);
+    g->n_tgt = (int)(nc * (r - 1));
+    g->n_ky_pos = (int)n_ky;
+    g->n_kx_pos = (int)n_kx;
+    g->n_pos = (int)(n_ky * n_kx);
+    if (g->n_pos == 0) {
+        *why = "ACS region too small (no kernel positions)";
+        return OKS_E_GRAPPA_ACS_TOO_SMALL;
+    }
+    return OKS_OK;
+}
+
+static void zg(oks_ctx* ctx, int batch, const double2* A, long long sA, int lda, int opA, const double2* B,
+               long long sB, int ldb, int opB, double2* C, long long sC, int ldc, int M, int N, int K, double alpha,
+               double beta, const int* skip) {
+    Zgemm64Args g{A, B, C, sA, sB, sC, M, N, K, lda, ldb, ldc, opA, opB, alpha, beta, skip};
+    OKS_HIP(launch_zgemm64(g, batch, ctx->stream));
+}
+
+// Calibrate the GRAPPA weights of every slice of `grid` from the ACS rows
+// [acs0, acs0 + ny_acs): w[z][d-1][ch][j], FP32.  Slices whose normal matrix
+// is not positive definite are marked in `fail[z]`.
+static void grappa_calibrate_batch(oks_ctx* ctx, const float2* grid, int nc, int nz, int ny, int nx, int acs0,
+                                   int ny_acs, int r, int kky, int kkx, float ridge, const GrGeom& g, float2* w,
+                                   int* fail, long long* launches) {
+    (void)ny_acs;
+    const int n = g.n_src, m = g.n_tgt;
+    float2* G32 = (float2*)ctx->buf("gr_G32", sizeof(float2) * (size_t)nz * n * n);
+    float2* R32 = (float2*)ctx->buf("gr_R32", sizeof(float2) * (size_t)nz * n * m);
+    CalibArgs ca{grid, G32, nc, nz, ny, nx, r, kky, kkx, acs0, g.n_kx_pos, g.n_pos, 0, n, n};
+    OKS_HIP(launch_calib_gemm(ca, ctx->stream));
+    CalibArgs cb{grid, R32, nc, nz, ny, nx, r, kky, kkx, acs0, g.n_kx_pos, g.n_pos, 1, n, m};
+    OKS_HIP(launch_calib_gemm(cb, ctx->stream));
+    const size_t nn = (size_t)n * n, nm = (size_t)n * m;
+    double2* G = (double2*)ctx->buf("gr_G64", sizeof(double2) * nz * nn);
+    double2* R = (double2*)ctx->buf("gr_R64", sizeof(double2) * nz * nm);
+    double2* L = (double2*)ctx->buf("gr_L64", sizeof(double2) * nz * nn);
+    double2* Y = (double2*)ctx->buf("gr_Y64", sizeof(double2) * nz * nm);
+    double2* X = (double2*)ctx->buf("gr_X64", sizeof(double2) * nz * nm);
+    const int nblk = (n + 63) / 64;
+    double2* D = (double2*)ctx->buf("gr_D64", sizeof(double2) * (size_t)nz * nblk * 64 * 64);
+    OKS_HIP(hipMemsetAsync(fail, 0, sizeof(int) * nz, ctx->stream));
+    Normal64Args na{G32, R32, G, R, n, m, ridge};
+    OKS_HIP(launch_normal64(na, nz, ctx->stream));
+    long long cnt = 3;
+    const long long snn = (long long)nn, snm = (long long)nm, sD = (long long)nblk * 64 * 64;
+    for (int kb = 0; kb < nblk; ++kb) {
+        const int k0 = kb * 64, kn = std::min(64, n - k0), r0 = k0 + kn, rn = n - r0;
+        Potrf64Args pa{G, L, D, fail, fail, ctx->d_status + 2, n, k0, kn, kb, nblk};
+        OKS_HIP(launch_potrf64_diag(pa, nz, ctx->stream));
+        const double2* Dk = D + (size_t)kb * 64 * 64;
+        if (rn > 0) {
+            // L[r0:, k0:k0+kn] = G[r0:, k0:k0+kn] Dk^H
+            zg(ctx, nz, G + (size_t)r0 * n + k0, snn, n, 0, Dk, sD, 64, 2, L + (size_t)r0 * n + k0, snn, n, rn, kn,
+               kn, 1.0, 0.0, fail);
+            // G[r0:, r0:] -= L[r0:, k0:k0+kn] L[r0:, k0:k0+kn]^H
+            zg(ctx, nz, L + (size_t)r0 * n + k0, snn, n, 0, L + (size_t)r0 * n + k0, snn, n, 2,
+               G + (size_t)r0 * n + r0, snn, n, rn, rn, kn, -1.0, 1.0, fail);
+        }
+        // Y_k = Dk R_k ; R[r0:] -= L[r0:, k0:k0+kn] Y_k
+        zg(ctx, nz, Dk, sD, 64, 0, R + (size_t)k0 * m, snm, m, 0, Y + (size_t)k0 * m, snm, m, kn, m, kn, 1.0, 0.0,
+           fail);
+        if (rn > 0)
+            zg(ctx, nz, L + (size_t)r0 * n + k0, snn, n, 0, Y + (size_t)k0 * m, snm, m, 0, R + (size_t)r0 * m, snm, m,
+               rn, m, kn, -1.0, 1.0, fail);
+        cnt += 5;
+    }
+    for (int kb = nblk - 1; kb >= 0; --kb) {
+        const int k0 = kb * 64, kn = std::min(64, n - k0);
+        const double2* Dk = D + (size_t)kb * 64 * 64;
+        // X_k = Dk^H Y_k ; Y[0:k0] -= L[k0:k0+kn, 0:k0]^H X_k
+        zg(ctx, nz, Dk, sD, 64, 2, Y + (size_t)k0 * m, snm, m, 0, X + (size_t)k0 * m, snm, m, kn, m, kn, 1.0, 0.0,
+           fail);
+        if (k0 > 0)
+            zg(ctx, nz, L + (size_t)k0 * n, snn, n, 2, X + (size_t)k0 * m, snm, m, 0, Y, snm, m, k0, m, kn, -1.0, 1.0,
+               fail);
+        cnt += 2;
+    }
+    OKS_HIP(launch_weights_out(X, w, nz, nc, r, n, ctx->stream));
+    if (launches) *launches = cnt + 1;
+}
+
+// Fill the missing rows of every slice of `grid` in place.
+static void grappa_apply_batch(oks_ctx* ctx, float2* grid, int nc, int nz, int ny, int nx, int r, int kky, int kkx,
+                               const float2* w, long long wz, const int* skip, const GrGeom& g) {
+    int2* list = (int2*)ctx->buf("gr_list", sizeof(int2) * (size_t)nz * (r - 1) * ny);
+    int* cnt = (int*)ctx->buf("gr_cnt", sizeof(int) * (size_t)nz * (r - 1));
+    GrappaFillArgs fa{grid, list, cnt, nz, ny, nx, r, kky};
+    OKS_HIP(launch_grappa_fills(fa, ctx->stream));
+    ApplyArgs aa{grid, w, list, cnt, skip, nc, nz, ny, nx, r, kky, kkx, nx - kkx + 1, g.n_src, wz};
+    OKS_HIP(launch_apply_gemm(aa, ctx->stream));
+}
+
+// GrappaRss::reconstruct steps 3-4 (recon/strategy.rs:349-399) on the
+// pre-processed grid.
+static void grappa_volume(oks_ctx* ctx, float2* grid, const oks_kspace* k, const oks_pattern& p, int kky, int kkx,
+                          float ridge, uint32_t* flags) {
+    const int nc = (int)k->nc, nz = (int)k->nz, ny = (int)k->ny, nx = (int)k->nx, r = (int)p.r;
+    const int ny_acs = (int)(p.acs_end - p.acs_start);
+    GrGeom g{};
+    std::string why;
+    if (grappa_geom(nc, ny_acs, nx, r, kky, kkx, &g, &why) != OKS_OK) {
+        *flags |= OKS_F_GRAPPA_CALIB_FAILED;  // every slice stays undersampled (recon/strategy.rs:389-395)
+        return;
+    }
+    float2* w = (float2*)ctx->buf("gr_w", sizeof(float2) * (size_t)nz * (r - 1) * nc * g.n_src);
+    int* fail = (int*)ctx->buf("gr_fail", sizeof(int) * nz);
+    grappa_calibrate_batch(ctx, grid, nc, nz, ny, nx, (int)p.acs_start, ny_acs, r, kky, kkx, ridge, g, w, fail,
+                           nullptr);
+    grappa_apply_batch(ctx, grid, nc, nz, ny, nx, r, kky, kkx, w, (long long)(r - 1) * nc * g.n_src, fail, g);
+}
+
+void recon_grappa_dev(oks_ctx* ctx, const oks_grappa_desc* d, const oks_kspace* k, oks_image* img) {
+    check_kspace(k);
+    if (!d || !img || !img->data) fail(OKS_E_INVALID_ARG, "null descriptor or image");
+    uint32_t flags = 0;
+    const float2* W = d->prewhiten ? whitener(ctx, (const float2*)k->noise, k->nc, k->n_noise, &flags, "W") : nullptr;
+    int os_mode = 0, rec = 0;
+    resolve_os(k, d->remove_oversampling, &os_mode, &rec, &flags);
+    float2* grid = prep_grid(ctx, k, W, os_mode, rec, "grid");
+    oks_pattern p;
+    if (!detect_pattern(k->mask, k->nz, k->ny, k->nx, &p)) {
+        flags |= OKS_F_NO_PATTERN;
+    } else {
+        grappa_volume(ctx, grid, k, p, d->kernel_ky, d->kernel_kx, d->ridge, &flags);
+    }
+    Win w[3];
+    output_windows(k, d->crop_to_recon_matrix, w, &flags);
+    ifft_rss_grid(ctx, grid, k->nc, k->nz, k->ny, k->nx, nullptr, 0, 0, three_d_of(d->fft_mode, k), w, img->data);
+    img->nz = w[0].len;
+    img->ny = w[1].len;
+    img->nx = w[2].len;
+    img->flags = flags;
+}
+
+}  // namespace oks
+
+using namespace oks;
+
+extern "C" {
+
+int oks_recon_grappa(oks_ctx* ctx, const oks_grappa_desc* d, const oks_kspace* k, oks_image* img) {
+    return host_call(ctx, d, k, img, recon_grappa_dev);
+}
+
+int oks_recon_grappa_dev(oks_ctx* ctx, const oks_grappa_desc* d, const oks_kspace* k, oks_image* img) {
+    return guarded(ctx, [&] { recon_grappa_dev(ctx, d, k, img); });
+}
+
+int oks_grappa_calibrate_dev(oks_ctx* ctx, const oks_c32* acs, int64_t nc, int64_t ny_acs, int64_t nx, int64_t r,
+                             int64_t kky, int64_t kkx, float ridge, oks_c32* weights) {
+    return guarded(ctx, [&] {
+        if (!acs || !weights || nc < 1 || nc > 32) fail(OKS_E_INVALID_ARG, "bad calibration arguments");
+        GrGeom g{};
+        std::string why;
+        const int st = grappa_geom(nc, ny_acs, nx, r, kky, kkx, &g, &why);
+        if (st != OKS_OK) fail(st, why);
+        int* f = (int*)ctx->buf("grc_fail", sizeof(int));
+        grappa_calibrate_batch(ctx, (const float2*)acs, (int)nc, 1, (int)ny_acs, (int)nx, 0, (int)ny_acs, (int)r,
+                               (int)kky, (int)kkx, ridge, g, (float2*)weights, f, nullptr);
+        int h = 0;
+        OKS_HIP(hipMemcpyAsync(&h, f, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        OKS_HIP(hipStreamSynchronize(ctx->stream));
+        if (h) fail(OKS_E_GRAPPA_CHOLESKY, "Cholesky factorization failed (matrix not positive definite)");
+    });
+}
+
+int oks_grappa_synthesize_dev(oks_ctx* ctx, const oks_c32* weights, oks_c32* kspace, int64_t nc, int64_t nz,
+                              int64_t ny, int64_t nx, int64_t r, int64_t kky, int64_t kkx) {
+    return guarded(ctx, [&] {
+        if (!weights || !kspace || nc < 1 || nc > 32 || r < 2 || kky < 2 || kkx < 1 || nx < kkx)
+            fail(OKS_E_INVALID_ARG, "bad synthesis arguments");
+        GrGeom g{};
+        g.n_src = (int)(nc * kky * kkx);
+        grappa_apply_batch(ctx, (float2*)kspace, (int)nc, (int)nz, (int)ny, (int)nx, (int)r, (int)kky, (int)kkx,
+                           (const float2*)weights, 0, nullptr, g);
+    });
+}
+
+}  // extern "C"

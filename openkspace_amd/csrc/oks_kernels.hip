@@ -203,6 +203,48 @@ __global__ void __launch_bounds__(512) k_rows_pw_os(RowPwArgs a) {
     }
 }
 
+// K-space form of the reader closure with oversampling removal: every row is
+// whitened, then each coil's readout goes through OversamplingRemover::apply
+// (recon/oversampling.rs:122-147) and lands re-centred at
+// [nx/2 - rec/2, nx/2 + rec/2) of a zeroed grid row (io/ismrmrd/reader.rs:392-399).
+// Rows that were never acquired (all-zero mask row, `acq[row] == 0`) stay zero.
+template <int NCMAX>
+__global__ void __launch_bounds__(512) k_prep_os(RowPwArgs a, const uint8_t* acq) {
+    extern __shared__ float2 lds[];
+    const int nx = a.nx, nc = a.nc;
+    const int WPB = blockDim.x >> 6;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float2* A = lds;
+    float2* B = lds + (size_t)nc * nx;
+    const size_t row = blockIdx.x;
+    load_whitened<NCMAX>(a, A, nx, row);
+    __syncthreads();
+    const int half = nx / 2, rec = a.rec, dst = half - rec / 2;
+    const bool on = acq[row] != 0;
+    for (int c = w; c < nc; c += WPB) {
+        float2* outp = a.out + (size_t)c * a.out_plane + row * nx;
+        if (!on) {
+            for (int x = lane; x < nx; x += 64) outp[x] = make_float2(0.f, 0.f);
+            continue;
+        }
+        float2* buf = A + (size_t)c * nx;
+        float2* scr = B + (size_t)w * nx;
+        float2* res = fft_wave<true>(buf, scr, a.p_enc, lane, 64);
+        const int crop0 = (nx - rec) / 2, rh = (rec + 1) / 2;
+        float2* q = (res == buf) ? scr : buf;
+        const float inv_nx = 1.0f / (float)nx;
+        for (int i = lane; i < rec; i += 64) q[i] = cscale(res[(((i + rh) % rec) + crop0 + half) % nx], inv_nx);
+        wave_sync();
+        float2* other = (q == buf) ? scr : buf;
+        float2* S = fft_wave<false>(q, other, a.p_rec, lane, 64);
+        for (int x = lane; x < nx; x += 64) {
+            const int m = x - dst;
+            outp[x] = (m >= 0 && m < rec) ? S[(m + rec / 2) % rec] : make_float2(0.f, 0.f);
+        }
+        wave_sync();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Generic row pass: centred 1-D transform along the contiguous axis of
 // [rows, n]; optional input mask multiply (mask row index = row % mask_rows),
@@ -330,14 +372,14 @@ __global__ void k_rss_finalize(const float* __restrict__ part, int G, long long 
 // Stand-alone pre-whitening y = W s over the sample columns of [nc, M]
 // (recon/prewhiten.rs:128-141), in place.
 template <int NCMAX>
-__global__ void __launch_bounds__(256) k_prewhiten(float2* __restrict__ d, const float2* __restrict__ W,
+__global__ void __launch_bounds__(256) k_prewhiten(const float2* in, float2* d, const float2* __restrict__ W,
                                                    int nc, long long M) {
     for (long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x; x < M;
          x += (long long)gridDim.x * blockDim.x) {
         float2 s[NCMAX];
 #pragma unroll
         for (int j = 0; j < NCMAX; ++j)
-            if (j < nc) s[j] = d[j * M + x];
+            if (j < nc) s[j] = in[j * M + x];
 #pragma unroll
         for (int i = 0; i < NCMAX; ++i) {
             if (i < nc) {
@@ -480,16 +522,31 @@ hipError_t launch_cols(const ColArgs& a, bool inverse, hipStream_t s) {
     return inverse ? go_cols_dir<true>(a, s) : go_cols_dir<false>(a, s);
 }
 
+hipError_t launch_prep_os(const RowPwArgs& a, const uint8_t* acq, hipStream_t s) {
+    const size_t lds = ((size_t)a.nc * a.nx + (size_t)8 * a.nx) * sizeof(float2);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    if (a.nc <= 16) {
+        OKS_ATTR(hipFuncSetAttribute((const void*)k_prep_os<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_prep_os<16>, dim3((unsigned)a.rows), dim3(512), lds, s, a, acq);
+    } else if (a.nc <= 32) {
+        OKS_ATTR(hipFuncSetAttribute((const void*)k_prep_os<32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_prep_os<32>, dim3((unsigned)a.rows), dim3(512), lds, s, a, acq);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_rss_finalize(const float* part, int G, long long M, float* out, float scale, hipStream_t s) {
     const unsigned grid = (unsigned)std::min<long long>((M + 255) / 256, 8192);
     hipLaunchKernelGGL(k_rss_finalize, dim3(grid), dim3(256), 0, s, part, G, M, out, scale);
     return hipGetLastError();
 }
 
-hipError_t launch_prewhiten(float2* d, const float2* W, int nc, long long M, hipStream_t s) {
-    const unsigned grid = (unsigned)std::min<long long>((M + 255) / 256, 4096);
-    if (nc <= 16) hipLaunchKernelGGL(k_prewhiten<16>, dim3(grid), dim3(256), 0, s, d, W, nc, M);
-    else if (nc <= 32) hipLaunchKernelGGL(k_prewhiten<32>, dim3(grid), dim3(256), 0, s, d, W, nc, M);
+hipError_t launch_prewhiten(const float2* in, float2* d, const float2* W, int nc, long long M, hipStream_t s) {
+    const unsigned grid = (unsigned)std::min<long long>((M + 255) / 256, 8192);
+    if (nc <= 16) hipLaunchKernelGGL(k_prewhiten<16>, dim3(grid), dim3(256), 0, s, in, d, W, nc, M);
+    else if (nc <= 32) hipLaunchKernelGGL(k_prewhiten<32>, dim3(grid), dim3(256), 0, s, in, d, W, nc, M);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

@@ -62,7 +62,8 @@ size_t cols_lds(int sx, int n, int ty, int mode);
 hipError_t launch_cols(const ColArgs& a, bool inverse, hipStream_t s);
 size_t rows_lds(int n);
 hipError_t launch_rss_finalize(const float* part, int G, long long M, float* out, float scale, hipStream_t s);
-hipError_t launch_prewhiten(float2* d, const float2* W, int nc, long long M, hipStream_t s);
+hipError_t launch_prewhiten(const float2* in, float2* d, const float2* W, int nc, long long M, hipStream_t s);
+hipError_t launch_prep_os(const RowPwArgs& a, const uint8_t* acq, hipStream_t s);
 hipError_t launch_rss(const float2* in, int C, long long M, float* out, float scale, hipStream_t s);
 
 }  // namespace oks

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "../../include/oks.h"
 #include "oks_ctx.hpp"
@@ -19,6 +20,10 @@ void row_pass(oks_ctx* ctx, const float2* in, float2* out, long long rows, int n
 void fft_axis(oks_ctx* ctx, float2* d, const int64_t* shape, int ndim, int axis, bool inverse, float scale,
               bool centred);
 bool three_d_of(int32_t mode, const oks_kspace* k);
+void check_kspace(const oks_kspace* k);
+void resolve_os(const oks_kspace* k, int enabled, int* os_mode, int* rec, uint32_t* flags);
+float2* prep_grid(oks_ctx* ctx, const oks_kspace* k, const float2* W, int os_mode, int rec, const char* slot);
+std::vector<uint8_t> os_mask(const oks_kspace* k, int rec);
 void output_windows(const oks_kspace* k, int crop, Win w[3], uint32_t* flags);
 void ifft_rss_grid(oks_ctx* ctx, const float2* k, int64_t nc, int64_t nz, int64_t ny, int64_t nx, const float2* W,
                    int os_mode, int rec, bool three_d, const Win w[3], float* out);

@@ -11,6 +11,7 @@
 #include "oks_ctx.hpp"
 #include "oks_host.hpp"
 #include "oks_kernels.hpp"
+#include "oks_abi_util.hpp"
 #include "oks_pipeline.hpp"
 
 namespace oks {
@@ -99,8 +100,7 @@ void fft_axis(oks_ctx* ctx, float2* d, const int64_t* shape, int ndim, int axis,
                  nullptr);
 }
 
-static void check_kspace(const oks_kspace* k, bool need_noise_ok = true) {
-    (void)need_noise_ok;
+void check_kspace(const oks_kspace* k) {
     if (!k || !k->kspace || !k->mask) fail(OKS_E_INVALID_ARG, "null k-space or mask");
     if (k->nc <= 0 || k->nz <= 0 || k->ny <= 0 || k->nx <= 0) fail(OKS_E_INVALID_ARG, "empty k-space");
     if (k->nc > 32) fail(OKS_E_UNSUPPORTED, "at most 32 channels");
@@ -120,6 +120,75 @@ void output_windows(const oks_kspace* k, int crop, Win w[3], uint32_t* flags) {
     w[0] = crop_win(k->nz, t[0]);
     w[1] = crop_win(k->ny, t[1]);
     w[2] = crop_win(k->nx, t[2]);
+}
+
+// OversamplingRemover::new validity for this grid (recon/oversampling.rs:50-70):
+// the lines must carry encoded_x samples; otherwise the reference skips it.
+void resolve_os(const oks_kspace* k, int enabled, int* os_mode, int* rec, uint32_t* flags) {
+    *os_mode = 0;
+    *rec = 0;
+    if (!enabled) return;
+    if (os_valid(k->hdr.encoded_x, k->hdr.recon_x) && k->nx == k->hdr.encoded_x) {
+        *os_mode = 1;
+        *rec = k->hdr.recon_x;
+    } else {
+        *flags |= OKS_F_OS_SKIPPED;
+    }
+}
+
+// The reader closure (whitening, oversampling removal; recon/strategy.rs:153-161)
+// applied to the raw grid, into workspace `slot` [nc, nz, ny, nx].
+float2* prep_grid(oks_ctx* ctx, const oks_kspace* k, const float2* W, int os_mode, int rec, const char* slot) {
+    const size_t plane = (size_t)k->nz * k->ny * k->nx, total = plane * k->nc;
+    float2* g = (float2*)ctx->buf(slot, sizeof(float2) * total);
+    if (os_mode) {
+        const int64_t rows = k->nz * k->ny;
+        ctx->h_acq.assign(rows, 0);
+        for (int64_t row = 0; row < rows; ++row) {
+            const uint8_t* m = k->mask + row * k->nx;
+            for (int64_t x = 0; x < k->nx; ++x)
+                if (m[x]) {
+                    ctx->h_acq[row] = 1;
+                    break;
+                }
+        }
+        uint8_t* acq = (uint8_t*)ctx->buf("prep_acq", rows);
+        OKS_HIP(hipMemcpyAsync(acq, ctx->h_acq.data(), rows, hipMemcpyHostToDevice, ctx->stream));
+        RowPwArgs a{};
+        a.in = (const float2*)k->kspace;
+        a.out = g;
+        a.W = W;
+        a.nc = (int)k->nc;
+        a.rows = rows;
+        a.in_plane = (long long)plane;
+        a.out_plane = (long long)plane;
+        a.nx = (int)k->nx;
+        a.rec = rec;
+        a.os_mode = 1;
+        a.p_enc = ctx->plan((int)k->nx);
+        a.p_rec = ctx->plan(rec);
+        OKS_HIP(launch_prep_os(a, acq, ctx->stream));
+    } else if (W) {
+        OKS_HIP(launch_prewhiten((const float2*)k->kspace, g, W, (int)k->nc, (long long)plane, ctx->stream));
+    } else {
+        OKS_HIP(hipMemcpyAsync(g, k->kspace, sizeof(float2) * total, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    return g;
+}
+
+// Per-cell mask after oversampling removal: acquired rows are marked on
+// [nx/2 - rec/2, nx/2 + rec/2) (io/ismrmrd/reader.rs:455-457).
+std::vector<uint8_t> os_mask(const oks_kspace* k, int rec) {
+    std::vector<uint8_t> m((size_t)k->nz * k->ny * k->nx, 0);
+    const int64_t dst = k->nx / 2 - rec / 2;
+    for (int64_t row = 0; row < k->nz * k->ny; ++row) {
+        const uint8_t* src = k->mask + row * k->nx;
+        bool acq = false;
+        for (int64_t x = 0; x < k->nx && !acq; ++x) acq = src[x] != 0;
+        if (acq)
+            for (int64_t x = dst; x < dst + rec; ++x) m[row * k->nx + x] = 1;
+    }
+    return m;
 }
 
 // Coil groups for an RSS column pass: enough workgroups to fill 256 CUs
@@ -196,14 +265,7 @@ void recon_ifft_rss_dev(oks_ctx* ctx, const oks_ifft_rss_desc* d, const oks_kspa
         fail(OKS_E_UNSUPPORTED, "partial-Fourier acquisition: homodyne reconstruction is not on this path");
     const float2* W = d->prewhiten ? whitener(ctx, (const float2*)k->noise, k->nc, k->n_noise, &flags, "W") : nullptr;
     int os_mode = 0, rec = 0;
-    if (d->remove_oversampling) {
-        if (os_valid(k->hdr.encoded_x, k->hdr.recon_x) && k->nx == k->hdr.encoded_x) {
-            os_mode = 1;
-            rec = k->hdr.recon_x;
-        } else {
-            flags |= OKS_F_OS_SKIPPED;
-        }
-    }
+    resolve_os(k, d->remove_oversampling, &os_mode, &rec, &flags);
     Win w[3];
     output_windows(k, d->crop_to_recon_matrix, w, &flags);
     ifft_rss_grid(ctx, (const float2*)k->kspace, k->nc, k->nz, k->ny, k->nx, W, os_mode, rec, td, w, img->data);
@@ -217,76 +279,6 @@ void recon_ifft_rss_dev(oks_ctx* ctx, const oks_ifft_rss_desc* d, const oks_kspa
 
 using namespace oks;
 
-namespace {
-
-template <class F>
-int guarded(oks_ctx* ctx, F f) {
-    if (!ctx) return OKS_E_INVALID_ARG;
-    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-    try {
-        ctx->err.clear();
-        OKS_HIP(hipSetDevice(ctx->device));
-        f();
-        return OKS_OK;
-    } catch (const Status& s) {
-        ctx->err = s.what();
-        return s.code;
-    } catch (const std::bad_alloc&) {
-        ctx->err = "host allocation failed";
-        return OKS_E_OOM;
-    } catch (const std::exception& e) {
-        ctx->err = e.what();
-        return OKS_E_INCONSISTENT;
-    } catch (...) {
-        ctx->err = "unknown error";
-        return OKS_E_INCONSISTENT;
-    }
-}
-
-// Host-buffer wrapper: upload, run the device composition, download, sync.
-template <class Desc, class Fn>
-int host_call(oks_ctx* ctx, const Desc* d, const oks_kspace* k, oks_image* img, Fn dev) {
-    return guarded(ctx, [&] {
-        if (!k || !k->kspace || !k->mask || !img || !img->data) fail(OKS_E_INVALID_ARG, "null argument");
-        const size_t nk = (size_t)k->nc * k->nz * k->ny * k->nx;
-        const size_t nimg = (size_t)k->nz * k->ny * k->nx;
-        float2* dk = (float2*)ctx->buf("host_k", nk * sizeof(float2));
-        OKS_HIP(hipMemcpyAsync(dk, k->kspace, nk * sizeof(float2), hipMemcpyHostToDevice, ctx->stream));
-        float2* dn = nullptr;
-        if (k->noise && k->n_noise > 0) {
-            const size_t nn = (size_t)k->nc * k->n_noise;
-            dn = (float2*)ctx->buf("host_noise", nn * sizeof(float2));
-            OKS_HIP(hipMemcpyAsync(dn, k->noise, nn * sizeof(float2), hipMemcpyHostToDevice, ctx->stream));
-        }
-        float* dimg = (float*)ctx->buf("host_img", nimg * sizeof(float));
-        float* dg = nullptr;
-        if (img->gfactor) dg = (float*)ctx->buf("host_g", nimg * sizeof(float));
-        oks_kspace kd = *k;
-        kd.kspace = (const oks_c32*)dk;
-        kd.noise = (const oks_c32*)dn;
-        oks_image id = *img;
-        id.data = dimg;
-        id.gfactor = dg;
-        dev(ctx, d, &kd, &id);
-        const size_t nout = (size_t)id.nz * id.ny * id.nx;
-        OKS_HIP(hipMemcpyAsync(img->data, dimg, nout * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-        if (dg) OKS_HIP(hipMemcpyAsync(img->gfactor, dg, nout * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-        int h[8];
-        OKS_HIP(hipMemcpyAsync(h, ctx->d_status, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
-        OKS_HIP(hipStreamSynchronize(ctx->stream));
-        OKS_HIP(hipMemsetAsync(ctx->d_status, 0, sizeof(h), ctx->stream));
-        uint32_t f = id.flags;
-        if (h[0]) f |= OKS_F_PREWHITEN_SKIPPED;
-        if (h[1]) f |= OKS_F_SENSE_PIXEL_FAILED;
-        if (h[2]) f |= OKS_F_GRAPPA_CALIB_FAILED;
-        img->nz = id.nz;
-        img->ny = id.ny;
-        img->nx = id.nx;
-        img->flags = f;
-    });
-}
-
-}  // namespace
 
 extern "C" {
 
@@ -330,7 +322,7 @@ int oks_prewhiten_calibrate_dev(oks_ctx* ctx, const oks_c32* noise, int32_t nc, 
 int oks_prewhiten_apply_dev(oks_ctx* ctx, const oks_c32* W, oks_c32* data, int32_t nc, int64_t ns) {
     return guarded(ctx, [&] {
         if (!W || !data || nc < 1 || nc > 32 || ns < 0) fail(OKS_E_INVALID_ARG, "bad whitening arguments");
-        OKS_HIP(launch_prewhiten((float2*)data, (const float2*)W, nc, ns, ctx->stream));
+        OKS_HIP(launch_prewhiten((const float2*)data, (float2*)data, (const float2*)W, nc, ns, ctx->stream));
     });
 }
 
