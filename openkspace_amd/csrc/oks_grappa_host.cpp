@@ -63,7 +63,7 @@ static void zg(oks_ctx* ctx, int batch, const double2* A, long long sA, int lda,
 // is not positive definite are marked in `fail[z]`.
 static void grappa_calibrate_batch(oks_ctx* ctx, const float2* grid, int nc, int nz, int ny, int nx, int acs0,
                                    int ny_acs, int r, int kky, int kkx, float ridge, const GrGeom& g, float2* w,
-                                   int* fail, long long* launches) {
+                                   int* fail, int* status, long long* launches) {
     (void)ny_acs;
     const int n = g.n_src, m = g.n_tgt;
     float2* G32 = (float2*)ctx->buf("gr_G32", sizeof(float2) * (size_t)nz * n * n);
@@ -87,7 +87,7 @@ static void grappa_calibrate_batch(oks_ctx* ctx, const float2* grid, int nc, int
     const long long snn = (long long)nn, snm = (long long)nm, sD = (long long)nblk * 64 * 64;
     for (int kb = 0; kb < nblk; ++kb) {
         const int k0 = kb * 64, kn = std::min(64, n - k0), r0 = k0 + kn, rn = n - r0;
-        Potrf64Args pa{G, L, D, fail, fail, ctx->d_status + 2, n, k0, kn, kb, nblk};
+        Potrf64Args pa{G, L, D, fail, fail, status, n, k0, kn, kb, nblk};
         OKS_HIP(launch_potrf64_diag(pa, nz, ctx->stream));
         const double2* Dk = D + (size_t)kb * 64 * 64;
         if (rn > 0) {
@@ -147,7 +147,7 @@ static void grappa_volume(oks_ctx* ctx, float2* grid, const oks_kspace* k, const
     float2* w = (float2*)ctx->buf("gr_w", sizeof(float2) * (size_t)nz * (r - 1) * nc * g.n_src);
     int* fail = (int*)ctx->buf("gr_fail", sizeof(int) * nz);
     grappa_calibrate_batch(ctx, grid, nc, nz, ny, nx, (int)p.acs_start, ny_acs, r, kky, kkx, ridge, g, w, fail,
-                           nullptr);
+                           ctx->d_status + 2, nullptr);
     grappa_apply_batch(ctx, grid, nc, nz, ny, nx, r, kky, kkx, w, (long long)(r - 1) * nc * g.n_src, fail, g);
 }
 
@@ -196,9 +196,9 @@ int oks_grappa_calibrate_dev(oks_ctx* ctx, const oks_c32* acs, int64_t nc, int64
         std::string why;
         const int st = grappa_geom(nc, ny_acs, nx, r, kky, kkx, &g, &why);
         if (st != OKS_OK) fail(st, why);
-        int* f = (int*)ctx->buf("grc_fail", sizeof(int));
+        int* f = (int*)ctx->buf("grc_fail", 2 * sizeof(int));
         grappa_calibrate_batch(ctx, (const float2*)acs, (int)nc, 1, (int)ny_acs, (int)nx, 0, (int)ny_acs, (int)r,
-                               (int)kky, (int)kkx, ridge, g, (float2*)weights, f, nullptr);
+                               (int)kky, (int)kkx, ridge, g, (float2*)weights, f, f + 1, nullptr);
         int h = 0;
         OKS_HIP(hipMemcpyAsync(&h, f, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         OKS_HIP(hipStreamSynchronize(ctx->stream));
