@@ -25,25 +25,31 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------------ fills
-// One workgroup per slice.  list[z][d-1][i] = {ty, ky0}, cnt[z][d-1].
-__global__ void __launch_bounds__(256) k_grappa_fills(GrappaFillArgs a) {
-    extern __shared__ unsigned char smem[];
-    unsigned char* sampled = smem;                    // ny
-    int* cnt = (int*)(smem + ((a.ny + 15) & ~15));   // r-1
-    const int z = blockIdx.x;
-    const int ny = a.ny, nx = a.nx, r = a.r;
-    if (threadIdx.x < r - 1) cnt[threadIdx.x] = 0;
-    const float2* row0 = a.grid + (size_t)z * ny * nx;  // coil 0 plane of slice z
-    for (int y = threadIdx.x >> 6; y < ny; y += blockDim.x >> 6) {
-        const int lane = threadIdx.x & 63;
-        int any = 0;
-        for (int x = lane; x < nx; x += 64) {
-            const float2 v = row0[(size_t)y * nx + x];
-            any |= (v.x != 0.f) | (v.y != 0.f);
-        }
-        any = __any(any);
-        if (lane == 0) sampled[y] = (unsigned char)(any != 0);
+// sampled[z][y] = coil-0 row of slice z has a non-zero sample
+// (recon/grappa.rs:338-344).  One wavefront per row.
+__global__ void __launch_bounds__(256) k_grappa_sampled(GrappaFillArgs a) {
+    const int lane = threadIdx.x & 63;
+    const long long row = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);  // z * ny + y
+    if (row >= (long long)a.nz * a.ny) return;
+    const float2* p = a.grid + (size_t)row * a.nx;  // coil 0 plane is the first nz*ny rows
+    int any = 0;
+    for (int x = lane; x < a.nx; x += 64) {
+        const float2 v = p[x];
+        any |= (v.x != 0.f) | (v.y != 0.f);
     }
+    any = __any(any);
+    if (lane == 0) a.sampled[row] = (unsigned char)(any != 0);
+}
+
+// Per missing target row, the fill written last in ascending-ky0 order
+// (recon/grappa.rs:363-420): the smallest d whose sources are all sampled.
+// One workgroup per slice; list[z][d-1][i] = {ty, ky0}, cnt[z][d-1].
+__global__ void __launch_bounds__(256) k_grappa_fills(GrappaFillArgs a) {
+    __shared__ int cnt[8];
+    const int z = blockIdx.x;
+    const int ny = a.ny, r = a.r;
+    const unsigned char* sampled = a.sampled + (size_t)z * ny;
+    if (threadIdx.x < r - 1) cnt[threadIdx.x] = 0;
     __syncthreads();
     const int kc = a.kky / 2 - 1, span = (a.kky - 1) * r + 1;
     for (int ty = threadIdx.x; ty < ny; ty += blockDim.x) {
@@ -360,10 +366,11 @@ __global__ void __launch_bounds__(256) k_zgemm64(Zgemm64Args g) {
 }
 
 // Normal matrix in FP64 with the ridge lambda = ridge * mean_i Re G_ii
-// (recon/grappa.rs:266-275); also the FP64 right-hand side.  One workgroup per slice.
+// (recon/grappa.rs:266-275); also the FP64 right-hand side.  Grid (chunks, nz);
+// every workgroup recomputes the (cheap) mean of the diagonal.
 __global__ void __launch_bounds__(256) k_normal64(Normal64Args a) {
     __shared__ double red[256];
-    const int z = blockIdx.x;
+    const int z = blockIdx.y;
     const float2* G = a.G32 + (size_t)z * a.n * a.n;
     double2* H = a.G64 + (size_t)z * a.n * a.n;
     double s = 0.0;
@@ -376,78 +383,77 @@ __global__ void __launch_bounds__(256) k_normal64(Normal64Args a) {
     }
     const double mean = red[0] / fmax((double)a.n, 1.0);
     const double lam = (double)a.ridge * fmax(mean, (double)1.1920929e-07f);
-    for (size_t e = threadIdx.x; e < (size_t)a.n * a.n; e += blockDim.x) {
-        const size_t i = e / a.n, j = e % a.n;
+    const size_t nn = (size_t)a.n * a.n, nm = (size_t)a.n * a.m;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nn; e += stride) {
+        const size_t i = e / a.n, j = e - i * a.n;
         const float2 v = G[e];
         H[e] = make_double2((double)v.x + (i == j ? lam : 0.0), (double)v.y);
     }
-    const float2* R = a.R32 + (size_t)z * a.n * a.m;
-    double2* Rd = a.R64 + (size_t)z * a.n * a.m;
-    for (size_t e = threadIdx.x; e < (size_t)a.n * a.m; e += blockDim.x) Rd[e] = make_double2(R[e].x, R[e].y);
+    const float2* R = a.R32 + (size_t)z * nm;
+    double2* Rd = a.R64 + (size_t)z * nm;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nm; e += stride)
+        Rd[e] = make_double2(R[e].x, R[e].y);
 }
 
 // Cholesky of the diagonal block G[k0:k0+kn, k0:k0+kn] (lower) and its
-// explicit inverse (recon/prewhiten.rs:148-210, in FP64).  A non-PD pivot
-// marks the slice failed (the reference leaves it undersampled).
-__global__ void __launch_bounds__(256) k_potrf64_diag(Potrf64Args a) {
+// explicit inverse (recon/prewhiten.rs:148-210, in FP64), by one wavefront:
+// lane i owns row i, so every step synchronises at wavefront scope only.  A
+// non-PD pivot marks the slice failed (the reference leaves it undersampled).
+__global__ void __launch_bounds__(64) k_potrf64_diag(Potrf64Args a) {
     __shared__ double2 L[64][65];
     __shared__ double2 I[64][65];
-    __shared__ int bad;
     const int z = blockIdx.x;
     if (a.skip[z]) return;
-    const int n = a.n, k0 = a.k0, kn = a.kn;
+    const int n = a.n, k0 = a.k0, kn = a.kn, lane = threadIdx.x;
     const double2* G = a.G + (size_t)z * n * n;
     double2* Lg = a.L + (size_t)z * n * n;
     double2* D = a.D + ((size_t)z * a.nblk + a.kb) * 64 * 64;
-    for (int e = threadIdx.x; e < kn * kn; e += 256) {
-        const int i = e / kn, j = e % kn;
-        L[i][j] = (j <= i) ? G[(size_t)(k0 + i) * n + k0 + j] : make_double2(0.0, 0.0);
+    for (int e = lane; e < 64 * 64; e += 64) {
+        const int i = e / 64, j = e % 64;
+        L[i][j] = (i < kn && j <= i) ? G[(size_t)(k0 + i) * n + k0 + j] : make_double2(0.0, 0.0);
         I[i][j] = make_double2(0.0, 0.0);
     }
-    if (threadIdx.x == 0) bad = 0;
-    __syncthreads();
+    wave_sync();
+    int bad = 0;
     for (int j = 0; j < kn; ++j) {
-        if (threadIdx.x == 0) {
-            const double d = L[j][j].x;
-            if (!(d > 0.0) || !isfinite(d)) bad = 1;
-            L[j][j] = make_double2(sqrt(fmax(d, 1e-300)), 0.0);
-        }
-        __syncthreads();
-        const double inv = 1.0 / L[j][j].x;
-        for (int i = j + 1 + threadIdx.x; i < kn; i += 256) L[i][j] = make_double2(L[i][j].x * inv, L[i][j].y * inv);
-        __syncthreads();
-        const int t = kn - j - 1;
-        for (int e = threadIdx.x; e < t * t; e += 256) {
-            const int i = j + 1 + e / t, c = j + 1 + e % t;
-            if (c <= i) {
-                const double2 p = zmul(L[i][j], zconj(L[c][j]));
-                L[i][c].x -= p.x;
-                L[i][c].y -= p.y;
+        const double d = L[j][j].x;
+        bad |= (!(d > 0.0) || !isfinite(d));
+        const double ljj = sqrt(fmax(d, 1e-300)), inv = 1.0 / ljj;
+        wave_sync();
+        if (lane == j) L[j][j] = make_double2(ljj, 0.0);
+        if (lane > j && lane < kn) L[lane][j] = make_double2(L[lane][j].x * inv, L[lane][j].y * inv);
+        wave_sync();
+        if (lane > j && lane < kn) {
+            const double2 lij = L[lane][j];
+            for (int c = j + 1; c <= lane; ++c) {
+                const double2 p = zmul(lij, zconj(L[c][j]));
+                L[lane][c].x -= p.x;
+                L[lane][c].y -= p.y;
             }
         }
-        __syncthreads();
+        wave_sync();
     }
-    // inverse of the lower-triangular block, one column per thread
-    if (threadIdx.x < kn) {
-        const int c = threadIdx.x;
+    if (lane < kn) {
+        const int c = lane;
         for (int i = c; i < kn; ++i) {
-            double2 s = make_double2(i == c ? 1.0 : 0.0, 0.0);
+            double2 acc = make_double2(i == c ? 1.0 : 0.0, 0.0);
             for (int j = c; j < i; ++j) {
                 const double2 p = zmul(L[i][j], I[j][c]);
-                s.x -= p.x;
-                s.y -= p.y;
+                acc.x -= p.x;
+                acc.y -= p.y;
             }
             const double dd = L[i][i].x;
-            I[i][c] = make_double2(s.x / dd, s.y / dd);
+            I[i][c] = make_double2(acc.x / dd, acc.y / dd);
         }
     }
-    __syncthreads();
-    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    wave_sync();
+    for (int e = lane; e < 64 * 64; e += 64) {
         const int i = e / 64, j = e % 64;
         D[e] = (i < kn && j < kn) ? I[i][j] : make_double2(0.0, 0.0);
         if (i < kn && j < kn) Lg[(size_t)(k0 + i) * n + k0 + j] = (j <= i) ? L[i][j] : make_double2(0.0, 0.0);
     }
-    if (threadIdx.x == 0 && bad) {
+    if (lane == 0 && bad) {
         a.fail[z] = 1;
         *a.status = 1;
     }
@@ -468,8 +474,10 @@ __global__ void k_weights_out(const double2* X, float2* w, int nz, int nc, int r
 
 // ------------------------------------------------------------ launchers
 hipError_t launch_grappa_fills(const GrappaFillArgs& a, hipStream_t s) {
-    const size_t lds = ((a.ny + 15) & ~15) + 64;
-    hipLaunchKernelGGL(k_grappa_fills, dim3(a.nz), dim3(256), lds, s, a);
+    if (a.r < 2 || a.r > 9) return hipErrorInvalidValue;
+    const unsigned rows = (unsigned)a.nz * a.ny;
+    hipLaunchKernelGGL(k_grappa_sampled, dim3((rows + 3) / 4), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_grappa_fills, dim3(a.nz), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -503,12 +511,13 @@ hipError_t launch_zgemm64(const Zgemm64Args& g, int batch, hipStream_t s) {
 }
 
 hipError_t launch_normal64(const Normal64Args& a, int nz, hipStream_t s) {
-    hipLaunchKernelGGL(k_normal64, dim3(nz), dim3(256), 0, s, a);
+    const unsigned chunks = (unsigned)std::min<size_t>(((size_t)a.n * a.n + 255) / 256, 128);
+    hipLaunchKernelGGL(k_normal64, dim3(chunks, nz), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_potrf64_diag(const Potrf64Args& a, int nz, hipStream_t s) {
-    hipLaunchKernelGGL(k_potrf64_diag, dim3(nz), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_potrf64_diag, dim3(nz), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

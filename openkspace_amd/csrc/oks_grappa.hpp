@@ -9,6 +9,7 @@ struct GrappaFillArgs {
     const float2* grid;  // [nc, nz, ny, nx]; coil 0 decides the sampled rows
     int2* list;          // [nz][r-1][ny] {ty, ky0}
     int* cnt;            // [nz][r-1]
+    unsigned char* sampled;  // [nz][ny] scratch
     int nz, ny, nx, r, kky;
 };
 

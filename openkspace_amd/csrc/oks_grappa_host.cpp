@@ -126,7 +126,8 @@ static void grappa_apply_batch(oks_ctx* ctx, float2* grid, int nc, int nz, int n
                                const float2* w, long long wz, const int* skip, const GrGeom& g) {
     int2* list = (int2*)ctx->buf("gr_list", sizeof(int2) * (size_t)nz * (r - 1) * ny);
     int* cnt = (int*)ctx->buf("gr_cnt", sizeof(int) * (size_t)nz * (r - 1));
-    GrappaFillArgs fa{grid, list, cnt, nz, ny, nx, r, kky};
+    unsigned char* sampled = (unsigned char*)ctx->buf("gr_sampled", (size_t)nz * ny);
+    GrappaFillArgs fa{grid, list, cnt, sampled, nz, ny, nx, r, kky};
     OKS_HIP(launch_grappa_fills(fa, ctx->stream));
     ApplyArgs aa{grid, w, list, cnt, skip, nc, nz, ny, nx, r, kky, kkx, nx - kkx + 1, g.n_src, wz};
     OKS_HIP(launch_apply_gemm(aa, ctx->stream));

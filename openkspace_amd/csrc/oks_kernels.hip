@@ -28,22 +28,37 @@ __global__ void __launch_bounds__(256) k_whitener(const float2* __restrict__ noi
     __shared__ double2 psi[32 * 32];
     __shared__ double2 L[32 * 32];
     __shared__ double2 Li[32 * 32];
+    __shared__ float2 chunk[32 * 256];  // noise samples staged 256 at a time
     __shared__ int bad;
     const int t = threadIdx.x;
     const double denom = fmax((double)ns - 1.0, 1.0);
-    for (int e = t; e < nc * nc; e += blockDim.x) {
-        const int i = e / nc, j = e % nc;
-        double re = 0.0, im = 0.0;
-        if (j <= i) {
-            const float2* a = noise + (size_t)i * ns;
-            const float2* b = noise + (size_t)j * ns;
-            for (int s = 0; s < ns; ++s) {
-                const double ar = a[s].x, ai = a[s].y, br = b[s].x, bi = b[s].y;
-                re += ar * br + ai * bi;  // a * conj(b)
-                im += ai * br - ar * bi;
+    double accr[4] = {0.0, 0.0, 0.0, 0.0}, acci[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int s0 = 0; s0 < ns; s0 += 256) {
+        const int cs = min(256, ns - s0);
+        __syncthreads();
+        for (int e = t; e < nc * 256; e += blockDim.x) {
+            const int c = e / 256, q = e % 256;
+            chunk[e] = q < cs ? noise[(size_t)c * ns + s0 + q] : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+        for (int k = 0; k < 4; ++k) {
+            const int e = t + k * 256;
+            if (e >= nc * nc) break;
+            const int i = e / nc, j = e % nc;
+            if (j > i) continue;
+            const float2* a = chunk + i * 256;
+            const float2* b = chunk + j * 256;
+            for (int q = 0; q < cs; ++q) {
+                const double ar = a[q].x, ai = a[q].y, br = b[q].x, bi = b[q].y;
+                accr[k] += ar * br + ai * bi;  // a * conj(b)
+                acci[k] += ai * br - ar * bi;
             }
         }
-        psi[e] = make_double2(re / denom, im / denom);
+    }
+    for (int k = 0; k < 4; ++k) {
+        const int e = t + k * 256;
+        if (e >= nc * nc) break;
+        psi[e] = make_double2(accr[k] / denom, acci[k] / denom);
         L[e] = make_double2(0.0, 0.0);
         Li[e] = make_double2(0.0, 0.0);
     }
