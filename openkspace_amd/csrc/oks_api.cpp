@@ -74,6 +74,11 @@ void* oks_ctx::buf(const std::string& name, size_t bytes) {
 
 void oks_ctx::release() {
     if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& kv : graphs) {
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+    }
+    graphs.clear();
     for (auto& kv : tw) (void)hipFree(kv.second);
     tw.clear();
     for (auto& kv : ws)
@@ -162,6 +167,21 @@ int oks_ctx_set_stream(oks_ctx* ctx, void* stream) {
         } else {
             OKS_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
             ctx->own_stream = true;
+        }
+        return OKS_OK;
+    });
+}
+
+int oks_ctx_set_option(oks_ctx* ctx, const char* name, int64_t value) {
+    return guarded(ctx, [&] {
+        const std::string n = name ? name : "";
+        if (n == "graphs") {
+            ctx->use_graphs = value != 0;
+        } else if (n == "zchunk") {
+            if (value < 1) fail(OKS_E_INVALID_ARG, "zchunk must be >= 1");
+            ctx->zchunk = (int)value;
+        } else {
+            fail(OKS_E_INVALID_ARG, "unknown option " + n);
         }
         return OKS_OK;
     });

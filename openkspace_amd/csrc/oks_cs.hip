@@ -1,0 +1,291 @@
+// oks_cs.hip -- FISTA L1-wavelet compressed sensing (fista_cs_single_coil,
+// recon/cs.rs:59-139) for every coil-slice plane at once.
+//
+// One iteration is three passes over the [planes, ny, nx] state:
+//   k_rows<FWD>   : T = row transform of z (centred, 1/sqrt(nx))
+//   k_cs_cols     : per column, forward transform, residual
+//                   r = mask ? (F z - y) : 0 (recon/cs.rs:101-111), inverse
+//                   transform, back into T
+//   k_cs_update   : per row pair, inverse row transform g, v = z - g, 1-level
+//                   Haar with soft-threshold of the detail bands and inverse
+//                   Haar (recon/wavelet.rs:32-99; every 2x2 block is local to a
+//                   row pair), then the FISTA momentum z = x+ + a (x+ - x)
+//                   (recon/cs.rs:114-136).
+#include <hip/hip_runtime.h>
+
+#include "oks_cs.hpp"
+#include "oks_fft.hpp"
+
+namespace oks {
+
+#define OKS_ATTR2(x)                      \
+    do {                                  \
+        hipError_t e_ = (x);              \
+        if (e_ != hipSuccess) return e_;  \
+    } while (0)
+
+// Column pass with the masked residual in k-space.  One workgroup per
+// (plane, strip of SX columns); the strip of T, y and the slice mask are staged
+// in LDS with coalesced loads.
+template <class F>
+__global__ void __launch_bounds__(512) k_cs_cols(CsColArgs a) {
+    extern __shared__ float2 lds[];
+    const int n = F::n ? F::n : a.p.n;
+    const int SX = a.sx, ls = n + 1;
+    const int WPB = blockDim.x >> 6;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const float2* tw = stage_twiddles<F>(lds, a.p);
+    float2* A = lds + tw_slots<F>();             // SX * ls
+    float2* B = F::scratch ? A + (size_t)SX * ls : nullptr;
+    float2* Yl = A + (size_t)(F::scratch ? 2 : 1) * SX * ls;  // SX * n (measured, centred index)
+    unsigned char* Ml = (unsigned char*)(Yl + (size_t)SX * n); // SX * n
+    const int strips = (a.nx + SX - 1) / SX;
+    const int plane = blockIdx.x / strips;        // c * nz + z
+    const int xs = (blockIdx.x - plane * strips) * SX;
+    const int z = plane % a.nz;
+    const int half = n / 2, hup = (n + 1) / 2;
+    const float2* T = a.T + (size_t)plane * n * a.nx;
+    const float2* Y = a.y + (size_t)plane * n * a.nx;
+    const unsigned char* M = a.mask + (size_t)z * n * a.nx;
+    for (int i = threadIdx.x; i < n * SX; i += blockDim.x) {
+        const int y = i / SX, col = i - y * SX;
+        float2 v = make_float2(0.f, 0.f), yv = make_float2(0.f, 0.f);
+        unsigned char m = 0;
+        if (xs + col < a.nx) {
+            const size_t o = (size_t)y * a.nx + xs + col;
+            v = T[o];
+            yv = Y[o];
+            m = M[o];
+        }
+        A[(size_t)col * ls + (y + half) % n] = v;
+        Yl[(size_t)col * n + y] = yv;
+        Ml[(size_t)col * n + y] = m;
+    }
+    __syncthreads();
+    for (int col = w; col < SX; col += WPB) {
+        float2* line = A + (size_t)col * ls;
+        float2* res = F::run(line, B ? B + (size_t)col * ls : nullptr, a.p_fwd, tw, lane);
+        if (res != line) {
+            for (int i = lane; i < n; i += 64) line[i] = res[i];
+            wave_sync();
+        }
+        // line[i] is the transform at centred index yo = (i + ceil(n/2)) mod n;
+        // the residual goes back to the same slot, which is the ifftshift
+        // position of yo for the inverse transform.
+        for (int i = lane; i < n; i += 64) {
+            const int yo = (i + hup) % n;
+            const float2 v = cscale(line[i], a.scale);
+            line[i] = Ml[(size_t)col * n + yo] ? csub(v, Yl[(size_t)col * n + yo]) : make_float2(0.f, 0.f);
+        }
+        wave_sync();
+        res = F::run_inv(line, B ? B + (size_t)col * ls : nullptr, a.p_inv, tw, lane);
+        if (res != line) {
+            for (int i = lane; i < n; i += 64) line[i] = res[i];
+        }
+        wave_sync();
+    }
+    __syncthreads();
+    float2* To = a.T + (size_t)plane * n * a.nx;
+    for (int i = threadIdx.x; i < n * SX; i += blockDim.x) {
+        const int yo = i / SX, col = i - yo * SX;
+        if (xs + col < a.nx) To[(size_t)yo * a.nx + xs + col] = cscale(A[(size_t)col * ls + (yo + half) % n], a.scale);
+    }
+}
+
+__device__ __forceinline__ float2 soft(float2 v, float lam) {
+    const float m = hypotf(v.x, v.y);
+    if (m <= lam) return make_float2(0.f, 0.f);
+    const float s = (m - lam) / m;
+    return make_float2(v.x * s - v.y * 0.f, v.x * 0.f + v.y * s);
+}
+
+// Row-pair pass: inverse row transform of T rows (2i, 2i+1), prox and momentum.
+// One wavefront per row pair.
+template <class F>
+__global__ void __launch_bounds__(256) k_cs_update(CsUpdArgs a) {
+    extern __shared__ float2 lds[];
+    const int n = F::n ? F::n : a.p.n;  // nx
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const float2* tw = stage_twiddles<F>(lds, a.p);
+    __syncthreads();
+    const long long pair = (long long)blockIdx.x * (blockDim.x >> 6) + w;
+    if (pair >= a.pairs) return;
+    float2* R0 = lds + tw_slots<F>() + (size_t)w * (F::scratch ? 4 : 2) * n;
+    float2* R1 = R0 + n;
+    float2* S = F::scratch ? R1 + n : nullptr;
+    const int hp = a.ny / 2;
+    const long long plane = pair / hp;
+    const int yi = (int)(pair - plane * hp);
+    const size_t base = ((size_t)plane * a.ny + 2 * yi) * n;
+    const int half = n / 2;
+    for (int x = lane; x < n; x += 64) {
+        R0[(x + half) % n] = a.T[base + x];
+        R1[(x + half) % n] = a.T[base + n + x];
+    }
+    wave_sync();
+    float2* g0 = F::run_inv(R0, S, a.p, tw, lane);
+    float2* g1 = F::run_inv(R1, S ? S + n : nullptr, a.p, tw, lane);
+    const float s = a.scale, lam = a.lambda, alpha = a.alpha;
+    const float2 c = make_float2(0.5f, 0.f);
+    float2* X = a.x + base;
+    float2* Z = a.z + base;
+    for (int j = lane; j < n / 2; j += 64) {
+        const int x0 = 2 * j, x1 = 2 * j + 1;
+        // v = z - g at the four pixels of block j
+        const float2 za = Z[x0], zb = Z[x1], zd = Z[n + x0], ze = Z[n + x1];
+        const float2 va = csub(za, cscale(g0[(x0 + half) % n], s));
+        const float2 vb = csub(zb, cscale(g0[(x1 + half) % n], s));
+        const float2 vd = csub(zd, cscale(g1[(x0 + half) % n], s));
+        const float2 ve = csub(ze, cscale(g1[(x1 + half) % n], s));
+        const float2 ll = cmul(cadd(cadd(cadd(va, vb), vd), ve), c);
+        const float2 hl = soft(cmul(csub(csub(cadd(va, vd), vb), ve), c), lam);
+        const float2 lh = soft(cmul(csub(csub(cadd(va, vb), vd), ve), c), lam);
+        const float2 hh = soft(cmul(cadd(csub(csub(va, vb), vd), ve), c), lam);
+        // inverse Haar (recon/wavelet.rs:55-75)
+        const float2 na = cmul(cadd(cadd(cadd(ll, hl), lh), hh), c);
+        const float2 nb = cmul(csub(csub(cadd(ll, lh), hl), hh), c);
+        const float2 nd = cmul(csub(csub(cadd(ll, hl), lh), hh), c);
+        const float2 ne = cmul(cadd(csub(csub(ll, hl), lh), hh), c);
+        const float2 xa = X[x0], xb = X[x1], xd = X[n + x0], xe = X[n + x1];
+        X[x0] = na;
+        X[x1] = nb;
+        X[n + x0] = nd;
+        X[n + x1] = ne;
+        const float2 al = make_float2(alpha, 0.f);
+        Z[x0] = cadd(na, cmul(al, csub(na, xa)));
+        Z[x1] = cadd(nb, cmul(al, csub(nb, xb)));
+        Z[n + x0] = cadd(nd, cmul(al, csub(nd, xd)));
+        Z[n + x1] = cadd(ne, cmul(al, csub(ne, xe)));
+    }
+}
+
+// RSS over coils with the crop window (recon/strategy.rs:871-879 + maybe_crop):
+// out[z][yo][xo] = sqrt(sum_c |x[c][z][y0+yo][x0+xo]|^2).
+__global__ void k_rss_crop(const float2* __restrict__ x, int nc, int nz, int ny, int nx, int z0, int tz, int y0,
+                           int ty, int x0, int tx, float* __restrict__ out) {
+    const long long total = (long long)tz * ty * tx;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int xo = (int)(i % tx);
+        const long long r = i / tx;
+        const int yo = (int)(r % ty), zo = (int)(r / ty);
+        const size_t off = ((size_t)(z0 + zo) * ny + y0 + yo) * nx + x0 + xo;
+        float s = 0.f;
+        for (int c = 0; c < nc; ++c) {
+            const float2 v = x[(size_t)c * nz * ny * nx + off];
+            s += v.x * v.x + v.y * v.y;
+        }
+        out[i] = sqrtf(s);
+    }
+}
+
+__global__ void k_copy2(const float2* __restrict__ a, float2* __restrict__ b, float2* __restrict__ c, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const float2 v = a[i];
+        b[i] = v;
+        c[i] = v;
+    }
+}
+
+// ------------------------------------------------------------ launchers
+#define OKS_CS_SIZES(X) X(16) X(32) X(64) X(128) X(160) X(256) X(320) X(368) X(384) X(512) X(640)
+
+static bool cs_fixed(int n) {
+    switch (n) {
+#define OKS_CASE(N) case N:
+        OKS_CS_SIZES(OKS_CASE)
+#undef OKS_CASE
+        return true;
+        default: return false;
+    }
+}
+
+template <int N>
+struct CsFixed {
+    static constexpr int n = N;
+    static constexpr bool scratch = false;
+    static constexpr bool lds_tw = true;
+    __device__ __forceinline__ static float2* run(float2* x, float2*, const FftPlan&, const float2* tw, int lane) {
+        CFft<N, false>::run(x, tw, lane);
+        return x;
+    }
+    __device__ __forceinline__ static float2* run_inv(float2* x, float2*, const FftPlan&, const float2* tw, int lane) {
+        CFft<N, true>::run(x, tw, lane);
+        return x;
+    }
+};
+struct CsDyn {
+    static constexpr int n = 0;
+    static constexpr bool scratch = true;
+    static constexpr bool lds_tw = false;
+    __device__ __forceinline__ static float2* run(float2* x, float2* s, const FftPlan& P, const float2*, int lane) {
+        return fft_wave<false>(x, s, P, lane, 64);
+    }
+    __device__ __forceinline__ static float2* run_inv(float2* x, float2* s, const FftPlan& P, const float2*, int lane) {
+        return fft_wave<true>(x, s, P, lane, 64);
+    }
+};
+
+static size_t tw_slots_n(int n) { return cs_fixed(n) ? (size_t)((n + 1) & ~1) : 0; }
+
+size_t cs_cols_lds(int sx, int n) {
+    const size_t lines = (size_t)(cs_fixed(n) ? 1 : 2) * sx * (n + 1);
+    return (tw_slots_n(n) + lines + (size_t)sx * n) * sizeof(float2) + (size_t)sx * n;
+}
+size_t cs_upd_lds(int n) { return (tw_slots_n(n) + (size_t)4 * (cs_fixed(n) ? 2 : 4) * n) * sizeof(float2); }
+
+template <class F>
+static hipError_t go_cs_cols(const CsColArgs& a, hipStream_t s) {
+    const size_t lds = cs_cols_lds(a.sx, a.p.n);
+    const unsigned grid = (unsigned)(a.planes * ((a.nx + a.sx - 1) / a.sx));
+    OKS_ATTR2(hipFuncSetAttribute((const void*)k_cs_cols<F>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_cs_cols<F>, dim3(grid), dim3(512), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_cs_cols(const CsColArgs& a, hipStream_t s) {
+    if (cs_cols_lds(a.sx, a.p.n) > 160 * 1024) return hipErrorInvalidValue;
+    switch (a.p.n) {
+#define OKS_CASE(N) \
+    case N: return go_cs_cols<CsFixed<N>>(a, s);
+        OKS_CS_SIZES(OKS_CASE)
+#undef OKS_CASE
+        default: return go_cs_cols<CsDyn>(a, s);
+    }
+}
+
+template <class F>
+static hipError_t go_cs_upd(const CsUpdArgs& a, hipStream_t s) {
+    const size_t lds = cs_upd_lds(a.p.n);
+    const unsigned grid = (unsigned)((a.pairs + 3) / 4);
+    OKS_ATTR2(hipFuncSetAttribute((const void*)k_cs_update<F>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_cs_update<F>, dim3(grid), dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_cs_update(const CsUpdArgs& a, hipStream_t s) {
+    if (cs_upd_lds(a.p.n) > 160 * 1024) return hipErrorInvalidValue;
+    switch (a.p.n) {
+#define OKS_CASE(N) \
+    case N: return go_cs_upd<CsFixed<N>>(a, s);
+        OKS_CS_SIZES(OKS_CASE)
+#undef OKS_CASE
+        default: return go_cs_upd<CsDyn>(a, s);
+    }
+}
+
+hipError_t launch_rss_crop(const float2* x, int nc, int nz, int ny, int nx, int z0, int tz, int y0, int ty, int x0,
+                           int tx, float* out, hipStream_t s) {
+    const long long total = (long long)tz * ty * tx;
+    const unsigned grid = (unsigned)std::min<long long>((total + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_rss_crop, dim3(grid), dim3(256), 0, s, x, nc, nz, ny, nx, z0, tz, y0, ty, x0, tx, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy2(const float2* a, float2* b, float2* c, long long n, hipStream_t s) {
+    const unsigned grid = (unsigned)std::min<long long>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_copy2, dim3(grid), dim3(256), 0, s, a, b, c, n);
+    return hipGetLastError();
+}
+
+}  // namespace oks

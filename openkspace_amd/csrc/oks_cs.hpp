@@ -1,0 +1,39 @@
+// oks_cs.hpp -- argument blocks and launchers of the CS (FISTA) kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "oks_fft.hpp"
+
+namespace oks {
+
+struct CsColArgs {
+    float2* T;                  // [planes, ny, nx] in/out
+    const float2* y;            // [planes, ny, nx] measured k-space
+    const unsigned char* mask;  // [nz, ny, nx]
+    long long planes;
+    int nz, nx, sx;
+    float scale;                // 1/sqrt(ny)
+    FftPlan p, p_fwd, p_inv;    // n = ny
+};
+
+struct CsUpdArgs {
+    const float2* T;  // [planes, ny, nx]
+    float2* x;
+    float2* z;
+    long long pairs;  // planes * ny / 2
+    int ny;
+    float scale;      // 1/sqrt(nx)
+    float lambda, alpha;
+    FftPlan p;        // n = nx
+};
+
+size_t cs_cols_lds(int sx, int n);
+size_t cs_upd_lds(int n);
+hipError_t launch_cs_cols(const CsColArgs& a, hipStream_t s);
+hipError_t launch_cs_update(const CsUpdArgs& a, hipStream_t s);
+hipError_t launch_rss_crop(const float2* x, int nc, int nz, int ny, int nx, int z0, int tz, int y0, int ty, int x0,
+                           int tx, float* out, hipStream_t s);
+hipError_t launch_copy2(const float2* a, float2* b, float2* c, long long n, hipStream_t s);
+
+}  // namespace oks

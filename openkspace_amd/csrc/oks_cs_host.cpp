@@ -1,0 +1,222 @@
+// oks_cs_host.cpp -- CsRss on the GPU (recon/strategy.rs:782-891) and the
+// batched function-level FISTA entry point (recon/cs.rs:59-139).  The
+// iteration body of every plane runs as three launches per iteration; the
+// whole iteration loop is captured once into a HIP graph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/oks.h"
+#include "oks_abi_util.hpp"
+#include "oks_cs.hpp"
+#include "oks_ctx.hpp"
+#include "oks_host.hpp"
+#include "oks_kernels.hpp"
+#include "oks_pipeline.hpp"
+
+namespace oks {
+
+static int cs_sx(int ny) {
+    int sx = 8;
+    while (sx > 1 && cs_cols_lds(sx, ny) > 160 * 1024) sx /= 2;
+    return sx;
+}
+
+// FISTA momentum sequence t_{k+1} = (1 + sqrt(1 + 4 t_k^2)) / 2,
+// alpha_k = (t_k - 1) / t_{k+1}, in f32 as the reference computes it.
+static std::vector<float> fista_alphas(int iters) {
+    std::vector<float> a(iters);
+    float t = 1.0f;
+    for (int i = 0; i < iters; ++i) {
+        const float tn = 0.5f * (1.0f + std::sqrt(1.0f + 4.0f * t * t));
+        a[i] = (t - 1.0f) / tn;
+        t = tn;
+    }
+    return a;
+}
+
+struct CsProblem {
+    float2* y;                   // measured k-space [planes, ny, nx]
+    const unsigned char* mask;   // [mask_planes, ny, nx]
+    float2 *x, *z, *T;
+    long long planes;
+    int mask_planes, ny, nx, iters;
+    float lambda;
+};
+
+static void cs_iterations(oks_ctx* ctx, const CsProblem& P, const std::vector<float>& alpha) {
+    const float sx_ = 1.0f / std::sqrt((float)P.nx), sy_ = 1.0f / std::sqrt((float)P.ny);
+    CsColArgs ca{};
+    ca.T = P.T;
+    ca.y = P.y;
+    ca.mask = P.mask;
+    ca.planes = P.planes;
+    ca.nz = P.mask_planes;
+    ca.nx = P.nx;
+    ca.sx = cs_sx(P.ny);
+    ca.scale = sy_;
+    ca.p = ctx->plan(P.ny);
+    ca.p_fwd = ca.p;
+    ca.p_inv = ca.p;
+    CsUpdArgs ua{};
+    ua.T = P.T;
+    ua.x = P.x;
+    ua.z = P.z;
+    ua.pairs = P.planes * (P.ny / 2);
+    ua.ny = P.ny;
+    ua.scale = sx_;
+    ua.lambda = P.lambda;
+    ua.p = ctx->plan(P.nx);
+    for (int it = 0; it < P.iters; ++it) {
+        row_pass(ctx, P.z, P.T, P.planes * P.ny, P.nx, P.nx, 0, sx_, false, true, nullptr, 0);
+        OKS_HIP(launch_cs_cols(ca, ctx->stream));
+        ua.alpha = alpha[it];
+        OKS_HIP(launch_cs_update(ua, ctx->stream));
+    }
+}
+
+// x = z = F^-1(mask y) (recon/cs.rs:84-97), then the iterations; result in P.x.
+static void run_fista(oks_ctx* ctx, const CsProblem& P) {
+    const float sx_ = 1.0f / std::sqrt((float)P.nx), sy_ = 1.0f / std::sqrt((float)P.ny);
+    if (cs_cols_lds(cs_sx(P.ny), P.ny) > 160 * 1024 || cs_upd_lds(P.nx) > 160 * 1024)
+        fail(OKS_E_UNSUPPORTED, "image too large for the CS passes");
+    // plans and attributes exist before any capture (no allocation inside a graph)
+    ctx->plan(P.ny);
+    ctx->plan(P.nx);
+    row_pass(ctx, P.y, P.T, P.planes * P.ny, P.nx, P.nx, 0, sx_, true, true, P.mask,
+             (long long)P.mask_planes * P.ny);
+    col_pass(ctx, P.T, P.T, nullptr, 1, (int)P.planes, 0, (int)P.planes, P.ny, P.nx, 0, 0, P.ny, sy_, true, true, 1,
+             nullptr);
+    OKS_HIP(launch_copy2(P.T, P.x, P.z, P.planes * P.ny * P.nx, ctx->stream));
+    if (P.iters <= 0) return;
+    const std::vector<float> alpha = fista_alphas(P.iters);
+    if (!ctx->use_graphs) {
+        cs_iterations(ctx, P, alpha);
+        return;
+    }
+    oks_ctx::GraphKey key{};
+    key.p[0] = P.y;
+    key.p[1] = P.mask;
+    key.p[2] = P.x;
+    key.p[3] = P.z;
+    key.p[4] = P.T;
+    key.i[0] = P.planes;
+    key.i[1] = P.mask_planes;
+    key.i[2] = P.ny;
+    key.i[3] = P.nx;
+    key.i[4] = P.iters;
+    uint32_t lb;
+    std::memcpy(&lb, &P.lambda, 4);
+    key.i[5] = lb;
+    hipGraphExec_t exec = ctx->graph_for("cs", key);
+    if (!exec) {
+        hipGraph_t graph = nullptr;
+        OKS_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            cs_iterations(ctx, P, alpha);
+        } catch (...) {
+            hipStreamEndCapture(ctx->stream, &graph);
+            if (graph) hipGraphDestroy(graph);
+            throw;
+        }
+        OKS_HIP(hipStreamEndCapture(ctx->stream, &graph));
+        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            hipGraphDestroy(graph);
+            hip_check(e, "hipGraphInstantiate");
+        }
+        ctx->graph_store("cs", key, graph, exec);
+    }
+    OKS_HIP(hipGraphLaunch(exec, ctx->stream));
+}
+
+void recon_cs_dev(oks_ctx* ctx, const oks_cs_desc* d, const oks_kspace* k, oks_image* img) {
+    check_kspace(k);
+    if (!d || !img || !img->data) fail(OKS_E_INVALID_ARG, "null descriptor or image");
+    uint32_t flags = 0;
+    const float2* W = d->prewhiten ? whitener(ctx, (const float2*)k->noise, k->nc, k->n_noise, &flags, "W") : nullptr;
+    int os_mode = 0, rec = 0;
+    resolve_os(k, d->remove_oversampling, &os_mode, &rec, &flags);
+    float2* grid = prep_grid(ctx, k, W, os_mode, rec, "grid");
+    const int nc = (int)k->nc, nz = (int)k->nz, ny = (int)k->ny, nx = (int)k->nx;
+    if (three_d_of(d->fft_mode, k)) {
+        int64_t shape[4] = {nc, nz, ny, nx};
+        if (nz >= 2) fft_axis(ctx, grid, shape, 4, 1, true, 1.0f / (float)nz, true);  // ifft1 along kz
+    }
+    Win w[3];
+    output_windows(k, d->crop_to_recon_matrix, w, &flags);
+    if (ny % 2 != 0 || nx % 2 != 0) {
+        flags |= OKS_F_CS_FALLBACK;  // recon/strategy.rs:829-842
+        ifft_rss_grid(ctx, grid, nc, nz, ny, nx, nullptr, 0, 0, false, w, img->data);
+    } else {
+        const size_t cells = (size_t)nz * ny * nx;
+        if (os_mode) {
+            ctx->h_mask = os_mask(k, rec);
+        } else {
+            ctx->h_mask.assign(k->mask, k->mask + cells);
+        }
+        unsigned char* dmask = (unsigned char*)ctx->buf("cs_mask", cells);
+        OKS_HIP(hipMemcpyAsync(dmask, ctx->h_mask.data(), cells, hipMemcpyHostToDevice, ctx->stream));
+        const size_t tot = (size_t)nc * cells;
+        CsProblem P{};
+        P.y = grid;
+        P.mask = dmask;
+        P.x = (float2*)ctx->buf("cs_x", sizeof(float2) * tot);
+        P.z = (float2*)ctx->buf("cs_z", sizeof(float2) * tot);
+        P.T = (float2*)ctx->buf("cs_t", sizeof(float2) * tot);
+        P.planes = (long long)nc * nz;
+        P.mask_planes = nz;
+        P.ny = ny;
+        P.nx = nx;
+        P.iters = d->iters;
+        P.lambda = d->lambda;
+        run_fista(ctx, P);
+        OKS_HIP(launch_rss_crop(P.x, nc, nz, ny, nx, (int)w[0].off, (int)w[0].len, (int)w[1].off, (int)w[1].len,
+                                (int)w[2].off, (int)w[2].len, img->data, ctx->stream));
+    }
+    img->nz = w[0].len;
+    img->ny = w[1].len;
+    img->nx = w[2].len;
+    img->flags = flags;
+}
+
+}  // namespace oks
+
+using namespace oks;
+
+extern "C" {
+
+int oks_recon_cs(oks_ctx* ctx, const oks_cs_desc* d, const oks_kspace* k, oks_image* img) {
+    return host_call(ctx, d, k, img, recon_cs_dev);
+}
+
+int oks_recon_cs_dev(oks_ctx* ctx, const oks_cs_desc* d, const oks_kspace* k, oks_image* img) {
+    return guarded(ctx, [&] { recon_cs_dev(ctx, d, k, img); });
+}
+
+int oks_fista_cs_dev(oks_ctx* ctx, const oks_c32* kzf, const uint8_t* mask, int64_t planes, int64_t mask_planes,
+                     int64_t ny, int64_t nx, int64_t iters, float lambda, oks_c32* out) {
+    return guarded(ctx, [&] {
+        if (!kzf || !mask || !out || planes < 1 || ny < 1 || nx < 1 || iters < 0) fail(OKS_E_INVALID_ARG, "bad CS arguments");
+        if (mask_planes != 1 && mask_planes != planes) fail(OKS_E_CS_SHAPE, "mask planes must be 1 or planes");
+        if (ny % 2 != 0 || nx % 2 != 0) fail(OKS_E_CS_ODD, "Ny and Nx must both be even for Haar wavelet");
+        const size_t tot = (size_t)planes * ny * nx;
+        CsProblem P{};
+        P.y = (float2*)kzf;
+        P.mask = mask;
+        P.x = (float2*)out;
+        P.z = (float2*)ctx->buf("fcs_z", sizeof(float2) * tot);
+        P.T = (float2*)ctx->buf("fcs_t", sizeof(float2) * tot);
+        P.planes = planes;
+        P.mask_planes = (int)mask_planes;
+        P.ny = (int)ny;
+        P.nx = (int)nx;
+        P.iters = (int)iters;
+        P.lambda = lambda;
+        run_fista(ctx, P);
+    });
+}
+
+}  // extern "C"

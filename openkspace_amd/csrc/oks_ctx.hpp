@@ -49,7 +49,38 @@ struct oks_ctx {
     int* d_status = nullptr;                       // device-side flags (whitener) [8]
     uint32_t pending_flags = 0;
     int zchunk = 4;                                // slices per row/column chunk (ifft-rss)
+    int use_graphs = 1;                            // capture iteration loops into HIP graphs
     std::vector<uint8_t> h_acq;                    // host staging: acquired rows
+    std::vector<uint8_t> h_mask;                   // host staging: per-cell mask
+
+    // Instantiated HIP graphs keyed by what their captured launches depend on.
+    struct GraphKey {
+        const void* p[8];
+        long long i[8];
+        bool operator==(const GraphKey& o) const {
+            for (int k = 0; k < 8; ++k)
+                if (p[k] != o.p[k] || i[k] != o.i[k]) return false;
+            return true;
+        }
+    };
+    struct GraphEntry {
+        GraphKey key;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::map<std::string, GraphEntry> graphs;
+    hipGraphExec_t graph_for(const std::string& name, const GraphKey& key) {
+        auto it = graphs.find(name);
+        return (it != graphs.end() && it->second.key == key) ? it->second.exec : nullptr;
+    }
+    void graph_store(const std::string& name, const GraphKey& key, hipGraph_t g, hipGraphExec_t e) {
+        auto it = graphs.find(name);
+        if (it != graphs.end()) {
+            if (it->second.exec) (void)hipGraphExecDestroy(it->second.exec);
+            if (it->second.graph) (void)hipGraphDestroy(it->second.graph);
+        }
+        graphs[name] = GraphEntry{key, g, e};
+    }
 
     oks::FftPlan plan(int n);
     void* buf(const std::string& name, size_t bytes);

@@ -397,63 +397,79 @@ __global__ void __launch_bounds__(256) k_normal64(Normal64Args a) {
 }
 
 // Cholesky of the diagonal block G[k0:k0+kn, k0:k0+kn] (lower) and its
-// explicit inverse (recon/prewhiten.rs:148-210, in FP64), by one wavefront:
-// lane i owns row i, so every step synchronises at wavefront scope only.  A
-// non-PD pivot marks the slice failed (the reference leaves it undersampled).
-__global__ void __launch_bounds__(64) k_potrf64_diag(Potrf64Args a) {
+// explicit inverse (recon/prewhiten.rs:148-210, in FP64).  256 threads; one
+// barrier per column step: every thread recomputes the pivot, the trailing
+// update uses the column scaled on the fly, and the column itself is scaled in
+// the next step.  The inverse goes row by row with each column's dot product
+// split over 4 lanes.  A non-PD pivot marks the slice failed (the reference
+// leaves it undersampled).
+__global__ void __launch_bounds__(256) k_potrf64_diag(Potrf64Args a) {
     __shared__ double2 L[64][65];
     __shared__ double2 I[64][65];
     const int z = blockIdx.x;
     if (a.skip[z]) return;
-    const int n = a.n, k0 = a.k0, kn = a.kn, lane = threadIdx.x;
+    const int n = a.n, k0 = a.k0, kn = a.kn, t = threadIdx.x;
     const double2* G = a.G + (size_t)z * n * n;
     double2* Lg = a.L + (size_t)z * n * n;
     double2* D = a.D + ((size_t)z * a.nblk + a.kb) * 64 * 64;
-    for (int e = lane; e < 64 * 64; e += 64) {
+    for (int e = t; e < 64 * 64; e += 256) {
         const int i = e / 64, j = e % 64;
         L[i][j] = (i < kn && j <= i) ? G[(size_t)(k0 + i) * n + k0 + j] : make_double2(0.0, 0.0);
         I[i][j] = make_double2(0.0, 0.0);
     }
-    wave_sync();
+    __syncthreads();
     int bad = 0;
+    double inv_prev = 1.0;
     for (int j = 0; j < kn; ++j) {
+        // scale column j-1 (nobody reads it during this step)
+        if (j > 0)
+            for (int i = j + t; i < kn; i += 256) L[i][j - 1] = make_double2(L[i][j - 1].x * inv_prev, L[i][j - 1].y * inv_prev);
         const double d = L[j][j].x;
         bad |= (!(d > 0.0) || !isfinite(d));
         const double ljj = sqrt(fmax(d, 1e-300)), inv = 1.0 / ljj;
-        wave_sync();
-        if (lane == j) L[j][j] = make_double2(ljj, 0.0);
-        if (lane > j && lane < kn) L[lane][j] = make_double2(L[lane][j].x * inv, L[lane][j].y * inv);
-        wave_sync();
-        if (lane > j && lane < kn) {
-            const double2 lij = L[lane][j];
-            for (int c = j + 1; c <= lane; ++c) {
-                const double2 p = zmul(lij, zconj(L[c][j]));
-                L[lane][c].x -= p.x;
-                L[lane][c].y -= p.y;
+        const int tt = kn - j - 1;
+        for (int e = t; e < tt * tt; e += 256) {
+            const int i = j + 1 + e / tt, c = j + 1 + e % tt;
+            if (c <= i) {
+                const double2 li = L[i][j], lc = L[c][j];
+                const double2 p = zmul(li, zconj(lc));
+                const double s2 = inv * inv;
+                L[i][c].x -= p.x * s2;
+                L[i][c].y -= p.y * s2;
             }
         }
-        wave_sync();
+        __syncthreads();
+        if (t == 0) L[j][j] = make_double2(ljj, 0.0);
+        inv_prev = inv;
     }
-    if (lane < kn) {
-        const int c = lane;
-        for (int i = c; i < kn; ++i) {
-            double2 acc = make_double2(i == c ? 1.0 : 0.0, 0.0);
-            for (int j = c; j < i; ++j) {
+    __syncthreads();  // the last column has no rows below its pivot
+    // inverse: row i, column c <= i; lanes 4c..4c+3 share column c
+    const int c = t >> 2, part = t & 3;
+    for (int i = 0; i < kn; ++i) {
+        double2 acc = make_double2(0.0, 0.0);
+        if (c < i)
+            for (int j = c + part; j < i; j += 4) {
                 const double2 p = zmul(L[i][j], I[j][c]);
-                acc.x -= p.x;
-                acc.y -= p.y;
+                acc.x += p.x;
+                acc.y += p.y;
             }
+        acc.x += __shfl_xor(acc.x, 1);
+        acc.y += __shfl_xor(acc.y, 1);
+        acc.x += __shfl_xor(acc.x, 2);
+        acc.y += __shfl_xor(acc.y, 2);
+        if (part == 0 && c <= i) {
             const double dd = L[i][i].x;
-            I[i][c] = make_double2(acc.x / dd, acc.y / dd);
+            const double re = (c == i ? 1.0 : 0.0) - acc.x, im = -acc.y;
+            I[i][c] = make_double2(re / dd, im / dd);
         }
+        __syncthreads();
     }
-    wave_sync();
-    for (int e = lane; e < 64 * 64; e += 64) {
+    for (int e = t; e < 64 * 64; e += 256) {
         const int i = e / 64, j = e % 64;
         D[e] = (i < kn && j < kn) ? I[i][j] : make_double2(0.0, 0.0);
         if (i < kn && j < kn) Lg[(size_t)(k0 + i) * n + k0 + j] = (j <= i) ? L[i][j] : make_double2(0.0, 0.0);
     }
-    if (lane == 0 && bad) {
+    if (t == 0 && bad) {
         a.fail[z] = 1;
         *a.status = 1;
     }
@@ -517,7 +533,7 @@ hipError_t launch_normal64(const Normal64Args& a, int nz, hipStream_t s) {
 }
 
 hipError_t launch_potrf64_diag(const Potrf64Args& a, int nz, hipStream_t s) {
-    hipLaunchKernelGGL(k_potrf64_diag, dim3(nz), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_potrf64_diag, dim3(nz), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
