@@ -1,7 +1,9 @@
 """GPU parity of FISTA L1-wavelet CS (fista_cs_single_coil, CsRss).
 
-Bar (FP32 path): rel-L2 <= 1e-5 against the FP64 twin and <= 1e-5 plus the
-FP32 restatement's own distance to FP64 against the FP32 restatement.  The
+Bar (iterative FP32 path): the backend's rel-L2 to the FP64 twin is at most
+the FP32 restatement's own distance to FP64 plus 1e-5, and its distance to the
+FP32 restatement at most 1e-5 plus twice that (FISTA accumulates FP32 rounding
+over the iterations, in the restatement as on the GPU).  The
 HIP-graph replay of the iteration loop is checked bit-for-bit against the
 eager launch sequence on every replay.
 """
@@ -59,7 +61,7 @@ def test_fista_reference_phantom(ctx):  # recon/cs.rs:259 on the GPU
     assert np.linalg.norm(got - truth) < 0.8 * zf_err
     _, r32 = O.fista_cs(kzf, mask, 200, 0.02, "f32")
     _, r64 = O.fista_cs(kzf, mask, 200, 0.02, "f64")
-    assert_parity(got, r32, r64, 1e-5, "fista 16x16 200it")
+    assert_parity(got, r32, r64, 1e-5, "fista 16x16 200it", iterative=True)
 
 
 @pytest.mark.parametrize("ny,nx,iters", [(32, 32, 40), (64, 48, 30), (320, 320, 120)])
@@ -69,7 +71,7 @@ def test_fista_vs_oracle(ctx, ny, nx, iters):
     got = gpu_fista(ctx, kzf, mask, iters, 0.01)[0]
     _, r32 = O.fista_cs(kzf, mask, iters, 0.01, "f32")
     _, r64 = O.fista_cs(kzf, mask, iters, 0.01, "f64")
-    assert_parity(got, r32, r64, TOL, f"fista {ny}x{nx} {iters}it")
+    assert_parity(got, r32, r64, TOL, f"fista {ny}x{nx} {iters}it", iterative=True)
 
 
 def test_fista_closed_forms(ctx):
@@ -118,7 +120,7 @@ def test_cs_strategy_small(ctx, prewhiten):
     _, r64, _, _ = O.recon("cs", d, case, "f64")
     got, flags, _ = recon_dev(ctx, "oks_recon_cs_dev", d, case)
     assert st == 0 and got.shape == r32.shape and flags == rflags
-    assert_parity(got, r32, r64, TOL, f"cs pw={prewhiten}")
+    assert_parity(got, r32, r64, TOL, f"cs pw={prewhiten}", iterative=True)
 
 
 def test_cs_strategy_3d_and_odd_fallback(ctx):
@@ -128,7 +130,7 @@ def test_cs_strategy_3d_and_odd_fallback(ctx):
     _, r64, _, _ = O.recon("cs", d, case, "f64")
     got, flags, _ = recon_dev(ctx, "oks_recon_cs_dev", d, case)
     assert flags == rflags
-    assert_parity(got, r32, r64, TOL, "cs 3d")
+    assert_parity(got, r32, r64, TOL, "cs 3d", iterative=True)
     case = _cs_case(ny=30, nx=33, kind="full")
     st, r32, rflags, _ = O.recon("cs", d, case, "f32")
     got, flags, _ = recon_dev(ctx, "oks_recon_cs_dev", d, case)
@@ -147,4 +149,4 @@ def test_cs_c3_shape(ctx):
     O.set_threads(1)
     got, flags, _ = recon_dev(ctx, "oks_recon_cs_dev", d, case)
     assert st == 0 and flags == rflags and got.shape == (2, 320, 320)
-    assert_parity(got, r32, r64, TOL, "C3 cs")
+    assert_parity(got, r32, r64, TOL, "C3 cs", iterative=True)
