@@ -118,8 +118,11 @@ __global__ void __launch_bounds__(256) k_zgemm_sbf16(typename L::Args args) {
     constexpr int MS = BM / (16 * WAVES_M);  // 16-row subtiles per wave
     __shared__ __attribute__((aligned(16))) unsigned short sA[4 * BM * BKP];
     __shared__ __attribute__((aligned(16))) unsigned short sB[4 * BN * BKP];
+    __shared__ int tab[L::kTab];
     L ld(args);
     if (!ld.active()) return;
+    ld.setup(tab);
+    __syncthreads();
     const Planes PA{sA, BM}, PB{sB, BN};
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wm = w / WAVES_N, wn = w % WAVES_N;
@@ -183,7 +186,11 @@ __global__ void __launch_bounds__(256) k_zgemm_sbf16(typename L::Args args) {
 //   A(m, k) = conj(P(k, m)), B(k, n) = P(k, n)  (Gram)   or  T(k, n)  (rhs)
 //   P(p, col) = grid[ch][z][acs0 + ky0 + a r][kx0 + b],  p = ky0 n_kx + kx0,
 //   col = ch kky kkx + a kkx + b;  T(p, (d-1) nc + ch) = grid[ch][z][acs0 + ky0 + kc r + d][kx0 + kx/2].
+// The column decompositions are tabulated once per workgroup (element offsets
+// from the slice's ACS origin).  Only the lower triangle of the Gram is ever
+// read by the Cholesky, so tiles strictly above the diagonal are skipped.
 struct CalibLoader {
+    static constexpr int kTab = 2048 + 512;  // source columns + target columns
     struct Args {
         const float2* grid;
         float2* out;  // [nz][M][N]
@@ -192,33 +199,46 @@ struct CalibLoader {
     };
     Args a;
     int m0, n0, z;
+    const float2* base;
+    const int* src_of;
+    const int* tgt_of;
     __device__ CalibLoader(const Args& x) : a(x) {
         m0 = blockIdx.x * 64;
         n0 = blockIdx.y * (a.rhs ? 32 : 64);
         z = blockIdx.z;
+        base = a.grid + ((size_t)z * a.ny + a.acs0) * a.nx;
     }
-    __device__ bool active() const { return true; }
+    __device__ bool active() const { return a.rhs || m0 + 64 > n0; }
+    __device__ void setup(int* tab) {
+        const long long cs = (long long)a.nz * a.ny * a.nx;
+        const int kk = a.kky * a.kkx, n_src = a.nc * kk;
+        for (int col = threadIdx.x; col < n_src && col < 2048; col += blockDim.x) {
+            const int ch = col / kk, rem = col - ch * kk;
+            const int ai = rem / a.kkx, bi = rem - ai * a.kkx;
+            tab[col] = (int)(ch * cs + (long long)ai * a.r * a.nx + bi);
+        }
+        const int kc = a.kky / 2 - 1, n_tgt = a.nc * (a.r - 1);
+        for (int t = threadIdx.x; t < n_tgt && t < 512; t += blockDim.x) {
+            const int d = t / a.nc + 1, ch = t - (d - 1) * a.nc;
+            tab[2048 + t] = (int)(ch * cs + (long long)(kc * a.r + d) * a.nx + a.kkx / 2);
+        }
+        src_of = tab;
+        tgt_of = tab + 2048;
+    }
     __device__ int K() const { return a.n_pos; }
-    __device__ float2 src(int p, int col) const {
-        const int ky0 = p / a.n_kx, kx0 = p - ky0 * a.n_kx;
-        const int kk = a.kky * a.kkx;
-        const int ch = col / kk, rem = col - ch * kk;
-        const int ai = rem / a.kkx, bi = rem - ai * a.kkx;
-        return a.grid[(((size_t)ch * a.nz + z) * a.ny + a.acs0 + ky0 + ai * a.r) * a.nx + kx0 + bi];
-    }
-    __device__ float2 tgt(int p, int t) const {
-        const int ky0 = p / a.n_kx, kx0 = p - ky0 * a.n_kx;
-        const int d = t / a.nc + 1, ch = t - (d - 1) * a.nc;
-        const int kc = a.kky / 2 - 1;
-        return a.grid[(((size_t)ch * a.nz + z) * a.ny + a.acs0 + ky0 + kc * a.r + d) * a.nx + kx0 + a.kkx / 2];
+    __device__ int pos(int p) const {
+        const int ky0 = p / a.n_kx;
+        return ky0 * a.nx + (p - ky0 * a.n_kx);
     }
     // k (position) fastest across threads: consecutive kx0 are contiguous.
     __device__ void load_a(const Planes& P, int k0) const {
         const int k = threadIdx.x & 31;
+        const bool kin = k0 + k < a.n_pos;
+        const int po = kin ? pos(k0 + k) : 0;
         for (int m = threadIdx.x >> 5; m < 64; m += 8) {
             float2 v = make_float2(0.f, 0.f);
-            if (m0 + m < a.M && k0 + k < a.n_pos) {
-                v = src(k0 + k, m0 + m);
+            if (m0 + m < a.M && kin) {
+                v = base[po + src_of[m0 + m]];
                 v.y = -v.y;  // conj
             }
             P.put(m, k, v);
@@ -227,9 +247,12 @@ struct CalibLoader {
     template <int BN>
     __device__ void load_b(const Planes& P, int k0) const {
         const int k = threadIdx.x & 31;
+        const bool kin = k0 + k < a.n_pos;
+        const int po = kin ? pos(k0 + k) : 0;
+        const int* of = a.rhs ? tgt_of : src_of;
         for (int n = threadIdx.x >> 5; n < BN; n += 8) {
             float2 v = make_float2(0.f, 0.f);
-            if (n0 + n < a.N && k0 + k < a.n_pos) v = a.rhs ? tgt(k0 + k, n0 + n) : src(k0 + k, n0 + n);
+            if (n0 + n < a.N && kin) v = base[po + of[n0 + n]];
             P.put(n, k, v);
         }
     }
@@ -244,6 +267,7 @@ struct CalibLoader {
 //   out[kx0, ch] = sum_j S(kx0, j) w_d[ch, j],  S(kx0, j) = grid[c'][z][ky0 + a r][kx0 + b].
 // blockIdx.z = z (r-1) + (d-1), blockIdx.y = fill index, blockIdx.x = kx0 tile.
 struct ApplyLoader {
+    static constexpr int kTab = 2048;
     struct Args {
         float2* grid;
         const float2* w;     // [nz][r-1][nc][n_src] (slice stride wz; 0 = one kernel for all slices)
@@ -256,6 +280,8 @@ struct ApplyLoader {
     Args a;
     int z, d, ty, ky0, m0;
     bool act;
+    const float2* base;
+    const int* src_of;
     __device__ ApplyLoader(const Args& x) : a(x) {
         const int zd = blockIdx.z;
         z = zd / (a.r - 1);
@@ -269,21 +295,28 @@ struct ApplyLoader {
             ky0 = e.y;
         }
         m0 = blockIdx.x * 64;
+        base = a.grid + ((size_t)z * a.ny + ky0) * a.nx + m0;
     }
     __device__ bool active() const { return act; }
+    __device__ void setup(int* tab) {
+        const long long cs = (long long)a.nz * a.ny * a.nx;
+        const int kk = a.kky * a.kkx;
+        for (int j = threadIdx.x; j < a.n_src && j < 2048; j += blockDim.x) {
+            const int ch = j / kk, rem = j - ch * kk;
+            const int ai = rem / a.kkx, bi = rem - ai * a.kkx;
+            tab[j] = (int)(ch * cs + (long long)ai * a.r * a.nx + bi);
+        }
+        src_of = tab;
+    }
     __device__ int K() const { return a.n_src; }
     // kx0 (m) fastest: contiguous samples of one source row.
     __device__ void load_a(const Planes& P, int k0) const {
         const int m = threadIdx.x & 63;
-        const int kk = a.kky * a.kkx;
+        const bool min_ = m0 + m < a.n_kx;
         for (int k = threadIdx.x >> 6; k < BK; k += 4) {
             float2 v = make_float2(0.f, 0.f);
             const int j = k0 + k;
-            if (m0 + m < a.n_kx && j < a.n_src) {
-                const int ch = j / kk, rem = j - ch * kk;
-                const int ai = rem / a.kkx, bi = rem - ai * a.kkx;
-                v = a.grid[(((size_t)ch * a.nz + z) * a.ny + ky0 + ai * a.r) * a.nx + m0 + m + bi];
-            }
+            if (min_ && j < a.n_src) v = base[src_of[j] + m];
             P.put(m, k, v);
         }
     }
@@ -397,53 +430,68 @@ __global__ void __launch_bounds__(256) k_normal64(Normal64Args a) {
 }
 
 // Cholesky of the diagonal block G[k0:k0+kn, k0:k0+kn] (lower) and its
-// explicit inverse (recon/prewhiten.rs:148-210, in FP64).  256 threads; one
-// barrier per column step: every thread recomputes the pivot, the trailing
-// update uses the column scaled on the fly, and the column itself is scaled in
-// the next step.  The inverse goes row by row with each column's dot product
-// split over 4 lanes.  A non-PD pivot marks the slice failed (the reference
-// leaves it undersampled).
+// explicit inverse (recon/prewhiten.rs:148-210, in FP64).  256 threads; thread
+// (ti, tc) keeps the 4 x 4 elements (ti + 16 a, tc + 16 b) of the block in
+// registers.  Step j: the owners of column j publish it (scaled) to LDS, every
+// thread updates its elements with j < c <= i.  The inverse then goes row by
+// row with each column's dot product split over 4 lanes.  A non-PD pivot
+// marks the slice failed (the reference leaves it undersampled).
 __global__ void __launch_bounds__(256) k_potrf64_diag(Potrf64Args a) {
     __shared__ double2 L[64][65];
     __shared__ double2 I[64][65];
+    __shared__ double2 col[64];
     const int z = blockIdx.x;
     if (a.skip[z]) return;
     const int n = a.n, k0 = a.k0, kn = a.kn, t = threadIdx.x;
+    const int ti = t >> 4, tc = t & 15;
     const double2* G = a.G + (size_t)z * n * n;
     double2* Lg = a.L + (size_t)z * n * n;
     double2* D = a.D + ((size_t)z * a.nblk + a.kb) * 64 * 64;
-    for (int e = t; e < 64 * 64; e += 256) {
-        const int i = e / 64, j = e % 64;
-        L[i][j] = (i < kn && j <= i) ? G[(size_t)(k0 + i) * n + k0 + j] : make_double2(0.0, 0.0);
-        I[i][j] = make_double2(0.0, 0.0);
-    }
-    __syncthreads();
+    double2 r[4][4];
+#pragma unroll
+    for (int ia = 0; ia < 4; ++ia)
+#pragma unroll
+        for (int ib = 0; ib < 4; ++ib) {
+            const int i = ti + 16 * ia, c = tc + 16 * ib;
+            r[ia][ib] = (i < kn && c <= i) ? G[(size_t)(k0 + i) * n + k0 + c] : make_double2(0.0, 0.0);
+        }
+    for (int e = t; e < 64 * 64; e += 256) I[e / 64][e % 64] = make_double2(0.0, 0.0);
     int bad = 0;
-    double inv_prev = 1.0;
     for (int j = 0; j < kn; ++j) {
-        // scale column j-1 (nobody reads it during this step)
-        if (j > 0)
-            for (int i = j + t; i < kn; i += 256) L[i][j - 1] = make_double2(L[i][j - 1].x * inv_prev, L[i][j - 1].y * inv_prev);
-        const double d = L[j][j].x;
+        const int jb = j >> 4;
+        if (tc == (j & 15)) {
+#pragma unroll
+            for (int ib = 0; ib < 4; ++ib)
+                if (ib == jb)
+#pragma unroll
+                    for (int ia = 0; ia < 4; ++ia) col[ti + 16 * ia] = r[ia][ib];
+        }
+        __syncthreads();
+        const double d = col[j].x;
         bad |= (!(d > 0.0) || !isfinite(d));
         const double ljj = sqrt(fmax(d, 1e-300)), inv = 1.0 / ljj;
-        const int tt = kn - j - 1;
-        for (int e = t; e < tt * tt; e += 256) {
-            const int i = j + 1 + e / tt, c = j + 1 + e % tt;
-            if (c <= i) {
-                const double2 li = L[i][j], lc = L[c][j];
-                const double2 p = zmul(li, zconj(lc));
-                const double s2 = inv * inv;
-                L[i][c].x -= p.x * s2;
-                L[i][c].y -= p.y * s2;
+        // scaled column j: L[i][j] = col[i] / ljj (i > j), L[j][j] = ljj
+        if (t < 64) L[t][j] = (t > j && t < kn) ? make_double2(col[t].x * inv, col[t].y * inv)
+                                                : (t == j ? make_double2(ljj, 0.0) : make_double2(0.0, 0.0));
+        const double s2 = inv * inv;
+#pragma unroll
+        for (int ia = 0; ia < 4; ++ia) {
+            const int i = ti + 16 * ia;
+            if (i <= j || i >= kn) continue;
+            const double2 li = col[i];
+#pragma unroll
+            for (int ib = 0; ib < 4; ++ib) {
+                const int c = tc + 16 * ib;
+                if (c > j && c <= i) {
+                    const double2 p = zmul(li, zconj(col[c]));
+                    r[ia][ib].x -= p.x * s2;
+                    r[ia][ib].y -= p.y * s2;
+                }
             }
         }
         __syncthreads();
-        if (t == 0) L[j][j] = make_double2(ljj, 0.0);
-        inv_prev = inv;
     }
-    __syncthreads();  // the last column has no rows below its pivot
-    // inverse: row i, column c <= i; lanes 4c..4c+3 share column c
+    // rows of L above the diagonal are zero; L[i][j] for i < j was set to 0 above
     const int c = t >> 2, part = t & 3;
     for (int i = 0; i < kn; ++i) {
         double2 acc = make_double2(0.0, 0.0);

@@ -145,6 +145,7 @@ static void grappa_volume(oks_ctx* ctx, float2* grid, const oks_kspace* k, const
         *flags |= OKS_F_GRAPPA_CALIB_FAILED;  // every slice stays undersampled (recon/strategy.rs:389-395)
         return;
     }
+    if (g.n_src > 2048 || g.n_tgt > 512) fail(OKS_E_UNSUPPORTED, "GRAPPA kernel too large (n_src > 2048)");
     float2* w = (float2*)ctx->buf("gr_w", sizeof(float2) * (size_t)nz * (r - 1) * nc * g.n_src);
     int* fail = (int*)ctx->buf("gr_fail", sizeof(int) * nz);
     grappa_calibrate_batch(ctx, grid, nc, nz, ny, nx, (int)p.acs_start, ny_acs, r, kky, kkx, ridge, g, w, fail,
@@ -197,6 +198,7 @@ int oks_grappa_calibrate_dev(oks_ctx* ctx, const oks_c32* acs, int64_t nc, int64
         std::string why;
         const int st = grappa_geom(nc, ny_acs, nx, r, kky, kkx, &g, &why);
         if (st != OKS_OK) fail(st, why);
+        if (g.n_src > 2048 || g.n_tgt > 512) fail(OKS_E_UNSUPPORTED, "GRAPPA kernel too large (n_src > 2048)");
         int* f = (int*)ctx->buf("grc_fail", 2 * sizeof(int));
         grappa_calibrate_batch(ctx, (const float2*)acs, (int)nc, 1, (int)ny_acs, (int)nx, 0, (int)ny_acs, (int)r,
                                (int)kky, (int)kkx, ridge, g, (float2*)weights, f, f + 1, nullptr);
@@ -214,6 +216,7 @@ int oks_grappa_synthesize_dev(oks_ctx* ctx, const oks_c32* weights, oks_c32* ksp
             fail(OKS_E_INVALID_ARG, "bad synthesis arguments");
         GrGeom g{};
         g.n_src = (int)(nc * kky * kkx);
+        if (g.n_src > 2048) fail(OKS_E_UNSUPPORTED, "GRAPPA kernel too large (n_src > 2048)");
         grappa_apply_batch(ctx, (float2*)kspace, (int)nc, (int)nz, (int)ny, (int)nx, (int)r, (int)kky, (int)kkx,
                            (const float2*)weights, 0, nullptr, g);
     });
