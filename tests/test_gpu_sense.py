@@ -1,0 +1,171 @@
+"""GPU parity of Walsh maps, the SENSE unfold / g-factor and SenseRss.
+
+Bars: the R x R solves run in FP64, so the unfold is within 1e-5 of the FP64
+oracle (and within 1e-5 plus the FP32 restatement's own distance to FP64 of
+the FP32 restatement); Walsh maps are FP32 like the reference (1e-5).
+Fallback decisions (no pattern, Ny % R != 0) and flags are exact.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+from openkspace_amd import abi, synth
+from openkspace_amd._lib import DeviceArray, OksError, check, lib
+from tests.gpu_util import assert_parity, recon_dev, rel
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from openkspace_amd._lib import Context
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def gpu_unfold(ctx, aliased, maps, r, ridge):
+    aliased = np.ascontiguousarray(aliased, dtype=np.complex64)
+    maps = np.ascontiguousarray(maps, dtype=np.complex64)
+    nc, ny, nx = maps.shape
+    da, dm = DeviceArray.from_host(ctx, aliased), DeviceArray.from_host(ctx, maps)
+    dr = DeviceArray(ctx, (ny, nx), np.complex64)
+    dg = DeviceArray(ctx, (ny, nx), np.float32)
+    check(ctx.ptr, lib().oks_sense_unfold_dev(ctx.ptr, da.addr, dm.addr, nc, ny, nx, r, C.c_float(ridge), dr.addr,
+                                              dg.addr))
+    ctx.synchronize()
+    rho, g = dr.download(), dg.download()
+    for d in (da, dm, dr, dg):
+        d.free()
+    return rho, g
+
+
+def gpu_walsh(ctx, img, window, iters):
+    img = np.ascontiguousarray(img, dtype=np.complex64)
+    nc, ny, nx = img.shape
+    di = DeviceArray.from_host(ctx, img)
+    dm = DeviceArray(ctx, img.shape, np.complex64)
+    check(ctx.ptr, lib().oks_walsh_maps_dev(ctx.ptr, di.addr, nc, 1, ny, nx, window, iters, dm.addr))
+    ctx.synchronize()
+    out = dm.download()
+    di.free()
+    dm.free()
+    return out
+
+
+def test_unfold_golden(ctx, golden):
+    r = int(golden["se_params"][0])
+    rho, g = gpu_unfold(ctx, golden["se_aliased"], golden["se_maps"], r, float(golden["se_ridge"]))
+    e_rho, e_g = rel(rho, golden["se_rho"]), rel(g, golden["se_g"])
+    print(f"unfold vs numpy FP64: rho {e_rho:.2e}, g {e_g:.2e}")
+    assert e_rho < 1e-5 and e_g < 1e-5
+
+
+def test_unfold_known_answers(ctx):
+    # R=1 with unit maps -> coil mean (recon/sense.rs:365)
+    maps = np.ones((2, 8, 6), np.complex64)
+    y, x = np.meshgrid(np.arange(8), np.arange(6), indexing="ij")
+    al = np.stack([np.sin(y * 0.1 + x * 0.05 + c) for c in range(2)]).astype(np.complex64)
+    rho, _ = gpu_unfold(ctx, al, maps, 1, 0.0)
+    assert np.abs(rho - al.mean(0)).max() < 1e-5
+    # orthogonal unit maps -> g = 1 (recon/sense.rs:406,436)
+    m2 = np.zeros((2, 8, 4), np.complex64)
+    m2[0, :4] = 1
+    m2[1, 4:] = 1
+    _, g = gpu_unfold(ctx, np.zeros_like(m2), m2, 2, 0.0)
+    assert np.abs(g - 1).max() < 1e-5
+    with pytest.raises(OksError) as e:
+        gpu_unfold(ctx, np.zeros((2, 10, 4)), np.zeros((2, 10, 4)), 3, 0.0)
+    assert e.value.code == abi.OKS_E_SENSE_INDIVISIBLE
+
+
+def test_unfold_vs_oracle_random(ctx):
+    rng = np.random.default_rng(8)
+    maps = (rng.standard_normal((6, 24, 10)) + 1j * rng.standard_normal((6, 24, 10))).astype(np.complex64)
+    al = (rng.standard_normal((6, 24, 10)) + 1j * rng.standard_normal((6, 24, 10))).astype(np.complex64)
+    for r in (2, 3, 4):
+        rho, g = gpu_unfold(ctx, al, maps, r, 1e-4)
+        _, r32, g32, _ = O.sense_unfold(al, maps, r, 1e-4, True, "f32")
+        _, r64, g64, _ = O.sense_unfold(al, maps, r, 1e-4, True, "f64")
+        assert_parity(rho, r32, r64, TOL, f"unfold R={r}")
+        assert_parity(g, g32, g64, TOL, f"gfactor R={r}")
+
+
+def test_walsh_golden_and_known_answers(ctx, golden):
+    win = int(golden["wa_params"][0])
+    maps = gpu_walsh(ctx, golden["wa_img"], win, 300)
+    print(f"walsh (300 it) vs eigh: {rel(maps, golden['wa_maps']):.2e}")
+    assert rel(maps, golden["wa_maps"]) < 1e-4
+    phi = 0.35 * np.pi  # recon/sensitivity.rs:219
+    m = gpu_walsh(ctx, np.full((1, 8, 8), np.exp(1j * phi), np.complex64), 1, 6)
+    assert np.all(m.real > 0.5) and np.abs(m.imag).max() < 1e-4
+
+
+def test_walsh_vs_oracle(ctx):
+    case = synth.make_case(nc=8, nz=1, ny=40, nx=36, kind="full", seed=12, ellipses=6)
+    img = O.ifft_centered(case["kspace"][:, 0], (1, 2), "f64")
+    got = gpu_walsh(ctx, img, 3, 6)
+    r32 = O.walsh_maps(img.astype(np.complex64), 3, 6, "f32")
+    r64 = O.walsh_maps(img, 3, 6, "f64")
+    assert_parity(got, r32, r64, TOL, "walsh 6 it")
+
+
+def _scase(**kw):
+    p = dict(nc=8, nz=2, ny=96, nx=48, kind="grappa", r=2, acs=24, seed=41, ellipses=6)
+    p.update(kw)
+    return synth.make_case(**p)
+
+
+@pytest.mark.parametrize("r,acs,gfac", [(2, 24, 0), (3, 24, 1), (4, 32, 1)])
+def test_sense_strategy(ctx, r, acs, gfac):
+    case = _scase(r=r, acs=acs)
+    d = abi.SenseDesc.default(compute_gfactor=gfac)
+    st, r32, rflags, g32 = O.recon("sense", d, case, "f32")
+    _, r64, _, g64 = O.recon("sense", d, case, "f64")
+    got, flags, g = recon_dev(ctx, "oks_recon_sense_dev", d, case)
+    assert st == 0 and got.shape == r32.shape and flags == rflags and not flags & abi.F_SENSE_FALLBACK
+    assert_parity(got, r32, r64, TOL, f"sense R={r}")
+    if gfac:
+        assert_parity(g, g32, g64, TOL, f"sense g R={r}")
+
+
+def test_sense_fallbacks(ctx):
+    d = abi.SenseDesc.default()
+    case = _scase(ny=100, r=3, acs=24)       # 100 % 3 != 0
+    _, ref, rflags, _ = O.recon("sense", d, case, "f32")
+    got, flags, _ = recon_dev(ctx, "oks_recon_sense_dev", d, case)
+    assert flags == rflags and flags & abi.F_SENSE_FALLBACK
+    assert rel(got, ref) < 2e-5
+    case = _scase(kind="full")
+    _, ref, rflags, _ = O.recon("sense", d, case, "f32")
+    got, flags, _ = recon_dev(ctx, "oks_recon_sense_dev", d, case)
+    assert flags == rflags and flags & abi.F_NO_PATTERN
+    assert rel(got, ref) < 2e-5
+
+
+def test_sense_3d_kz_decoupling(ctx):
+    case = synth.make_case(nc=6, nz=8, ny=48, nx=32, kind="grappa", r=2, acs=16, three_d=True, seed=5, ellipses=4)
+    d = abi.SenseDesc.default()
+    st, r32, rflags, _ = O.recon("sense", d, case, "f32")
+    _, r64, _, _ = O.recon("sense", d, case, "f64")
+    got, flags, _ = recon_dev(ctx, "oks_recon_sense_dev", d, case)
+    assert flags == rflags
+    assert_parity(got, r32, r64, TOL, "sense 3d")
+
+
+def test_sense_c4_shape_fallback(ctx):
+    # C4 (32 coils, 256 ky, R=3, 32 ACS lines, 3-D): 256 % 3 != 0, so the
+    # reference falls back to the 3-D IFFT + RSS (recon/strategy.rs:631-647);
+    # 8 of the 160 kz partitions keep the check within seconds on the CPU.
+    case = synth.make_config("c4_sense3d", nz=8)
+    d = abi.SenseDesc.default()
+    O.set_threads(16)
+    st, r32, rflags, _ = O.recon("sense", d, case, "f32")
+    _, r64, _, _ = O.recon("sense", d, case, "f64")
+    O.set_threads(1)
+    got, flags, _ = recon_dev(ctx, "oks_recon_sense_dev", d, case)
+    assert st == 0 and flags == rflags and flags & abi.F_SENSE_FALLBACK
+    assert_parity(got, r32, r64, TOL, "C4 sense fallback")
