@@ -1,12 +1,21 @@
 """Benchmark of the MI355X ReconStrategy backend (one GPU).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|...]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload served|c1|c2]
 
-A step is one reconstruction call over one batch of synthetic k-space already
-resident in HBM.  Rank 0 prints one JSON line with slices/s (whole job),
-p50/p99 step latency, rel-L2 of the output against the FP32 oracle and its
-FP64 twin, the roofline of the timed launch sequence and the CPU baseline
-(the oracle, single-threaded, on a bounded sample of the same workload).
+served (default): the served mixed stream -- single-slice requests, shapes
+uniform over {256^2 x 8, 320^2 x 16, 640 x 368 x 15 (2x readout
+oversampling), 384^2 x 32}, strategies 50 % ifft-rss / 30 % GRAPPA R=4 / 20 %
+CS (60 iterations), k-space resident in HBM.  A step is one pass of the
+stream (all requests arriving at once) through the continuous-batching
+server; `value` is served slices/s.  The line also carries p50/p99 latency
+under Poisson arrivals at half the served rate, the plain GPU path (the same
+requests one at a time, one launch chain per slice) with its slices/s and
+latency, the alone-vs-served and oracle rel-L2 per strategy and shape, the
+roofline of the dominant stage (HIP events on the slot streams) and the CPU
+baseline (the FP32 oracle, one thread, on a bounded sample of the stream).
+
+c1 / c2: one multi-slice reconstruction call (ifft-rss 16 x 15 x 640 x 368,
+GRAPPA 8 x 32 x 384^2 R=4) per step.
 """
 import argparse
 import ctypes as C
@@ -30,7 +39,14 @@ def _args():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=5)
-    p.add_argument("--workload", default="c1")
+    p.add_argument("--workload", default="served")
+    p.add_argument("--requests", type=int, default=512, help="served: requests per stream pass")
+    p.add_argument("--slots", type=int, default=4, help="served: batches in flight")
+    p.add_argument("--max-batch", type=int, default=32, help="served: requests per batch")
+    p.add_argument("--pool", type=int, default=3, help="served: distinct slices per shape and strategy")
+    p.add_argument("--load", type=float, default=0.5, help="served: Poisson rate as a fraction of served rate")
+    p.add_argument("--stage", default="cs_iterations", help="served: stage timed for the roofline")
+    p.add_argument("--no-graphs", action="store_true", help="served: launch CS iterations eagerly")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-check", action="store_true")
@@ -84,6 +100,194 @@ def workload_c2():
 
 WORKLOADS = {"c1": workload_c1, "c2": workload_c2}
 
+# Stages the served roofline can be taken on: (oks_stage id, bound, unit, peak).
+STAGES = {
+    "ifft_rss": (1, "hbm", "GB/s", HBM_PEAK_GBS, 1e9),
+    "grappa_apply": (3, "mfma", "TFLOP/s", BF16_PEAK_TFLOPS, 1e12),
+    "cs_iterations": (4, "hbm", "GB/s", HBM_PEAK_GBS, 1e9),
+}
+
+
+def serve_descs():
+    """Descriptors of the served stream: reference defaults, CS at 60
+    iterations, readout-oversampling removal off (the x crop to the recon
+    matrix is the fused crop of the IFFT)."""
+    from openkspace_amd import abi
+    return {"ifft_rss": abi.IfftRssDesc.default(remove_oversampling=0),
+            "grappa": abi.GrappaDesc.default(remove_oversampling=0),
+            "cs": abi.CsDesc.default(remove_oversampling=0, iters=60)}
+
+
+def _pct(v, q):
+    return float(np.percentile(np.asarray(v), q)) if len(v) else None
+
+
+def run_served(a):
+    from openkspace_amd import abi, synth
+    from openkspace_amd._lib import Context, DeviceArray
+    from openkspace_amd.server import Server
+
+    ctx = Context(0)
+    descs = serve_descs()
+    pool = synth.serve_pool(a.pool)
+    dev = {}
+    for key, cases in pool.items():
+        lst = []
+        for case in cases:
+            nc, _, ny, nx = case["kspace"].shape
+            k = DeviceArray.from_host(ctx, case["kspace"])
+            n = DeviceArray.from_host(ctx, case["noise"])
+            m = np.ascontiguousarray(case["mask"], np.uint8)
+            ks = abi.KSpace(k.addr, m.ctypes.data, nc, 1, ny, nx, n.addr, case["noise"].shape[1],
+                            abi.Header(**case["hdr"]))
+            lst.append(dict(ks=ks, bufs=(k, n), m=m, case=case))
+        dev[key] = lst
+    seq = synth.serve_requests(a.requests)
+    seen = {}
+    spec = []
+    for key in seq:
+        j = seen.get(key, 0)
+        seen[key] = j + 1
+        spec.append((key, j % a.pool))
+    max_out = max(ny * nx for ny, nx, _, _ in synth.SERVE_SHAPES)
+    outs = [DeviceArray(ctx, (max_out,), np.float32) for _ in spec]
+    N = len(spec)
+
+    def submit_all(srv, arrivals):
+        for i, (key, j) in enumerate(spec):
+            d = dev[key][j]
+            srv.submit(Server.request(descs[key[1]], d["ks"], outs[i].addr, None, i, arrivals[i]))
+
+    def one_pass(srv, arrivals):
+        submit_all(srv, arrivals)
+        got = srv.collect(N)
+        bad = [c for c in got.values() if c.status != 0]
+        if bad:
+            kinds = sorted({(spec[int(c.tag)][0][1], synth.SERVE_SHAPES[spec[int(c.tag)][0][0]][:3], int(c.batch))
+                            for c in bad})
+            raise RuntimeError(f"{len(bad)} requests failed, status {bad[0].status}: {srv.last_error()}; "
+                               f"(strategy, shape, batch) of failures: {kinds[:12]}")
+        return got
+
+    stage_id, bound, unit, peak, scale = STAGES[a.stage]
+    srv = Server(slots=a.slots, max_batch=a.max_batch, time_stage=stage_id, graphs=not a.no_graphs)
+    for _ in range(a.warmup):
+        one_pass(srv, [srv.now() + 0.005] * N)
+    st0 = srv.stats()
+    walls, batch_sizes, sat_lat = [], [], []
+    for _ in range(a.steps):
+        t0 = srv.now() + 0.005
+        got = one_pass(srv, [t0] * N)
+        walls.append(max(c.t_done for c in got.values()) - t0)
+        batch_sizes += [c.batch for c in got.values()]
+        sat_lat += [(c.t_done - c.t_arrival) * 1e3 for c in got.values()]
+    st1 = srv.stats()
+    value = N * a.steps / sum(walls)
+    # one request per (shape, strategy) for the checks: its served image now
+    check_idx = {}
+    for i, (key, j) in enumerate(spec):
+        check_idx.setdefault(key, i)
+    served_img = {key: (outs[i].download(), got[i]) for key, i in check_idx.items()}
+    # latency under Poisson arrivals at a fraction of the served rate
+    rate = a.load * value
+    t0 = srv.now() + 0.01
+    arr = t0 + synth.poisson_arrivals(N, rate)
+    got = one_pass(srv, list(arr))
+    p_lat = [(c.t_done - c.t_arrival) * 1e3 for c in got.values()]
+    p_thr = N / (max(c.t_done for c in got.values()) - t0)
+    stats = srv.stats()
+    srv.close()
+    # the plain GPU path: one request per launch chain, one at a time
+    plain = Server(slots=1, max_batch=1, graphs=not a.no_graphs)
+    one_pass(plain, [plain.now() + 0.005] * N)  # warm-up pass: plans, workspaces, graphs
+    t0 = plain.now() + 0.005
+    gotp = one_pass(plain, [t0] * N)
+    plain_wall = max(c.t_done for c in gotp.values()) - t0
+    plain_value = N / plain_wall
+    plain_service = [(c.t_done - c.t_start) * 1e3 for c in gotp.values()]
+    plain.close()
+    plain_img = {key: outs[i].download() for key, i in check_idx.items()}
+
+    d_ms = st1.stage_ms - st0.stage_ms
+    d_units = st1.stage_units - st0.stage_units
+    d_n = st1.stage_launches - st0.stage_launches
+    avg_s = d_ms * 1e-3 / max(d_n, 1)
+    achieved = (d_units / max(d_n, 1)) / avg_s / scale if d_n else None
+    shapes = [f"{ny}x{nx}x{nc}" + (" (2x readout oversampling)" if os_x > 1 else "")
+              for ny, nx, nc, os_x in synth.SERVE_SHAPES]
+    out = {
+        "metric": METRIC, "value": value, "unit": "slices/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": sum(walls) * 1e3 / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c64/f32 (GRAPPA split-bf16 MFMA, fp64 solves)",
+        "data": "synthetic",
+        "config": {"workload": "served_mixed_stream", "requests_per_step": N, "shapes": shapes,
+                   "mix": {"ifft_rss": 0.5, "grappa_r4": 0.3, "cs_60it": 0.2}, "slots": a.slots,
+                   "max_batch": a.max_batch, "distinct_slices_per_shape_and_strategy": a.pool,
+                   "remove_oversampling": False, "prewhiten": True, "arrivals": "all at once per step",
+                   "cs_iterations_as_hip_graphs": not a.no_graphs},
+        "latency_ms": {"p50": _pct(p_lat, 50), "p99": _pct(p_lat, 99), "mean": float(np.mean(p_lat)),
+                       "load": f"Poisson, {rate:.0f} requests/s ({a.load:g} x served rate)",
+                       "achieved_slices_per_s": p_thr},
+        "saturation_latency_ms": {"p50": _pct(sat_lat, 50), "p99": _pct(sat_lat, 99)},
+        "plain_gpu": {"value": plain_value, "unit": "slices/s",
+                      "service_latency_ms": {"p50": _pct(plain_service, 50), "p99": _pct(plain_service, 99)},
+                      "path": "same requests one at a time (batch of one, one slot)"},
+        "served_vs_plain": value / plain_value,
+        "batches": {"mean_requests": float(np.mean(batch_sizes)), "count": int(st1.batches - st0.batches),
+                    "peak_inflight_bytes": int(stats.peak_inflight_bytes), "hbm_budget": int(stats.hbm_budget)},
+    }
+    out["roofline"] = {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+                       "frac": (achieved / peak) if achieved else None, "traffic": None,
+                       "kernel": f"stage {a.stage} on the slot streams", "launches": int(d_n),
+                       "avg_launch_ms": avg_s * 1e3, "algorithmic_units_per_launch": d_units / max(d_n, 1)}
+    rl = {}
+    for key, i in check_idx.items():
+        c = served_img[key][1]
+        n = int(c.nz * c.ny * c.nx)
+        s_img = served_img[key][0][:n]
+        p_img = plain_img[key][:n]
+        rl[f"{key[1]} {synth.SERVE_SHAPES[key[0]][0]}x{synth.SERVE_SHAPES[key[0]][1]}"] = {
+            "served_vs_alone": float(np.linalg.norm(s_img - p_img) / np.linalg.norm(p_img)),
+            "batch": int(c.batch)}
+    if not a.no_check:
+        import oracle as O
+        O.set_threads(min(16, os.cpu_count() or 1))
+        for key, i in check_idx.items():
+            name = f"{key[1]} {synth.SERVE_SHAPES[key[0]][0]}x{synth.SERVE_SHAPES[key[0]][1]}"
+            case = dev[key][spec[i][1]]["case"]
+            c = served_img[key][1]
+            n = int(c.nz * c.ny * c.nx)
+            s_img = served_img[key][0][:n]
+            _, r32, _, _ = O.recon(key[1], descs[key[1]], case, "f32")
+            _, r64, _, _ = O.recon(key[1], descs[key[1]], case, "f64")
+            r32, r64 = r32.reshape(-1), r64.reshape(-1)
+            rl[name].update({"vs_fp32_oracle": float(np.linalg.norm(s_img - r32) / np.linalg.norm(r32)),
+                             "vs_fp64_oracle": float(np.linalg.norm(s_img - r64) / np.linalg.norm(r64)),
+                             "fp32_oracle_vs_fp64": float(np.linalg.norm(r32 - r64) / np.linalg.norm(r64))})
+    out["rel_l2"] = rl
+    if not a.no_cpu:
+        import oracle as O
+        O.set_threads(1)
+        done, t0c = 0, time.perf_counter()
+        for key, j in spec:
+            st, _, _, _ = O.recon(key[1], descs[key[1]], dev[key][j]["case"], "f32")
+            assert st == 0
+            done += 1
+            if time.perf_counter() - t0c > a.cpu_seconds:
+                break
+        dt = time.perf_counter() - t0c
+        out["cpu_baseline"] = {"value": done / dt, "unit": "slices/s", "cores": 1, "kind": "port",
+                               "sample": f"first {done} requests of the served stream through the FP32 oracle, "
+                                         f"single thread, {dt:.1f} s"}
+    for lst in dev.values():
+        for d in lst:
+            for b in d["bufs"]:
+                b.free()
+    for o in outs:
+        o.free()
+    ctx.close()
+    print(json.dumps(out))
+
 
 def cpu_baseline(wl, seconds):
     import oracle as O
@@ -109,6 +313,13 @@ def main():
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if a.workload == "served":
+        if a.steps == 50 and "--steps" not in sys.argv:
+            a.steps = 5
+        if a.warmup == 5 and "--warmup" not in sys.argv:
+            a.warmup = 1
+        run_served(a)
+        return
     from openkspace_amd._lib import Context, Event
     from openkspace_amd.device import DevCase
 

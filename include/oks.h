@@ -154,6 +154,188 @@ typedef struct oks_pattern {
     int64_t r, acs_start, acs_end, ky_lo, ky_hi;
 } oks_pattern;
 
+/* SamplingPattern::detect via detect_pattern (recon/grappa.rs:50-129,499-509):
+ * OKS_OK and *out filled when a GRAPPA pattern is found, OKS_E_UNSUPPORTED
+ * where the reference returns None.  Host-only integer logic. */
+int oks_detect_pattern(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx, oks_pattern* out);
+
+/* ------------------------------------------------------------- context */
+/* Opaque per-device state: stream, FFT plans, twiddles, workspaces, cached
+ * HIP graphs (replaces the reference's per-call FftPlanner construction,
+ * recon/fft.rs:65-66).  One owner at a time; calls on one ctx serialise. */
+typedef struct oks_ctx oks_ctx;
+
+int oks_ctx_create(int device, oks_ctx** out);
+void oks_ctx_destroy(oks_ctx* ctx);
+const char* oks_last_error(const oks_ctx* ctx);     /* message of the last failing call */
+int oks_ctx_set_stream(oks_ctx* ctx, void* hip_stream); /* NULL = a private stream */
+void* oks_ctx_stream(oks_ctx* ctx);
+/* Options: "graphs" (0/1, capture the CS iteration loop), "zchunk" (slices per
+ * IFFT+RSS chunk, >= 1). */
+int oks_ctx_set_option(oks_ctx* ctx, const char* name, int64_t value);
+int oks_ctx_synchronize(oks_ctx* ctx);
+/* Flags raised on the device by the *_dev calls since the last take
+ * (whitener not PD, SENSE pixel failure, GRAPPA calibration failure);
+ * synchronises the stream and clears them. */
+int oks_ctx_take_flags(oks_ctx* ctx, uint32_t* flags);
+
+/* Stage timing (option "time_stage" = an oks_stage): every launch sequence of
+ * that stage is bracketed by HIP events on the ctx stream and charged its
+ * algorithmic work (bytes for the HBM-bound stages, flops for GRAPPA).
+ * oks_ctx_stage_time synchronises and returns the totals since the option
+ * was set. */
+enum oks_stage {
+    OKS_STAGE_NONE = 0,
+    OKS_STAGE_IFFT_RSS = 1,          /* IFFT + crop + RSS tail; bytes: grid read + image write      */
+    OKS_STAGE_GRAPPA_CALIBRATE = 2,  /* Gram, rhs, FP64 ridge solve; flops of recon/grappa.rs:156-308 */
+    OKS_STAGE_GRAPPA_APPLY = 3,      /* synthesis GEMM; flops 8 x filled points x nc x n_src          */
+    OKS_STAGE_CS_ITERATIONS = 4      /* FISTA iterations; bytes 5*8 + 1 per pixel, plane, iteration   */
+};
+int oks_ctx_stage_time(oks_ctx* ctx, double* ms, double* units, int64_t* launches);
+
+/* Device memory and timing through the library's own HIP runtime. kind:
+ * 0 host->device, 1 device->host, 2 device->device (synchronous). */
+int oks_device_malloc(oks_ctx* ctx, size_t bytes, void** ptr);
+int oks_device_free(oks_ctx* ctx, void* ptr);
+int oks_memcpy(oks_ctx* ctx, void* dst, const void* src, size_t bytes, int32_t kind);
+int oks_memset(oks_ctx* ctx, void* dst, int value, size_t bytes);
+int oks_event_create(oks_ctx* ctx, void** ev);
+int oks_event_record(oks_ctx* ctx, void* ev);          /* on the ctx stream */
+int oks_event_elapsed_ms(oks_ctx* ctx, void* start, void* stop, float* ms);
+int oks_event_destroy(oks_ctx* ctx, void* ev);
+
+/* ---------------------------------------------------- strategy level */
+/* One entry point per ReconStrategy impl: `reconstruct(&self, file)`
+ * (recon/strategy.rs:58-67) with the file replaced by the tensors
+ * read_kspace_with_mask and read_calibration produce.  The plain form takes
+ * host buffers and blocks until the image is on the host; the _dev form takes
+ * device kspace/noise/image (mask stays on the host), is asynchronous on the
+ * ctx stream, and leaves device-raised flags for oks_ctx_take_flags. */
+int oks_recon_ifft_rss(oks_ctx* ctx, const oks_ifft_rss_desc* d, const oks_kspace* k, oks_image* img);
+int oks_recon_ifft_rss_dev(oks_ctx* ctx, const oks_ifft_rss_desc* d, const oks_kspace* k, oks_image* img);
+    /* IfftRss::reconstruct, recon/strategy.rs:113-223 */
+int oks_recon_grappa(oks_ctx* ctx, const oks_grappa_desc* d, const oks_kspace* k, oks_image* img);
+int oks_recon_grappa_dev(oks_ctx* ctx, const oks_grappa_desc* d, const oks_kspace* k, oks_image* img);
+    /* GrappaRss::reconstruct, recon/strategy.rs:306-453 */
+int oks_recon_sense(oks_ctx* ctx, const oks_sense_desc* d, const oks_kspace* k, oks_image* img);
+int oks_recon_sense_dev(oks_ctx* ctx, const oks_sense_desc* d, const oks_kspace* k, oks_image* img);
+    /* SenseRss::reconstruct, recon/strategy.rs:542-733 (Walsh maps; ESPIRiT
+     * returns OKS_E_UNSUPPORTED) */
+int oks_recon_cs(oks_ctx* ctx, const oks_cs_desc* d, const oks_kspace* k, oks_image* img);
+int oks_recon_cs_dev(oks_ctx* ctx, const oks_cs_desc* d, const oks_kspace* k, oks_image* img);
+    /* CsRss::reconstruct, recon/strategy.rs:782-891 */
+
+/* ---------------------------------------------------- function level */
+/* All device pointers, asynchronous on the ctx stream. */
+
+/* ifft{1,2,3}_inplace / centred transforms (recon/fft.rs:19-56,
+ * recon/cs.rs:141-218): centred transform of `data` (C-contiguous, `ndim`
+ * dims) along `axes` in the order given; norm 0 none, 1 1/n (ifft_axis,
+ * recon/fft.rs:73-83), 2 1/sqrt(n) (recon/cs.rs:166-176). */
+int oks_fft_centered_dev(oks_ctx* ctx, oks_c32* data, const int64_t* shape, int32_t ndim, const int32_t* axes,
+                         int32_t naxes, int32_t inverse, int32_t norm);
+/* rss_combine_4d (recon/coil.rs:24-27): out[i] = sqrt(sum_c |in[c][i]|^2). */
+int oks_rss_dev(oks_ctx* ctx, const oks_c32* in, int64_t nc, int64_t inner, float* out);
+/* NoisePrewhitener::from_noise_acqs (recon/prewhiten.rs:48-111): W = chol(Psi)^-1
+ * [nc, nc] lower, FP64 inside; W = I and a device flag when not PD. */
+int oks_prewhiten_calibrate_dev(oks_ctx* ctx, const oks_c32* noise, int32_t nc, int64_t ns, oks_c32* W);
+/* NoisePrewhitener::apply (recon/prewhiten.rs:118-142) on [nc, ns] in place. */
+int oks_prewhiten_apply_dev(oks_ctx* ctx, const oks_c32* W, oks_c32* data, int32_t nc, int64_t ns);
+/* GrappaKernel::calibrate (recon/grappa.rs:156-308): acs [nc, ny_acs, nx] ->
+ * weights [r-1][nc][nc*kky*kkx]; OKS_E_GRAPPA_* as the reference errors. */
+int oks_grappa_calibrate_dev(oks_ctx* ctx, const oks_c32* acs, int64_t nc, int64_t ny_acs, int64_t nx, int64_t r,
+                             int64_t kky, int64_t kkx, float ridge, oks_c32* weights);
+/* GrappaKernel::synthesize (recon/grappa.rs:318-424) on every kz slice of
+ * kspace [nc, nz, ny, nx], in place, one weight set for all slices. */
+int oks_grappa_synthesize_dev(oks_ctx* ctx, const oks_c32* weights, oks_c32* kspace, int64_t nc, int64_t nz,
+                              int64_t ny, int64_t nx, int64_t r, int64_t kky, int64_t kkx);
+/* sense_unfold_1d + sense_gfactor_1d (recon/sense.rs:64-162,181-243):
+ * aliased, maps [nc, ny, nx] -> rho [ny, nx] (and g [ny, nx] when g != NULL);
+ * R x R solves in FP64. */
+int oks_sense_unfold_dev(oks_ctx* ctx, const oks_c32* aliased, const oks_c32* maps, int64_t nc, int64_t ny,
+                         int64_t nx, int64_t r, float ridge, oks_c32* rho, float* g);
+/* walsh_sensitivity_maps (recon/sensitivity.rs:37-142) per z: img, maps
+ * [nc, nz, ny, nx]. */
+int oks_walsh_maps_dev(oks_ctx* ctx, const oks_c32* img, int64_t nc, int64_t nz, int64_t ny, int64_t nx,
+                       int64_t window, int64_t iters, oks_c32* maps);
+/* fista_cs_single_coil (recon/cs.rs:59-139) on `planes` independent images:
+ * kzf [planes, ny, nx], mask [mask_planes (1 or planes), ny, nx] -> out. */
+int oks_fista_cs_dev(oks_ctx* ctx, const oks_c32* kzf, const uint8_t* mask, int64_t planes, int64_t mask_planes,
+                     int64_t ny, int64_t nx, int64_t iters, float lambda, oks_c32* out);
+
+/* ------------------------------------------------------------- serving */
+/* No counterpart in the reference, whose CLI reconstructs one file per call
+ * (cli/main.rs:678-784).  A request loop with continuous batching: requests
+ * (one reconstruction each, k-space and noise resident in device memory) are
+ * admitted by their working-set footprint against an HBM budget; arrived
+ * requests with the same strategy, descriptor, shape, header and ky sampling
+ * run together as one multi-slice reconstruction on one of `slots` streams,
+ * while other slots run other batches.  The only asynchronous surface: it owns
+ * its worker threads, contexts and streams. */
+#define OKS_SERVER_MAX_SLOTS 16
+#define OKS_SERVER_MAX_BATCH 64
+
+enum oks_strategy_id {
+    OKS_STRATEGY_IFFT_RSS = 0,
+    OKS_STRATEGY_GRAPPA = 1,
+    OKS_STRATEGY_SENSE = 2,
+    OKS_STRATEGY_CS = 3
+};
+
+typedef struct oks_server oks_server;
+
+typedef struct oks_server_config {
+    int32_t device;
+    int32_t slots;        /* batches in flight, one stream each (1..16)                */
+    int32_t max_batch;    /* requests per batch (1..64); 1 = one request per launch chain */
+    int32_t max_pending;  /* queued requests before submit returns OKS_E_BUSY; 0 = 65536 */
+    int64_t hbm_budget;   /* bytes of batch working set in flight; 0 = half the free HBM */
+    int32_t time_stage;   /* oks_stage timed on every slot (stats.stage_*), 0 = none    */
+    int32_t no_graphs;    /* 1: launch iteration loops eagerly instead of as HIP graphs */
+} oks_server_config;
+
+typedef struct oks_request {
+    uint64_t tag;         /* returned in the completion                                  */
+    int32_t strategy;     /* oks_strategy_id                                             */
+    const void* desc;     /* the strategy's descriptor; copied at submit                 */
+    oks_kspace k;         /* kspace/noise in device memory, mask on the host (copied)     */
+    float* out;           /* device image, oks_output_shape() floats                      */
+    float* gfactor;       /* device g-factor map (SENSE) or NULL                          */
+    double not_before;    /* arrival time on the server clock (s); earlier = now          */
+} oks_request;
+
+typedef struct oks_completion {
+    uint64_t tag;
+    int32_t status;       /* oks_status of the request's batch                          */
+    uint32_t flags;       /* oks_flags of this request                                  */
+    int64_t nz, ny, nx;   /* image shape                                                */
+    int32_t batch, slot;  /* requests in its batch, slot it ran on                      */
+    double t_arrival, t_start, t_done; /* server clock (s): latency = t_done - t_arrival */
+} oks_completion;
+
+typedef struct oks_server_stats {
+    int64_t submitted, completed, failed, batches, pending;
+    int64_t inflight_bytes, peak_inflight_bytes, hbm_budget, workspace_releases;
+    int64_t slot_workspace_bytes[OKS_SERVER_MAX_SLOTS]; /* device workspace held per slot */
+    double stage_ms, stage_units;     /* config.time_stage totals over all slots */
+    int64_t stage_launches;
+} oks_server_stats;
+
+void oks_server_default(oks_server_config* c);
+int oks_server_create(const oks_server_config* cfg, oks_server** out);
+void oks_server_destroy(oks_server* s);              /* finishes running batches, drops queued ones */
+const char* oks_server_last_error(oks_server* s);
+double oks_server_now(const oks_server* s);          /* server clock, seconds since create */
+/* OKS_E_OOM when the request alone exceeds the budget, OKS_E_BUSY when full. */
+int oks_server_submit(oks_server* s, const oks_request* r);
+/* Up to `max` completions; waits up to timeout_ms (<0 forever) for the first. */
+int oks_server_poll(oks_server* s, oks_completion* out, int32_t max, int32_t timeout_ms, int32_t* n);
+/* Wait until nothing is queued or running (timeout_ms < 0: forever); OKS_E_BUSY on timeout. */
+int oks_server_drain(oks_server* s, int32_t timeout_ms);
+int oks_server_get_stats(oks_server* s, oks_server_stats* out);
+/* Working-set bytes the server charges a request at admission. */
+int oks_request_footprint(const oks_request* r, int64_t* bytes);
+
 #ifdef __cplusplus
 }
 #endif

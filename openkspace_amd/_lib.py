@@ -64,6 +64,21 @@ def lib():
     L.oks_rss_dev.argtypes = [_P, _P, _I64, _I64, _P]
     L.oks_prewhiten_calibrate_dev.argtypes = [_P, _P, _I32, _I64, _P]
     L.oks_prewhiten_apply_dev.argtypes = [_P, _P, _P, _I32, _I64]
+    L.oks_ctx_stage_time.argtypes = [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(_I64)]
+    L.oks_server_default.argtypes = [_P]
+    L.oks_server_default.restype = None
+    L.oks_server_create.argtypes = [_P, C.POINTER(_P)]
+    L.oks_server_destroy.argtypes = [_P]
+    L.oks_server_destroy.restype = None
+    L.oks_server_last_error.argtypes = [_P]
+    L.oks_server_last_error.restype = C.c_char_p
+    L.oks_server_now.argtypes = [_P]
+    L.oks_server_now.restype = C.c_double
+    L.oks_server_submit.argtypes = [_P, _P]
+    L.oks_server_poll.argtypes = [_P, _P, _I32, _I32, C.POINTER(_I32)]
+    L.oks_server_drain.argtypes = [_P, _I32]
+    L.oks_server_get_stats.argtypes = [_P, _P]
+    L.oks_request_footprint.argtypes = [_P, C.POINTER(_I64)]
     _lib = L
     return L
 
@@ -107,6 +122,12 @@ class Context:
 
     def set_option(self, name, value):
         check(self.ptr, lib().oks_ctx_set_option(self.ptr, name.encode(), int(value)))
+
+    def stage_time(self):
+        """(ms, units, launches) of the stage selected by set_option("time_stage", ...)."""
+        ms, units, n = C.c_double(0), C.c_double(0), _I64(0)
+        check(self.ptr, lib().oks_ctx_stage_time(self.ptr, C.byref(ms), C.byref(units), C.byref(n)))
+        return float(ms.value), float(units.value), int(n.value)
 
     def take_flags(self):
         f = C.c_uint32(0)

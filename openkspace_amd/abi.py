@@ -118,6 +118,45 @@ class Pattern(C.Structure):
                 ("ky_lo", C.c_int64), ("ky_hi", C.c_int64)]
 
 
+# --------------------------------------------------------------- serving
+STRATEGY_IFFT_RSS, STRATEGY_GRAPPA, STRATEGY_SENSE, STRATEGY_CS = 0, 1, 2, 3
+SERVER_MAX_SLOTS = 16
+STAGE_NONE, STAGE_IFFT_RSS, STAGE_GRAPPA_CALIBRATE, STAGE_GRAPPA_APPLY, STAGE_CS_ITERATIONS = 0, 1, 2, 3, 4
+SERVER_MAX_BATCH = 64
+
+# descriptor class -> (strategy id, reference strategy name, oracle name)
+STRATEGY_OF = {IfftRssDesc: (STRATEGY_IFFT_RSS, "ifft-rss", "ifft_rss"),
+               GrappaDesc: (STRATEGY_GRAPPA, "grappa", "grappa"),
+               SenseDesc: (STRATEGY_SENSE, "sense", "sense"),
+               CsDesc: (STRATEGY_CS, "cs", "cs")}
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("max_batch", C.c_int32),
+                ("max_pending", C.c_int32), ("hbm_budget", C.c_int64), ("time_stage", C.c_int32),
+                ("no_graphs", C.c_int32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("strategy", C.c_int32), ("desc", C.c_void_p), ("k", KSpace),
+                ("out", C.c_void_p), ("gfactor", C.c_void_p), ("not_before", C.c_double)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("status", C.c_int32), ("flags", C.c_uint32),
+                ("nz", C.c_int64), ("ny", C.c_int64), ("nx", C.c_int64),
+                ("batch", C.c_int32), ("slot", C.c_int32),
+                ("t_arrival", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("batches", C.c_int64), ("pending", C.c_int64), ("inflight_bytes", C.c_int64),
+                ("peak_inflight_bytes", C.c_int64), ("hbm_budget", C.c_int64),
+                ("workspace_releases", C.c_int64), ("slot_workspace_bytes", C.c_int64 * SERVER_MAX_SLOTS),
+                ("stage_ms", C.c_double), ("stage_units", C.c_double), ("stage_launches", C.c_int64)]
+
+
 def crop_target(shape, recon):
     """`IfftRss::maybe_crop` target (recon/strategy.rs:227-253); None if no crop."""
     nz, ny, nx = shape

@@ -18,6 +18,11 @@
 
 using namespace oks;
 
+std::shared_mutex& oks::device_alloc_mutex() {
+    static std::shared_mutex m;
+    return m;
+}
+
 // ------------------------------------------------------------ context
 oks::FftPlan oks_ctx::plan(int n) {
     FftPlan P{};
@@ -48,9 +53,13 @@ oks::FftPlan oks_ctx::plan(int n) {
             h[k] = make_float2((float)cosl(a), (float)sinl(a));
         }
         float2* d = nullptr;
-        OKS_HIP(hipMalloc(&d, sizeof(float2) * h.size()));
-        OKS_HIP(hipMemcpy(d, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice));
+        {
+            std::unique_lock<std::shared_mutex> lk(device_alloc_mutex());
+            OKS_HIP(hipMalloc(&d, sizeof(float2) * h.size()));
+        }
         it = tw.emplace(n, d).first;
+        OKS_HIP(hipMemcpyAsync(d, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice, stream));
+        OKS_HIP(hipStreamSynchronize(stream));
     }
     P.tw = it->second;
     return P;
@@ -59,6 +68,7 @@ oks::FftPlan oks_ctx::plan(int n) {
 void* oks_ctx::buf(const std::string& name, size_t bytes) {
     DevBuf& b = ws[name];
     if (bytes > b.cap) {
+        std::unique_lock<std::shared_mutex> lk(device_alloc_mutex());
         if (b.p) {
             OKS_HIP(hipStreamSynchronize(stream));
             OKS_HIP(hipFree(b.p));
@@ -72,12 +82,64 @@ void* oks_ctx::buf(const std::string& name, size_t bytes) {
     return b.p;
 }
 
+hipEvent_t oks_ctx::take_event() {
+    if (!event_pool.empty()) {
+        hipEvent_t e = event_pool.back();
+        event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void oks_ctx::timed_collect() {
+    std::vector<int> cnt;
+    // the device counters are read in stream order (no device-wide copy)
+    for (auto& t : timed_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+            double u = t.units;
+            if (t.dev_count && t.n_count > 0) {
+                cnt.assign(t.n_count, 0);
+                if (hipMemcpyAsync(cnt.data(), t.dev_count, sizeof(int) * t.n_count, hipMemcpyDeviceToHost,
+                                   stream) == hipSuccess &&
+                    hipStreamSynchronize(stream) == hipSuccess)
+                    for (int v : cnt) u += (double)v * t.per_count;
+            }
+            timed_ms += ms;
+            timed_units += u;
+            ++timed_launches;
+        }
+        event_pool.push_back(t.a);
+        event_pool.push_back(t.b);
+    }
+    timed_pending.clear();
+}
+
+void oks_ctx::release_workspace() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    std::unique_lock<std::shared_mutex> lk(device_alloc_mutex());
+    for (auto& kv : graphs)
+        for (auto& e : kv.second) {
+            if (e.exec) (void)hipGraphExecDestroy(e.exec);
+            if (e.graph) (void)hipGraphDestroy(e.graph);
+        }
+    graphs.clear();
+    for (auto& kv : ws)
+        if (kv.second.p) (void)hipFree(kv.second.p);
+    ws.clear();
+    last_calib_fail = nullptr;
+}
+
 void oks_ctx::release() {
     if (stream) (void)hipStreamSynchronize(stream);
-    for (auto& kv : graphs) {
-        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
-        if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
-    }
+    std::unique_lock<std::shared_mutex> lk(device_alloc_mutex());
+    for (auto& kv : graphs)
+        for (auto& e : kv.second) {
+            if (e.exec) (void)hipGraphExecDestroy(e.exec);
+            if (e.graph) (void)hipGraphDestroy(e.graph);
+        }
     graphs.clear();
     for (auto& kv : tw) (void)hipFree(kv.second);
     tw.clear();
@@ -86,6 +148,13 @@ void oks_ctx::release() {
     ws.clear();
     if (d_status) (void)hipFree(d_status);
     d_status = nullptr;
+    for (auto& t : timed_pending) {
+        if (t.a) (void)hipEventDestroy(t.a);
+        if (t.b) (void)hipEventDestroy(t.b);
+    }
+    timed_pending.clear();
+    for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
+    event_pool.clear();
     if (own_stream && stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
 }
@@ -129,6 +198,7 @@ int oks_ctx_create(int device, oks_ctx** out) {
         int n = 0;
         OKS_HIP(hipGetDeviceCount(&n));
         if (device < 0 || device >= n) fail(OKS_E_DEVICE, "no such HIP device");
+        std::unique_lock<std::shared_mutex> lk(device_alloc_mutex());
         OKS_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         c->own_stream = true;
         OKS_HIP(hipMalloc(&c->d_status, 8 * sizeof(int)));
@@ -160,6 +230,7 @@ const char* oks_last_error(const oks_ctx* ctx) { return ctx ? ctx->err.c_str() :
 int oks_ctx_set_stream(oks_ctx* ctx, void* stream) {
     return guarded(ctx, [&] {
         OKS_HIP(hipStreamSynchronize(ctx->stream));
+        std::unique_lock<std::shared_mutex> lk(device_alloc_mutex());
         if (ctx->own_stream && ctx->stream) OKS_HIP(hipStreamDestroy(ctx->stream));
         if (stream) {
             ctx->stream = (hipStream_t)stream;
@@ -177,6 +248,13 @@ int oks_ctx_set_option(oks_ctx* ctx, const char* name, int64_t value) {
         const std::string n = name ? name : "";
         if (n == "graphs") {
             ctx->use_graphs = value != 0;
+        } else if (n == "time_stage") {
+            if (value < OKS_STAGE_NONE || value > OKS_STAGE_CS_ITERATIONS) fail(OKS_E_INVALID_ARG, "unknown stage");
+            OKS_HIP(hipStreamSynchronize(ctx->stream));
+            ctx->timed_collect();
+            ctx->timed_stage = (int)value;
+            ctx->timed_ms = ctx->timed_units = 0.0;
+            ctx->timed_launches = 0;
         } else if (n == "zchunk") {
             if (value < 1) fail(OKS_E_INVALID_ARG, "zchunk must be >= 1");
             ctx->zchunk = (int)value;
@@ -208,6 +286,17 @@ int oks_ctx_take_flags(oks_ctx* ctx, uint32_t* flags) {
         if (h[2]) f |= OKS_F_GRAPPA_CALIB_FAILED;
         ctx->pending_flags = 0;
         if (flags) *flags = f;
+        return OKS_OK;
+    });
+}
+
+int oks_ctx_stage_time(oks_ctx* ctx, double* ms, double* units, int64_t* launches) {
+    return guarded(ctx, [&] {
+        OKS_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->timed_collect();
+        if (ms) *ms = ctx->timed_ms;
+        if (units) *units = ctx->timed_units;
+        if (launches) *launches = ctx->timed_launches;
         return OKS_OK;
     });
 }
@@ -244,6 +333,7 @@ int oks_device_malloc(oks_ctx* ctx, size_t bytes, void** ptr) {
     return guarded(ctx, [&] {
         if (!ptr) fail(OKS_E_INVALID_ARG, "null out pointer");
         *ptr = nullptr;
+        std::unique_lock<std::shared_mutex> lk(device_alloc_mutex());
         OKS_HIP(hipMalloc(ptr, std::max<size_t>(bytes, 1)));
         return OKS_OK;
     });
@@ -252,6 +342,7 @@ int oks_device_malloc(oks_ctx* ctx, size_t bytes, void** ptr) {
 int oks_device_free(oks_ctx* ctx, void* ptr) {
     return guarded(ctx, [&] {
         OKS_HIP(hipStreamSynchronize(ctx->stream));
+        std::unique_lock<std::shared_mutex> lk(device_alloc_mutex());
         if (ptr) OKS_HIP(hipFree(ptr));
         return OKS_OK;
     });

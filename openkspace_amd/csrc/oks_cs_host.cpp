@@ -92,7 +92,11 @@ static void run_fista(oks_ctx* ctx, const CsProblem& P) {
     OKS_HIP(launch_copy2(P.T, P.x, P.z, P.planes * P.ny * P.nx, ctx->stream));
     if (P.iters <= 0) return;
     const std::vector<float> alpha = fista_alphas(P.iters);
+    // algorithmic bytes per pixel, plane and iteration: read z, x, y and the
+    // mask, write x and z
+    const double it_bytes = (double)P.planes * P.iters * ((double)P.ny * P.nx * (5.0 * 8.0) + (double)P.ny * P.nx);
     if (!ctx->use_graphs) {
+        StageTimer timer(ctx, OKS_STAGE_CS_ITERATIONS, it_bytes);
         cs_iterations(ctx, P, alpha);
         return;
     }
@@ -113,22 +117,27 @@ static void run_fista(oks_ctx* ctx, const CsProblem& P) {
     hipGraphExec_t exec = ctx->graph_for("cs", key);
     if (!exec) {
         hipGraph_t graph = nullptr;
-        OKS_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        try {
-            cs_iterations(ctx, P, alpha);
-        } catch (...) {
-            hipStreamEndCapture(ctx->stream, &graph);
-            if (graph) hipGraphDestroy(graph);
-            throw;
-        }
-        OKS_HIP(hipStreamEndCapture(ctx->stream, &graph));
-        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        if (e != hipSuccess) {
-            hipGraphDestroy(graph);
-            hip_check(e, "hipGraphInstantiate");
+        {
+            // no device allocation or release in any thread while capturing
+            std::shared_lock<std::shared_mutex> lk(device_alloc_mutex());
+            OKS_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                cs_iterations(ctx, P, alpha);
+            } catch (...) {
+                hipStreamEndCapture(ctx->stream, &graph);
+                if (graph) hipGraphDestroy(graph);
+                throw;
+            }
+            OKS_HIP(hipStreamEndCapture(ctx->stream, &graph));
+            hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                hipGraphDestroy(graph);
+                hip_check(e, "hipGraphInstantiate");
+            }
         }
         ctx->graph_store("cs", key, graph, exec);
     }
+    StageTimer timer(ctx, OKS_STAGE_CS_ITERATIONS, it_bytes);
     OKS_HIP(hipGraphLaunch(exec, ctx->stream));
 }
 

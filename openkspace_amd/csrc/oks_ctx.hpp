@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <map>
 #include <mutex>
+#include <shared_mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -30,6 +31,12 @@ inline void hip_check(hipError_t e, const char* what) {
 
 inline void fail(int code, const std::string& m) { throw Status(code, m); }
 
+// Device allocation, release and synchronous copies synchronise the device,
+// which must not happen while another thread's stream is being captured into
+// a HIP graph (contexts of one server run on concurrent threads).  Captures
+// hold this lock shared; allocations and releases hold it exclusively.
+std::shared_mutex& device_alloc_mutex();
+
 // Grow-only device buffer.
 struct DevBuf {
     void* p = nullptr;
@@ -50,8 +57,10 @@ struct oks_ctx {
     uint32_t pending_flags = 0;
     int zchunk = 4;                                // slices per row/column chunk (ifft-rss)
     int use_graphs = 1;                            // capture iteration loops into HIP graphs
+    bool kspace_is_scratch = false;                // input grid may be modified in place (server batches)
     std::vector<uint8_t> h_acq;                    // host staging: acquired rows
     std::vector<uint8_t> h_mask;                   // host staging: per-cell mask
+    std::vector<uint8_t> h_batch_mask;             // host staging: a server batch's masks
 
     // Instantiated HIP graphs keyed by what their captured launches depend on.
     struct GraphKey {
@@ -68,21 +77,81 @@ struct oks_ctx {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
     };
-    std::map<std::string, GraphEntry> graphs;
+    // Up to kGraphsPerName instantiated graphs per name, newest last; a server
+    // slot alternates between batch shapes, so one entry would recapture often.
+    static constexpr size_t kGraphsPerName = 8;
+    std::map<std::string, std::vector<GraphEntry>> graphs;
     hipGraphExec_t graph_for(const std::string& name, const GraphKey& key) {
         auto it = graphs.find(name);
-        return (it != graphs.end() && it->second.key == key) ? it->second.exec : nullptr;
+        if (it == graphs.end()) return nullptr;
+        for (auto& e : it->second)
+            if (e.key == key) return e.exec;
+        return nullptr;
     }
     void graph_store(const std::string& name, const GraphKey& key, hipGraph_t g, hipGraphExec_t e) {
-        auto it = graphs.find(name);
-        if (it != graphs.end()) {
-            if (it->second.exec) (void)hipGraphExecDestroy(it->second.exec);
-            if (it->second.graph) (void)hipGraphDestroy(it->second.graph);
+        auto& v = graphs[name];
+        if (v.size() >= kGraphsPerName) {
+            std::unique_lock<std::shared_mutex> lk(oks::device_alloc_mutex());
+            if (v.front().exec) (void)hipGraphExecDestroy(v.front().exec);
+            if (v.front().graph) (void)hipGraphDestroy(v.front().graph);
+            v.erase(v.begin());
         }
-        graphs[name] = GraphEntry{key, g, e};
+        v.push_back(GraphEntry{key, g, e});
     }
+    // Per-slice calibration failures of the last GRAPPA volume ([nz] on the
+    // device), or nullptr when it ran no calibration.
+    const int* last_calib_fail = nullptr;
+
+    // Stage timing: launches of stage `timed_stage` (an oks_stage) are
+    // bracketed by events on the ctx stream; `units` are the algorithmic bytes
+    // or flops of the launch, optionally completed after the fact from device
+    // counters (count[i] * per_count summed over n_count entries).
+    struct TimedLaunch {
+        hipEvent_t a = nullptr, b = nullptr;
+        double units = 0.0;
+        const int* dev_count = nullptr;
+        int n_count = 0;
+        double per_count = 0.0;
+    };
+    int timed_stage = 0;
+    std::vector<TimedLaunch> timed_pending;
+    std::vector<hipEvent_t> event_pool;
+    double timed_ms = 0.0, timed_units = 0.0;
+    long long timed_launches = 0;
+    hipEvent_t take_event();
+    void timed_collect();  // call once the stream has synchronised
 
     oks::FftPlan plan(int n);
     void* buf(const std::string& name, size_t bytes);
     void release();
+    void release_workspace();  // named workspaces and the graphs that use them
 };
+
+namespace oks {
+
+// RAII bracket of one launch (or launch sequence) of stage `stage`.
+struct StageTimer {
+    oks_ctx* c;
+    bool on;
+    oks_ctx::TimedLaunch t;
+    StageTimer(oks_ctx* ctx, int stage, double units, const int* dev_count = nullptr, int n_count = 0,
+               double per_count = 0.0)
+        : c(ctx), on(ctx->timed_stage == stage && stage != 0) {
+        if (!on) return;
+        t.units = units;
+        t.dev_count = dev_count;
+        t.n_count = n_count;
+        t.per_count = per_count;
+        t.a = c->take_event();
+        OKS_HIP(hipEventRecord(t.a, c->stream));
+    }
+    ~StageTimer() {
+        if (!on) return;
+        t.b = c->take_event();
+        if (t.b && hipEventRecord(t.b, c->stream) == hipSuccess) c->timed_pending.push_back(t);
+    }
+    StageTimer(const StageTimer&) = delete;
+    StageTimer& operator=(const StageTimer&) = delete;
+};
+
+}  // namespace oks

@@ -66,6 +66,11 @@ static void grappa_calibrate_batch(oks_ctx* ctx, const float2* grid, int nc, int
                                    int* fail, int* status, long long* launches) {
     (void)ny_acs;
     const int n = g.n_src, m = g.n_tgt;
+    // flops of GrappaKernel::calibrate per slice: Gram (Hermitian half), A^H B,
+    // Cholesky, and the two triangular solves, 8 flops per complex MAC
+    const double dn = n, dm = m, dp = g.n_pos;
+    StageTimer timer(ctx, OKS_STAGE_GRAPPA_CALIBRATE,
+                     8.0 * nz * (dp * dn * (dn + 1) / 2 + dp * dn * dm + dn * dn * dn / 3 + dn * dn * dm));
     float2* G32 = (float2*)ctx->buf("gr_G32", sizeof(float2) * (size_t)nz * n * n);
     float2* R32 = (float2*)ctx->buf("gr_R32", sizeof(float2) * (size_t)nz * n * m);
     CalibArgs ca{grid, G32, nc, nz, ny, nx, r, kky, kkx, acs0, g.n_kx_pos, g.n_pos, 0, n, n};
@@ -130,6 +135,9 @@ static void grappa_apply_batch(oks_ctx* ctx, float2* grid, int nc, int nz, int n
     GrappaFillArgs fa{grid, list, cnt, sampled, nz, ny, nx, r, kky};
     OKS_HIP(launch_grappa_fills(fa, ctx->stream));
     ApplyArgs aa{grid, w, list, cnt, skip, nc, nz, ny, nx, r, kky, kkx, nx - kkx + 1, g.n_src, wz};
+    // flops: 8 per complex MAC, nc x n_src MACs per filled point, (nx-kkx+1)
+    // points per filled row; the filled rows are counted on the device
+    StageTimer timer(ctx, OKS_STAGE_GRAPPA_APPLY, 0.0, cnt, nz * (r - 1), 8.0 * (nx - kkx + 1) * nc * g.n_src);
     OKS_HIP(launch_apply_gemm(aa, ctx->stream));
 }
 
@@ -139,6 +147,7 @@ static void grappa_volume(oks_ctx* ctx, float2* grid, const oks_kspace* k, const
                           float ridge, uint32_t* flags) {
     const int nc = (int)k->nc, nz = (int)k->nz, ny = (int)k->ny, nx = (int)k->nx, r = (int)p.r;
     const int ny_acs = (int)(p.acs_end - p.acs_start);
+    ctx->last_calib_fail = nullptr;
     GrGeom g{};
     std::string why;
     if (grappa_geom(nc, ny_acs, nx, r, kky, kkx, &g, &why) != OKS_OK) {
@@ -150,6 +159,7 @@ static void grappa_volume(oks_ctx* ctx, float2* grid, const oks_kspace* k, const
     int* fail = (int*)ctx->buf("gr_fail", sizeof(int) * nz);
     grappa_calibrate_batch(ctx, grid, nc, nz, ny, nx, (int)p.acs_start, ny_acs, r, kky, kkx, ridge, g, w, fail,
                            ctx->d_status + 2, nullptr);
+    ctx->last_calib_fail = fail;
     grappa_apply_batch(ctx, grid, nc, nz, ny, nx, r, kky, kkx, w, (long long)(r - 1) * nc * g.n_src, fail, g);
 }
 
@@ -162,6 +172,7 @@ void recon_grappa_dev(oks_ctx* ctx, const oks_grappa_desc* d, const oks_kspace* 
     resolve_os(k, d->remove_oversampling, &os_mode, &rec, &flags);
     float2* grid = prep_grid(ctx, k, W, os_mode, rec, "grid");
     oks_pattern p;
+    ctx->last_calib_fail = nullptr;
     if (!detect_pattern(k->mask, k->nz, k->ny, k->nx, &p)) {
         flags |= OKS_F_NO_PATTERN;
     } else {

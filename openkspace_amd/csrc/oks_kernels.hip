@@ -18,19 +18,22 @@ namespace oks {
     } while (0)
 
 // ---------------------------------------------------------------------------
-// Noise whitener, FP64 (recon/prewhiten.rs:48-111).
-// Psi = (1/max(N-1,1)) sum_s n_s n_s^H, L = chol(Psi), W = L^-1, all in double
-// in LDS; W is stored as complex64.  If Psi is not positive definite (the
-// reference returns None and skips pre-whitening) W = I and *status = 1.
-// One workgroup of 256 threads, nc <= 32.
-__global__ void __launch_bounds__(256) k_whitener(const float2* __restrict__ noise, int nc, int ns,
-                                                  float2* __restrict__ W, int* __restrict__ status) {
+// Noise whitener, FP64 (recon/prewhiten.rs:48-111), one workgroup per
+// request b = blockIdx.x.  Psi = (1/max(N-1,1)) sum_s n_s n_s^H, L = chol(Psi),
+// W = L^-1, all in double in LDS; W is stored as complex64.  Where the
+// reference returns None (fewer samples than channels, or a covariance that is
+// not positive definite: recon/prewhiten.rs:49-66,160-164) W = I and the
+// request's status is raised.  256 threads, nc <= 32.
+__global__ void __launch_bounds__(256) k_whitener(WhitenArgs wa) {
     __shared__ double2 psi[32 * 32];
     __shared__ double2 L[32 * 32];
     __shared__ double2 Li[32 * 32];
     __shared__ float2 chunk[32 * 256];  // noise samples staged 256 at a time
     __shared__ int bad;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, b = blockIdx.x, nc = wa.nc;
+    const float2* __restrict__ noise = wa.noise[b];
+    const int ns = (noise && wa.ns[b] >= nc) ? wa.ns[b] : 0;
+    float2* __restrict__ W = wa.W + (size_t)b * nc * nc;
     const double denom = fmax((double)ns - 1.0, 1.0);
     double accr[4] = {0.0, 0.0, 0.0, 0.0}, acci[4] = {0.0, 0.0, 0.0, 0.0};
     for (int s0 = 0; s0 < ns; s0 += 256) {
@@ -62,7 +65,7 @@ __global__ void __launch_bounds__(256) k_whitener(const float2* __restrict__ noi
         L[e] = make_double2(0.0, 0.0);
         Li[e] = make_double2(0.0, 0.0);
     }
-    if (t == 0) bad = 0;
+    if (t == 0) bad = ns == 0;
     __syncthreads();
     // Row-wise Cholesky (recon/prewhiten.rs:148-179) on the lower triangle.
     for (int i = 0; i < nc; ++i) {
@@ -110,7 +113,12 @@ __global__ void __launch_bounds__(256) k_whitener(const float2* __restrict__ noi
         double2 v = bad ? make_double2(i == j ? 1.0 : 0.0, 0.0) : Li[e];
         W[e] = make_float2((float)v.x, (float)v.y);
     }
-    if (t == 0 && bad) *status = 1;  // sticky until the host takes the flags
+    if (t == 0) {
+        if (wa.per_request)
+            wa.status[b] = bad;
+        else if (bad)
+            wa.status[0] = 1;  // sticky until the host takes the flags
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -422,7 +430,19 @@ __global__ void k_rss(const float2* __restrict__ in, int C, long long M, float* 
 
 // ------------------------------------------------------------- launchers
 hipError_t launch_whitener(const float2* noise, int nc, int ns, float2* W, int* status, hipStream_t s) {
-    hipLaunchKernelGGL(k_whitener, dim3(1), dim3(256), 0, s, noise, nc, ns, W, status);
+    WhitenArgs a{};
+    a.noise[0] = noise;
+    a.ns[0] = ns;
+    a.W = W;
+    a.status = status;
+    a.nc = nc;
+    a.per_request = 0;
+    return launch_whitener_batch(a, 1, s);
+}
+
+hipError_t launch_whitener_batch(const WhitenArgs& a, int B, hipStream_t s) {
+    if (a.nc < 1 || a.nc > 32 || B < 1 || B > OKS_MAX_BATCH) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_whitener, dim3((unsigned)B), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

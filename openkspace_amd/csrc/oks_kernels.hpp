@@ -10,6 +10,19 @@
 
 namespace oks {
 
+// Requests one server batch can hold (pointer tables travel as kernel arguments).
+constexpr int OKS_MAX_BATCH = 64;
+
+// Noise whiteners of up to OKS_MAX_BATCH requests, one per workgroup.
+struct WhitenArgs {
+    const float2* noise[OKS_MAX_BATCH];  // [nc, ns[b]] per request, or nullptr
+    int ns[OKS_MAX_BATCH];
+    float2* W;          // [B, nc, nc]
+    int* status;        // per_request ? [B] (0/1 written) : [1] (sticky, raised only)
+    int nc;
+    int per_request;
+};
+
 // Fused pre-whitening + centred kx IFFT (+ oversampling chain) + x crop.
 struct RowPwArgs {
     const float2* in;   // first row of the chunk, coil 0; coil c at in + c * in_plane
@@ -55,6 +68,7 @@ struct ColArgs {
 };
 
 hipError_t launch_whitener(const float2* noise, int nc, int ns, float2* W, int* status, hipStream_t s);
+hipError_t launch_whitener_batch(const WhitenArgs& a, int B, hipStream_t s);
 size_t rows_prewhiten_lds(int nc, int nx, int os_mode);
 hipError_t launch_rows_prewhiten_ifft(const RowPwArgs& a, hipStream_t s);
 hipError_t launch_rows(const RowArgs& a, bool inverse, hipStream_t s);

@@ -140,6 +140,7 @@ void resolve_os(const oks_kspace* k, int enabled, int* os_mode, int* rec, uint32
 // applied to the raw grid, into workspace `slot` [nc, nz, ny, nx].
 float2* prep_grid(oks_ctx* ctx, const oks_kspace* k, const float2* W, int os_mode, int rec, const char* slot) {
     const size_t plane = (size_t)k->nz * k->ny * k->nx, total = plane * k->nc;
+    if (!os_mode && !W && ctx->kspace_is_scratch) return (float2*)k->kspace;  // caller's private copy
     float2* g = (float2*)ctx->buf(slot, sizeof(float2) * total);
     if (os_mode) {
         const int64_t rows = k->nz * k->ny;
@@ -206,6 +207,10 @@ static int rss_groups(int nc, long long wgs_per_group) {
 void ifft_rss_grid(oks_ctx* ctx, const float2* k, int64_t nc, int64_t nz, int64_t ny, int64_t nx, const float2* W,
                    int os_mode, int rec, bool three_d, const Win w[3], float* out) {
     const int tx = (int)w[2].len, ty = (int)w[1].len;
+    // algorithmic bytes: the k-space slices read once, the image written once
+    const int64_t zread = three_d ? nz : w[0].len;
+    StageTimer timer(ctx, OKS_STAGE_IFFT_RSS,
+                     (double)nc * zread * ny * nx * 8.0 + (double)w[0].len * ty * tx * 4.0);
     RowPwArgs a{};
     a.W = W;
     a.nc = (int)nc;

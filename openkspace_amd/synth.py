@@ -172,8 +172,40 @@ CONFIGS = {
                        seed=SEED_BASE + 3),
 }
 
-# Served mixed stream request shapes (ny, nx, nc, os_x).
+# Served mixed stream: request shapes (ny, nx, nc, os_x) drawn uniformly, and
+# the strategy mix (50 % ifft-rss, 30 % GRAPPA R=4, 20 % CS).
 SERVE_SHAPES = [(256, 256, 8, 1), (320, 320, 16, 1), (368, 640, 15, 2), (384, 384, 32, 1)]
+SERVE_MIX = (("ifft_rss", 0.5), ("grappa", 0.3), ("cs", 0.2))
+SERVE_SEED = SEED_BASE + 4
+_SERVE_KIND = {"ifft_rss": "full", "grappa": "grappa", "cs": "cs"}
+
+
+def serve_pool(per_combo=3, seed=SERVE_SEED):
+    """Distinct single-slice acquisitions for every (shape, strategy) pair of
+    the served stream: {(shape_index, strategy): [case, ...]}."""
+    pool = {}
+    for si, (ny, nx, nc, os_x) in enumerate(SERVE_SHAPES):
+        for mi, (name, _) in enumerate(SERVE_MIX):
+            pool[(si, name)] = [make_case(nc=nc, nz=1, ny=ny, nx=nx, os_x=os_x, kind=_SERVE_KIND[name], r=4,
+                                          acs=24, seed=seed + 1000 * si + 100 * mi + j)
+                                for j in range(per_combo)]
+    return pool
+
+
+def serve_requests(n, seed=SERVE_SEED):
+    """The request sequence: [(shape_index, strategy)] * n, seeded."""
+    rng = np.random.default_rng(seed)
+    names = [m[0] for m in SERVE_MIX]
+    probs = [m[1] for m in SERVE_MIX]
+    strat = rng.choice(len(names), size=n, p=probs)
+    shape = rng.integers(0, len(SERVE_SHAPES), size=n)
+    return [(int(shape[i]), names[int(strat[i])]) for i in range(n)]
+
+
+def poisson_arrivals(n, rate, seed=SERVE_SEED + 1):
+    """Arrival offsets (s) of a Poisson process of `rate` requests/s."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, size=n))
 
 
 def make_config(name, **over):

@@ -394,7 +394,8 @@ int grappa_calibrate(const cx<T>* acs, int64_t nc, int64_t ny_acs, int64_t nx, i
     T sum = T(0);
     for (int64_t i = 0; i < n_src; ++i) sum += ata[i * n_src + i].re;
     T mean = sum / std::max((T)n_src, T(1));
-    T lam = ridge * std::max(mean, (T)std::numeric_limits<float>::epsilon());
+    // f32::max returns the non-NaN operand, as fmax does (std::max would not).
+    T lam = ridge * std::fmax(mean, (T)std::numeric_limits<float>::epsilon());
     for (int64_t i = 0; i < n_src; ++i) ata[i * n_src + i] += cx<T>{lam, T(0)};
     std::vector<cx<T>> L((size_t)n_src * n_src), Li((size_t)n_src * n_src);
     if (!cholesky_lower(ata.data(), (int)n_src, L.data())) return GR_CHOLESKY;
@@ -521,8 +522,8 @@ int sense_unfold(const cx<T>* aliased, const cx<T>* maps, int64_t nc, int64_t ny
                 for (int64_t k = 0; k < r; ++k) {
                     T inv_diag = T(0);
                     for (int64_t j = k; j < r; ++j) inv_diag += norm_sqr(Li[j * r + k]);
-                    T d = std::max(chc[k * r + k].re, T(0));
-                    gfac[(y0 + k * nyr) * nx + x] = std::sqrt(std::max(inv_diag * d, T(0)));
+                    T d = std::fmax(chc[k * r + k].re, T(0));  // f32::max: NaN yields 0
+                    gfac[(y0 + k * nyr) * nx + x] = std::sqrt(std::fmax(inv_diag * d, T(0)));
                 }
         }
     return SE_OK;
