@@ -46,7 +46,7 @@ def _args():
     p.add_argument("--pool", type=int, default=3, help="served: distinct slices per shape and strategy")
     p.add_argument("--load", type=float, default=0.5, help="served: Poisson rate as a fraction of served rate")
     p.add_argument("--stage", default="cs_iterations", help="served: stage timed for the roofline")
-    p.add_argument("--no-graphs", action="store_true", help="served: launch CS iterations eagerly")
+    p.add_argument("--no-graphs", action="store_true", help="launch CS iterations eagerly (no HIP graph)")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-check", action="store_true")
@@ -98,7 +98,32 @@ def workload_c2():
     )
 
 
-WORKLOADS = {"c1": workload_c1, "c2": workload_c2}
+def workload_c3():
+    from openkspace_amd import abi, synth
+    case = synth.make_config("c3_cs")
+    nc, nz, ny, nx = case["kspace"].shape
+    iters = 120
+    desc = abi.CsDesc.default(iters=iters, lambda_=0.01)
+    # per coil-slice-iteration: read z, x, y and the mask, write x and z
+    algo_bytes = nc * nz * iters * (5 * ny * nx * 8 + ny * nx)
+    return dict(
+        name="c3_cs",
+        config={"workload": "c3_cs", "slices": nz, "coils": nc, "ky": ny, "kx": nx, "mask": "variable-density R=6, "
+                "24x24 centre", "iterations": iters, "lambda": 0.01, "wavelet": "Haar, 1 level", "strategy": "CsRss",
+                "prewhiten": True, "iteration_loop": "HIP graph"},
+        case=case, desc=desc, fn="oks_recon_cs_dev", oracle="cs", slices=nz,
+        bound="hbm", algo=algo_bytes, kernel="CsRss launch sequence (FISTA graph + RSS)",
+        cpu_unit=lambda c, z: _slice_case(c, z), dtype="c64/f32",
+    )
+
+
+WORKLOADS = {"c1": workload_c1, "c2": workload_c2, "c3": workload_c3}
+
+# HBM bytes the dominant kernels move per algorithmic unit, measured with the
+# FETCH_SIZE / WRITE_SIZE counters scaled by a copy kernel of known size
+# (profiles/hbm_traffic_counters.md): CS 73 B per pixel-iteration (41 algorithmic),
+# ifft-rss 61.9 MB per 15 x 368 x 640 slice (28.73 MB algorithmic).
+TRAFFIC_PER_ALGO_BYTE = {"c1": 61.9e6 / 28.73e6, "c3": 73.0 / 41.0, "served": 73.0 / 41.0}
 
 # Stages the served roofline can be taken on: (oks_stage id, bound, unit, peak).
 STAGES = {
@@ -116,6 +141,37 @@ def serve_descs():
     return {"ifft_rss": abi.IfftRssDesc.default(remove_oversampling=0),
             "grappa": abi.GrappaDesc.default(remove_oversampling=0),
             "cs": abi.CsDesc.default(remove_oversampling=0, iters=60)}
+
+
+def run_hbm_copy(a):
+    """Copy-bandwidth ceiling and the calibration kernel of the byte counters:
+    k_prewhiten with one channel and W = [1] reads and writes 64 Mi complex64
+    (512 MiB each way) per launch."""
+    import ctypes as C
+    from openkspace_amd._lib import Context, DeviceArray, Event, check, lib
+    M = 64 * 1024 * 1024
+    ctx = Context(0)
+    buf = DeviceArray(ctx, (M,), np.complex64)
+    buf.zero()
+    w = DeviceArray.from_host(ctx, np.ones(1, np.complex64))
+    L = lib()
+    for _ in range(a.warmup):
+        check(ctx.ptr, L.oks_prewhiten_apply_dev(ctx.ptr, w.addr, buf.addr, 1, M))
+    e0, e1 = Event(ctx), Event(ctx)
+    e0.record()
+    for _ in range(a.steps):
+        check(ctx.ptr, L.oks_prewhiten_apply_dev(ctx.ptr, w.addr, buf.addr, 1, M))
+    e1.record()
+    ctx.synchronize()
+    ms = e0.elapsed_ms(e1) / a.steps
+    gbs = 2 * 8 * M / (ms * 1e-3) / 1e9
+    print(json.dumps({"metric": "copy bandwidth (read + write)", "value": gbs, "unit": "GB/s", "n_gpus": 1,
+                      "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True,
+                      "config": {"workload": "hbm_copy", "bytes_read_per_launch": 8 * M,
+                                 "bytes_written_per_launch": 8 * M}}))
+    buf.free()
+    w.free()
+    ctx.close()
 
 
 def _pct(v, q):
@@ -236,8 +292,11 @@ def run_served(a):
         "batches": {"mean_requests": float(np.mean(batch_sizes)), "count": int(st1.batches - st0.batches),
                     "peak_inflight_bytes": int(stats.peak_inflight_bytes), "hbm_budget": int(stats.hbm_budget)},
     }
+    units_per_launch = d_units / max(d_n, 1)
+    tr = TRAFFIC_PER_ALGO_BYTE.get("served") if a.stage == "cs_iterations" else None
     out["roofline"] = {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
-                       "frac": (achieved / peak) if achieved else None, "traffic": None,
+                       "frac": (achieved / peak) if achieved else None,
+                       "traffic": units_per_launch * tr if tr else None,
                        "kernel": f"stage {a.stage} on the slot streams", "launches": int(d_n),
                        "avg_launch_ms": avg_s * 1e3, "algorithmic_units_per_launch": d_units / max(d_n, 1)}
     rl = {}
@@ -313,6 +372,9 @@ def main():
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if a.workload == "hbm_copy":
+        run_hbm_copy(a)
+        return
     if a.workload == "served":
         if a.steps == 50 and "--steps" not in sys.argv:
             a.steps = 5
@@ -325,6 +387,9 @@ def main():
 
     wl = WORKLOADS[a.workload]()
     ctx = Context(0)
+    if a.no_graphs:
+        ctx.set_option("graphs", 0)
+        wl["config"]["iteration_loop"] = "eager launches"
     dc = DevCase(ctx, wl["case"])
     ctx.synchronize()
 
@@ -356,8 +421,10 @@ def main():
     avg_s = float(lat.mean()) * 1e-3
     if wl["bound"] == "hbm":
         achieved = wl["algo"] / avg_s / 1e9
+        tr = TRAFFIC_PER_ALGO_BYTE.get(a.workload)
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": wl["kernel"],
+                           "frac": achieved / HBM_PEAK_GBS, "traffic": wl["algo"] * tr if tr else None,
+                           "kernel": wl["kernel"],
                            "algorithmic_bytes": wl["algo"], "avg_launch_ms": avg_s * 1e3}
     else:
         achieved = wl["algo"] / avg_s / 1e12

@@ -171,7 +171,10 @@ const char* oks_last_error(const oks_ctx* ctx);     /* message of the last faili
 int oks_ctx_set_stream(oks_ctx* ctx, void* hip_stream); /* NULL = a private stream */
 void* oks_ctx_stream(oks_ctx* ctx);
 /* Options: "graphs" (0/1, capture the CS iteration loop), "zchunk" (slices per
- * IFFT+RSS chunk, >= 1). */
+ * IFFT+RSS chunk, >= 1), "time_stage" (an oks_stage, below), and the CS
+ * column-pass launch shape "cs_sx" (columns per workgroup: 0 = automatic,
+ * 4, 8, 16, 32) and "cs_threads" (128, 256, 512).  Launch shapes change
+ * speed only, never results. */
 int oks_ctx_set_option(oks_ctx* ctx, const char* name, int64_t value);
 int oks_ctx_synchronize(oks_ctx* ctx);
 /* Flags raised on the device by the *_dev calls since the last take

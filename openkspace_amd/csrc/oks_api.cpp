@@ -255,6 +255,17 @@ int oks_ctx_set_option(oks_ctx* ctx, const char* name, int64_t value) {
             ctx->timed_stage = (int)value;
             ctx->timed_ms = ctx->timed_units = 0.0;
             ctx->timed_launches = 0;
+        } else if (n == "cs_sx") {
+            if (value != 0 && value != 4 && value != 8 && value != 16 && value != 32)
+                fail(OKS_E_INVALID_ARG, "cs_sx must be 0, 4, 8, 16 or 32");
+            ctx->cs_sx = (int)value;
+        } else if (n == "cs_threads") {
+            if (value != 128 && value != 256 && value != 512 && value != 1024)
+                fail(OKS_E_INVALID_ARG, "cs_threads must be 128, 256, 512 or 1024");
+            ctx->cs_threads = (int)value;
+        } else if (n == "cs_chunk") {
+            if (value < 0) fail(OKS_E_INVALID_ARG, "cs_chunk must be >= 0");
+            ctx->cs_chunk = value;
         } else if (n == "zchunk") {
             if (value < 1) fail(OKS_E_INVALID_ARG, "zchunk must be >= 1");
             ctx->zchunk = (int)value;

@@ -1,17 +1,21 @@
 // oks_cs.hip -- FISTA L1-wavelet compressed sensing (fista_cs_single_coil,
 // recon/cs.rs:59-139) for every coil-slice plane at once.
 //
-// One iteration is three passes over the [planes, ny, nx] state:
-//   k_rows<FWD>   : T = row transform of z (centred, 1/sqrt(nx))
+// One iteration is two passes over the [planes, ny, nx] state; T holds the
+// centred row transform of z on entry to an iteration:
 //   k_cs_cols     : per column, forward transform, residual
 //                   r = mask ? (F z - y) : 0 (recon/cs.rs:101-111), inverse
-//                   transform, back into T
+//                   transform, back into T.  The measured k-space and the mask
+//                   are read from column-major copies made once per call.
 //   k_cs_update   : per row pair, inverse row transform g, v = z - g, 1-level
 //                   Haar with soft-threshold of the detail bands and inverse
 //                   Haar (recon/wavelet.rs:32-99; every 2x2 block is local to a
-//                   row pair), then the FISTA momentum z = x+ + a (x+ - x)
-//                   (recon/cs.rs:114-136).
+//                   row pair), the FISTA momentum z = x+ + a (x+ - x)
+//                   (recon/cs.rs:114-136), and the forward row transform of the
+//                   new z into T for the next iteration.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "oks_cs.hpp"
 #include "oks_fft.hpp"
@@ -25,10 +29,11 @@ namespace oks {
     } while (0)
 
 // Column pass with the masked residual in k-space.  One workgroup per
-// (plane, strip of SX columns); the strip of T, y and the slice mask are staged
-// in LDS with coalesced loads.
+// (plane, strip of SX columns); the strip of T is staged in LDS with
+// row-segment loads of SX * 8 bytes, the measured k-space and mask are read
+// per column from their column-major copies (contiguous along the transform).
 template <class F>
-__global__ void __launch_bounds__(512) k_cs_cols(CsColArgs a) {
+__global__ void __launch_bounds__(1024) k_cs_cols(CsColArgs a) {
     extern __shared__ float2 lds[];
     const int n = F::n ? F::n : a.p.n;
     const int SX = a.sx, ls = n + 1;
@@ -37,45 +42,66 @@ __global__ void __launch_bounds__(512) k_cs_cols(CsColArgs a) {
     const float2* tw = stage_twiddles<F>(lds, a.p);
     float2* A = lds + tw_slots<F>();             // SX * ls
     float2* B = F::scratch ? A + (size_t)SX * ls : nullptr;
-    float2* Yl = A + (size_t)(F::scratch ? 2 : 1) * SX * ls;  // SX * n (measured, centred index)
-    unsigned char* Ml = (unsigned char*)(Yl + (size_t)SX * n); // SX * n
     const int strips = (a.nx + SX - 1) / SX;
-    const int plane = blockIdx.x / strips;        // c * nz + z
-    const int xs = (blockIdx.x - plane * strips) * SX;
-    const int z = plane % a.nz;
+    const long long plane = blockIdx.x / strips;  // c * nz + z
+    const int xs = (int)(blockIdx.x - plane * strips) * SX;
+    const int z = (int)((a.plane0 + plane) % a.nz);
     const int half = n / 2, hup = (n + 1) / 2;
-    const float2* T = a.T + (size_t)plane * n * a.nx;
-    const float2* Y = a.y + (size_t)plane * n * a.nx;
-    const unsigned char* M = a.mask + (size_t)z * n * a.nx;
+    float2* T = a.T + (size_t)plane * n * a.nx;
+    const float2* YT = a.yT + (size_t)plane * a.nx * n;
+    const unsigned char* MT = a.mT + (size_t)z * a.nx * n;
     for (int i = threadIdx.x; i < n * SX; i += blockDim.x) {
         const int y = i / SX, col = i - y * SX;
-        float2 v = make_float2(0.f, 0.f), yv = make_float2(0.f, 0.f);
-        unsigned char m = 0;
-        if (xs + col < a.nx) {
-            const size_t o = (size_t)y * a.nx + xs + col;
-            v = T[o];
-            yv = Y[o];
-            m = M[o];
-        }
+        float2 v = make_float2(0.f, 0.f);
+        if (xs + col < a.nx) v = T[(size_t)y * a.nx + xs + col];
         A[(size_t)col * ls + (y + half) % n] = v;
-        Yl[(size_t)col * n + y] = yv;
-        Ml[(size_t)col * n + y] = m;
     }
     __syncthreads();
     for (int col = w; col < SX; col += WPB) {
+        if (xs + col >= a.nx) break;
         float2* line = A + (size_t)col * ls;
+        // line[i] will hold the transform at centred index yo = (i + ceil(n/2))
+        // mod n; the residual goes back to the same slot, which is the
+        // ifftshift position of yo for the inverse transform.  The measured
+        // values and mask bits of this lane's slots are fetched before the
+        // forward transform so their latency hides behind it.
+        const float2* ycol = YT + (size_t)(xs + col) * n;
+        const unsigned char* mcol = MT + (size_t)(xs + col) * n;
+        constexpr int PER = F::n ? (F::n + 63) / 64 : 1;
+        float2 yv[PER];
+        unsigned char mv[PER];
+        if constexpr (F::n != 0) {
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const int i = lane + 64 * k;
+                mv[k] = 0;
+                if (i < n) {
+                    const int yo = (i + hup) % n;
+                    mv[k] = mcol[yo];
+                    yv[k] = ycol[yo];
+                }
+            }
+        }
         float2* res = F::run(line, B ? B + (size_t)col * ls : nullptr, a.p_fwd, tw, lane);
         if (res != line) {
             for (int i = lane; i < n; i += 64) line[i] = res[i];
             wave_sync();
         }
-        // line[i] is the transform at centred index yo = (i + ceil(n/2)) mod n;
-        // the residual goes back to the same slot, which is the ifftshift
-        // position of yo for the inverse transform.
-        for (int i = lane; i < n; i += 64) {
-            const int yo = (i + hup) % n;
-            const float2 v = cscale(line[i], a.scale);
-            line[i] = Ml[(size_t)col * n + yo] ? csub(v, Yl[(size_t)col * n + yo]) : make_float2(0.f, 0.f);
+        if constexpr (F::n != 0) {
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const int i = lane + 64 * k;
+                if (i < n) {
+                    const float2 v = cscale(line[i], a.scale);
+                    line[i] = mv[k] ? csub(v, yv[k]) : make_float2(0.f, 0.f);
+                }
+            }
+        } else {
+            for (int i = lane; i < n; i += 64) {
+                const int yo = (i + hup) % n;
+                const float2 v = cscale(line[i], a.scale);
+                line[i] = mcol[yo] ? csub(v, ycol[yo]) : make_float2(0.f, 0.f);
+            }
         }
         wave_sync();
         res = F::run_inv(line, B ? B + (size_t)col * ls : nullptr, a.p_inv, tw, lane);
@@ -85,10 +111,31 @@ __global__ void __launch_bounds__(512) k_cs_cols(CsColArgs a) {
         wave_sync();
     }
     __syncthreads();
-    float2* To = a.T + (size_t)plane * n * a.nx;
     for (int i = threadIdx.x; i < n * SX; i += blockDim.x) {
         const int yo = i / SX, col = i - yo * SX;
-        if (xs + col < a.nx) To[(size_t)yo * a.nx + xs + col] = cscale(A[(size_t)col * ls + (yo + half) % n], a.scale);
+        if (xs + col < a.nx) T[(size_t)yo * a.nx + xs + col] = cscale(A[(size_t)col * ls + (yo + half) % n], a.scale);
+    }
+}
+
+// out[b][c][r] = in[b][r][c] for a batch of rows x cols matrices, through
+// 32 x 32 LDS tiles (coalesced on both sides).
+template <class T>
+__global__ void __launch_bounds__(256) k_transpose(const T* __restrict__ in, T* __restrict__ out, int rows,
+                                                   int cols) {
+    __shared__ T tile[32][33];
+    const size_t b = blockIdx.z;
+    const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    const T* src = in + b * rows * cols;
+    T* dst = out + b * rows * cols;
+    for (int k = ty; k < 32; k += 8) {
+        const int r = r0 + k, c = c0 + tx;
+        if (r < rows && c < cols) tile[k][tx] = src[(size_t)r * cols + c];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int c = c0 + k, r = r0 + tx;
+        if (r < rows && c < cols) dst[(size_t)c * rows + r] = tile[tx][k];
     }
 }
 
@@ -152,10 +199,33 @@ __global__ void __launch_bounds__(256) k_cs_update(CsUpdArgs a) {
         X[n + x0] = nd;
         X[n + x1] = ne;
         const float2 al = make_float2(alpha, 0.f);
-        Z[x0] = cadd(na, cmul(al, csub(na, xa)));
-        Z[x1] = cadd(nb, cmul(al, csub(nb, xb)));
-        Z[n + x0] = cadd(nd, cmul(al, csub(nd, xd)));
-        Z[n + x1] = cadd(ne, cmul(al, csub(ne, xe)));
+        const float2 z0 = cadd(na, cmul(al, csub(na, xa)));
+        const float2 z1 = cadd(nb, cmul(al, csub(nb, xb)));
+        const float2 z2 = cadd(nd, cmul(al, csub(nd, xd)));
+        const float2 z3 = cadd(ne, cmul(al, csub(ne, xe)));
+        Z[x0] = z0;
+        Z[x1] = z1;
+        Z[n + x0] = z2;
+        Z[n + x1] = z3;
+        if (a.write_t) {
+            // the new z into the slots this lane read g from: the ifftshift
+            // positions of the forward row transform
+            g0[(x0 + half) % n] = z0;
+            g0[(x1 + half) % n] = z1;
+            g1[(x0 + half) % n] = z2;
+            g1[(x1 + half) % n] = z3;
+        }
+    }
+    if (!a.write_t) return;
+    wave_sync();
+    // T = centred forward row transform of the new z, 1/sqrt(nx)
+    float2* f0 = F::run(g0, F::scratch ? (g0 == R0 ? S : R0) : nullptr, a.p, tw, lane);
+    float2* f1 = F::run(g1, F::scratch ? (g1 == R1 ? S + n : R1) : nullptr, a.p, tw, lane);
+    wave_sync();
+    float2* To = a.Tn + base;
+    for (int x = lane; x < n; x += 64) {
+        To[x] = cscale(f0[(x + half) % n], s);
+        To[n + x] = cscale(f1[(x + half) % n], s);
     }
 }
 
@@ -230,7 +300,7 @@ static size_t tw_slots_n(int n) { return cs_fixed(n) ? (size_t)((n + 1) & ~1) : 
 
 size_t cs_cols_lds(int sx, int n) {
     const size_t lines = (size_t)(cs_fixed(n) ? 1 : 2) * sx * (n + 1);
-    return (tw_slots_n(n) + lines + (size_t)sx * n) * sizeof(float2) + (size_t)sx * n;
+    return (tw_slots_n(n) + lines) * sizeof(float2);
 }
 size_t cs_upd_lds(int n) { return (tw_slots_n(n) + (size_t)4 * (cs_fixed(n) ? 2 : 4) * n) * sizeof(float2); }
 
@@ -238,8 +308,9 @@ template <class F>
 static hipError_t go_cs_cols(const CsColArgs& a, hipStream_t s) {
     const size_t lds = cs_cols_lds(a.sx, a.p.n);
     const unsigned grid = (unsigned)(a.planes * ((a.nx + a.sx - 1) / a.sx));
+    const int threads = (a.threads == 1024 || a.threads == 256 || a.threads == 128) ? a.threads : 512;
     OKS_ATTR2(hipFuncSetAttribute((const void*)k_cs_cols<F>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_cs_cols<F>, dim3(grid), dim3(512), lds, s, a);
+    hipLaunchKernelGGL(k_cs_cols<F>, dim3(grid), dim3(threads), lds, s, a);
     return hipGetLastError();
 }
 
@@ -280,6 +351,29 @@ hipError_t launch_rss_crop(const float2* x, int nc, int nz, int ny, int nx, int 
     const unsigned grid = (unsigned)std::min<long long>((total + 255) / 256, 8192);
     hipLaunchKernelGGL(k_rss_crop, dim3(grid), dim3(256), 0, s, x, nc, nz, ny, nx, z0, tz, y0, ty, x0, tx, out);
     return hipGetLastError();
+}
+
+template <class T>
+static hipError_t go_transpose(const T* in, T* out, long long batch, int rows, int cols, hipStream_t s) {
+    if (batch < 1 || rows < 1 || cols < 1) return hipErrorInvalidValue;
+    const size_t mat = (size_t)rows * cols;
+    for (long long b0 = 0; b0 < batch; b0 += 65535) {
+        const long long nb = std::min<long long>(65535, batch - b0);
+        const dim3 grid((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32), (unsigned)nb);
+        hipLaunchKernelGGL(k_transpose<T>, grid, dim3(256), 0, s, in + b0 * mat, out + b0 * mat, rows, cols);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_transpose_c64(const float2* in, float2* out, long long batch, int rows, int cols, hipStream_t s) {
+    return go_transpose(in, out, batch, rows, cols, s);
+}
+
+hipError_t launch_transpose_u8(const unsigned char* in, unsigned char* out, long long batch, int rows, int cols,
+                               hipStream_t s) {
+    return go_transpose(in, out, batch, rows, cols, s);
 }
 
 hipError_t launch_copy2(const float2* a, float2* b, float2* c, long long n, hipStream_t s) {

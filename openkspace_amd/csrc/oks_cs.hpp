@@ -9,18 +9,22 @@ namespace oks {
 
 struct CsColArgs {
     float2* T;                  // [planes, ny, nx] in/out
-    const float2* y;            // [planes, ny, nx] measured k-space
-    const unsigned char* mask;  // [nz, ny, nx]
+    const float2* yT;           // [planes, nx, ny] measured k-space, column-major
+    const unsigned char* mT;    // [nz, nx, ny] mask, column-major
     long long planes;
+    long long plane0;           // index of the first plane (mask plane = (plane0 + p) % nz)
     int nz, nx, sx;
+    int threads;                // workgroup size: 1024, 512 (default), 256 or 128
     float scale;                // 1/sqrt(ny)
     FftPlan p, p_fwd, p_inv;    // n = ny
 };
 
 struct CsUpdArgs {
     const float2* T;  // [planes, ny, nx]
+    float2* Tn;       // row transform of the new z (may alias T), when write_t
     float2* x;
     float2* z;
+    int write_t;
     long long pairs;  // planes * ny / 2
     int ny;
     float scale;      // 1/sqrt(nx)
@@ -35,5 +39,8 @@ hipError_t launch_cs_update(const CsUpdArgs& a, hipStream_t s);
 hipError_t launch_rss_crop(const float2* x, int nc, int nz, int ny, int nx, int z0, int tz, int y0, int ty, int x0,
                            int tx, float* out, hipStream_t s);
 hipError_t launch_copy2(const float2* a, float2* b, float2* c, long long n, hipStream_t s);
+hipError_t launch_transpose_c64(const float2* in, float2* out, long long batch, int rows, int cols, hipStream_t s);
+hipError_t launch_transpose_u8(const unsigned char* in, unsigned char* out, long long batch, int rows, int cols,
+                               hipStream_t s);
 
 }  // namespace oks

@@ -1,9 +1,10 @@
 // oks_cs_host.cpp -- CsRss on the GPU (recon/strategy.rs:782-891) and the
 // batched function-level FISTA entry point (recon/cs.rs:59-139).  The
-// iteration body of every plane runs as three launches per iteration; the
+// iteration body of every plane runs as two launches per iteration; the
 // whole iteration loop is captured once into a HIP graph and replayed.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -18,9 +19,11 @@
 
 namespace oks {
 
-static int cs_sx(int ny) {
-    int sx = 8;
-    while (sx > 1 && cs_cols_lds(sx, ny) > 160 * 1024) sx /= 2;
+// Columns per column-pass workgroup: 16 (128-byte row segments) while two
+// workgroups still fit in one CU's LDS, unless the context fixes it.
+static int cs_sx(const oks_ctx* ctx, int ny) {
+    int sx = ctx->cs_sx > 0 ? ctx->cs_sx : 16;
+    while (sx > 1 && cs_cols_lds(sx, ny) > (ctx->cs_sx > 0 ? 160 : 80) * 1024) sx /= 2;
     return sx;
 }
 
@@ -41,46 +44,69 @@ struct CsProblem {
     float2* y;                   // measured k-space [planes, ny, nx]
     const unsigned char* mask;   // [mask_planes, ny, nx]
     float2 *x, *z, *T;
+    float2* yT;                  // [planes, nx, ny] column-major copy of y
+    unsigned char* mT;           // [mask_planes, nx, ny] column-major copy of the mask
     long long planes;
     int mask_planes, ny, nx, iters;
     float lambda;
 };
 
+// Planes per chunk of the iteration loop: all, unless the context sets a
+// chunk (a chunk whose state fits in the last-level cache keeps it resident
+// across iterations).
+static long long cs_chunk(const oks_ctx* ctx, long long planes) {
+    return ctx->cs_chunk > 0 ? std::min<long long>(ctx->cs_chunk, planes) : planes;
+}
+
 static void cs_iterations(oks_ctx* ctx, const CsProblem& P, const std::vector<float>& alpha) {
     const float sx_ = 1.0f / std::sqrt((float)P.nx), sy_ = 1.0f / std::sqrt((float)P.ny);
+    const size_t plane = (size_t)P.ny * P.nx;
+    OKS_HIP(launch_transpose_u8(P.mask, P.mT, P.mask_planes, P.ny, P.nx, ctx->stream));
     CsColArgs ca{};
-    ca.T = P.T;
-    ca.y = P.y;
-    ca.mask = P.mask;
-    ca.planes = P.planes;
+    ca.mT = P.mT;
     ca.nz = P.mask_planes;
     ca.nx = P.nx;
-    ca.sx = cs_sx(P.ny);
+    ca.sx = cs_sx(ctx, P.ny);
+    ca.threads = ctx->cs_threads;
     ca.scale = sy_;
     ca.p = ctx->plan(P.ny);
     ca.p_fwd = ca.p;
     ca.p_inv = ca.p;
     CsUpdArgs ua{};
-    ua.T = P.T;
-    ua.x = P.x;
-    ua.z = P.z;
-    ua.pairs = P.planes * (P.ny / 2);
     ua.ny = P.ny;
     ua.scale = sx_;
     ua.lambda = P.lambda;
     ua.p = ctx->plan(P.nx);
-    for (int it = 0; it < P.iters; ++it) {
-        row_pass(ctx, P.z, P.T, P.planes * P.ny, P.nx, P.nx, 0, sx_, false, true, nullptr, 0);
-        OKS_HIP(launch_cs_cols(ca, ctx->stream));
-        ua.alpha = alpha[it];
-        OKS_HIP(launch_cs_update(ua, ctx->stream));
+    const long long chunk = cs_chunk(ctx, P.planes);
+    for (long long p0 = 0; p0 < P.planes; p0 += chunk) {
+        const long long np = std::min(chunk, P.planes - p0);
+        const size_t off = (size_t)p0 * plane;
+        float2 *T = P.T + off, *x = P.x + off, *z = P.z + off, *yT = P.yT + off;
+        OKS_HIP(launch_transpose_c64(P.y + off, yT, np, P.ny, P.nx, ctx->stream));
+        // T = centred row transform of z; later iterations get it from the update pass
+        row_pass(ctx, z, T, np * P.ny, P.nx, P.nx, 0, sx_, false, true, nullptr, 0);
+        ca.T = T;
+        ca.yT = yT;
+        ca.planes = np;
+        ca.plane0 = p0;
+        ua.T = T;
+        ua.Tn = T;
+        ua.x = x;
+        ua.z = z;
+        ua.pairs = np * (P.ny / 2);
+        for (int it = 0; it < P.iters; ++it) {
+            OKS_HIP(launch_cs_cols(ca, ctx->stream));
+            ua.alpha = alpha[it];
+            ua.write_t = it + 1 < P.iters;
+            OKS_HIP(launch_cs_update(ua, ctx->stream));
+        }
     }
 }
 
 // x = z = F^-1(mask y) (recon/cs.rs:84-97), then the iterations; result in P.x.
 static void run_fista(oks_ctx* ctx, const CsProblem& P) {
     const float sx_ = 1.0f / std::sqrt((float)P.nx), sy_ = 1.0f / std::sqrt((float)P.ny);
-    if (cs_cols_lds(cs_sx(P.ny), P.ny) > 160 * 1024 || cs_upd_lds(P.nx) > 160 * 1024)
+    if (cs_cols_lds(cs_sx(ctx, P.ny), P.ny) > 160 * 1024 || cs_upd_lds(P.nx) > 160 * 1024)
         fail(OKS_E_UNSUPPORTED, "image too large for the CS passes");
     // plans and attributes exist before any capture (no allocation inside a graph)
     ctx->plan(P.ny);
@@ -106,6 +132,8 @@ static void run_fista(oks_ctx* ctx, const CsProblem& P) {
     key.p[2] = P.x;
     key.p[3] = P.z;
     key.p[4] = P.T;
+    key.p[5] = P.yT;
+    key.p[6] = P.mT;
     key.i[0] = P.planes;
     key.i[1] = P.mask_planes;
     key.i[2] = P.ny;
@@ -114,6 +142,8 @@ static void run_fista(oks_ctx* ctx, const CsProblem& P) {
     uint32_t lb;
     std::memcpy(&lb, &P.lambda, 4);
     key.i[5] = lb;
+    key.i[6] = ((long long)ctx->cs_sx << 16) | ctx->cs_threads;
+    key.i[7] = ctx->cs_chunk;
     hipGraphExec_t exec = ctx->graph_for("cs", key);
     if (!exec) {
         hipGraph_t graph = nullptr;
@@ -175,6 +205,8 @@ void recon_cs_dev(oks_ctx* ctx, const oks_cs_desc* d, const oks_kspace* k, oks_i
         P.x = (float2*)ctx->buf("cs_x", sizeof(float2) * tot);
         P.z = (float2*)ctx->buf("cs_z", sizeof(float2) * tot);
         P.T = (float2*)ctx->buf("cs_t", sizeof(float2) * tot);
+        P.yT = (float2*)ctx->buf("cs_yT", sizeof(float2) * tot);
+        P.mT = (unsigned char*)ctx->buf("cs_mT", cells);
         P.planes = (long long)nc * nz;
         P.mask_planes = nz;
         P.ny = ny;
@@ -218,6 +250,8 @@ int oks_fista_cs_dev(oks_ctx* ctx, const oks_c32* kzf, const uint8_t* mask, int6
         P.x = (float2*)out;
         P.z = (float2*)ctx->buf("fcs_z", sizeof(float2) * tot);
         P.T = (float2*)ctx->buf("fcs_t", sizeof(float2) * tot);
+        P.yT = (float2*)ctx->buf("cs_yT", sizeof(float2) * tot);
+        P.mT = (unsigned char*)ctx->buf("cs_mT", (size_t)mask_planes * ny * nx);
         P.planes = planes;
         P.mask_planes = (int)mask_planes;
         P.ny = (int)ny;

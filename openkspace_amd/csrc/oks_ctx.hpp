@@ -58,6 +58,9 @@ struct oks_ctx {
     int zchunk = 4;                                // slices per row/column chunk (ifft-rss)
     int use_graphs = 1;                            // capture iteration loops into HIP graphs
     bool kspace_is_scratch = false;                // input grid may be modified in place (server batches)
+    int cs_sx = 0;                                 // CS column-pass strip width (0 = automatic)
+    int cs_threads = 512;                          // CS column-pass workgroup size
+    long long cs_chunk = 0;                        // CS planes per iteration-loop chunk (0 = all)
     std::vector<uint8_t> h_acq;                    // host staging: acquired rows
     std::vector<uint8_t> h_mask;                   // host staging: per-cell mask
     std::vector<uint8_t> h_batch_mask;             // host staging: a server batch's masks

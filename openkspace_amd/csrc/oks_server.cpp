@@ -6,8 +6,8 @@
 // request becomes visible at its arrival time.  Each of `slots` worker
 // threads owns a backend context with its own stream and workspace; whenever
 // one is free it takes the oldest arrived request and every other arrived
-// request with the same batch key (strategy, descriptor, shape, header and ky
-// sampling projection), up to `max_batch`, as long as the working-set
+// request with the same batch key (strategy, descriptor, shape, header and,
+// except for CS, the ky sampling projection), up to `max_batch`, as long as the working-set
 // footprint of everything in flight stays within the HBM budget.  The batch
 // runs as one multi-slice reconstruction: per-request FP64 whiteners, a gather
 // into [nc, B, ny, nx] with each request's whitening applied, the strategy on
@@ -96,7 +96,10 @@ bool same_key(const Req& a, const Req& b) {
     const oks_kspace &x = a.r.k, &y = b.r.k;
     if (x.nc != y.nc || x.nz != y.nz || x.ny != y.ny || x.nx != y.nx) return false;
     if (std::memcmp(&x.hdr, &y.hdr, sizeof(oks_header)) != 0) return false;
-    return a.ky == b.ky;
+    // The ky projection drives the pattern / partial-Fourier decisions of
+    // ifft-rss, GRAPPA and SENSE; CS makes no host decision from the mask
+    // (each request's mask goes to the device as its own plane).
+    return a.r.strategy == OKS_STRATEGY_CS || a.ky == b.ky;
 }
 
 // Working-set bytes one request adds to its batch on the device: the packed
@@ -118,7 +121,7 @@ int64_t footprint(const Req& q) {
             }
             break;
         }
-        case OKS_STRATEGY_CS: f += 3 * grid + nz * ny * nx; break;
+        case OKS_STRATEGY_CS: f += 4 * grid + 2 * nz * ny * nx; break;  // x, z, T, column-major y; masks
         case OKS_STRATEGY_SENSE: f += 3 * grid + 2 * out; break;
         default: break;
     }
