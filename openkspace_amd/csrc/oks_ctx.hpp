@@ -55,7 +55,7 @@ struct oks_ctx {
     std::map<std::string, oks::DevBuf> ws;         // named workspaces
     int* d_status = nullptr;                       // device-side flags (whitener) [8]
     uint32_t pending_flags = 0;
-    int zchunk = 4;                                // slices per row/column chunk (ifft-rss)
+    int zchunk = 16;                               // slices per row/column chunk (ifft-rss)
     int use_graphs = 1;                            // capture iteration loops into HIP graphs
     bool kspace_is_scratch = false;                // input grid may be modified in place (server batches)
     int cs_sx = 0;                                 // CS column-pass strip width (0 = automatic)

@@ -84,16 +84,32 @@ constexpr int BKP = BK + 8;  // padded LDS row (bf16 elements)
 struct Planes {
     unsigned short* p;
     int rows;
-    __device__ __forceinline__ void put(int row, int k, float2 v) const {
-        const unsigned short rh = bf16_bits_rne(v.x), ih = bf16_bits_rne(v.y);
-        const unsigned short rl = bf16_bits_rne(v.x - bf16_to_f(rh));
-        const unsigned short il = bf16_bits_rne(v.y - bf16_to_f(ih));
-        const int o = row * BKP + k;
+    // Eight consecutive k of one row: each plane written with one 16-byte store.
+    __device__ __forceinline__ void put8(int row, int k8, const float2 (&v)[8]) const {
+        u32x4 rh, rl, ih, il;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            unsigned int w[4][2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const float2 x = v[2 * q + h];
+                const unsigned short a = bf16_bits_rne(x.x), b = bf16_bits_rne(x.y);
+                w[0][h] = a;
+                w[1][h] = bf16_bits_rne(x.x - bf16_to_f(a));
+                w[2][h] = b;
+                w[3][h] = bf16_bits_rne(x.y - bf16_to_f(b));
+            }
+            rh[q] = w[0][0] | (w[0][1] << 16);
+            rl[q] = w[1][0] | (w[1][1] << 16);
+            ih[q] = w[2][0] | (w[2][1] << 16);
+            il[q] = w[3][0] | (w[3][1] << 16);
+        }
+        const int o = row * BKP + k8;
         const int ps = rows * BKP;
-        p[o] = rh;
-        p[ps + o] = rl;
-        p[2 * ps + o] = ih;
-        p[3 * ps + o] = il;
+        *(u32x4*)(p + o) = rh;
+        *(u32x4*)(p + ps + o) = rl;
+        *(u32x4*)(p + 2 * ps + o) = ih;
+        *(u32x4*)(p + 3 * ps + o) = il;
     }
     __device__ __forceinline__ bf16x8 get(int plane, int row, int k8) const {
         return *(const bf16x8*)(p + plane * rows * BKP + row * BKP + k8);
@@ -226,35 +242,37 @@ struct CalibLoader {
         tgt_of = tab + 2048;
     }
     __device__ int K() const { return a.n_pos; }
-    __device__ int pos(int p) const {
-        const int ky0 = p / a.n_kx;
-        return ky0 * a.nx + (p - ky0 * a.n_kx);
-    }
-    // k (position) fastest across threads: consecutive kx0 are contiguous.
-    __device__ void load_a(const Planes& P, int k0) const {
-        const int k = threadIdx.x & 31;
-        const bool kin = k0 + k < a.n_pos;
-        const int po = kin ? pos(k0 + k) : 0;
-        for (int m = threadIdx.x >> 5; m < 64; m += 8) {
-            float2 v = make_float2(0.f, 0.f);
-            if (m0 + m < a.M && kin) {
-                v = base[po + src_of[m0 + m]];
-                v.y = -v.y;  // conj
+    // One thread: eight consecutive positions (contiguous kx0 within an ACS
+    // row) of one source column, stored as one 16-byte write per plane.
+    __device__ void gather8(const int* of, int col_base, int lim, int rows, int k0, const Planes& P, bool conj) const {
+        const int t = threadIdx.x;
+        if (t >= rows * 4) return;
+        const int m = t >> 2, k8 = (t & 3) * 8;
+        float2 v[8];
+        const int col = col_base + m;
+        const int p0 = k0 + k8;
+        int ky0 = p0 / a.n_kx, kx0 = p0 - ky0 * a.n_kx;
+        const bool cin = col < lim;
+        const int off = cin ? of[col] : 0;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            float2 x = make_float2(0.f, 0.f);
+            if (cin && p0 + e < a.n_pos) {
+                x = base[ky0 * a.nx + kx0 + off];
+                if (conj) x.y = -x.y;
             }
-            P.put(m, k, v);
+            v[e] = x;
+            if (++kx0 == a.n_kx) {
+                kx0 = 0;
+                ++ky0;
+            }
         }
+        P.put8(m, k8, v);
     }
+    __device__ void load_a(const Planes& P, int k0) const { gather8(src_of, m0, a.M, 64, k0, P, true); }
     template <int BN>
     __device__ void load_b(const Planes& P, int k0) const {
-        const int k = threadIdx.x & 31;
-        const bool kin = k0 + k < a.n_pos;
-        const int po = kin ? pos(k0 + k) : 0;
-        const int* of = a.rhs ? tgt_of : src_of;
-        for (int n = threadIdx.x >> 5; n < BN; n += 8) {
-            float2 v = make_float2(0.f, 0.f);
-            if (n0 + n < a.N && kin) v = base[po + of[n0 + n]];
-            P.put(n, k, v);
-        }
+        gather8(a.rhs ? tgt_of : src_of, n0, a.N, BN, k0, P, false);
     }
     __device__ void store(int m, int n, float2 v) const {
         m += m0;
@@ -309,26 +327,32 @@ struct ApplyLoader {
         src_of = tab;
     }
     __device__ int K() const { return a.n_src; }
-    // kx0 (m) fastest: contiguous samples of one source row.
+    // kx0 (m) fastest across lanes (coalesced), eight consecutive source
+    // columns j per thread, stored as one 16-byte write per plane.
     __device__ void load_a(const Planes& P, int k0) const {
-        const int m = threadIdx.x & 63;
+        const int m = threadIdx.x & 63, k8 = (threadIdx.x >> 6) * 8;
         const bool min_ = m0 + m < a.n_kx;
-        for (int k = threadIdx.x >> 6; k < BK; k += 4) {
-            float2 v = make_float2(0.f, 0.f);
-            const int j = k0 + k;
-            if (min_ && j < a.n_src) v = base[src_of[j] + m];
-            P.put(m, k, v);
+        float2 v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int j = k0 + k8 + e;
+            v[e] = (min_ && j < a.n_src) ? base[src_of[j] + m] : make_float2(0.f, 0.f);
         }
+        P.put8(m, k8, v);
     }
     template <int BN>
     __device__ void load_b(const Planes& P, int k0) const {
-        const int k = threadIdx.x & 31;
-        const float2* wd = a.w + (size_t)z * a.wz + (size_t)(d - 1) * a.nc * a.n_src;
-        for (int n = threadIdx.x >> 5; n < BN; n += 8) {
-            float2 v = make_float2(0.f, 0.f);
-            if (n < a.nc && k0 + k < a.n_src) v = wd[(size_t)n * a.n_src + k0 + k];
-            P.put(n, k, v);
+        const int t = threadIdx.x;
+        if (t >= BN * 4) return;
+        const int n = t >> 2, k8 = (t & 3) * 8;
+        const float2* wd = a.w + (size_t)z * a.wz + (size_t)(d - 1) * a.nc * a.n_src + (size_t)n * a.n_src;
+        float2 v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int j = k0 + k8 + e;
+            v[e] = (n < a.nc && j < a.n_src) ? wd[j] : make_float2(0.f, 0.f);
         }
+        P.put8(n, k8, v);
     }
     __device__ void store(int m, int n, float2 v) const {
         m += m0;

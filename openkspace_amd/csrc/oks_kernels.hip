@@ -394,22 +394,33 @@ __global__ void k_rss_finalize(const float* __restrict__ part, int G, long long 
 // ---------------------------------------------------------------------------
 // Stand-alone pre-whitening y = W s over the sample columns of [nc, M]
 // (recon/prewhiten.rs:128-141), in place.
-template <int NCMAX>
+template <int NCMAX, int U>
 __global__ void __launch_bounds__(256) k_prewhiten(const float2* in, float2* d, const float2* __restrict__ W,
                                                    int nc, long long M) {
-    for (long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x; x < M;
-         x += (long long)gridDim.x * blockDim.x) {
-        float2 s[NCMAX];
+    // U positions per thread and step, all loads issued before any store (the
+    // call may be in place, so the compiler cannot reorder them itself).
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long x0 = (long long)blockIdx.x * blockDim.x + threadIdx.x; x0 < M; x0 += stride * U) {
+        float2 s[U][NCMAX];
 #pragma unroll
-        for (int j = 0; j < NCMAX; ++j)
-            if (j < nc) s[j] = in[j * M + x];
+        for (int u = 0; u < U; ++u) {
+            const long long x = x0 + u * stride;
 #pragma unroll
-        for (int i = 0; i < NCMAX; ++i) {
-            if (i < nc) {
-                float2 acc = make_float2(0.f, 0.f);
+            for (int j = 0; j < NCMAX; ++j)
+                if (j < nc && x < M) s[u][j] = in[j * M + x];
+        }
 #pragma unroll
-                for (int j = 0; j <= i; ++j) acc = cadd(acc, cmul(W[i * nc + j], s[j]));
-                d[i * M + x] = acc;
+        for (int u = 0; u < U; ++u) {
+            const long long x = x0 + u * stride;
+            if (x >= M) break;
+#pragma unroll
+            for (int i = 0; i < NCMAX; ++i) {
+                if (i < nc) {
+                    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+                    for (int j = 0; j <= i; ++j) acc = cadd(acc, cmul(W[i * nc + j], s[u][j]));
+                    d[i * M + x] = acc;
+                }
             }
         }
     }
@@ -580,8 +591,10 @@ hipError_t launch_rss_finalize(const float* part, int G, long long M, float* out
 
 hipError_t launch_prewhiten(const float2* in, float2* d, const float2* W, int nc, long long M, hipStream_t s) {
     const unsigned grid = (unsigned)std::min<long long>((M + 255) / 256, 8192);
-    if (nc <= 16) hipLaunchKernelGGL(k_prewhiten<16>, dim3(grid), dim3(256), 0, s, in, d, W, nc, M);
-    else if (nc <= 32) hipLaunchKernelGGL(k_prewhiten<32>, dim3(grid), dim3(256), 0, s, in, d, W, nc, M);
+    if (nc <= 4) hipLaunchKernelGGL((k_prewhiten<4, 8>), dim3(grid), dim3(256), 0, s, in, d, W, nc, M);
+    else if (nc <= 8) hipLaunchKernelGGL((k_prewhiten<8, 4>), dim3(grid), dim3(256), 0, s, in, d, W, nc, M);
+    else if (nc <= 16) hipLaunchKernelGGL((k_prewhiten<16, 2>), dim3(grid), dim3(256), 0, s, in, d, W, nc, M);
+    else if (nc <= 32) hipLaunchKernelGGL((k_prewhiten<32, 1>), dim3(grid), dim3(256), 0, s, in, d, W, nc, M);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
