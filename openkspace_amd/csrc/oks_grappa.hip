@@ -368,48 +368,69 @@ __device__ __forceinline__ double2 zmul(double2 a, double2 b) {
 __device__ __forceinline__ double2 zconj(double2 a) { return make_double2(a.x, -a.y); }
 
 // C = alpha op(A) op(B) + beta C, complex double, batched over blockIdx.z.
-// op 0 = N, 2 = conjugate transpose.  Tile 32 x 32, 256 threads x (2 x 2).
+// op 0 = N, 2 = conjugate transpose.  Tile TS x TS (64 or 32), 256 threads x
+// (TS/16)^2 outputs, K step 16; operand tiles are loaded with the contiguous
+// index fastest.  lower: only tiles that reach the lower triangle
+// (n0 <= m0 + TS - 1) are computed -- enough for the Hermitian trailing update
+// of the factorisation, which only ever reads the lower triangle.
+template <int TS>
 __global__ void __launch_bounds__(256) k_zgemm64(Zgemm64Args g) {
-    __shared__ double2 As[16][33];
-    __shared__ double2 Bs[16][33];
+    constexpr int R = TS / 16;  // outputs per thread along m and n
+    __shared__ double2 As[16][TS + 1];  // [k][m]
+    __shared__ double2 Bs[16][TS + 1];  // [k][n]
     const int b = blockIdx.z;
+    if (g.skip && g.skip[b]) return;
+    const int m0 = blockIdx.x * TS, n0 = blockIdx.y * TS;
+    if (g.lower && n0 > m0 + TS - 1) return;
     const double2* A = g.A + (size_t)b * g.sA;
     const double2* B = g.B + (size_t)b * g.sB;
     double2* C = g.C + (size_t)b * g.sC;
-    if (g.skip && g.skip[b]) return;
-    const int m0 = blockIdx.x * 32, n0 = blockIdx.y * 32;
-    const int tm = threadIdx.x / 16, tn = threadIdx.x % 16;
-    double2 acc[2][2];
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) acc[i][j] = make_double2(0.0, 0.0);
+    const int t = threadIdx.x, tm = t >> 4, tn = t & 15;
+    double2 acc[R][R];
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc[i][j] = make_double2(0.0, 0.0);
     for (int k0 = 0; k0 < g.K; k0 += 16) {
-        for (int e = threadIdx.x; e < 16 * 32; e += 256) {
-            const int kk = e / 32, mm = e % 32;
+#pragma unroll
+        for (int q = 0; q < TS / 16; ++q) {
+            const int e = t + 256 * q;  // 16 * TS elements per operand tile
+            int kk, mm;
+            if (g.opA) { mm = e % TS; kk = e / TS; } else { kk = e & 15; mm = e >> 4; }
             const int m = m0 + mm, k = k0 + kk;
             double2 v = make_double2(0.0, 0.0);
             if (m < g.M && k < g.K) v = g.opA ? zconj(A[(size_t)k * g.lda + m]) : A[(size_t)m * g.lda + k];
             As[kk][mm] = v;
-            const int n = n0 + mm;
+            int kb, nn;
+            if (g.opB) { kb = e & 15; nn = e >> 4; } else { nn = e % TS; kb = e / TS; }
+            const int n = n0 + nn, kq = k0 + kb;
             double2 u = make_double2(0.0, 0.0);
-            if (n < g.N && k < g.K) u = g.opB ? zconj(B[(size_t)n * g.ldb + k]) : B[(size_t)k * g.ldb + n];
-            Bs[kk][mm] = u;
+            if (n < g.N && kq < g.K) u = g.opB ? zconj(B[(size_t)n * g.ldb + kq]) : B[(size_t)kq * g.ldb + n];
+            Bs[kb][nn] = u;
         }
         __syncthreads();
 #pragma unroll 4
         for (int kk = 0; kk < 16; ++kk) {
-            const double2 a0 = As[kk][tm * 2], a1 = As[kk][tm * 2 + 1];
-            const double2 b0 = Bs[kk][tn * 2], b1 = Bs[kk][tn * 2 + 1];
-            double2 p;
-            p = zmul(a0, b0); acc[0][0].x += p.x; acc[0][0].y += p.y;
-            p = zmul(a0, b1); acc[0][1].x += p.x; acc[0][1].y += p.y;
-            p = zmul(a1, b0); acc[1][0].x += p.x; acc[1][0].y += p.y;
-            p = zmul(a1, b1); acc[1][1].x += p.x; acc[1][1].y += p.y;
+            double2 av[R], bv[R];
+#pragma unroll
+            for (int i = 0; i < R; ++i) av[i] = As[kk][tm + 16 * i];
+#pragma unroll
+            for (int j = 0; j < R; ++j) bv[j] = Bs[kk][tn + 16 * j];
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    acc[i][j].x = fma(av[i].x, bv[j].x, fma(-av[i].y, bv[j].y, acc[i][j].x));
+                    acc[i][j].y = fma(av[i].x, bv[j].y, fma(av[i].y, bv[j].x, acc[i][j].y));
+                }
         }
         __syncthreads();
     }
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) {
-            const int m = m0 + tm * 2 + i, n = n0 + tn * 2 + j;
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int m = m0 + tm + 16 * i, n = n0 + tn + 16 * j;
             if (m < g.M && n < g.N) {
                 double2* c = &C[(size_t)m * g.ldc + n];
                 double2 v = make_double2(g.alpha * acc[i][j].x, g.alpha * acc[i][j].y);
@@ -463,7 +484,7 @@ __global__ void __launch_bounds__(256) k_normal64(Normal64Args a) {
 __global__ void __launch_bounds__(256) k_potrf64_diag(Potrf64Args a) {
     __shared__ double2 L[64][65];
     __shared__ double2 I[64][65];
-    __shared__ double2 col[64];
+    __shared__ double2 colb[2][64];
     const int z = blockIdx.x;
     if (a.skip[z]) return;
     const int n = a.n, k0 = a.k0, kn = a.kn, t = threadIdx.x;
@@ -480,17 +501,16 @@ __global__ void __launch_bounds__(256) k_potrf64_diag(Potrf64Args a) {
             r[ia][ib] = (i < kn && c <= i) ? G[(size_t)(k0 + i) * n + k0 + c] : make_double2(0.0, 0.0);
         }
     for (int e = t; e < 64 * 64; e += 256) I[e / 64][e % 64] = make_double2(0.0, 0.0);
+    // Column 0 is published by its owners; afterwards the owners of column
+    // j + 1 publish it right after their step-j update, so each step needs
+    // one barrier (two column buffers alternate).
+    if (tc == 0)
+#pragma unroll
+        for (int ia = 0; ia < 4; ++ia) colb[0][ti + 16 * ia] = r[ia][0];
+    __syncthreads();
     int bad = 0;
     for (int j = 0; j < kn; ++j) {
-        const int jb = j >> 4;
-        if (tc == (j & 15)) {
-#pragma unroll
-            for (int ib = 0; ib < 4; ++ib)
-                if (ib == jb)
-#pragma unroll
-                    for (int ia = 0; ia < 4; ++ia) col[ti + 16 * ia] = r[ia][ib];
-        }
-        __syncthreads();
+        const double2* col = colb[j & 1];
         const double d = col[j].x;
         bad |= (!(d > 0.0) || !isfinite(d));
         const double ljj = sqrt(fmax(d, 1e-300)), inv = 1.0 / ljj;
@@ -513,29 +533,45 @@ __global__ void __launch_bounds__(256) k_potrf64_diag(Potrf64Args a) {
                 }
             }
         }
+        const int jn = j + 1;
+        if (jn < kn && tc == (jn & 15)) {
+            const int jb = jn >> 4;
+#pragma unroll
+            for (int ib = 0; ib < 4; ++ib)
+                if (ib == jb)
+#pragma unroll
+                    for (int ia = 0; ia < 4; ++ia) colb[jn & 1][ti + 16 * ia] = r[ia][ib];
+        }
         __syncthreads();
     }
-    // rows of L above the diagonal are zero; L[i][j] for i < j was set to 0 above
-    const int c = t >> 2, part = t & 3;
-    for (int i = 0; i < kn; ++i) {
-        double2 acc = make_double2(0.0, 0.0);
-        if (c < i)
+    // I = L^-1 by forward substitution.  Columns are independent: four lanes
+    // of one wavefront own column c and split each dot product, so no
+    // workgroup barrier is needed between rows.
+    {
+        const int c = t >> 2, part = t & 3;
+        for (int i = c; i < kn; ++i) {
+            double2 acc = make_double2(0.0, 0.0);
+#pragma unroll 4
             for (int j = c + part; j < i; j += 4) {
                 const double2 p = zmul(L[i][j], I[j][c]);
                 acc.x += p.x;
                 acc.y += p.y;
             }
-        acc.x += __shfl_xor(acc.x, 1);
-        acc.y += __shfl_xor(acc.y, 1);
-        acc.x += __shfl_xor(acc.x, 2);
-        acc.y += __shfl_xor(acc.y, 2);
-        if (part == 0 && c <= i) {
-            const double dd = L[i][i].x;
-            const double re = (c == i ? 1.0 : 0.0) - acc.x, im = -acc.y;
-            I[i][c] = make_double2(re / dd, im / dd);
+            acc.x += __shfl_xor(acc.x, 1);
+            acc.y += __shfl_xor(acc.y, 1);
+            acc.x += __shfl_xor(acc.x, 2);
+            acc.y += __shfl_xor(acc.y, 2);
+            if (part == 0) {
+                const double dd = L[i][i].x;
+                const double re = (c == i ? 1.0 : 0.0) - acc.x, im = -acc.y;
+                I[i][c] = make_double2(re / dd, im / dd);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         }
-        __syncthreads();
     }
+    __syncthreads();
     for (int e = t; e < 64 * 64; e += 256) {
         const int i = e / 64, j = e % 64;
         D[e] = (i < kn && j < kn) ? I[i][j] : make_double2(0.0, 0.0);
@@ -593,8 +629,15 @@ hipError_t launch_apply_gemm(const ApplyArgs& c, hipStream_t s) {
 
 hipError_t launch_zgemm64(const Zgemm64Args& g, int batch, hipStream_t s) {
     if (g.M <= 0 || g.N <= 0) return hipSuccess;
-    dim3 grid((g.M + 31) / 32, (g.N + 31) / 32, batch);
-    hipLaunchKernelGGL(k_zgemm64, grid, dim3(256), 0, s, g);
+    // 64 x 64 tiles when they alone give two workgroups per CU, else 32 x 32
+    const long long tiles64 = (long long)((g.M + 63) / 64) * ((g.N + 63) / 64) * batch;
+    if (tiles64 >= 512) {
+        dim3 grid((g.M + 63) / 64, (g.N + 63) / 64, batch);
+        hipLaunchKernelGGL(k_zgemm64<64>, grid, dim3(256), 0, s, g);
+    } else {
+        dim3 grid((g.M + 31) / 32, (g.N + 31) / 32, batch);
+        hipLaunchKernelGGL(k_zgemm64<32>, grid, dim3(256), 0, s, g);
+    }
     return hipGetLastError();
 }
 

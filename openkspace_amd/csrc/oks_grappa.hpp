@@ -38,6 +38,7 @@ struct Zgemm64Args {
     int opA, opB;          // 0 = N, 2 = conjugate transpose
     double alpha, beta;
     const int* skip;       // per batch, or null
+    int lower;             // only tiles reaching the lower triangle (M == N Hermitian update)
 };
 
 struct Normal64Args {

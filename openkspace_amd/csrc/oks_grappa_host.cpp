@@ -53,8 +53,8 @@ static int grappa_geom(int64_t nc, int64_t ny_acs, int64_t nx, int64_t r, int64_
 
 static void zg(oks_ctx* ctx, int batch, const double2* A, long long sA, int lda, int opA, const double2* B,
                long long sB, int ldb, int opB, double2* C, long long sC, int ldc, int M, int N, int K, double alpha,
-               double beta, const int* skip) {
-    Zgemm64Args g{A, B, C, sA, sB, sC, M, N, K, lda, ldb, ldc, opA, opB, alpha, beta, skip};
+               double beta, const int* skip, int lower = 0) {
+    Zgemm64Args g{A, B, C, sA, sB, sC, M, N, K, lda, ldb, ldc, opA, opB, alpha, beta, skip, lower};
     OKS_HIP(launch_zgemm64(g, batch, ctx->stream));
 }
 
@@ -101,7 +101,7 @@ static void grappa_calibrate_batch(oks_ctx* ctx, const float2* grid, int nc, int
                kn, 1.0, 0.0, fail);
             // G[r0:, r0:] -= L[r0:, k0:k0+kn] L[r0:, k0:k0+kn]^H
             zg(ctx, nz, L + (size_t)r0 * n + k0, snn, n, 0, L + (size_t)r0 * n + k0, snn, n, 2,
-               G + (size_t)r0 * n + r0, snn, n, rn, rn, kn, -1.0, 1.0, fail);
+               G + (size_t)r0 * n + r0, snn, n, rn, rn, kn, -1.0, 1.0, fail, 1);
         }
         // Y_k = Dk R_k ; R[r0:] -= L[r0:, k0:k0+kn] Y_k
         zg(ctx, nz, Dk, sD, 64, 0, R + (size_t)k0 * m, snm, m, 0, Y + (size_t)k0 * m, snm, m, kn, m, kn, 1.0, 0.0,
