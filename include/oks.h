@@ -222,8 +222,7 @@ int oks_recon_grappa_dev(oks_ctx* ctx, const oks_grappa_desc* d, const oks_kspac
     /* GrappaRss::reconstruct, recon/strategy.rs:306-453 */
 int oks_recon_sense(oks_ctx* ctx, const oks_sense_desc* d, const oks_kspace* k, oks_image* img);
 int oks_recon_sense_dev(oks_ctx* ctx, const oks_sense_desc* d, const oks_kspace* k, oks_image* img);
-    /* SenseRss::reconstruct, recon/strategy.rs:542-733 (Walsh maps; ESPIRiT
-     * returns OKS_E_UNSUPPORTED) */
+    /* SenseRss::reconstruct, recon/strategy.rs:542-733 (Walsh or ESPIRiT maps) */
 int oks_recon_cs(oks_ctx* ctx, const oks_cs_desc* d, const oks_kspace* k, oks_image* img);
 int oks_recon_cs_dev(oks_ctx* ctx, const oks_cs_desc* d, const oks_kspace* k, oks_image* img);
     /* CsRss::reconstruct, recon/strategy.rs:782-891 */
@@ -261,6 +260,14 @@ int oks_sense_unfold_dev(oks_ctx* ctx, const oks_c32* aliased, const oks_c32* ma
  * [nc, nz, ny, nx]. */
 int oks_walsh_maps_dev(oks_ctx* ctx, const oks_c32* img, int64_t nc, int64_t nz, int64_t ny, int64_t nx,
                        int64_t window, int64_t iters, oks_c32* maps);
+/* espirit_sensitivity_maps (recon/espirit.rs:52-282) on one ACS block
+ * acs [nc, acs_y, acs_x] -> maps [nc, ny, nx]; FP64 inside.  `kept` (may be
+ * NULL) receives the number of calibration kernels above the threshold.
+ * OKS_E_INVALID_ARG where the reference asserts (kernel larger than the ACS
+ * block or the image), OKS_E_UNSUPPORTED beyond nc * kernel^2 = 2048 or a
+ * kernel above 8. */
+int oks_espirit_maps_dev(oks_ctx* ctx, const oks_c32* acs, int64_t nc, int64_t acs_y, int64_t acs_x, int64_t ny,
+                         int64_t nx, int64_t kernel, float threshold, int64_t iters, oks_c32* maps, int64_t* kept);
 /* fista_cs_single_coil (recon/cs.rs:59-139) on `planes` independent images:
  * kzf [planes, ny, nx], mask [mask_planes (1 or planes), ny, nx] -> out. */
 int oks_fista_cs_dev(oks_ctx* ctx, const oks_c32* kzf, const uint8_t* mask, int64_t planes, int64_t mask_planes,

@@ -60,6 +60,7 @@ def lib():
     L.oks_fista_cs_dev.argtypes = [_P, _P, _P] + [_I64] * 5 + [C.c_float, _P]
     L.oks_sense_unfold_dev.argtypes = [_P, _P, _P] + [_I64] * 4 + [C.c_float, _P, _P]
     L.oks_walsh_maps_dev.argtypes = [_P, _P] + [_I64] * 6 + [_P]
+    L.oks_espirit_maps_dev.argtypes = [_P, _P] + [_I64] * 6 + [C.c_float, _I64, _P, _P]
     L.oks_fft_centered_dev.argtypes = [_P, _P, _P, _I32, _P, _I32, _I32, _I32]
     L.oks_rss_dev.argtypes = [_P, _P, _I64, _I64, _P]
     L.oks_prewhiten_calibrate_dev.argtypes = [_P, _P, _I32, _I64, _P]

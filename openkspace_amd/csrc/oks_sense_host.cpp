@@ -1,5 +1,6 @@
 // oks_sense_host.cpp -- SenseRss on the GPU (recon/strategy.rs:542-733) with
-// Walsh maps, and the function-level SENSE / Walsh entry points.
+// Walsh or ESPIRiT maps, and the function-level SENSE / Walsh / ESPIRiT
+// entry points.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -8,6 +9,7 @@
 #include "../../include/oks.h"
 #include "oks_abi_util.hpp"
 #include "oks_ctx.hpp"
+#include "oks_espirit.hpp"
 #include "oks_host.hpp"
 #include "oks_kernels.hpp"
 #include "oks_pipeline.hpp"
@@ -21,6 +23,40 @@ static void ifft2_planes(oks_ctx* ctx, float2* d, long long planes, int ny, int 
     if (ny >= 2)
         col_pass(ctx, d, d, nullptr, 1, (int)planes, 0, (int)planes, ny, nx, 0, 0, ny, 1.0f / (float)ny, true, true, 1,
                  nullptr);
+}
+
+// espirit_sensitivity_maps (recon/espirit.rs:52-282) for nz slices: maps
+// [nc][nz][ny][nx] with coil stride map_cs.  Slices go in chunks whose FP64
+// calibration matrices stay within 4 GiB.  Returns the kernels kept by the
+// first slice (device-side counts stay in the workspace).
+static void espirit_batch(oks_ctx* ctx, const EspAcs& acs, int nc, int nz, int ny, int nx, int kw, float thr,
+                          int iters, float2* maps, long long map_cs) {
+    // the reference asserts these (recon/espirit.rs:67-78)
+    if (kw < 1 || kw > acs.ay || kw > acs.ax || kw > ny || kw > nx)
+        fail(OKS_E_INVALID_ARG, "ESPIRiT: kernel larger than the ACS block or the image");
+    if (kw > 8) fail(OKS_E_UNSUPPORTED, "ESPIRiT: kernel size above 8");
+    if (nc < 1 || nc > 32) fail(OKS_E_UNSUPPORTED, "ESPIRiT: at most 32 channels");
+    if (iters < 0) fail(OKS_E_INVALID_ARG, "ESPIRiT: negative iteration count");
+    const int n = nc * kw * kw, D = 2 * kw - 1;
+    if (n > kEspMaxN) fail(OKS_E_UNSUPPORTED, "ESPIRiT: nc * kernel^2 above 2048");
+    const size_t per = (size_t)n * n * sizeof(double2);
+    const int zc = std::max(1, std::min(nz, (int)((size_t(4) << 30) / per)));
+    double2* M = (double2*)ctx->buf("esp_M", per * zc);
+    double* lam = (double*)ctx->buf("esp_lam", sizeof(double) * kEspMaxKept * zc);
+    double2* vec = (double2*)ctx->buf("esp_vec", sizeof(double2) * kEspMaxKept * n * zc);
+    int* kept = (int*)ctx->buf("esp_kept", sizeof(int) * zc);
+    double2* Q = (double2*)ctx->buf("esp_Q", sizeof(double2) * (size_t)nc * nc * D * D * zc);
+    const double s = std::sqrt((double)ny * nx) / ((double)kw * kw) / ((double)ny * nx);
+    for (int z0 = 0; z0 < nz; z0 += zc) {
+        const int nzc = std::min(zc, nz - z0);
+        EspAcs sa = acs;
+        sa.acs = acs.acs + (size_t)z0 * acs.zs;
+        OKS_HIP(launch_espirit_gram(EspGramArgs{sa, nc, kw, n, M}, nzc, ctx->stream));
+        OKS_HIP(launch_espirit_eig(EspEigArgs{M, lam, vec, kept, n, iters, (double)thr}, nzc, ctx->stream));
+        OKS_HIP(launch_espirit_q(EspQArgs{vec, kept, Q, nc, kw, n, s * s}, nzc, ctx->stream));
+        EspVoxArgs va{Q, vec, kept, maps + (size_t)z0 * ny * nx, map_cs, nc, kw, n, nzc, ny, nx, iters, s};
+        OKS_HIP(launch_espirit_voxel(va, ctx->stream));
+    }
 }
 
 void recon_sense_dev(oks_ctx* ctx, const oks_sense_desc* d, const oks_kspace* k, oks_image* img) {
@@ -44,16 +80,22 @@ void recon_sense_dev(oks_ctx* ctx, const oks_sense_desc* d, const oks_kspace* k,
         flags |= OKS_F_SENSE_FALLBACK | (have ? 0u : (uint32_t)OKS_F_NO_PATTERN);
         ifft_rss_grid(ctx, grid, nc, nz, ny, nx, nullptr, 0, 0, false, w, img->data);
     } else {
-        if (d->map_source != OKS_MAPS_WALSH)
-            fail(OKS_E_UNSUPPORTED, "ESPIRiT sensitivity maps are not on this backend yet; use map_source = Walsh");
         const long long planes = (long long)nc * nz;
         const size_t tot = (size_t)planes * ny * nx;
-        float2* low = (float2*)ctx->buf("se_low", sizeof(float2) * tot);
-        OKS_HIP(launch_rows_window(grid, low, planes, ny, nx, (int)p.acs_start, (int)p.acs_end, ctx->stream));
-        ifft2_planes(ctx, low, planes, ny, nx);
         float2* maps = (float2*)ctx->buf("se_maps", sizeof(float2) * tot);
-        WalshArgs wa{low, maps, nc, nz, ny, nx, d->walsh_window, d->walsh_iters};
-        OKS_HIP(launch_walsh(wa, ctx->stream));
+        if (d->map_source == OKS_MAPS_ESPIRIT) {
+            // the contiguous ACS block [nc, acs_len, nx] of every slice (recon/strategy.rs:673-688)
+            const EspAcs ea{grid + (size_t)p.acs_start * nx, (long long)ny * nx, (long long)nz * ny * nx, nx,
+                            (int)(p.acs_end - p.acs_start), nx};
+            espirit_batch(ctx, ea, nc, nz, ny, nx, d->espirit_kernel, d->espirit_threshold, d->espirit_iters, maps,
+                          (long long)nz * ny * nx);
+        } else {
+            float2* low = (float2*)ctx->buf("se_low", sizeof(float2) * tot);
+            OKS_HIP(launch_rows_window(grid, low, planes, ny, nx, (int)p.acs_start, (int)p.acs_end, ctx->stream));
+            ifft2_planes(ctx, low, planes, ny, nx);
+            WalshArgs wa{low, maps, nc, nz, ny, nx, d->walsh_window, d->walsh_iters};
+            OKS_HIP(launch_walsh(wa, ctx->stream));
+        }
         ifft2_planes(ctx, grid, planes, ny, nx);  // aliased coil images
         float* full = (float*)ctx->buf("se_img", sizeof(float) * (size_t)nz * ny * nx);
         float* gfull = (d->compute_gfactor && img->gfactor) ? (float*)ctx->buf("se_g", sizeof(float) * (size_t)nz * ny * nx)
@@ -104,6 +146,25 @@ int oks_sense_unfold_dev(oks_ctx* ctx, const oks_c32* aliased, const oks_c32* ma
         UnfoldArgs ua{(const float2*)aliased, (const float2*)maps, nullptr, (float2*)rho, g, ctx->d_status + 1,
                       (int)nc, 1, (int)ny, (int)nx, ridge};
         OKS_HIP(launch_sense_unfold(ua, (int)r, ctx->stream));
+    });
+}
+
+int oks_espirit_maps_dev(oks_ctx* ctx, const oks_c32* acs, int64_t nc, int64_t acs_y, int64_t acs_x, int64_t ny,
+                         int64_t nx, int64_t kernel, float threshold, int64_t iters, oks_c32* maps, int64_t* kept) {
+    return guarded(ctx, [&] {
+        if (!acs || !maps || nc < 1 || acs_y < 1 || acs_x < 1 || ny < 1 || nx < 1)
+            fail(OKS_E_INVALID_ARG, "bad ESPIRiT arguments");
+        const EspAcs ea{(const float2*)acs, (long long)nc * acs_y * acs_x, (long long)acs_y * acs_x, (int)acs_x,
+                        (int)acs_y, (int)acs_x};
+        espirit_batch(ctx, ea, (int)nc, 1, (int)ny, (int)nx, (int)kernel, threshold, (int)iters, (float2*)maps,
+                      (long long)ny * nx);
+        if (kept) {
+            int h = 0;
+            OKS_HIP(hipMemcpyAsync(&h, ctx->buf("esp_kept", sizeof(int)), sizeof(int), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+            OKS_HIP(hipStreamSynchronize(ctx->stream));
+            *kept = h;
+        }
     });
 }
 

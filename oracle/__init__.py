@@ -46,6 +46,10 @@ def lib():
             getattr(L, f"oko_grappa_synthesize_{s}").argtypes = [_P, _P] + [_I64] * 7
             getattr(L, f"oko_sense_unfold_{s}").argtypes = [_P, _P] + [_I64] * 4 + [C.c_double, _P, _P, _P]
             getattr(L, f"oko_walsh_maps_{s}").argtypes = [_P] + [_I64] * 5 + [_P]
+            getattr(L, f"oko_eigpairs_{s}").argtypes = [_P, _I64, _I64, _I64, _P, _P]
+            getattr(L, f"oko_eigpairs_{s}").restype = _I64
+            getattr(L, f"oko_espirit_maps_{s}").argtypes = [_P] + [_I64] * 6 + [C.c_double, _I64, _P]
+            getattr(L, f"oko_espirit_maps_{s}").restype = _I64
             getattr(L, f"oko_haar_{s}").argtypes = [_P, _I64, _I64, C.c_int, _P]
             getattr(L, f"oko_soft_threshold_{s}").argtypes = [_P, _I64, _I64, C.c_double]
             getattr(L, f"oko_centred_fft2_{s}").argtypes = [_P, _I64, _I64, C.c_int]
@@ -226,6 +230,26 @@ def walsh_maps(img, window, iters, prec="f32"):
     out = np.zeros_like(a)
     getattr(lib(), f"oko_walsh_maps_{prec}")(a.ctypes.data, *a.shape, window, iters, out.ctypes.data)
     return out
+
+
+def eigpairs(a, num, iters, prec="f32"):
+    """top_hermitian_eigpairs (recon/espirit.rs:291-360): (eigenvalues, vectors [k, n])."""
+    m = _c(a, prec)
+    n = m.shape[0]
+    lam = np.zeros(min(num, n), dtype=_rt(prec))
+    vec = np.zeros((min(num, n), n), dtype=m.dtype)
+    k = getattr(lib(), f"oko_eigpairs_{prec}")(m.ctypes.data, n, num, iters, lam.ctypes.data, vec.ctypes.data)
+    return lam[:k], vec[:k]
+
+
+def espirit_maps(acs, ny, nx, kernel, threshold, iters, prec="f32"):
+    """espirit_sensitivity_maps (recon/espirit.rs:52-282): (maps [nc, ny, nx], kept kernels)."""
+    a = _c(acs, prec)
+    nc, ay, ax = a.shape
+    out = np.zeros((nc, ny, nx), dtype=a.dtype)
+    k = getattr(lib(), f"oko_espirit_maps_{prec}")(a.ctypes.data, nc, ay, ax, ny, nx, kernel, threshold, iters,
+                                                  out.ctypes.data)
+    return out, int(k)
 
 
 def haar(x, inverse=False, prec="f32"):

@@ -312,7 +312,7 @@ static int recon_sense(const oks_sense_desc* d, const oks_kspace* in, T* out, T*
         finish(img, g.nz, g.ny, g.nx, d->crop_to_recon_matrix, in, out, shape, flags);
         return OKS_OK;
     }
-    if (d->map_source != OKS_MAPS_WALSH) return OKS_E_UNSUPPORTED;  // ESPIRiT: not built yet
+    const bool espirit = d->map_source == OKS_MAPS_ESPIRIT;
     const int64_t nc = g.nc, ny = g.ny, nx = g.nx, nz = g.nz, plane = ny * nx;
     std::vector<T> img(nz * plane, T(0)), gimg(d->compute_gfactor ? nz * plane : 0, T(0));
     int64_t s3[3] = {nc, ny, nx};
@@ -326,9 +326,23 @@ static int recon_sense(const oks_sense_desc* d, const oks_kspace* in, T* out, T*
                 for (int64_t x = 0; x < nx; ++x) low[c * plane + y * nx + x] = src[y * nx + x];
             std::copy(src, src + plane, &ali[c * plane]);
         }
-        ifft_centered(low.data(), s3, 3, ax2, 2);
-        walsh_maps(low.data(), nc, ny, nx, (int64_t)d->walsh_window, (int64_t)d->walsh_iters,
-                   maps.data());
+        if (espirit) {
+            // the contiguous ACS block [nc, acs_len, nx] of this slice (recon/strategy.rs:673-688)
+            const int64_t al = p.acs_end - p.acs_start;
+            std::vector<cx<T>> acs(nc * al * nx);
+            for (int64_t c = 0; c < nc; ++c)
+                for (int64_t i = 0; i < al; ++i)
+                    for (int64_t x = 0; x < nx; ++x)
+                        acs[(c * al + i) * nx + x] = g.k[((c * nz + kz) * ny + p.acs_start + i) * nx + x];
+            const int64_t kw = d->espirit_kernel;
+            if (kw < 1 || kw > al || kw > nx || kw > ny) return OKS_E_INVALID_ARG;  // the reference asserts
+            espirit_maps(acs.data(), nc, al, nx, ny, nx, kw, (T)d->espirit_threshold, (int64_t)d->espirit_iters,
+                         maps.data());
+        } else {
+            ifft_centered(low.data(), s3, 3, ax2, 2);
+            walsh_maps(low.data(), nc, ny, nx, (int64_t)d->walsh_window, (int64_t)d->walsh_iters,
+                       maps.data());
+        }
         ifft_centered(ali.data(), s3, 3, ax2, 2);
         int64_t failed = 0;
         sense_unfold(ali.data(), maps.data(), nc, ny, nx, p.r, (T)d->ridge, rho.data(),
@@ -463,6 +477,16 @@ OKO_EXPORT int oko_get_threads(void) { return n_threads(); }
                                         int64_t* failed) {                                   \
         return sense_unfold((const cx<T>*)ali, (const cx<T>*)maps, nc, ny, nx, r,            \
                             (T)(float)ridge, (cx<T>*)out, g, failed);                        \
+    }                                                                                        \
+    OKO_EXPORT int64_t oko_eigpairs_##S(const T* a, int64_t n, int64_t num, int64_t iters,    \
+                                        T* lam, T* vec) {                                   \
+        return top_hermitian_eigpairs((const cx<T>*)a, n, num, iters, lam, (cx<T>*)vec);     \
+    }                                                                                        \
+    OKO_EXPORT int64_t oko_espirit_maps_##S(const T* acs, int64_t nc, int64_t ay, int64_t ax, \
+                                            int64_t ny, int64_t nx, int64_t kw, double thr,  \
+                                            int64_t iters, T* maps) {                        \
+        return espirit_maps((const cx<T>*)acs, nc, ay, ax, ny, nx, kw, (T)thr, iters,        \
+                            (cx<T>*)maps);                                                   \
     }                                                                                        \
     OKO_EXPORT void oko_walsh_maps_##S(const T* img, int64_t nc, int64_t ny, int64_t nx,     \
                                        int64_t window, int64_t iters, T* out) {              \

@@ -15,6 +15,7 @@
 // uses rustfft 6.4.1, whose FP32 rounding differs at the 1e-7 level.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -589,6 +590,180 @@ void walsh_maps(const cx<T>* img, int64_t nc, int64_t ny, int64_t nx, int64_t wi
             for (int64_t c = 0; c < nc; ++c) out[(c * ny + y) * nx + x] = v[c] * cx<T>{sc, T(0)};
         }
     });
+}
+
+// Seed angle of eigenpair k, component i (recon/espirit.rs:306): evaluated in
+// T, left to right, no contraction.
+template <class T>
+inline T espirit_seed_theta(int64_t i, int64_t k) {
+    const T a = T(0.37) * (T)i;
+    const T b = T(0.13) + a;
+    const T c = T(0.91) * (T)k;
+    return b + c;
+}
+
+// top_hermitian_eigpairs (recon/espirit.rs:291-360): the `num` largest
+// eigenpairs of a Hermitian n x n matrix by power iteration (Rayleigh
+// quotient, early exit once |dl| < 1e-6 max(|l|, 1e-6) after iteration 2) with
+// Hotelling deflation.  Returns the number of pairs found; lam[k], vec[k*n+i].
+template <class T>
+int64_t top_hermitian_eigpairs(const cx<T>* a, int64_t n, int64_t num, int64_t iters, T* lam, cx<T>* vec) {
+    std::vector<cx<T>> m(a, a + n * n), v(n), w(n);
+    int64_t count = 0;
+    for (int64_t k = 0; k < std::min(num, n); ++k) {
+        for (int64_t i = 0; i < n; ++i) {
+            const T th = espirit_seed_theta<T>(i, k);
+            v[i] = {std::cos(th), std::sin(th)};
+        }
+        T nrm0 = T(0);
+        for (int64_t i = 0; i < n; ++i) nrm0 += norm_sqr(v[i]);
+        nrm0 = std::sqrt(nrm0);
+        if (nrm0 < (T)1e-30) break;
+        for (int64_t i = 0; i < n; ++i) v[i] = v[i] * cx<T>{T(1) / nrm0, T(0)};
+        T lambda = T(0), prev = T(0);
+        for (int64_t it = 0; it < iters; ++it) {
+            parallel_for(n, [&](int64_t i) {
+                cx<T> acc{T(0), T(0)};
+                for (int64_t j = 0; j < n; ++j) acc += m[i * n + j] * v[j];
+                w[i] = acc;
+            });
+            T rq = T(0);
+            for (int64_t i = 0; i < n; ++i) rq += (conj(v[i]) * w[i]).re;
+            T nrm = T(0);
+            for (int64_t i = 0; i < n; ++i) nrm += norm_sqr(w[i]);
+            nrm = std::sqrt(nrm);
+            if (nrm < (T)1e-30) {
+                lambda = T(0);
+                break;
+            }
+            for (int64_t i = 0; i < n; ++i) v[i] = w[i] * cx<T>{T(1) / nrm, T(0)};
+            lambda = rq;
+            if (it > 2 && std::abs(lambda - prev) < (T)1e-6 * std::fmax(std::abs(lambda), (T)1e-6)) break;
+            prev = lambda;
+        }
+        if (lambda <= T(0)) break;
+        parallel_for(n, [&](int64_t i) {
+            const cx<T> li = cx<T>{lambda, T(0)} * v[i];
+            for (int64_t j = 0; j < n; ++j) m[i * n + j] -= li * conj(v[j]);
+        });
+        lam[count] = lambda;
+        std::copy(v.begin(), v.end(), vec + count * n);
+        ++count;
+    }
+    return count;
+}
+
+// espirit_sensitivity_maps (recon/espirit.rs:52-282): acs [nc, ay, ax] ->
+// maps [nc, ny, nx].  Returns the number of kept kernels.
+template <class T>
+int64_t espirit_maps(const cx<T>* acs, int64_t nc, int64_t ay, int64_t ax, int64_t ny, int64_t nx, int64_t kw,
+                     T threshold, int64_t iters, cx<T>* maps) {
+    for (int64_t i = 0; i < nc * ny * nx; ++i) maps[i] = {T(0), T(0)};
+    const int64_t n = nc * kw * kw, wy = ay - kw + 1, wx = ax - kw + 1, rows = wy * wx;
+    // A^H A with A[r][c kw^2 + dy kw + dx] = acs[c][iy + dy][ix + dx], r = iy wx + ix (:80-112)
+    std::vector<cx<T>> aha(n * n, cx<T>{T(0), T(0)});
+    auto A = [&](int64_t r, int64_t col) {
+        const int64_t iy = r / wx, ix = r % wx, c = col / (kw * kw), q = col % (kw * kw);
+        return acs[(c * ay + iy + q / kw) * ax + ix + q % kw];
+    };
+    parallel_for(n, [&](int64_t i) {
+        for (int64_t j = i; j < n; ++j) {
+            cx<T> acc{T(0), T(0)};
+            for (int64_t r = 0; r < rows; ++r) acc += conj(A(r, i)) * A(r, j);
+            aha[i * n + j] = acc;
+            if (i != j) aha[j * n + i] = conj(acc);
+        }
+    });
+    const int64_t max_kept = std::min<int64_t>(n, 256);
+    std::vector<T> lam(max_kept);
+    std::vector<cx<T>> vec(max_kept * n);
+    const int64_t found = top_hermitian_eigpairs(aha.data(), n, max_kept, iters, lam.data(), vec.data());
+    const T smax = found ? std::sqrt(std::fmax(lam[0], T(0))) : T(0);
+    const T cutoff = std::fmax(threshold * smax, T(0));
+    int64_t K = 0;
+    while (K < found && !(std::sqrt(std::fmax(lam[K], T(0))) < cutoff)) ++K;
+    if (K == 0) return 0;
+    // kernel images: centred IFFT2 of the zero-padded conjugate kernel,
+    // times inv_n * sqrt(ny nx) / kw^2 (:141-190)
+    const T scale = std::sqrt((T)ny * (T)nx) / ((T)kw * (T)kw);
+    const T inv_n = T(1) / ((T)ny * (T)nx);
+    const T f = inv_n * scale;
+    Fft<T> fx((int)nx, +1), fy((int)ny, +1);
+    std::vector<cx<T>> img(K * nc * ny * nx);
+    const int64_t y0 = (ny - kw) / 2, x0 = (nx - kw) / 2;
+    parallel_for(K * nc, [&](int64_t kc) {
+        const int64_t k = kc / nc, c = kc % nc;
+        cx<T>* pad = &img[kc * ny * nx];
+        for (int64_t i = 0; i < ny * nx; ++i) pad[i] = {T(0), T(0)};
+        for (int64_t dy = 0; dy < kw; ++dy)
+            for (int64_t dx = 0; dx < kw; ++dx) pad[(y0 + dy) * nx + x0 + dx] = conj(vec[k * n + c * kw * kw + dy * kw + dx]);
+        const int64_t sh[2] = {ny, nx};
+        ifftshift_axis(pad, sh, 2, 0);
+        ifftshift_axis(pad, sh, 2, 1);
+        std::vector<cx<T>> buf(std::max(ny, nx));
+        for (int64_t y = 0; y < ny; ++y) fx.run(pad + y * nx);
+        for (int64_t x = 0; x < nx; ++x) {
+            for (int64_t y = 0; y < ny; ++y) buf[y] = pad[y * nx + x];
+            fy.run(buf.data());
+            for (int64_t y = 0; y < ny; ++y) pad[y * nx + x] = buf[y];
+        }
+        fftshift_axis(pad, sh, 2, 0);
+        fftshift_axis(pad, sh, 2, 1);
+        for (int64_t i = 0; i < ny * nx; ++i) pad[i] = pad[i] * cx<T>{f, T(0)};
+    });
+    // per-voxel dominant eigenvector of G = sum_k m_k m_k^H (:199-272)
+    const int64_t plane = ny * nx;
+    parallel_for(plane, [&](int64_t p) {
+        std::vector<cx<T>> g(nc * nc), v(nc), w(nc);
+        for (int64_t c1 = 0; c1 < nc; ++c1)
+            for (int64_t c2 = 0; c2 < nc; ++c2) {
+                cx<T> acc{T(0), T(0)};
+                for (int64_t k = 0; k < K; ++k) acc += img[(k * nc + c1) * plane + p] * conj(img[(k * nc + c2) * plane + p]);
+                g[c1 * nc + c2] = acc;
+            }
+        T nrm0 = T(0);
+        for (int64_t c = 0; c < nc; ++c) {
+            v[c] = img[c * plane + p];
+            nrm0 += norm_sqr(v[c]);
+        }
+        if (nrm0 < (T)1e-30) {
+            v[0] = {T(1), T(0)};
+            nrm0 = T(1);
+        }
+        const T inv0 = T(1) / std::sqrt(nrm0);
+        for (int64_t c = 0; c < nc; ++c) v[c] = v[c] * cx<T>{inv0, T(0)};
+        T lambda = T(0);
+        for (int64_t it = 0; it < iters; ++it) {
+            for (int64_t c1 = 0; c1 < nc; ++c1) {
+                cx<T> acc{T(0), T(0)};
+                for (int64_t c2 = 0; c2 < nc; ++c2) acc += g[c1 * nc + c2] * v[c2];
+                w[c1] = acc;
+            }
+            T nrm = T(0);
+            for (int64_t c = 0; c < nc; ++c) nrm += norm_sqr(w[c]);
+            nrm = std::sqrt(nrm);
+            if (nrm < (T)1e-30) break;
+            const T inv = T(1) / nrm;
+            for (int64_t c = 0; c < nc; ++c) v[c] = w[c] * cx<T>{inv, T(0)};
+            lambda = nrm;
+        }
+        int64_t imax = 0;
+        T mmax = T(0);
+        for (int64_t c = 0; c < nc; ++c) {
+            const T mm = cnorm(v[c]);
+            if (mm > mmax) {
+                mmax = mm;
+                imax = c;
+            }
+        }
+        if (mmax > T(0)) {
+            const cx<T> ph{v[imax].re / mmax, -v[imax].im / mmax};
+            for (int64_t c = 0; c < nc; ++c) v[c] = v[c] * ph;
+        }
+        const T sc = std::sqrt(lambda);
+        for (int64_t c = 0; c < nc; ++c) maps[c * plane + p] = v[c] * cx<T>{sc, T(0)};
+    });
+    return K;
 }
 
 // ------------------------------------------------------------------- CS

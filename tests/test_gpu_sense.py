@@ -169,3 +169,79 @@ def test_sense_c4_shape_fallback(ctx):
     got, flags, _ = recon_dev(ctx, "oks_recon_sense_dev", d, case)
     assert st == 0 and flags == rflags and flags & abi.F_SENSE_FALLBACK
     assert_parity(got, r32, r64, TOL, "C4 sense fallback")
+
+
+# ------------------------------------------------------------------ ESPIRiT
+def gpu_espirit(ctx, acs, ny, nx, kernel, thr, iters):
+    acs = np.ascontiguousarray(acs, dtype=np.complex64)
+    nc, ay, ax = acs.shape
+    da = DeviceArray.from_host(ctx, acs)
+    dm = DeviceArray(ctx, (nc, ny, nx), np.complex64)
+    kept = np.zeros(1, np.int64)
+    check(ctx.ptr, lib().oks_espirit_maps_dev(ctx.ptr, da.addr, nc, ay, ax, ny, nx, kernel, C.c_float(thr), iters,
+                                              dm.addr, kept.ctypes.data))
+    ctx.synchronize()
+    out = dm.download()
+    da.free()
+    dm.free()
+    return out, int(kept[0])
+
+
+def _cfft2(img):
+    return np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(img, axes=(-2, -1))), axes=(-2, -1))
+
+
+def test_espirit_vs_oracle(ctx):
+    # ACS rows of a synthetic acquisition, as SenseRss hands them over
+    case = synth.make_case(nc=5, nz=1, ny=48, nx=40, kind="full", seed=31, ellipses=5)
+    acs = case["kspace"][:, 0, 14:34, :]
+    got, kept = gpu_espirit(ctx, acs, 48, 40, 5, 0.02, 30)
+    r32, k32 = O.espirit_maps(acs, 48, 40, 5, 0.02, 30, "f32")
+    r64, k64 = O.espirit_maps(acs, 48, 40, 5, 0.02, 30, "f64")
+    assert kept == k64
+    assert_parity(got, r32, r64, TOL, f"espirit ({kept} kernels)")
+
+
+def test_espirit_known_answers(ctx):
+    # recon/espirit.rs:387 on the GPU: magnitude ratio of two Gaussian coils
+    ny = nx = 32
+    yy, xx = np.meshgrid(np.arange(ny) - ny / 2.0, np.arange(nx), indexing="ij")
+    s1 = np.exp(-(yy ** 2 + (xx - nx / 3.0) ** 2) / 250.0)
+    s2 = np.exp(-(yy ** 2 + (xx - 2.0 * nx / 3.0) ** 2) / 250.0)
+    ph = np.zeros((ny, nx))
+    ph[6:26, 6:26] = 1.0
+    k = _cfft2(np.stack([s1 * ph, s2 * ph]).astype(np.complex64))
+    maps, _ = gpu_espirit(ctx, k[:, 8:24, 8:24], ny, nx, 5, 0.02, 40)
+    m0, m1 = np.abs(maps[0, 8:24, 8:24]), np.abs(maps[1, 8:24, 8:24])
+    ok = m0 >= 1e-4
+    want = (s2 / s1)[8:24, 8:24][ok]
+    assert np.mean(np.abs(m1[ok] / m0[ok] - want) / np.maximum(want, 1e-6)) < 0.05
+    # constant complex coil weights: maps proportional to the weights, phase included
+    rng = np.random.default_rng(7)
+    obj = np.zeros((ny, nx))
+    obj[8:24, 6:26] = 1.0 + 0.3 * rng.random((16, 20))
+    wts = np.array([1.0, 0.6 * np.exp(0.7j), 0.3 * np.exp(-1.9j)])
+    k = _cfft2((wts[:, None, None] * obj[None]).astype(np.complex64))
+    maps, _ = gpu_espirit(ctx, k[:, 8:24, 8:24], ny, nx, 5, 0.02, 40)
+    ratio = maps[1:, 12:20, 10:22] / maps[0, 12:20, 10:22]
+    assert np.max(np.abs(ratio - (wts[1:] / wts[0])[:, None, None])) < 2e-3
+
+
+def test_espirit_argument_errors(ctx):
+    acs = np.ones((2, 4, 16), np.complex64)
+    with pytest.raises(OksError) as e:
+        gpu_espirit(ctx, acs, 16, 16, 5, 0.02, 10)  # kernel taller than the ACS block
+    assert e.value.code == abi.OKS_E_INVALID_ARG
+
+
+@pytest.mark.parametrize("r,gfac", [(2, 1), (3, 0)])
+def test_sense_strategy_espirit(ctx, r, gfac):
+    case = _scase(nc=6, r=r, acs=24)
+    d = abi.SenseDesc.default(map_source=abi.MAPS_ESPIRIT, compute_gfactor=gfac)
+    st, r32, rflags, g32 = O.recon("sense", d, case, "f32")
+    _, r64, _, g64 = O.recon("sense", d, case, "f64")
+    got, flags, g = recon_dev(ctx, "oks_recon_sense_dev", d, case)
+    assert st == 0 and flags == rflags and not flags & abi.F_SENSE_FALLBACK
+    assert_parity(got, r32, r64, TOL, f"sense espirit R={r}")
+    if gfac:
+        assert_parity(g, g32, g64, TOL, f"sense espirit g R={r}")
