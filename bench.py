@@ -117,7 +117,27 @@ def workload_c3():
     )
 
 
-WORKLOADS = {"c1": workload_c1, "c2": workload_c2, "c3": workload_c3}
+def workload_c4():
+    from openkspace_amd import abi, synth
+    case = synth.make_config("c4_sense3d")
+    nc, nz, ny, nx = case["kspace"].shape
+    desc = abi.SenseDesc.default(map_source=abi.MAPS_ESPIRIT, espirit_kernel=6, espirit_threshold=0.02,
+                                 espirit_iters=30, ridge=1e-4, compute_gfactor=1)
+    # Ny = 256 is not divisible by R = 3: the reference falls back to the 3-D
+    # IFFT + RSS (recon/strategy.rs:631-647).  Bytes: read the grid, write the volume.
+    algo_bytes = nc * nz * ny * nx * 8 + nz * ny * nx * 4
+    return dict(
+        name="c4_sense3d",
+        config={"workload": "c4_sense3d", "volumes": 1, "coils": nc, "kx": nx, "ky": ny, "kz": nz, "R": 3,
+                "acs_lines": 32, "strategy": "SenseRss", "maps": "ESPIRiT (kernel 6, threshold 0.02, 30 it)",
+                "ridge": 1e-4, "gfactor": True, "path_taken": "fallback 3-D IFFT + RSS (Ny % R != 0)"},
+        case=case, desc=desc, fn="oks_recon_sense_dev", oracle="sense", slices=nz,
+        bound="hbm", algo=algo_bytes, kernel="SenseRss launch sequence (kz IFFT, ky/kx IFFT, RSS)",
+        cpu_unit=None, dtype="c64/f32",
+    )
+
+
+WORKLOADS = {"c1": workload_c1, "c2": workload_c2, "c3": workload_c3, "c4": workload_c4}
 
 # HBM bytes the dominant kernels move per algorithmic unit, measured with the
 # FETCH_SIZE / WRITE_SIZE counters scaled by a copy kernel of known size
@@ -352,6 +372,18 @@ def cpu_baseline(wl, seconds):
     import oracle as O
     O.set_threads(1)
     case = wl["case"]
+    if wl["cpu_unit"] is None:
+        # one 3-D volume cannot be cut into slices: time the oracle on the
+        # same volume with kz reduced to 8 partitions, per slice
+        from openkspace_amd import synth
+        sub = synth.make_config(wl["name"], nz=8)
+        t0 = time.perf_counter()
+        st, _, _, _ = O.recon(wl["oracle"], wl["desc"], sub, "f32")
+        dt = time.perf_counter() - t0
+        assert st == 0
+        return {"value": 8 / dt, "unit": "slices/s", "cores": 1, "kind": "port",
+                "sample": f"the same acquisition with 8 kz partitions through the FP32 oracle, single thread, "
+                          f"{dt:.1f} s"}
     nz = case["kspace"].shape[1]
     done, t0 = 0, time.perf_counter()
     while done < nz:

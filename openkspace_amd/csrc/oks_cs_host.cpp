@@ -178,18 +178,25 @@ void recon_cs_dev(oks_ctx* ctx, const oks_cs_desc* d, const oks_kspace* k, oks_i
     const float2* W = d->prewhiten ? whitener(ctx, (const float2*)k->noise, k->nc, k->n_noise, &flags, "W") : nullptr;
     int os_mode = 0, rec = 0;
     resolve_os(k, d->remove_oversampling, &os_mode, &rec, &flags);
-    float2* grid = prep_grid(ctx, k, W, os_mode, rec, "grid");
     const int nc = (int)k->nc, nz = (int)k->nz, ny = (int)k->ny, nx = (int)k->nx;
-    if (three_d_of(d->fft_mode, k)) {
-        int64_t shape[4] = {nc, nz, ny, nx};
-        if (nz >= 2) fft_axis(ctx, grid, shape, 4, 1, true, 1.0f / (float)nz, true);  // ifft1 along kz
-    }
+    const bool td = three_d_of(d->fft_mode, k) && nz >= 2;
     Win w[3];
     output_windows(k, d->crop_to_recon_matrix, w, &flags);
     if (ny % 2 != 0 || nx % 2 != 0) {
-        flags |= OKS_F_CS_FALLBACK;  // recon/strategy.rs:829-842
-        ifft_rss_grid(ctx, grid, nc, nz, ny, nx, nullptr, 0, 0, false, w, img->data);
+        // recon/strategy.rs:829-842: (kz IFFT,) 2-D IFFT + RSS, fused with the whitening
+        flags |= OKS_F_CS_FALLBACK;
+        if (os_mode) {  // never-acquired rows are zeroed by the reader closure, not by the fused row pass
+            const float2* g = prep_grid(ctx, k, W, os_mode, rec, "grid");
+            ifft_rss_grid(ctx, g, nc, nz, ny, nx, nullptr, 0, 0, td, w, img->data);
+        } else {
+            ifft_rss_grid(ctx, (const float2*)k->kspace, nc, nz, ny, nx, W, 0, 0, td, w, img->data);
+        }
     } else {
+        float2* grid = prep_grid(ctx, k, W, os_mode, rec, "grid");
+        if (td) {
+            int64_t shape[4] = {nc, nz, ny, nx};
+            fft_axis(ctx, grid, shape, 4, 1, true, 1.0f / (float)nz, true);  // ifft1 along kz
+        }
         const size_t cells = (size_t)nz * ny * nx;
         if (os_mode) {
             ctx->h_mask = os_mask(k, rec);

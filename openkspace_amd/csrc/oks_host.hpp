@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstring>
 #include <vector>
 
 #include "../../include/oks.h"
@@ -78,19 +79,27 @@ inline bool pattern_detect(const std::vector<uint8_t>& any, oks_pattern* p) {
     return true;
 }
 
+// True when any of the n mask bytes at m is non-zero.  Never-acquired rows
+// have to be read to the end anyway, so the row is ORed 8 bytes at a time
+// without an early exit, which the compiler vectorises.
+inline bool row_any(const uint8_t* m, int64_t n) {
+    uint64_t acc = 0;
+    int64_t x = 0;
+    for (; x + 8 <= n; x += 8) {
+        uint64_t w;
+        std::memcpy(&w, m + x, sizeof(w));
+        acc |= w;
+    }
+    for (; x < n; ++x) acc |= m[x];
+    return acc != 0;
+}
+
 // ky projection of a [nz, ny, nx] mask (recon/grappa.rs:499-509).
 inline std::vector<uint8_t> ky_any(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx) {
     std::vector<uint8_t> any(ny, 0);
     for (int64_t z = 0; z < nz; ++z)
-        for (int64_t y = 0; y < ny; ++y) {
-            if (any[y]) continue;
-            const uint8_t* m = mask + (z * ny + y) * nx;
-            for (int64_t x = 0; x < nx; ++x)
-                if (m[x]) {
-                    any[y] = 1;
-                    break;
-                }
-        }
+        for (int64_t y = 0; y < ny; ++y)
+            if (!any[y]) any[y] = row_any(mask + (z * ny + y) * nx, nx);
     return any;
 }
 

@@ -66,20 +66,30 @@ void recon_sense_dev(oks_ctx* ctx, const oks_sense_desc* d, const oks_kspace* k,
     const float2* W = d->prewhiten ? whitener(ctx, (const float2*)k->noise, k->nc, k->n_noise, &flags, "W") : nullptr;
     int os_mode = 0, rec = 0;
     resolve_os(k, d->remove_oversampling, &os_mode, &rec, &flags);
-    float2* grid = prep_grid(ctx, k, W, os_mode, rec, "grid");
     const int nc = (int)k->nc, nz = (int)k->nz, ny = (int)k->ny, nx = (int)k->nx;
-    if (three_d_of(d->fft_mode, k) && nz >= 2) {
-        int64_t shape[4] = {nc, nz, ny, nx};
-        fft_axis(ctx, grid, shape, 4, 1, true, 1.0f / (float)nz, true);  // ifft1 along kz (recon/strategy.rs:594-597)
-    }
+    const bool td = three_d_of(d->fft_mode, k) && nz >= 2;
     Win w[3];
     output_windows(k, d->crop_to_recon_matrix, w, &flags);
+    // The fallback decision depends on the mask only, so it is taken before
+    // the data is touched; the fallback (recon/strategy.rs:602-647: kz IFFT,
+    // 2-D IFFT, RSS) then runs as one fused IFFT + RSS with the whitening in
+    // its row pass.
     oks_pattern p;
     const bool have = detect_pattern(k->mask, k->nz, k->ny, k->nx, &p);
-    if (!have || ny % p.r != 0) {  // recon/strategy.rs:602-647
+    if (!have || ny % p.r != 0) {
         flags |= OKS_F_SENSE_FALLBACK | (have ? 0u : (uint32_t)OKS_F_NO_PATTERN);
-        ifft_rss_grid(ctx, grid, nc, nz, ny, nx, nullptr, 0, 0, false, w, img->data);
+        if (os_mode) {  // never-acquired rows are zeroed by the reader closure, not by the fused row pass
+            const float2* g = prep_grid(ctx, k, W, os_mode, rec, "grid");
+            ifft_rss_grid(ctx, g, nc, nz, ny, nx, nullptr, 0, 0, td, w, img->data);
+        } else {
+            ifft_rss_grid(ctx, (const float2*)k->kspace, nc, nz, ny, nx, W, 0, 0, td, w, img->data);
+        }
     } else {
+        float2* grid = prep_grid(ctx, k, W, os_mode, rec, "grid");
+        if (td) {
+            int64_t shape[4] = {nc, nz, ny, nx};
+            fft_axis(ctx, grid, shape, 4, 1, true, 1.0f / (float)nz, true);  // ifft1 along kz (recon/strategy.rs:594-597)
+        }
         const long long planes = (long long)nc * nz;
         const size_t tot = (size_t)planes * ny * nx;
         float2* maps = (float2*)ctx->buf("se_maps", sizeof(float2) * tot);

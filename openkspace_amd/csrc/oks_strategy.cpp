@@ -145,14 +145,7 @@ float2* prep_grid(oks_ctx* ctx, const oks_kspace* k, const float2* W, int os_mod
     if (os_mode) {
         const int64_t rows = k->nz * k->ny;
         ctx->h_acq.assign(rows, 0);
-        for (int64_t row = 0; row < rows; ++row) {
-            const uint8_t* m = k->mask + row * k->nx;
-            for (int64_t x = 0; x < k->nx; ++x)
-                if (m[x]) {
-                    ctx->h_acq[row] = 1;
-                    break;
-                }
-        }
+        for (int64_t row = 0; row < rows; ++row) ctx->h_acq[row] = row_any(k->mask + row * k->nx, k->nx);
         uint8_t* acq = (uint8_t*)ctx->buf("prep_acq", rows);
         OKS_HIP(hipMemcpyAsync(acq, ctx->h_acq.data(), rows, hipMemcpyHostToDevice, ctx->stream));
         RowPwArgs a{};
@@ -183,10 +176,7 @@ std::vector<uint8_t> os_mask(const oks_kspace* k, int rec) {
     std::vector<uint8_t> m((size_t)k->nz * k->ny * k->nx, 0);
     const int64_t dst = k->nx / 2 - rec / 2;
     for (int64_t row = 0; row < k->nz * k->ny; ++row) {
-        const uint8_t* src = k->mask + row * k->nx;
-        bool acq = false;
-        for (int64_t x = 0; x < k->nx && !acq; ++x) acq = src[x] != 0;
-        if (acq)
+        if (row_any(k->mask + row * k->nx, k->nx))
             for (int64_t x = dst; x < dst + rec; ++x) m[row * k->nx + x] = 1;
     }
     return m;

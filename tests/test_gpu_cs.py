@@ -123,6 +123,19 @@ def test_cs_strategy_small(ctx, prewhiten):
     assert_parity(got, r32, r64, TOL, f"cs pw={prewhiten}", iterative=True)
 
 
+@pytest.mark.parametrize("three_d,os_x", [(False, 2), (True, 1), (True, 2)])
+def test_cs_fallback_fused(ctx, three_d, os_x):
+    # odd ny: whitened, (oversampling-removed,) kz-transformed IFFT + RSS in one pass
+    case = synth.make_case(nc=4, nz=4 if three_d else 2, ny=33, nx=20 * os_x, os_x=os_x, kind="cs",
+                           three_d=three_d, seed=13, ellipses=4)
+    d = abi.CsDesc.default(iters=5, prewhiten=1, remove_oversampling=1)
+    st, r32, rflags, _ = O.recon("cs", d, case, "f32")
+    _, r64, _, _ = O.recon("cs", d, case, "f64")
+    got, flags, _ = recon_dev(ctx, "oks_recon_cs_dev", d, case)
+    assert st == 0 and flags == rflags and flags & abi.F_CS_FALLBACK
+    assert_parity(got, r32, r64, TOL, f"cs fallback 3d={three_d} os={os_x}")
+
+
 def test_cs_strategy_3d_and_odd_fallback(ctx):
     case = synth.make_case(nc=3, nz=6, ny=32, nx=32, kind="cs", three_d=True, seed=3, ellipses=4)
     d = abi.CsDesc.default(iters=15)

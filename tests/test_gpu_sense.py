@@ -146,6 +146,20 @@ def test_sense_fallbacks(ctx):
     assert rel(got, ref) < 2e-5
 
 
+@pytest.mark.parametrize("three_d,os_x,pw", [(False, 2, 1), (True, 1, 1), (True, 2, 1), (True, 1, 0)])
+def test_sense_fallback_fused(ctx, three_d, os_x, pw):
+    # indivisible ny: the fallback runs whitening, (oversampling removal,) kz
+    # IFFT, 2-D IFFT and RSS as one fused pass over the raw k-space
+    case = synth.make_case(nc=5, nz=4 if three_d else 2, ny=50, nx=24 * os_x, os_x=os_x, kind="grappa", r=3,
+                           acs=16, three_d=three_d, seed=11, ellipses=4)
+    d = abi.SenseDesc.default(prewhiten=pw, remove_oversampling=1)
+    st, r32, rflags, _ = O.recon("sense", d, case, "f32")
+    _, r64, _, _ = O.recon("sense", d, case, "f64")
+    got, flags, _ = recon_dev(ctx, "oks_recon_sense_dev", d, case)
+    assert st == 0 and flags == rflags and flags & abi.F_SENSE_FALLBACK
+    assert_parity(got, r32, r64, TOL, f"sense fallback 3d={three_d} os={os_x} pw={pw}")
+
+
 def test_sense_3d_kz_decoupling(ctx):
     case = synth.make_case(nc=6, nz=8, ny=48, nx=32, kind="grappa", r=2, acs=16, three_d=True, seed=5, ellipses=4)
     d = abi.SenseDesc.default()
