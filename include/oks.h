@@ -174,7 +174,10 @@ void* oks_ctx_stream(oks_ctx* ctx);
  * IFFT+RSS chunk, >= 1), "time_stage" (an oks_stage, below), and the CS
  * column-pass launch shape "cs_sx" (columns per workgroup: 0 = automatic,
  * 4, 8, 16, 32) and "cs_threads" (128, 256, 512).  Launch shapes change
- * speed only, never results. */
+ * speed only, never results.  "rows_pipe" (0/1, default 1) selects the
+ * pipelined IFFT+RSS row pass, which whitens after the kx IFFT instead of
+ * before it (the same linear map, FP32 rounding in another order);
+ * "cols_pipe" (0/1, default 1) lets column passes prefetch the next coil. */
 int oks_ctx_set_option(oks_ctx* ctx, const char* name, int64_t value);
 int oks_ctx_synchronize(oks_ctx* ctx);
 /* Flags raised on the device by the *_dev calls since the last take

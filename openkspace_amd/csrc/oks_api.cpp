@@ -266,6 +266,9 @@ int oks_ctx_set_option(oks_ctx* ctx, const char* name, int64_t value) {
         } else if (n == "cs_chunk") {
             if (value < 0) fail(OKS_E_INVALID_ARG, "cs_chunk must be >= 0");
             ctx->cs_chunk = value;
+        } else if (n == "rows_pipe" || n == "cols_pipe") {
+            if (value != 0 && value != 1) fail(OKS_E_INVALID_ARG, n + " must be 0 or 1");
+            (n == "rows_pipe" ? ctx->rows_pipe : ctx->cols_pipe) = (int)value;
         } else if (n == "zchunk") {
             if (value < 1) fail(OKS_E_INVALID_ARG, "zchunk must be >= 1");
             ctx->zchunk = (int)value;

@@ -32,8 +32,13 @@ namespace oks {
 // (plane, strip of SX columns); the strip of T is staged in LDS with
 // row-segment loads of SX * 8 bytes, the measured k-space and mask are read
 // per column from their column-major copies (contiguous along the transform).
+// Largest workgroup a column kernel is built for: 512 for the lengths whose
+// transform needs more than the 128 registers per lane a 1024-thread
+// workgroup leaves (they would spill to scratch).
+constexpr int cs_cols_max_threads(int n) { return (n == 368 || n == 640) ? 512 : 1024; }
+
 template <class F>
-__global__ void __launch_bounds__(1024) k_cs_cols(CsColArgs a) {
+__global__ void __launch_bounds__(cs_cols_max_threads(F::n)) k_cs_cols(CsColArgs a) {
     extern __shared__ float2 lds[];
     const int n = F::n ? F::n : a.p.n;
     const int SX = a.sx, ls = n + 1;
@@ -308,7 +313,8 @@ template <class F>
 static hipError_t go_cs_cols(const CsColArgs& a, hipStream_t s) {
     const size_t lds = cs_cols_lds(a.sx, a.p.n);
     const unsigned grid = (unsigned)(a.planes * ((a.nx + a.sx - 1) / a.sx));
-    const int threads = (a.threads == 1024 || a.threads == 256 || a.threads == 128) ? a.threads : 512;
+    int threads = (a.threads == 1024 || a.threads == 256 || a.threads == 128) ? a.threads : 512;
+    threads = std::min(threads, cs_cols_max_threads(F::n));
     OKS_ATTR2(hipFuncSetAttribute((const void*)k_cs_cols<F>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_cs_cols<F>, dim3(grid), dim3(threads), lds, s, a);
     return hipGetLastError();

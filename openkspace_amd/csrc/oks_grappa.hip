@@ -475,38 +475,46 @@ __global__ void __launch_bounds__(256) k_normal64(Normal64Args a) {
 }
 
 // Cholesky of the diagonal block G[k0:k0+kn, k0:k0+kn] (lower) and its
-// explicit inverse (recon/prewhiten.rs:148-210, in FP64).  256 threads; thread
-// (ti, tc) keeps the 4 x 4 elements (ti + 16 a, tc + 16 b) of the block in
-// registers.  Step j: the owners of column j publish it (scaled) to LDS, every
-// thread updates its elements with j < c <= i.  The inverse then goes row by
-// row with each column's dot product split over 4 lanes.  A non-PD pivot
-// marks the slice failed (the reference leaves it undersampled).
-__global__ void __launch_bounds__(256) k_potrf64_diag(Potrf64Args a) {
+// explicit inverse (recon/prewhiten.rs:148-210, in FP64).  1024 threads;
+// thread (ti, tc) keeps the 2 x 2 elements (ti + 32 a, tc + 32 b) of the
+// block in registers.  Step j: the owners of column j publish it to LDS (two
+// buffers alternate, so one barrier per step), every thread updates its
+// elements with j < c <= i.  The inverse is built by doubling: four
+// wavefronts invert the 16 x 16 diagonal blocks by forward substitution, then
+//   [[A, 0], [B, C]]^-1 = [[A^-1, 0], [-C^-1 B A^-1, C^-1]]
+// joins them into 32 x 32 and 64 x 64 blocks, with B A^-1 parked in the
+// (zero) upper triangle of the inverse.  A non-PD pivot marks the slice
+// failed (the reference leaves it undersampled).
+constexpr int kPotrfThreads = 1024;
+
+__global__ void __launch_bounds__(kPotrfThreads) k_potrf64_diag(Potrf64Args a) {
     __shared__ double2 L[64][65];
     __shared__ double2 I[64][65];
     __shared__ double2 colb[2][64];
+    __shared__ double dinv[64];
     const int z = blockIdx.x;
     if (a.skip[z]) return;
     const int n = a.n, k0 = a.k0, kn = a.kn, t = threadIdx.x;
-    const int ti = t >> 4, tc = t & 15;
+    const int ti = t >> 5, tc = t & 31;
     const double2* G = a.G + (size_t)z * n * n;
     double2* Lg = a.L + (size_t)z * n * n;
     double2* D = a.D + ((size_t)z * a.nblk + a.kb) * 64 * 64;
-    double2 r[4][4];
+    double2 r[2][2];
 #pragma unroll
-    for (int ia = 0; ia < 4; ++ia)
+    for (int ia = 0; ia < 2; ++ia)
 #pragma unroll
-        for (int ib = 0; ib < 4; ++ib) {
-            const int i = ti + 16 * ia, c = tc + 16 * ib;
+        for (int ib = 0; ib < 2; ++ib) {
+            const int i = ti + 32 * ia, c = tc + 32 * ib;
             r[ia][ib] = (i < kn && c <= i) ? G[(size_t)(k0 + i) * n + k0 + c] : make_double2(0.0, 0.0);
         }
-    for (int e = t; e < 64 * 64; e += 256) I[e / 64][e % 64] = make_double2(0.0, 0.0);
-    // Column 0 is published by its owners; afterwards the owners of column
-    // j + 1 publish it right after their step-j update, so each step needs
-    // one barrier (two column buffers alternate).
+    for (int e = t; e < 64 * 64; e += kPotrfThreads) {
+        L[e / 64][e % 64] = make_double2(0.0, 0.0);
+        I[e / 64][e % 64] = make_double2(0.0, 0.0);
+    }
+    if (t < 64) dinv[t] = 0.0;
     if (tc == 0)
 #pragma unroll
-        for (int ia = 0; ia < 4; ++ia) colb[0][ti + 16 * ia] = r[ia][0];
+        for (int ia = 0; ia < 2; ++ia) colb[0][ti + 32 * ia] = r[ia][0];
     __syncthreads();
     int bad = 0;
     for (int j = 0; j < kn; ++j) {
@@ -515,17 +523,22 @@ __global__ void __launch_bounds__(256) k_potrf64_diag(Potrf64Args a) {
         bad |= (!(d > 0.0) || !isfinite(d));
         const double ljj = sqrt(fmax(d, 1e-300)), inv = 1.0 / ljj;
         // scaled column j: L[i][j] = col[i] / ljj (i > j), L[j][j] = ljj
-        if (t < 64) L[t][j] = (t > j && t < kn) ? make_double2(col[t].x * inv, col[t].y * inv)
-                                                : (t == j ? make_double2(ljj, 0.0) : make_double2(0.0, 0.0));
+        if (t < 64) {
+            if (t > j && t < kn) L[t][j] = make_double2(col[t].x * inv, col[t].y * inv);
+            if (t == j) {
+                L[j][j] = make_double2(ljj, 0.0);
+                dinv[j] = inv;
+            }
+        }
         const double s2 = inv * inv;
 #pragma unroll
-        for (int ia = 0; ia < 4; ++ia) {
-            const int i = ti + 16 * ia;
+        for (int ia = 0; ia < 2; ++ia) {
+            const int i = ti + 32 * ia;
             if (i <= j || i >= kn) continue;
             const double2 li = col[i];
 #pragma unroll
-            for (int ib = 0; ib < 4; ++ib) {
-                const int c = tc + 16 * ib;
+            for (int ib = 0; ib < 2; ++ib) {
+                const int c = tc + 32 * ib;
                 if (c > j && c <= i) {
                     const double2 p = zmul(li, zconj(col[c]));
                     r[ia][ib].x -= p.x * s2;
@@ -534,24 +547,24 @@ __global__ void __launch_bounds__(256) k_potrf64_diag(Potrf64Args a) {
             }
         }
         const int jn = j + 1;
-        if (jn < kn && tc == (jn & 15)) {
-            const int jb = jn >> 4;
+        if (jn < kn && tc == (jn & 31)) {
+            const int jb = jn >> 5;
 #pragma unroll
-            for (int ib = 0; ib < 4; ++ib)
+            for (int ib = 0; ib < 2; ++ib)
                 if (ib == jb)
 #pragma unroll
-                    for (int ia = 0; ia < 4; ++ia) colb[jn & 1][ti + 16 * ia] = r[ia][ib];
+                    for (int ia = 0; ia < 2; ++ia) colb[jn & 1][ti + 32 * ia] = r[ia][ib];
         }
         __syncthreads();
     }
-    // I = L^-1 by forward substitution.  Columns are independent: four lanes
-    // of one wavefront own column c and split each dot product, so no
-    // workgroup barrier is needed between rows.
-    {
-        const int c = t >> 2, part = t & 3;
-        for (int i = c; i < kn; ++i) {
+    // Level 0: wavefront b inverts the 16 x 16 block at 16 b; four lanes own
+    // one column and split its dot products.
+    const int w = t >> 6, lane = t & 63;
+    if (w < 4) {
+        const int b0 = 16 * w, c = b0 + (lane >> 2), part = lane & 3;
+        const int end = min(b0 + 16, kn);
+        for (int i = c; i < end; ++i) {
             double2 acc = make_double2(0.0, 0.0);
-#pragma unroll 4
             for (int j = c + part; j < i; j += 4) {
                 const double2 p = zmul(L[i][j], I[j][c]);
                 acc.x += p.x;
@@ -561,20 +574,51 @@ __global__ void __launch_bounds__(256) k_potrf64_diag(Potrf64Args a) {
             acc.y += __shfl_xor(acc.y, 1);
             acc.x += __shfl_xor(acc.x, 2);
             acc.y += __shfl_xor(acc.y, 2);
-            if (part == 0) {
-                const double dd = L[i][i].x;
-                const double re = (c == i ? 1.0 : 0.0) - acc.x, im = -acc.y;
-                I[i][c] = make_double2(re / dd, im / dd);
-            }
+            if (part == 0) I[i][c] = make_double2(((c == i ? 1.0 : 0.0) - acc.x) * dinv[i], -acc.y * dinv[i]);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         }
     }
     __syncthreads();
-    for (int e = t; e < 64 * 64; e += 256) {
+    // Levels 1 and 2: join blocks of size h into blocks of size 2h.  With
+    // A = I[o..o+h), C = I[o+h..o+2h) already inverted and B = L[o+h.., o..]:
+    //   T = B A^-1 (A^-1 lower: k >= q), stored at I[o + p][o + h + q];
+    //   I[o + h + p][o + q] = -sum_{k <= p} C^-1[p][k] T[k][q].
+    for (int h = 16; h < 64; h *= 2) {
+        const int nb = 64 / (2 * h);        // blocks joined at this level
+        const int outs = nb * h * h;        // 512 (h = 16) or 1024 (h = 32)
+        int o = 0, p = 0, q = 0;
+        const bool act = t < outs;
+        if (act) {
+            const int blk = t / (h * h), e = t - blk * h * h;
+            o = blk * 2 * h;
+            p = e / h;
+            q = e - p * h;
+            double2 acc = make_double2(0.0, 0.0);
+            for (int k = q; k < h; ++k) {
+                const double2 v = zmul(L[o + h + p][o + k], I[o + k][o + q]);
+                acc.x += v.x;
+                acc.y += v.y;
+            }
+            I[o + p][o + h + q] = acc;
+        }
+        __syncthreads();
+        if (act) {
+            double2 acc = make_double2(0.0, 0.0);
+            for (int k = 0; k <= p; ++k) {
+                const double2 v = zmul(I[o + h + p][o + h + k], I[o + k][o + h + q]);
+                acc.x += v.x;
+                acc.y += v.y;
+            }
+            I[o + h + p][o + q] = make_double2(-acc.x, -acc.y);
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < 64 * 64; e += kPotrfThreads) {
         const int i = e / 64, j = e % 64;
-        D[e] = (i < kn && j < kn) ? I[i][j] : make_double2(0.0, 0.0);
+        const bool in = i < kn && j < kn && j <= i;
+        D[e] = in ? I[i][j] : make_double2(0.0, 0.0);
         if (i < kn && j < kn) Lg[(size_t)(k0 + i) * n + k0 + j] = (j <= i) ? L[i][j] : make_double2(0.0, 0.0);
     }
     if (t == 0 && bad) {
@@ -648,7 +692,7 @@ hipError_t launch_normal64(const Normal64Args& a, int nz, hipStream_t s) {
 }
 
 hipError_t launch_potrf64_diag(const Potrf64Args& a, int nz, hipStream_t s) {
-    hipLaunchKernelGGL(k_potrf64_diag, dim3(nz), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_potrf64_diag, dim3(nz), dim3(kPotrfThreads), 0, s, a);
     return hipGetLastError();
 }
 

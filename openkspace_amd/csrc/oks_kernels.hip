@@ -6,10 +6,18 @@
 // complex64, image [nz, ty, tx] float32.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "oks_fft.hpp"
 #include "oks_kernels.hpp"
 
 namespace oks {
+
+// Workgroup barrier that orders LDS accesses only.  Global loads issued
+// before it stay in flight across it (__syncthreads() also waits for them),
+// which is what lets a workgroup prefetch its next input during a transform.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 
 #define OKS_ATTR(x)                         \
     do {                                    \
@@ -178,6 +186,125 @@ __global__ void __launch_bounds__(512) k_rows_pw(RowPwArgs a) {
     }
 }
 
+// Pipelined row pass for the fixed transform lengths.  Persistent workgroups
+// walk rows blockIdx.x, blockIdx.x + gridDim.x, ...; while the coils of one
+// row are transformed in LDS, the raw samples of the next row are already on
+// their way into registers.  The whitening y = W s (recon/prewhiten.rs:128-141)
+// is applied after the kx IFFT, to the kept columns only: both are linear
+// maps on different axes, so they commute.  In the output phase P thread
+// groups share each kept column, group g taking the coil rows [b_g, b_g+1)
+// chosen so the lower-triangular multiply counts balance; a thread forms 8
+// output rows at a time from the transformed coils in LDS and W^T in LDS.
+constexpr int kRowsPipeThreads = 1024;
+
+// Lengths the pipelined kernel is built for.  368 and 640 are left to the
+// one-row-per-workgroup kernel: their transforms need more registers than a
+// 1024-thread workgroup leaves, and on 640-point rows with 15 coils the
+// pipelined form measured 20 % slower.
+#define OKS_PIPE_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(160) X(256) X(320) X(384) X(512)
+
+// Elements per thread of the next row that are prefetched into registers.
+constexpr int rows_pipe_prefetch(int ncmax, int n, int kmax) {
+    const int cap = (ncmax == 32 && n == 512) ? 8 : kmax;
+    return kmax < cap ? kmax : cap;
+}
+
+__device__ __forceinline__ int tri_bound(int g, int P, int nc, bool tri) {
+    if (!tri) return g * nc / P;
+    int s = 0;  // smallest s with s (s + 1) P >= g nc (nc + 1)
+    while (s < nc && s * (s + 1) * P < g * nc * (nc + 1)) ++s;
+    return s;
+}
+
+template <int NCMAX, int N>
+__global__ void __launch_bounds__(kRowsPipeThreads) k_rows_pw_pipe(RowPwArgs a) {
+    using F = FixedFft<N, true>;
+    extern __shared__ float2 lds[];
+    constexpr int T = kRowsPipeThreads;
+    constexpr int KMAX = (NCMAX * N + T - 1) / T;
+    // register-hungry transform lengths prefetch part of the next row only
+    constexpr int KP = rows_pipe_prefetch(NCMAX, N, KMAX);
+    constexpr int U = 8;
+    const int nc = a.nc, tx = a.tx, x0 = a.x0, half = N / 2;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const bool white = a.W != nullptr;
+    const float2* tw = stage_twiddles<F>(lds, a.p_enc);
+    float2* Wt = lds + tw_slots<F>();            // nc * nc, Wt[j nc + i] = W[i nc + j]
+    float2* A = Wt + (white ? nc * nc : 0);      // nc * N
+    if (white)
+        for (int e = threadIdx.x; e < nc * nc; e += T) {
+            const int i = e / nc, j = e - i * nc;
+            Wt[j * nc + i] = a.W[e];
+        }
+    const int total = nc * N;
+    float2 pf[KP];
+    auto prefetch = [&](long long row) {
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            const int e = (int)threadIdx.x + k * T;
+            if (e < total) {
+                const int c = e / N, x = e - c * N;
+                pf[k] = a.in[(size_t)c * a.in_plane + (size_t)row * N + x];
+            }
+        }
+    };
+    int P = 1;
+    while (P < 8 && 2 * P <= nc && 2 * P * tx <= T) P *= 2;
+    const int span = T / P, q = (int)threadIdx.x / span, xt = (int)threadIdx.x - q * span;
+    const int i0 = tri_bound(q, P, nc, white), i1 = tri_bound(q + 1, P, nc, white);
+    long long row = blockIdx.x;
+    if (row < a.rows) prefetch(row);
+    for (; row < a.rows; row += gridDim.x) {
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            const int e = (int)threadIdx.x + k * T;
+            if (e < total) {
+                const int c = e / N, x = e - c * N;
+                const int pos = x + half < N ? x + half : x + half - N;  // ifftshift
+                A[c * N + pos] = pf[k];
+            }
+        }
+#pragma unroll 4
+        for (int k = KP; k < KMAX; ++k) {
+            const int e = (int)threadIdx.x + k * T;
+            if (e < total) {
+                const int c = e / N, x = e - c * N;
+                const int pos = x + half < N ? x + half : x + half - N;
+                A[c * N + pos] = a.in[(size_t)c * a.in_plane + (size_t)row * N + x];
+            }
+        }
+        lds_barrier();
+        if (row + gridDim.x < a.rows) prefetch(row + gridDim.x);
+        for (int c = w; c < nc; c += T / 64) CFft<N, true>::run(A + c * N, tw, lane);
+        lds_barrier();
+        for (int xo = xt; xo < tx; xo += span) {
+            const int src = (x0 + xo + half) % N;
+            float2* dst = a.out + (size_t)row * tx + xo;
+            if (white) {
+                for (int ib = i0; ib < i1; ib += U) {
+                    const int ie = min(ib + U, i1);
+                    float2 acc[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) acc[u] = make_float2(0.f, 0.f);
+                    for (int j = 0; j < ie; ++j) {
+                        const float2 f = A[j * N + src];
+                        const float2* wr = Wt + j * nc + ib;
+#pragma unroll
+                        for (int u = 0; u < U; ++u)
+                            if (ib + u < ie && j <= ib + u) acc[u] = cadd(acc[u], cmul(wr[u], f));
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        if (ib + u < ie) dst[(size_t)(ib + u) * a.out_plane] = cscale(acc[u], a.scale);
+                }
+            } else {
+                for (int i = i0; i < i1; ++i) dst[(size_t)i * a.out_plane] = cscale(A[i * N + src], a.scale);
+            }
+        }
+        lds_barrier();
+    }
+}
+
 // Same row pass with the reference's oversampling-removal chain
 // (recon/oversampling.rs:122-147) and its re-centring into the nx grid
 // (io/ismrmrd/reader.rs:392-399) run on every line first.
@@ -304,7 +431,7 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
 // accumulates in LDS.  mode 0 writes the transformed complex data (window
 // [y0, y0+ty) along n); mode 1 writes sqrt(sum_c |.|^2) * scale, or with
 // G > 1 coil groups the per-group partial sums for k_rss_finalize.
-template <class F>
+template <class F, bool PIPE>
 __global__ void __launch_bounds__(512) k_cols(ColArgs a) {
     extern __shared__ float2 lds[];
     const int n = F::n ? F::n : a.p.n;
@@ -328,15 +455,48 @@ __global__ void __launch_bounds__(512) k_cols(ColArgs a) {
     const int ty = a.ty, y0 = a.y0;
     if (a.mode == 1)
         for (int i = threadIdx.x; i < SX * ty; i += blockDim.x) acc[i] = 0.f;
-    for (int c = c0; c < c1; ++c) {
+    // PIPE (fixed lengths, several coils per workgroup): coil c + 1's strip is
+    // loaded into registers while coil c is transformed (the launch uses 512
+    // threads and SX <= 16).
+    constexpr bool pipe = PIPE && F::n != 0;
+    constexpr int PF = pipe ? (F::n * 16 + 511) / 512 : 1;
+    float2 pf[PF];
+    auto prefetch = [&](int c) {
         const float2* __restrict__ src = a.in + ((size_t)c * a.Z + z) * n * a.X;
-        for (int i = threadIdx.x; i < n * SX; i += blockDim.x) {
-            const int y = i / SX, col = i - y * SX;
-            float2 v = make_float2(0.f, 0.f);
-            if (xs + col < a.X) v = src[(size_t)y * a.X + xs + col];
-            A[(size_t)col * ls + (y + half) % n] = v;
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+            const int i = (int)threadIdx.x + k * 512;
+            if (i < n * SX) {
+                const int y = i / SX, col = i - y * SX;
+                pf[k] = xs + col < a.X ? src[(size_t)y * a.X + xs + col] : make_float2(0.f, 0.f);
+            }
         }
-        __syncthreads();
+    };
+    if constexpr (pipe) {
+        if (c0 < c1) prefetch(c0);
+    }
+    for (int c = c0; c < c1; ++c) {
+        if constexpr (pipe) {
+#pragma unroll
+            for (int k = 0; k < PF; ++k) {
+                const int i = (int)threadIdx.x + k * 512;
+                if (i < n * SX) {
+                    const int y = i / SX, col = i - y * SX;
+                    A[(size_t)col * ls + (y + half) % n] = pf[k];
+                }
+            }
+            lds_barrier();
+            if (c + 1 < c1) prefetch(c + 1);
+        } else {
+            const float2* __restrict__ src = a.in + ((size_t)c * a.Z + z) * n * a.X;
+            for (int i = threadIdx.x; i < n * SX; i += blockDim.x) {
+                const int y = i / SX, col = i - y * SX;
+                float2 v = make_float2(0.f, 0.f);
+                if (xs + col < a.X) v = src[(size_t)y * a.X + xs + col];
+                A[(size_t)col * ls + (y + half) % n] = v;
+            }
+            __syncthreads();
+        }
         for (int col = w; col < SX; col += WPB) {
             float2* line = A + (size_t)col * ls;
             float2* res = F::run(line, B ? B + (size_t)col * ls : nullptr, a.p, tw, lane);
@@ -351,7 +511,7 @@ __global__ void __launch_bounds__(512) k_cols(ColArgs a) {
             }
             wave_sync();
         }
-        __syncthreads();
+        lds_barrier();
         if (a.mode == 0) {
             float2* dst = a.out + ((size_t)c * a.Zo + zi) * ty * a.X;
             for (int i = threadIdx.x; i < ty * SX; i += blockDim.x) {
@@ -359,7 +519,7 @@ __global__ void __launch_bounds__(512) k_cols(ColArgs a) {
                 if (xs + col < a.X)
                     dst[(size_t)yo * a.X + xs + col] = cscale(A[(size_t)col * ls + (y0 + yo + half) % n], a.scale);
             }
-            __syncthreads();
+            lds_barrier();
         }
     }
     if (a.mode == 1) {
@@ -471,10 +631,24 @@ static bool is_fixed(int n) {
 
 static size_t tw_lds_slots(int n) { return is_fixed(n) ? (size_t)((n + 1) & ~1) : 0; }
 
-size_t rows_prewhiten_lds(int nc, int nx, int os_mode) {
-    if (os_mode) return ((size_t)nc * nx + (size_t)8 * nx) * sizeof(float2);
-    const bool scratch = !is_fixed(nx);
-    return (tw_lds_slots(nx) + (size_t)nc * nx + (scratch ? (size_t)8 * nx : 0)) * sizeof(float2);
+static bool pipe_size(int n) {
+    switch (n) {
+#define OKS_CASE(N) case N:
+        OKS_PIPE_SIZES(OKS_CASE)
+#undef OKS_CASE
+        return true;
+        default: return false;
+    }
+}
+static bool rows_pipe_used(bool pipe, int os_mode, int nx) { return pipe && !os_mode && pipe_size(nx); }
+static bool rows_pipe_used(const RowPwArgs& a) { return rows_pipe_used(a.pipe != 0, a.os_mode, a.nx); }
+
+size_t rows_prewhiten_lds(int nc, int nx, int os_mode, bool white, bool pipe) {
+    if (os_mode || !is_fixed(nx)) return ((size_t)nc * nx + (size_t)8 * nx) * sizeof(float2);
+    // pipelined kernel: twiddles, W^T when whitening, coils
+    if (rows_pipe_used(pipe, os_mode, nx))
+        return (tw_lds_slots(nx) + (white ? (size_t)nc * nc : 0) + (size_t)nc * nx) * sizeof(float2);
+    return (tw_lds_slots(nx) + (size_t)nc * nx) * sizeof(float2);
 }
 
 template <int NCMAX, class F>
@@ -484,12 +658,33 @@ static hipError_t go_rows_pw(const RowPwArgs& a, size_t lds, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Persistent grid: as many workgroups as fit on the 256 CUs at once (LDS and
+// the 2048-thread limit), never more than there are rows.
+template <int NCMAX, int N>
+static hipError_t go_rows_pw_pipe(const RowPwArgs& a, size_t lds, hipStream_t s) {
+    OKS_ATTR(hipFuncSetAttribute((const void*)k_rows_pw_pipe<NCMAX, N>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)lds));
+    const long long per_cu = std::max<long long>(1, std::min<long long>(2, (160 * 1024) / (long long)lds));
+    const long long grid = std::min<long long>(a.rows, 256 * per_cu);
+    hipLaunchKernelGGL((k_rows_pw_pipe<NCMAX, N>), dim3((unsigned)grid), dim3(kRowsPipeThreads), lds, s, a);
+    return hipGetLastError();
+}
+
 template <int NCMAX>
 static hipError_t go_rows_pw_n(const RowPwArgs& a, size_t lds, hipStream_t s) {
     if (a.os_mode) {
         OKS_ATTR(hipFuncSetAttribute((const void*)k_rows_pw_os<NCMAX>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(k_rows_pw_os<NCMAX>, dim3((unsigned)a.rows), dim3(512), lds, s, a);
         return hipGetLastError();
+    }
+    if (rows_pipe_used(a)) {
+        switch (a.nx) {
+#define OKS_CASE(N) \
+    case N: return go_rows_pw_pipe<NCMAX, N>(a, lds, s);
+            OKS_PIPE_SIZES(OKS_CASE)
+#undef OKS_CASE
+            default: break;
+        }
     }
     switch (a.nx) {
 #define OKS_CASE(N) \
@@ -501,7 +696,7 @@ static hipError_t go_rows_pw_n(const RowPwArgs& a, size_t lds, hipStream_t s) {
 }
 
 hipError_t launch_rows_prewhiten_ifft(const RowPwArgs& a, hipStream_t s) {
-    const size_t lds = rows_prewhiten_lds(a.nc, a.nx, a.os_mode);
+    const size_t lds = rows_prewhiten_lds(a.nc, a.nx, a.os_mode, a.W != nullptr, a.pipe != 0);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     if (a.nc <= 16) return go_rows_pw_n<16>(a, lds, s);
     if (a.nc <= 32) return go_rows_pw_n<32>(a, lds, s);
@@ -540,14 +735,22 @@ size_t cols_lds(int sx, int n, int ty, int mode) {
            (mode == 1 ? (size_t)sx * ty * sizeof(float) : 0);
 }
 
+template <class F, bool PIPE>
+static hipError_t go_cols_k(const ColArgs& a, size_t lds, unsigned grid, hipStream_t s) {
+    OKS_ATTR(hipFuncSetAttribute((const void*)k_cols<F, PIPE>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_cols<F, PIPE>), dim3(grid), dim3(512), lds, s, a);
+    return hipGetLastError();
+}
+
 template <class F>
 static hipError_t go_cols(const ColArgs& a, hipStream_t s) {
     const size_t lds = cols_lds(a.sx, a.p.n, a.ty, a.mode);
     const int strips = (a.X + a.sx - 1) / a.sx;
     const unsigned grid = (unsigned)(a.G * a.Zo * strips);
-    OKS_ATTR(hipFuncSetAttribute((const void*)k_cols<F>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_cols<F>, dim3(grid), dim3(512), lds, s, a);
-    return hipGetLastError();
+    if (a.sx > 16) return hipErrorInvalidValue;
+    // the smallest coil group has C / G coils
+    if (a.pipe && F::n != 0 && a.C / a.G >= 2) return go_cols_k<F, true>(a, lds, grid, s);
+    return go_cols_k<F, false>(a, lds, grid, s);
 }
 
 template <bool INV>

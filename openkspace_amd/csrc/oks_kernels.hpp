@@ -34,6 +34,7 @@ struct RowPwArgs {
     int nx, tx, x0;     // output = centred columns [x0, x0 + tx)
     float scale;
     int os_mode, rec;   // reference oversampling removal to `rec` samples
+    int pipe;           // fixed lengths: pipelined persistent kernel (1) or one row per workgroup (0)
     FftPlan p_enc;      // n = nx, inverse
     FftPlan p_rec;      // n = rec, forward (os_mode)
 };
@@ -64,12 +65,13 @@ struct ColArgs {
     int y0, ty;         // centred output window along n
     int centred;
     float scale;
+    int pipe;           // prefetch the next coil's strip when a workgroup loops over coils
     FftPlan p;
 };
 
 hipError_t launch_whitener(const float2* noise, int nc, int ns, float2* W, int* status, hipStream_t s);
 hipError_t launch_whitener_batch(const WhitenArgs& a, int B, hipStream_t s);
-size_t rows_prewhiten_lds(int nc, int nx, int os_mode);
+size_t rows_prewhiten_lds(int nc, int nx, int os_mode, bool white, bool pipe);
 hipError_t launch_rows_prewhiten_ifft(const RowPwArgs& a, hipStream_t s);
 hipError_t launch_rows(const RowArgs& a, bool inverse, hipStream_t s);
 size_t cols_lds(int sx, int n, int ty, int mode);

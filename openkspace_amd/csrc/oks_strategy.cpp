@@ -55,6 +55,7 @@ void col_pass(oks_ctx* ctx, const float2* in, float2* out_c, float* out_r, int C
     a.ty = ty;
     a.centred = centred;
     a.scale = scale;
+    a.pipe = ctx->cols_pipe;
     a.p = ctx->plan(n);
     a.sx = pick_sx(n, ty, mode, X);
     if (cols_lds(a.sx, n, ty, mode) > 160 * 1024) fail(OKS_E_UNSUPPORTED, "transform length too large for LDS");
@@ -202,6 +203,7 @@ void ifft_rss_grid(oks_ctx* ctx, const float2* k, int64_t nc, int64_t nz, int64_
     StageTimer timer(ctx, OKS_STAGE_IFFT_RSS,
                      (double)nc * zread * ny * nx * 8.0 + (double)w[0].len * ty * tx * 4.0);
     RowPwArgs a{};
+    a.pipe = ctx->rows_pipe;
     a.W = W;
     a.nc = (int)nc;
     a.nx = (int)nx;
@@ -213,7 +215,7 @@ void ifft_rss_grid(oks_ctx* ctx, const float2* k, int64_t nc, int64_t nz, int64_
     a.in_plane = nz * ny * nx;
     a.p_enc = ctx->plan((int)nx);
     if (os_mode) a.p_rec = ctx->plan(rec);
-    if (rows_prewhiten_lds((int)nc, (int)nx, os_mode) > 160 * 1024)
+    if (rows_prewhiten_lds((int)nc, (int)nx, os_mode, W != nullptr, ctx->rows_pipe != 0) > 160 * 1024)
         fail(OKS_E_UNSUPPORTED, "readout too long for the fused row pass");
     if (!three_d) {
         // slices actually written: the z window of the crop

@@ -130,3 +130,24 @@ def test_host_buffer_entry_point(ctx):
     _, ref64, _, _ = O.recon("ifft_rss", d, case, "f64")
     assert vol.strategy == "ifft-rss" and vol.flags == rflags
     assert_parity(vol.data, ref, ref64, TOL32, "host entry")
+
+
+@pytest.mark.parametrize("rows_pipe,cols_pipe", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("nc,nx,prewhiten", [(32, 384, 1), (17, 256, 1), (5, 128, 0), (15, 640, 1)])
+def test_ifft_rss_pass_forms(ctx, rows_pipe, cols_pipe, nc, nx, prewhiten):
+    # both row-pass forms (whitening before or after the kx IFFT; persistent
+    # pipelined or one row per workgroup) and both column-pass forms meet the
+    # same bar; nc = 17 and 5 leave the coil loops and thread groups ragged
+    case = _case(nc=nc, nz=2, ny=40, nx=nx, os_x=2)
+    d = abi.IfftRssDesc.default(remove_oversampling=0, prewhiten=prewhiten)
+    st, ref32, rflags, _ = O.recon("ifft_rss", d, case, "f32")
+    _, ref64, _, _ = O.recon("ifft_rss", d, case, "f64")
+    ctx.set_option("rows_pipe", rows_pipe)
+    ctx.set_option("cols_pipe", cols_pipe)
+    try:
+        got, flags, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", d, case)
+    finally:
+        ctx.set_option("rows_pipe", 1)
+        ctx.set_option("cols_pipe", 1)
+    assert st == 0 and flags == rflags and got.shape == ref32.shape
+    assert_parity(got, ref32, ref64, TOL32, f"rows_pipe={rows_pipe} cols_pipe={cols_pipe} nc={nc} nx={nx}")
