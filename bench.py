@@ -14,8 +14,11 @@ latency, the alone-vs-served and oracle rel-L2 per strategy and shape, the
 roofline of the dominant stage (HIP events on the slot streams) and the CPU
 baseline (the FP32 oracle, one thread, on a bounded sample of the stream).
 
-c1 / c2: one multi-slice reconstruction call (ifft-rss 16 x 15 x 640 x 368,
-GRAPPA 8 x 32 x 384^2 R=4) per step.
+c1 / c2 / c3 / c4: one multi-slice reconstruction call (ifft-rss 16 x 15 x
+640 x 368, GRAPPA 8 x 32 x 384^2 R=4, CS 32 x 8 x 320^2 120 iterations, 3-D
+SENSE 32 x 160 x 256^2 R=3, which falls back to IFFT + RSS) per step.
+c4u: the SENSE unfold + g-factor of the C4 volume at ky = 258 (divisible by
+R = 3), one launch per volume.  hbm_copy: the copy-bandwidth ceiling.
 """
 import argparse
 import ctypes as C
@@ -192,6 +195,105 @@ def run_hbm_copy(a):
     buf.free()
     w.free()
     ctx.close()
+
+
+def c4u_volume(nc=32, nz=160, ny=258, nx=256, r=3, seed=None):
+    """The C4 acquisition at a divisible ky count (258 = 3 x 86): per kz the
+    coil images of a 10-ellipse phantom folded R times along ky (the aliased
+    image the unfold reads, rows 0..ny/R, repeated periodically), and the
+    smooth coil sensitivities used as the maps."""
+    from openkspace_amd import synth
+    rng = np.random.default_rng(synth.SEED_BASE + 3 if seed is None else seed)
+    maps2 = synth.coil_maps(rng, nc, ny, nx).astype(np.complex64)
+    obj = synth._ellipses(rng, ny, nx, 1, 10).astype(np.float32)
+    coil = maps2 * obj[None]
+    nr = ny // r
+    fold = sum(coil[:, k * nr:(k + 1) * nr] for k in range(r))
+    al2 = np.concatenate([fold] * r, axis=1).astype(np.complex64)
+    scale = (1.0 + 0.25 * np.cos(np.arange(nz) * 2 * np.pi / nz)).astype(np.float32)
+    aliased = al2[:, None] * scale[None, :, None, None]
+    maps = np.broadcast_to(maps2[:, None], (nc, nz, ny, nx))
+    return np.ascontiguousarray(aliased), np.ascontiguousarray(maps)
+
+
+def run_c4u(a):
+    """SENSE unfold of the C4 volume at a divisible ky count: 32 coils x 160
+    kz x 258 ky x 256 kx, R = 3, ridge 1e-4, g-factor on, the R x R solves in
+    FP64, one launch per volume with maps and aliased data resident in HBM.
+    The ESPIRiT maps feeding it are not part of this line (the maps are the
+    generator's coil sensitivities)."""
+    from openkspace_amd._lib import Context, DeviceArray, Event, check, lib
+    nc, nz, ny, nx, r, ridge = 32, 160, 258, 256, 3, 1e-4
+    aliased, maps = c4u_volume(nc, nz, ny, nx, r)
+    ctx = Context(0)
+    da, dm = DeviceArray.from_host(ctx, aliased), DeviceArray.from_host(ctx, maps)
+    dr = DeviceArray(ctx, (nz, ny, nx), np.complex64)
+    dg = DeviceArray(ctx, (nz, ny, nx), np.float32)
+    L = lib()
+
+    def step():
+        check(ctx.ptr, L.oks_sense_unfold_volume_dev(ctx.ptr, da.addr, dm.addr, nc, nz, ny, nx, r, C.c_float(ridge),
+                                                      dr.addr, dg.addr))
+
+    for _ in range(a.warmup):
+        step()
+    ctx.synchronize()
+    evs = [(Event(ctx), Event(ctx)) for _ in range(a.steps)]
+    t0 = time.perf_counter()
+    for s, e in evs:
+        s.record()
+        step()
+        e.record()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    lat = np.array([s.elapsed_ms(e) for s, e in evs])
+    # per kz: aliased rows 0..ny/R and the maps read, rho (complex) and g written
+    algo = nz * (nc * (ny // r) * nx * 8 + nc * ny * nx * 8 + ny * nx * 8 + ny * nx * 4)
+    avg_s = float(lat.mean()) * 1e-3
+    out = {
+        "metric": METRIC, "value": nz * a.steps / wall, "unit": "slices/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": wall * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64/f32 (R x R solves f64)", "data": "synthetic",
+        "config": {"workload": "c4u_sense_unfold", "volumes": 1, "coils": nc, "kx": nx, "ky": ny, "kz": nz, "R": r,
+                   "ridge": ridge, "gfactor": True,
+                   "note": "C4 at a ky count divisible by R (258); unfold + g-factor only, maps given"},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "mean": float(lat.mean())},
+        "roofline": {"bound": "hbm", "achieved": algo / avg_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": algo / avg_s / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_sense_unfold<3> (one launch per volume)", "algorithmic_bytes": algo,
+                     "avg_launch_ms": avg_s * 1e3},
+    }
+    if not a.no_check or not a.no_cpu:
+        import oracle as O
+        rho, g = dr.download(), dg.download()
+    if not a.no_check:
+        errs = {}
+        for z in (0, nz // 2):
+            _, o32, g32, _ = O.sense_unfold(aliased[:, z], maps[:, z], r, ridge, True, "f32")
+            _, o64, g64, _ = O.sense_unfold(aliased[:, z], maps[:, z], r, ridge, True, "f64")
+            errs[f"kz={z}"] = {
+                "rho_vs_fp32_oracle": float(np.linalg.norm(rho[z] - o32) / np.linalg.norm(o32)),
+                "rho_vs_fp64_oracle": float(np.linalg.norm(rho[z] - o64) / np.linalg.norm(o64)),
+                "g_vs_fp64_oracle": float(np.linalg.norm(g[z] - g64) / np.linalg.norm(g64)),
+                "fp32_oracle_vs_fp64": float(np.linalg.norm(o32 - o64) / np.linalg.norm(o64))}
+        out["rel_l2"] = errs
+    if not a.no_cpu:
+        O.set_threads(1)
+        done, t0 = 0, time.perf_counter()
+        while done < nz:
+            O.sense_unfold(aliased[:, done], maps[:, done], r, ridge, True, "f32")
+            done += 1
+            if time.perf_counter() - t0 > a.cpu_seconds:
+                break
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": done / dt, "unit": "slices/s", "cores": 1, "kind": "port",
+                               "sample": f"{done} of {nz} kz planes through the FP32 oracle unfold + g-factor, "
+                                         f"single thread, {dt:.1f} s"}
+    for d in (da, dm, dr, dg):
+        d.free()
+    ctx.close()
+    print(json.dumps(out))
 
 
 def _pct(v, q):
@@ -406,6 +508,9 @@ def main():
         sys.exit(2)
     if a.workload == "hbm_copy":
         run_hbm_copy(a)
+        return
+    if a.workload == "c4u":
+        run_c4u(a)
         return
     if a.workload == "served":
         if a.steps == 50 and "--steps" not in sys.argv:

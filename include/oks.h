@@ -259,6 +259,11 @@ int oks_grappa_synthesize_dev(oks_ctx* ctx, const oks_c32* weights, oks_c32* ksp
  * R x R solves in FP64. */
 int oks_sense_unfold_dev(oks_ctx* ctx, const oks_c32* aliased, const oks_c32* maps, int64_t nc, int64_t ny,
                          int64_t nx, int64_t r, float ridge, oks_c32* rho, float* g);
+/* The per-kz loop of SenseRss over sense_unfold_1d + sense_gfactor_1d
+ * (recon/strategy.rs:656-720) in one launch: aliased, maps [nc, nz, ny, nx]
+ * -> rho [nz, ny, nx] (and g [nz, ny, nx] when g != NULL). */
+int oks_sense_unfold_volume_dev(oks_ctx* ctx, const oks_c32* aliased, const oks_c32* maps, int64_t nc, int64_t nz,
+                                int64_t ny, int64_t nx, int64_t r, float ridge, oks_c32* rho, float* g);
 /* walsh_sensitivity_maps (recon/sensitivity.rs:37-142) per z: img, maps
  * [nc, nz, ny, nx]. */
 int oks_walsh_maps_dev(oks_ctx* ctx, const oks_c32* img, int64_t nc, int64_t nz, int64_t ny, int64_t nx,

@@ -59,6 +59,7 @@ def lib():
     L.oks_ctx_set_option.argtypes = [_P, C.c_char_p, _I64]
     L.oks_fista_cs_dev.argtypes = [_P, _P, _P] + [_I64] * 5 + [C.c_float, _P]
     L.oks_sense_unfold_dev.argtypes = [_P, _P, _P] + [_I64] * 4 + [C.c_float, _P, _P]
+    L.oks_sense_unfold_volume_dev.argtypes = [_P, _P, _P] + [_I64] * 5 + [C.c_float, _P, _P]
     L.oks_walsh_maps_dev.argtypes = [_P, _P] + [_I64] * 6 + [_P]
     L.oks_espirit_maps_dev.argtypes = [_P, _P] + [_I64] * 6 + [C.c_float, _I64, _P, _P]
     L.oks_fft_centered_dev.argtypes = [_P, _P, _P, _I32, _P, _I32, _I32, _I32]

@@ -44,6 +44,11 @@ __device__ __forceinline__ float2 twid(const float2* tw, int idx) {
     return w;
 }
 
+// Workgroup barrier that orders LDS accesses only.  Global loads issued
+// before it stay in flight across it (__syncthreads() also waits for them),
+// which is what lets a workgroup prefetch its next input while it computes.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // Wavefront-scope LDS ordering between Stockham stages.
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -152,10 +152,19 @@ __global__ void __launch_bounds__(256) k_zgemm_sbf16(typename L::Args args) {
         }
     const int K = ld.K();
     const int lr = lane & 15, lk = (lane >> 4) * 8;
+    // Step k0's operands are fetched into registers during step k0 - BK's
+    // MFMAs; the LDS-only barriers leave those loads in flight.
+    float2 va[8], vb[8];
+    ld.fetch_a(0, va);
+    ld.template fetch_b<BN>(0, vb);
     for (int k0 = 0; k0 < K; k0 += BK) {
-        ld.load_a(PA, k0);
-        ld.template load_b<BN>(PB, k0);
-        __syncthreads();
+        ld.commit_a(PA, va);
+        ld.template commit_b<BN>(PB, vb);
+        lds_barrier();
+        if (k0 + BK < K) {
+            ld.fetch_a(k0 + BK, va);
+            ld.template fetch_b<BN>(k0 + BK, vb);
+        }
 #pragma unroll
         for (int ns = 0; ns < 2; ++ns) {
             const int nrow = wn * 32 + ns * 16 + lr;
@@ -184,7 +193,7 @@ __global__ void __launch_bounds__(256) k_zgemm_sbf16(typename L::Args args) {
                 ci[ms][ns] = i;
             }
         }
-        __syncthreads();
+        lds_barrier();
     }
 #pragma unroll
     for (int ms = 0; ms < MS; ++ms)
@@ -243,12 +252,12 @@ struct CalibLoader {
     }
     __device__ int K() const { return a.n_pos; }
     // One thread: eight consecutive positions (contiguous kx0 within an ACS
-    // row) of one source column, stored as one 16-byte write per plane.
-    __device__ void gather8(const int* of, int col_base, int lim, int rows, int k0, const Planes& P, bool conj) const {
+    // row) of one source column; the commit stores them as one 16-byte write
+    // per plane.  Thread t holds row t / 4, positions k0 + 8 (t % 4) + [0, 8).
+    __device__ void fetch8(const int* of, int col_base, int lim, int rows, int k0, float2 (&v)[8], bool conj) const {
         const int t = threadIdx.x;
         if (t >= rows * 4) return;
         const int m = t >> 2, k8 = (t & 3) * 8;
-        float2 v[8];
         const int col = col_base + m;
         const int p0 = k0 + k8;
         int ky0 = p0 / a.n_kx, kx0 = p0 - ky0 * a.n_kx;
@@ -267,12 +276,20 @@ struct CalibLoader {
                 ++ky0;
             }
         }
-        P.put8(m, k8, v);
     }
-    __device__ void load_a(const Planes& P, int k0) const { gather8(src_of, m0, a.M, 64, k0, P, true); }
+    __device__ static void commit8(const Planes& P, int rows, const float2 (&v)[8]) {
+        const int t = threadIdx.x;
+        if (t < rows * 4) P.put8(t >> 2, (t & 3) * 8, v);
+    }
+    __device__ void fetch_a(int k0, float2 (&v)[8]) const { fetch8(src_of, m0, a.M, 64, k0, v, true); }
+    __device__ void commit_a(const Planes& P, const float2 (&v)[8]) const { commit8(P, 64, v); }
     template <int BN>
-    __device__ void load_b(const Planes& P, int k0) const {
-        gather8(a.rhs ? tgt_of : src_of, n0, a.N, BN, k0, P, false);
+    __device__ void fetch_b(int k0, float2 (&v)[8]) const {
+        fetch8(a.rhs ? tgt_of : src_of, n0, a.N, BN, k0, v, false);
+    }
+    template <int BN>
+    __device__ void commit_b(const Planes& P, const float2 (&v)[8]) const {
+        commit8(P, BN, v);
     }
     __device__ void store(int m, int n, float2 v) const {
         m += m0;
@@ -328,31 +345,37 @@ struct ApplyLoader {
     }
     __device__ int K() const { return a.n_src; }
     // kx0 (m) fastest across lanes (coalesced), eight consecutive source
-    // columns j per thread, stored as one 16-byte write per plane.
-    __device__ void load_a(const Planes& P, int k0) const {
+    // columns j per thread (thread t: m = t % 64, j = k0 + 8 (t / 64) + [0, 8)),
+    // committed as one 16-byte write per plane.
+    __device__ void fetch_a(int k0, float2 (&v)[8]) const {
         const int m = threadIdx.x & 63, k8 = (threadIdx.x >> 6) * 8;
         const bool min_ = m0 + m < a.n_kx;
-        float2 v[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int j = k0 + k8 + e;
             v[e] = (min_ && j < a.n_src) ? base[src_of[j] + m] : make_float2(0.f, 0.f);
         }
-        P.put8(m, k8, v);
     }
+    __device__ void commit_a(const Planes& P, const float2 (&v)[8]) const {
+        P.put8(threadIdx.x & 63, (threadIdx.x >> 6) * 8, v);
+    }
+    // Row n of the weights (one target coil), eight consecutive j per thread.
     template <int BN>
-    __device__ void load_b(const Planes& P, int k0) const {
+    __device__ void fetch_b(int k0, float2 (&v)[8]) const {
         const int t = threadIdx.x;
         if (t >= BN * 4) return;
         const int n = t >> 2, k8 = (t & 3) * 8;
         const float2* wd = a.w + (size_t)z * a.wz + (size_t)(d - 1) * a.nc * a.n_src + (size_t)n * a.n_src;
-        float2 v[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int j = k0 + k8 + e;
             v[e] = (n < a.nc && j < a.n_src) ? wd[j] : make_float2(0.f, 0.f);
         }
-        P.put8(n, k8, v);
+    }
+    template <int BN>
+    __device__ void commit_b(const Planes& P, const float2 (&v)[8]) const {
+        const int t = threadIdx.x;
+        if (t < BN * 4) P.put8(t >> 2, (t & 3) * 8, v);
     }
     __device__ void store(int m, int n, float2 v) const {
         m += m0;

@@ -13,11 +13,6 @@
 
 namespace oks {
 
-// Workgroup barrier that orders LDS accesses only.  Global loads issued
-// before it stay in flight across it (__syncthreads() also waits for them),
-// which is what lets a workgroup prefetch its next input during a transform.
-__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
-
 
 #define OKS_ATTR(x)                         \
     do {                                    \

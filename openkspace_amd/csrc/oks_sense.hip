@@ -276,6 +276,7 @@ hipError_t launch_walsh(const WalshArgs& a, hipStream_t s) {
 
 hipError_t launch_sense_unfold(const UnfoldArgs& a, int r, hipStream_t s) {
     const long long total = (long long)a.nz * (a.ny / r) * a.nx;
+    if ((total + 255) / 256 > 0x7fffffffLL) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((total + 255) / 256)), blk(256);
     switch (r) {
         case 1: hipLaunchKernelGGL(k_sense_unfold<1>, grid, blk, 0, s, a); break;

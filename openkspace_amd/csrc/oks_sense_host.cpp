@@ -142,21 +142,32 @@ int oks_recon_sense_dev(oks_ctx* ctx, const oks_sense_desc* d, const oks_kspace*
 // rho (complex, [ny, nx]) and/or g ([ny, nx]) may be null.  Returns
 // OKS_E_SENSE_* like SenseError; pixels whose system is not positive definite
 // are zero (reported through oks_ctx_take_flags as OKS_F_SENSE_PIXEL_FAILED).
+// sense_unfold_1d / sense_gfactor_1d on nz planes of [nc, nz, ny, nx]; the
+// argument errors are the reference's (recon/sense.rs:71-82).
+static void unfold_planes(oks_ctx* ctx, const oks_c32* aliased, const oks_c32* maps, int64_t nc, int64_t nz,
+                          int64_t ny, int64_t nx, int64_t r, float ridge, oks_c32* rho, float* g) {
+    if (r < 1) fail(OKS_E_SENSE_BAD_ACCEL, "SENSE: acceleration factor must be >= 1");
+    if (!maps || nc < 0 || nz < 1 || ny < 0 || nx < 0) fail(OKS_E_INVALID_ARG, "bad SENSE arguments");
+    if (ny % r != 0) fail(OKS_E_SENSE_INDIVISIBLE, "SENSE: Ny must be divisible by R");
+    if (r > 8) fail(OKS_E_UNSUPPORTED, "SENSE: R > 8");
+    if (nc > 32 || nz > 65535 || ny > 4096 || nx > 4096) fail(OKS_E_UNSUPPORTED, "SENSE: volume too large");
+    const size_t cells = (size_t)nz * ny * nx;
+    if (rho) OKS_HIP(hipMemsetAsync(rho, 0, cells * sizeof(oks_c32), ctx->stream));
+    if (g) OKS_HIP(hipMemsetAsync(g, 0, cells * sizeof(float), ctx->stream));
+    if (nc == 0 || nx == 0 || ny / r == 0) return;
+    UnfoldArgs ua{(const float2*)aliased, (const float2*)maps, nullptr, (float2*)rho, g, ctx->d_status + 1,
+                  (int)nc, (int)nz, (int)ny, (int)nx, ridge};
+    OKS_HIP(launch_sense_unfold(ua, (int)r, ctx->stream));
+}
+
 int oks_sense_unfold_dev(oks_ctx* ctx, const oks_c32* aliased, const oks_c32* maps, int64_t nc, int64_t ny, int64_t nx,
                          int64_t r, float ridge, oks_c32* rho, float* g) {
-    return guarded(ctx, [&] {
-        if (r < 1) fail(OKS_E_SENSE_BAD_ACCEL, "SENSE: acceleration factor must be >= 1");
-        if (!maps || nc < 0 || ny < 0 || nx < 0) fail(OKS_E_INVALID_ARG, "bad SENSE arguments");
-        if (ny % r != 0) fail(OKS_E_SENSE_INDIVISIBLE, "SENSE: Ny must be divisible by R");
-        if (r > 8) fail(OKS_E_UNSUPPORTED, "SENSE: R > 8");
-        const size_t plane = (size_t)ny * nx;
-        if (rho) OKS_HIP(hipMemsetAsync(rho, 0, plane * sizeof(oks_c32), ctx->stream));
-        if (g) OKS_HIP(hipMemsetAsync(g, 0, plane * sizeof(float), ctx->stream));
-        if (nc == 0 || nx == 0 || ny / r == 0) return;
-        UnfoldArgs ua{(const float2*)aliased, (const float2*)maps, nullptr, (float2*)rho, g, ctx->d_status + 1,
-                      (int)nc, 1, (int)ny, (int)nx, ridge};
-        OKS_HIP(launch_sense_unfold(ua, (int)r, ctx->stream));
-    });
+    return guarded(ctx, [&] { unfold_planes(ctx, aliased, maps, nc, 1, ny, nx, r, ridge, rho, g); });
+}
+
+int oks_sense_unfold_volume_dev(oks_ctx* ctx, const oks_c32* aliased, const oks_c32* maps, int64_t nc, int64_t nz,
+                                int64_t ny, int64_t nx, int64_t r, float ridge, oks_c32* rho, float* g) {
+    return guarded(ctx, [&] { unfold_planes(ctx, aliased, maps, nc, nz, ny, nx, r, ridge, rho, g); });
 }
 
 int oks_espirit_maps_dev(oks_ctx* ctx, const oks_c32* acs, int64_t nc, int64_t acs_y, int64_t acs_x, int64_t ny,
