@@ -241,6 +241,16 @@ int oks_fft_centered_dev(oks_ctx* ctx, oks_c32* data, const int64_t* shape, int3
                          int32_t naxes, int32_t inverse, int32_t norm);
 /* rss_combine_4d (recon/coil.rs:24-27): out[i] = sqrt(sum_c |in[c][i]|^2). */
 int oks_rss_dev(oks_ctx* ctx, const oks_c32* in, int64_t nc, int64_t inner, float* out);
+/* center_crop_3d (recon/crop.rs:13-27): each target clamped to [1, n], window
+ * offset (n - t) / 2; in [nz, ny, nx] -> out [tz', ty', tx'], the clamped
+ * shape written to out_shape. */
+int oks_crop_dev(oks_ctx* ctx, const float* in, int64_t nz, int64_t ny, int64_t nx, int64_t tz, int64_t ty, int64_t tx,
+                 float* out, int64_t out_shape[3]);
+/* OversamplingRemover::apply (recon/oversampling.rs:103-154) on `rows`
+ * readouts: in [rows, enc] -> out [rows, rec].  OKS_E_UNSUPPORTED where
+ * OversamplingRemover::new returns None (recon/oversampling.rs:50-70: rec = 0,
+ * enc <= rec, enc % rec != 0 or a ratio above 8). */
+int oks_os_remove_dev(oks_ctx* ctx, const oks_c32* in, int64_t rows, int64_t enc, int64_t rec, oks_c32* out);
 /* NoisePrewhitener::from_noise_acqs (recon/prewhiten.rs:48-111): W = chol(Psi)^-1
  * [nc, nc] lower, FP64 inside; W = I and a device flag when not PD. */
 int oks_prewhiten_calibrate_dev(oks_ctx* ctx, const oks_c32* noise, int32_t nc, int64_t ns, oks_c32* W);

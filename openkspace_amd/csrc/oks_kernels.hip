@@ -390,6 +390,34 @@ __global__ void __launch_bounds__(512) k_prep_os(RowPwArgs a, const uint8_t* acq
     }
 }
 
+// OversamplingRemover::apply (recon/oversampling.rs:103-154) on single
+// readouts, [rows, enc] -> [rows, rec], one wavefront per readout: rotate by
+// ceil(enc/2), IFFT(enc) / enc, rotate by enc/2, keep [(enc-rec)/2, +rec),
+// rotate by ceil(rec/2), FFT(rec) unscaled, rotate by rec/2.
+__global__ void __launch_bounds__(256) k_os_remove(const float2* __restrict__ in, float2* __restrict__ out,
+                                                   long long rows, int enc, int rec, FftPlan p_enc, FftPlan p_rec) {
+    extern __shared__ float2 lds[];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long long row = (long long)blockIdx.x * (blockDim.x >> 6) + w;
+    if (row >= rows) return;  // wavefront-local from here on
+    float2* buf = lds + (size_t)w * 2 * enc;
+    float2* scr = buf + enc;
+    const float2* src = in + row * enc;
+    const int half = enc / 2;
+    for (int x = lane; x < enc; x += 64) buf[(x + half) % enc] = src[x];
+    wave_sync();
+    float2* res = fft_wave<true>(buf, scr, p_enc, lane, 64);
+    float2* q = (res == buf) ? scr : buf;
+    const int crop0 = (enc - rec) / 2, rh = (rec + 1) / 2;
+    const float inv = 1.0f / (float)enc;
+    for (int i = lane; i < rec; i += 64) q[i] = cscale(res[(((i + rh) % rec) + crop0 + half) % enc], inv);
+    wave_sync();
+    float2* other = (q == buf) ? scr : buf;
+    float2* S = fft_wave<false>(q, other, p_rec, lane, 64);
+    float2* dst = out + row * rec;
+    for (int m = lane; m < rec; m += 64) dst[m] = S[(m + rec / 2) % rec];
+}
+
 // ---------------------------------------------------------------------------
 // Generic row pass: centred 1-D transform along the contiguous axis of
 // [rows, n]; optional input mask multiply (mask row index = row % mask_rows),
@@ -778,6 +806,21 @@ hipError_t launch_prep_os(const RowPwArgs& a, const uint8_t* acq, hipStream_t s)
     } else {
         return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_os_remove(const float2* in, float2* out, long long rows, int enc, int rec, const FftPlan& p_enc,
+                            const FftPlan& p_rec, hipStream_t s) {
+    if (rows <= 0) return hipSuccess;
+    const size_t per_wave = (size_t)2 * enc * sizeof(float2);
+    const int wpb = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / per_wave));
+    if (per_wave > 160 * 1024 || rec > enc) return hipErrorInvalidValue;
+    const long long grid = (rows + wpb - 1) / wpb;
+    if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
+    OKS_ATTR(hipFuncSetAttribute((const void*)k_os_remove, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(per_wave * wpb)));
+    hipLaunchKernelGGL(k_os_remove, dim3((unsigned)grid), dim3(64 * wpb), per_wave * wpb, s, in, out, rows, enc, rec,
+                       p_enc, p_rec);
     return hipGetLastError();
 }
 

@@ -81,5 +81,7 @@ hipError_t launch_rss_finalize(const float* part, int G, long long M, float* out
 hipError_t launch_prewhiten(const float2* in, float2* d, const float2* W, int nc, long long M, hipStream_t s);
 hipError_t launch_prep_os(const RowPwArgs& a, const uint8_t* acq, hipStream_t s);
 hipError_t launch_rss(const float2* in, int C, long long M, float* out, float scale, hipStream_t s);
+hipError_t launch_os_remove(const float2* in, float2* out, long long rows, int enc, int rec, const FftPlan& p_enc,
+                            const FftPlan& p_rec, hipStream_t s);
 
 }  // namespace oks

@@ -13,6 +13,7 @@
 #include "oks_kernels.hpp"
 #include "oks_abi_util.hpp"
 #include "oks_pipeline.hpp"
+#include "oks_sense.hpp"
 
 namespace oks {
 
@@ -298,6 +299,32 @@ int oks_fft_centered_dev(oks_ctx* ctx, oks_c32* data, const int64_t* shape, int3
             const float s = norm == 1 ? 1.0f / (float)n : norm == 2 ? 1.0f / std::sqrt((float)n) : 1.0f;
             if (n >= 2) fft_axis(ctx, (float2*)data, shape, ndim, axes[i], inverse != 0, s, true);
         }
+    });
+}
+
+int oks_os_remove_dev(oks_ctx* ctx, const oks_c32* in, int64_t rows, int64_t enc, int64_t rec, oks_c32* out) {
+    return guarded(ctx, [&] {
+        if (!in || !out || rows < 0 || enc < 1 || rec < 0) fail(OKS_E_INVALID_ARG, "bad oversampling arguments");
+        if (!os_valid(enc, rec))  // OversamplingRemover::new returns None (recon/oversampling.rs:50-70)
+            fail(OKS_E_UNSUPPORTED, "no oversampling remover for these encoded/recon sizes");
+        if (enc > 4096) fail(OKS_E_UNSUPPORTED, "readout longer than 4096 samples");
+        if (rows == 0) return;
+        OKS_HIP(launch_os_remove((const float2*)in, (float2*)out, rows, (int)enc, (int)rec, ctx->plan((int)enc),
+                                 ctx->plan((int)rec), ctx->stream));
+    });
+}
+
+int oks_crop_dev(oks_ctx* ctx, const float* in, int64_t nz, int64_t ny, int64_t nx, int64_t tz, int64_t ty, int64_t tx,
+                 float* out, int64_t out_shape[3]) {
+    return guarded(ctx, [&] {
+        if (!in || !out || !out_shape || nz < 1 || ny < 1 || nx < 1) fail(OKS_E_INVALID_ARG, "bad crop arguments");
+        if (nz > 65535 || ny > 65535 || nx > 65535) fail(OKS_E_UNSUPPORTED, "volume too large");
+        const Win wz = crop_win(nz, tz), wy = crop_win(ny, ty), wx = crop_win(nx, tx);
+        OKS_HIP(launch_crop3d(in, (int)ny, (int)nx, (int)wz.off, (int)wy.off, (int)wx.off, (int)wz.len, (int)wy.len,
+                              (int)wx.len, out, ctx->stream));
+        out_shape[0] = wz.len;
+        out_shape[1] = wy.len;
+        out_shape[2] = wx.len;
     });
 }
 
