@@ -70,12 +70,6 @@ __global__ void __launch_bounds__(256) k_grappa_fills(GrappaFillArgs a) {
 }
 
 // ----------------------------------------------------- split-bf16 GEMM
-__device__ __forceinline__ unsigned short bf16_bits_rne(float x) {
-    unsigned int u = __float_as_uint(x);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
-__device__ __forceinline__ float bf16_to_f(unsigned short h) { return __uint_as_float((unsigned int)h << 16); }
 
 constexpr int BK = 32;
 constexpr int BKP = BK + 8;  // padded LDS row (bf16 elements)
@@ -85,31 +79,35 @@ struct Planes {
     unsigned short* p;
     int rows;
     // Eight consecutive k of one row: each plane written with one 16-byte store.
+    // hi = bf16(x) and lo = bf16(x - hi), both round-to-nearest-even in the
+    // hardware conversion (v_cvt_pk_bf16_f32); element 2q sits in the low half
+    // of word q.
     __device__ __forceinline__ void put8(int row, int k8, const float2 (&v)[8]) const {
-        u32x4 rh, rl, ih, il;
+        unsigned int wrh[4], wrl[4], wih[4], wil[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            unsigned int w[4][2];
+            unsigned int h[4][2];
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const float2 x = v[2 * q + h];
-                const unsigned short a = bf16_bits_rne(x.x), b = bf16_bits_rne(x.y);
-                w[0][h] = a;
-                w[1][h] = bf16_bits_rne(x.x - bf16_to_f(a));
-                w[2][h] = b;
-                w[3][h] = bf16_bits_rne(x.y - bf16_to_f(b));
+            for (int e = 0; e < 2; ++e) {
+                const float2 x = v[2 * q + e];
+                const __bf16 a = (__bf16)x.x, b = (__bf16)x.y;
+                const __bf16 al = (__bf16)(x.x - (float)a), bl = (__bf16)(x.y - (float)b);
+                h[0][e] = __builtin_bit_cast(unsigned short, a);
+                h[1][e] = __builtin_bit_cast(unsigned short, al);
+                h[2][e] = __builtin_bit_cast(unsigned short, b);
+                h[3][e] = __builtin_bit_cast(unsigned short, bl);
             }
-            rh[q] = w[0][0] | (w[0][1] << 16);
-            rl[q] = w[1][0] | (w[1][1] << 16);
-            ih[q] = w[2][0] | (w[2][1] << 16);
-            il[q] = w[3][0] | (w[3][1] << 16);
+            wrh[q] = h[0][0] | (h[0][1] << 16);
+            wrl[q] = h[1][0] | (h[1][1] << 16);
+            wih[q] = h[2][0] | (h[2][1] << 16);
+            wil[q] = h[3][0] | (h[3][1] << 16);
         }
         const int o = row * BKP + k8;
         const int ps = rows * BKP;
-        *(u32x4*)(p + o) = rh;
-        *(u32x4*)(p + ps + o) = rl;
-        *(u32x4*)(p + 2 * ps + o) = ih;
-        *(u32x4*)(p + 3 * ps + o) = il;
+        *(u32x4*)(p + o) = (u32x4){wrh[0], wrh[1], wrh[2], wrh[3]};
+        *(u32x4*)(p + ps + o) = (u32x4){wrl[0], wrl[1], wrl[2], wrl[3]};
+        *(u32x4*)(p + 2 * ps + o) = (u32x4){wih[0], wih[1], wih[2], wih[3]};
+        *(u32x4*)(p + 3 * ps + o) = (u32x4){wil[0], wil[1], wil[2], wil[3]};
     }
     __device__ __forceinline__ bf16x8 get(int plane, int row, int k8) const {
         return *(const bf16x8*)(p + plane * rows * BKP + row * BKP + k8);
@@ -123,6 +121,28 @@ __device__ __forceinline__ bf16x8 neg8(bf16x8 v) {
 }
 
 #define MFMA(a, b, c) __builtin_amdgcn_mfma_f32_16x16x32_bf16((a), (b), (c), 0, 0, 0)
+
+// Block coordinates in an XCD-aware order.  Workgroups go to the 8 XCDs
+// round-robin by linear id (x fastest), so ids L, L + 8, ... share one XCD and
+// its L2.  The remap hands XCD k one contiguous range of a z-major tile order
+// (G = 8 q + r tiles: XCDs below r take q + 1, the others q), so the tiles of
+// one slice share an L2 instead of every L2 holding every slice.  It is a
+// bijection of the grid: which tile a workgroup computes changes, not what.
+struct Blk {
+    int x, y, z;
+};
+__device__ __forceinline__ Blk xcd_block() {
+    const unsigned X = gridDim.x, Y = gridDim.y, Z = gridDim.z;
+    const unsigned G = X * Y * Z;
+    const unsigned L = blockIdx.x + X * (blockIdx.y + Y * blockIdx.z);
+    const unsigned q = G / 8, r = G % 8, xcd = L % 8, slot = L / 8;
+    const unsigned id = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + slot;
+    Blk b;
+    b.x = (int)(id % X);
+    b.y = (int)((id / X) % Y);
+    b.z = (int)(id / (X * Y));
+    return b;
+}
 
 // C[m, n] = sum_k A(m, k) B(k, n), complex.  BM = 64, BN = 32 * WN_TILES;
 // 4 wavefronts: BN = 64 -> 2 x 2 waves of 32 x 32; BN = 32 -> 4 x 1 waves of 16 x 32.
@@ -228,9 +248,10 @@ struct CalibLoader {
     const int* src_of;
     const int* tgt_of;
     __device__ CalibLoader(const Args& x) : a(x) {
-        m0 = blockIdx.x * 64;
-        n0 = blockIdx.y * (a.rhs ? 32 : 64);
-        z = blockIdx.z;
+        const Blk b = xcd_block();
+        m0 = b.x * 64;
+        n0 = b.y * (a.rhs ? 32 : 64);
+        z = b.z;
         base = a.grid + ((size_t)z * a.ny + a.acs0) * a.nx;
     }
     __device__ bool active() const { return a.rhs || m0 + 64 > n0; }
@@ -318,10 +339,11 @@ struct ApplyLoader {
     const float2* base;
     const int* src_of;
     __device__ ApplyLoader(const Args& x) : a(x) {
-        const int zd = blockIdx.z;
+        const Blk b = xcd_block();
+        const int zd = b.z;
         z = zd / (a.r - 1);
         d = zd - z * (a.r - 1) + 1;
-        const int f = blockIdx.y;
+        const int f = b.y;
         act = f < a.cnt[zd] && !(a.skip && a.skip[z]);
         ty = ky0 = 0;
         if (act) {
@@ -329,7 +351,7 @@ struct ApplyLoader {
             ty = e.x;
             ky0 = e.y;
         }
-        m0 = blockIdx.x * 64;
+        m0 = b.x * 64;
         base = a.grid + ((size_t)z * a.ny + ky0) * a.nx + m0;
     }
     __device__ bool active() const { return act; }
@@ -401,9 +423,10 @@ __global__ void __launch_bounds__(256) k_zgemm64(Zgemm64Args g) {
     constexpr int R = TS / 16;  // outputs per thread along m and n
     __shared__ double2 As[16][TS + 1];  // [k][m]
     __shared__ double2 Bs[16][TS + 1];  // [k][n]
-    const int b = blockIdx.z;
+    const Blk bk = xcd_block();
+    const int b = bk.z;
     if (g.skip && g.skip[b]) return;
-    const int m0 = blockIdx.x * TS, n0 = blockIdx.y * TS;
+    const int m0 = bk.x * TS, n0 = bk.y * TS;
     if (g.lower && n0 > m0 + TS - 1) return;
     const double2* A = g.A + (size_t)b * g.sA;
     const double2* B = g.B + (size_t)b * g.sB;
