@@ -34,8 +34,11 @@ namespace oks {
 // per column from their column-major copies (contiguous along the transform).
 // Largest workgroup a column kernel is built for: 512 for the lengths whose
 // transform needs more than the 128 registers per lane a 1024-thread
-// workgroup leaves (they would spill to scratch).
-constexpr int cs_cols_max_threads(int n) { return (n == 368 || n == 640) ? 512 : 1024; }
+// workgroup leaves (they would spill to scratch), 256 for 368, whose kernel
+// needs about 190 registers: at two wavefronts per SIMD, 256-thread
+// workgroups let two strips share a CU (one loading while the other
+// transforms) where a 512-thread workgroup would hold the CU alone.
+constexpr int cs_cols_max_threads(int n) { return n == 368 ? 256 : n == 640 ? 512 : 1024; }
 
 template <class F>
 __global__ void __launch_bounds__(cs_cols_max_threads(F::n)) k_cs_cols(CsColArgs a) {
