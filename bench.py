@@ -146,7 +146,9 @@ WORKLOADS = {"c1": workload_c1, "c2": workload_c2, "c3": workload_c3, "c4": work
 # FETCH_SIZE / WRITE_SIZE counters scaled by a copy kernel of known size
 # (profiles/hbm_traffic_counters.md): CS 73 B per pixel-iteration (41 algorithmic),
 # ifft-rss 61.9 MB per 15 x 368 x 640 slice (28.73 MB algorithmic).
-TRAFFIC_PER_ALGO_BYTE = {"c1": 61.9e6 / 28.73e6, "c3": 73.0 / 41.0, "served": 73.0 / 41.0}
+# HBM bytes moved per algorithmic byte, from the FETCH_SIZE / WRITE_SIZE
+# counter runs (calibrated on a copy kernel of known size)
+TRAFFIC_PER_ALGO_BYTE = {"c1": 60.8e6 / 28.73e6, "c3": 65.4 / 41.0, "served": 65.4 / 41.0}
 
 # Stages the served roofline can be taken on: (oks_stage id, bound, unit, peak).
 STAGES = {

@@ -54,10 +54,10 @@ __global__ void __launch_bounds__(cs_cols_max_threads(F::n)) k_cs_cols(CsColArgs
     const long long plane = blockIdx.x / strips;  // c * nz + z
     const int xs = (int)(blockIdx.x - plane * strips) * SX;
     const int z = (int)((a.plane0 + plane) % a.nz);
-    const int half = n / 2, hup = (n + 1) / 2;
+    const int half = n / 2;
     float2* T = a.T + (size_t)plane * n * a.nx;
-    const float2* YT = a.yT + (size_t)plane * a.nx * n;
-    const unsigned char* MT = a.mT + (size_t)z * a.nx * n;
+    const long long coil = (a.plane0 + plane) / a.nz;
+    const float2* YC = a.yC + coil * a.base[(long long)a.nz * a.nx];  // this coil's compact data
     for (int i = threadIdx.x; i < n * SX; i += blockDim.x) {
         const int y = i / SX, col = i - y * SX;
         float2 v = make_float2(0.f, 0.f);
@@ -70,24 +70,27 @@ __global__ void __launch_bounds__(cs_cols_max_threads(F::n)) k_cs_cols(CsColArgs
         float2* line = A + (size_t)col * ls;
         // line[i] will hold the transform at centred index yo = (i + ceil(n/2))
         // mod n; the residual goes back to the same slot, which is the
-        // ifftshift position of yo for the inverse transform.  The measured
-        // values and mask bits of this lane's slots are fetched before the
-        // forward transform so their latency hides behind it.
-        const float2* ycol = YT + (size_t)(xs + col) * n;
-        const unsigned char* mcol = MT + (size_t)(xs + col) * n;
+        // ifftshift position of yo for the inverse transform.  Chunk k's mask
+        // word has bit l set when slot l + 64 k is sampled; the measured value
+        // of a sampled slot is the compact entry at the number of sampled
+        // slots before it.  Both are fetched before the forward transform so
+        // their latency hides behind it.
+        const long long zx = (long long)z * a.nx + __builtin_amdgcn_readfirstlane(xs + col);
+        const unsigned long long* wcol = a.mB + zx * a.kw;
+        const float2* ycol = YC + a.base[zx];
         constexpr int PER = F::n ? (F::n + 63) / 64 : 1;
         float2 yv[PER];
-        unsigned char mv[PER];
+        bool mv[PER];
         if constexpr (F::n != 0) {
+            int run = 0;
 #pragma unroll
             for (int k = 0; k < PER; ++k) {
-                const int i = lane + 64 * k;
-                mv[k] = 0;
-                if (i < n) {
-                    const int yo = (i + hup) % n;
-                    mv[k] = mcol[yo];
-                    yv[k] = ycol[yo];
-                }
+                const unsigned long long word = wcol[k];
+                const int r = run + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(word >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((unsigned)word, 0u));
+                mv[k] = (word >> lane) & 1ull;
+                yv[k] = mv[k] ? ycol[r] : make_float2(0.f, 0.f);
+                run += __popcll(word);
             }
         }
         float2* res = F::run(line, B ? B + (size_t)col * ls : nullptr, a.p_fwd, tw, lane);
@@ -105,10 +108,17 @@ __global__ void __launch_bounds__(cs_cols_max_threads(F::n)) k_cs_cols(CsColArgs
                 }
             }
         } else {
-            for (int i = lane; i < n; i += 64) {
-                const int yo = (i + hup) % n;
-                const float2 v = cscale(line[i], a.scale);
-                line[i] = mcol[yo] ? csub(v, ycol[yo]) : make_float2(0.f, 0.f);
+            int run = 0;
+            for (int k = 0; k < a.kw; ++k) {
+                const unsigned long long word = wcol[k];
+                const int i = lane + 64 * k;
+                const int r = run + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(word >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((unsigned)word, 0u));
+                if (i < n) {
+                    const float2 v = cscale(line[i], a.scale);
+                    line[i] = ((word >> lane) & 1ull) ? csub(v, ycol[r]) : make_float2(0.f, 0.f);
+                }
+                run += __popcll(word);
             }
         }
         wave_sync();
@@ -122,6 +132,79 @@ __global__ void __launch_bounds__(cs_cols_max_threads(F::n)) k_cs_cols(CsColArgs
     for (int i = threadIdx.x; i < n * SX; i += blockDim.x) {
         const int yo = i / SX, col = i - yo * SX;
         if (xs + col < a.nx) T[(size_t)yo * a.nx + xs + col] = cscale(A[(size_t)col * ls + (yo + half) % n], a.scale);
+    }
+}
+
+// Mask bits in the column pass's lane order, from the column-major mask
+// [nz, nx, ny]: word k of column (z, x) has bit l set when the centred index
+// yo = (l + 64 k + ceil(ny/2)) mod ny is sampled (l + 64 k < ny).  cnt[(z, x)]
+// = sampled positions of the column.  One wavefront per column.
+__global__ void __launch_bounds__(256) k_cs_maskbits(const unsigned char* __restrict__ mT, unsigned long long* mB,
+                                                     long long* cnt, long long cols, int ny, int kw) {
+    const long long col = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (col >= cols) return;
+    const int lane = threadIdx.x & 63, hup = (ny + 1) / 2;
+    const unsigned char* m = mT + col * ny;
+    long long total = 0;
+    for (int k = 0; k < kw; ++k) {
+        const int i = lane + 64 * k;
+        const bool b = i < ny && m[(i + hup) % ny] != 0;
+        const unsigned long long word = __ballot(b);
+        if (lane == 0) mB[col * kw + k] = word;
+        total += __popcll(word);
+    }
+    if (lane == 0) cnt[col] = total;
+}
+
+// Exclusive prefix sum of v[0, n) in place, the total into v[n]; one workgroup.
+__global__ void __launch_bounds__(1024) k_cs_scan(long long* v, long long n) {
+    __shared__ long long buf[1024];
+    __shared__ long long carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (long long b0 = 0; b0 < n; b0 += 1024) {
+        const long long i = b0 + threadIdx.x;
+        const long long x = i < n ? v[i] : 0;
+        buf[threadIdx.x] = x;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            const long long t = (int)threadIdx.x >= o ? buf[threadIdx.x - o] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < n) v[i] = carry + buf[threadIdx.x] - x;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += buf[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) v[n] = carry;
+}
+
+// The measured k-space at the sampled positions only, in the column pass's
+// lane order: for plane p = c nz + z and column x, the r-th sampled slot of
+// the column goes to yC[c S + base[(z, x)] + r] (S = base[nz nx], one coil's
+// total).  yT is the column-major copy [planes, nx, ny].  One wavefront per
+// (plane, column).
+__global__ void __launch_bounds__(256) k_cs_compact(const float2* __restrict__ yT, const unsigned long long* __restrict__ mB,
+                                                    const long long* __restrict__ base, float2* __restrict__ yC,
+                                                    long long planes, int nz, int ny, int nx, int kw) {
+    const long long id = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);  // p nx + x
+    if (id >= planes * nx) return;
+    const long long p = id / nx;
+    const int x = (int)(id - p * nx), z = (int)(p % nz);
+    const long long c = p / nz, zx = (long long)z * nx + x;
+    const int lane = threadIdx.x & 63, hup = (ny + 1) / 2;
+    const unsigned long long* wb = mB + zx * kw;
+    float2* dst = yC + c * base[(long long)nz * nx] + base[zx];
+    const float2* src = yT + id * ny;
+    int run = 0;
+    for (int k = 0; k < kw; ++k) {
+        const unsigned long long word = wb[k];
+        const int r = run + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(word >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((unsigned)word, 0u));
+        if ((word >> lane) & 1ull) dst[r] = src[(lane + 64 * k + hup) % ny];
+        run += __popcll(word);
     }
 }
 
@@ -383,6 +466,29 @@ hipError_t launch_transpose_c64(const float2* in, float2* out, long long batch, 
 hipError_t launch_transpose_u8(const unsigned char* in, unsigned char* out, long long batch, int rows, int cols,
                                hipStream_t s) {
     return go_transpose(in, out, batch, rows, cols, s);
+}
+
+hipError_t launch_cs_maskbits(const unsigned char* mT, unsigned long long* mB, long long* cnt, long long cols, int ny,
+                              int kw, hipStream_t s) {
+    if (cols <= 0) return hipSuccess;
+    const long long grid = (cols + 3) / 4;
+    if (grid > 0x7fffffffLL || kw != (ny + 63) / 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_cs_maskbits, dim3((unsigned)grid), dim3(256), 0, s, mT, mB, cnt, cols, ny, kw);
+    return hipGetLastError();
+}
+
+hipError_t launch_cs_scan(long long* v, long long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_cs_scan, dim3(1), dim3(1024), 0, s, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_cs_compact(const float2* yT, const unsigned long long* mB, const long long* base, float2* yC,
+                             long long planes, int nz, int ny, int nx, int kw, hipStream_t s) {
+    const long long grid = (planes * nx + 3) / 4;
+    if (grid <= 0) return hipSuccess;
+    if (grid > 0x7fffffffLL || kw != (ny + 63) / 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_cs_compact, dim3((unsigned)grid), dim3(256), 0, s, yT, mB, base, yC, planes, nz, ny, nx, kw);
+    return hipGetLastError();
 }
 
 hipError_t launch_copy2(const float2* a, float2* b, float2* c, long long n, hipStream_t s) {

@@ -9,8 +9,10 @@ namespace oks {
 
 struct CsColArgs {
     float2* T;                  // [planes, ny, nx] in/out
-    const float2* yT;           // [planes, nx, ny] measured k-space, column-major
-    const unsigned char* mT;    // [nz, nx, ny] mask, column-major
+    const float2* yC;           // measured k-space at the sampled positions only (k_cs_compact)
+    const unsigned long long* mB;  // [nz, nx, kw] mask bits in the pass's lane order (k_cs_maskbits)
+    const long long* base;      // [nz * nx + 1] first compact index of column (z, x); [nz nx] = one coil's total
+    int kw;                     // 64-position chunks per column, ceil(ny / 64)
     long long planes;
     long long plane0;           // index of the first plane (mask plane = (plane0 + p) % nz)
     int nz, nx, sx;
@@ -39,6 +41,11 @@ hipError_t launch_cs_update(const CsUpdArgs& a, hipStream_t s);
 hipError_t launch_rss_crop(const float2* x, int nc, int nz, int ny, int nx, int z0, int tz, int y0, int ty, int x0,
                            int tx, float* out, hipStream_t s);
 hipError_t launch_copy2(const float2* a, float2* b, float2* c, long long n, hipStream_t s);
+hipError_t launch_cs_maskbits(const unsigned char* mT, unsigned long long* mB, long long* cnt, long long cols, int ny,
+                              int kw, hipStream_t s);
+hipError_t launch_cs_scan(long long* v, long long n, hipStream_t s);
+hipError_t launch_cs_compact(const float2* yT, const unsigned long long* mB, const long long* base, float2* yC,
+                             long long planes, int nz, int ny, int nx, int kw, hipStream_t s);
 hipError_t launch_transpose_c64(const float2* in, float2* out, long long batch, int rows, int cols, hipStream_t s);
 hipError_t launch_transpose_u8(const unsigned char* in, unsigned char* out, long long batch, int rows, int cols,
                                hipStream_t s);

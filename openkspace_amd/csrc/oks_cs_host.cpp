@@ -44,8 +44,9 @@ struct CsProblem {
     float2* y;                   // measured k-space [planes, ny, nx]
     const unsigned char* mask;   // [mask_planes, ny, nx]
     float2 *x, *z, *T;
-    float2* yT;                  // [planes, nx, ny] column-major copy of y
+    float2* yC;                  // y at the sampled positions only (planes * ny * nx slots at most)
     unsigned char* mT;           // [mask_planes, nx, ny] column-major copy of the mask
+    long long* bits;             // column offsets [mask_planes * nx + 1], then mask words [mask_planes * nx * kw]
     long long planes;
     int mask_planes, ny, nx, iters;
     float lambda;
@@ -58,12 +59,31 @@ static long long cs_chunk(const oks_ctx* ctx, long long planes) {
     return ctx->cs_chunk > 0 ? std::min<long long>(ctx->cs_chunk, planes) : planes;
 }
 
+// 8-byte slots of CsProblem::bits.
+static size_t cs_bits_slots(long long mask_planes, int ny, int nx) {
+    const size_t cols = (size_t)mask_planes * nx;
+    return cols + 1 + cols * (size_t)((ny + 63) / 64);
+}
+
 static void cs_iterations(oks_ctx* ctx, const CsProblem& P, const std::vector<float>& alpha) {
     const float sx_ = 1.0f / std::sqrt((float)P.nx), sy_ = 1.0f / std::sqrt((float)P.ny);
     const size_t plane = (size_t)P.ny * P.nx;
+    const int kw = (P.ny + 63) / 64;
+    const long long cols = (long long)P.mask_planes * P.nx;
+    long long* base = P.bits;
+    unsigned long long* words = (unsigned long long*)(P.bits + cols + 1);
+    // mask words, per-column offsets and the compact measured data, once per
+    // call (T is free until the first row pass and holds the column-major y)
     OKS_HIP(launch_transpose_u8(P.mask, P.mT, P.mask_planes, P.ny, P.nx, ctx->stream));
+    OKS_HIP(launch_cs_maskbits(P.mT, words, base, cols, P.ny, kw, ctx->stream));
+    OKS_HIP(launch_cs_scan(base, cols, ctx->stream));
+    OKS_HIP(launch_transpose_c64(P.y, P.T, P.planes, P.ny, P.nx, ctx->stream));
+    OKS_HIP(launch_cs_compact(P.T, words, base, P.yC, P.planes, P.mask_planes, P.ny, P.nx, kw, ctx->stream));
     CsColArgs ca{};
-    ca.mT = P.mT;
+    ca.yC = P.yC;
+    ca.mB = words;
+    ca.base = base;
+    ca.kw = kw;
     ca.nz = P.mask_planes;
     ca.nx = P.nx;
     ca.sx = cs_sx(ctx, P.ny);
@@ -81,12 +101,10 @@ static void cs_iterations(oks_ctx* ctx, const CsProblem& P, const std::vector<fl
     for (long long p0 = 0; p0 < P.planes; p0 += chunk) {
         const long long np = std::min(chunk, P.planes - p0);
         const size_t off = (size_t)p0 * plane;
-        float2 *T = P.T + off, *x = P.x + off, *z = P.z + off, *yT = P.yT + off;
-        OKS_HIP(launch_transpose_c64(P.y + off, yT, np, P.ny, P.nx, ctx->stream));
+        float2 *T = P.T + off, *x = P.x + off, *z = P.z + off;
         // T = centred row transform of z; later iterations get it from the update pass
         row_pass(ctx, z, T, np * P.ny, P.nx, P.nx, 0, sx_, false, true, nullptr, 0);
         ca.T = T;
-        ca.yT = yT;
         ca.planes = np;
         ca.plane0 = p0;
         ua.T = T;
@@ -132,8 +150,9 @@ static void run_fista(oks_ctx* ctx, const CsProblem& P) {
     key.p[2] = P.x;
     key.p[3] = P.z;
     key.p[4] = P.T;
-    key.p[5] = P.yT;
+    key.p[5] = P.yC;
     key.p[6] = P.mT;
+    key.p[7] = P.bits;
     key.i[0] = P.planes;
     key.i[1] = P.mask_planes;
     key.i[2] = P.ny;
@@ -212,8 +231,9 @@ void recon_cs_dev(oks_ctx* ctx, const oks_cs_desc* d, const oks_kspace* k, oks_i
         P.x = (float2*)ctx->buf("cs_x", sizeof(float2) * tot);
         P.z = (float2*)ctx->buf("cs_z", sizeof(float2) * tot);
         P.T = (float2*)ctx->buf("cs_t", sizeof(float2) * tot);
-        P.yT = (float2*)ctx->buf("cs_yT", sizeof(float2) * tot);
+        P.yC = (float2*)ctx->buf("cs_yC", sizeof(float2) * tot);
         P.mT = (unsigned char*)ctx->buf("cs_mT", cells);
+        P.bits = (long long*)ctx->buf("cs_bits", sizeof(long long) * cs_bits_slots(nz, ny, nx));
         P.planes = (long long)nc * nz;
         P.mask_planes = nz;
         P.ny = ny;
@@ -257,8 +277,9 @@ int oks_fista_cs_dev(oks_ctx* ctx, const oks_c32* kzf, const uint8_t* mask, int6
         P.x = (float2*)out;
         P.z = (float2*)ctx->buf("fcs_z", sizeof(float2) * tot);
         P.T = (float2*)ctx->buf("fcs_t", sizeof(float2) * tot);
-        P.yT = (float2*)ctx->buf("cs_yT", sizeof(float2) * tot);
+        P.yC = (float2*)ctx->buf("cs_yC", sizeof(float2) * tot);
         P.mT = (unsigned char*)ctx->buf("cs_mT", (size_t)mask_planes * ny * nx);
+        P.bits = (long long*)ctx->buf("cs_bits", sizeof(long long) * cs_bits_slots(mask_planes, (int)ny, (int)nx));
         P.planes = planes;
         P.mask_planes = (int)mask_planes;
         P.ny = (int)ny;

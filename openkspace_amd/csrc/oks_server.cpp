@@ -121,7 +121,9 @@ int64_t footprint(const Req& q) {
             }
             break;
         }
-        case OKS_STRATEGY_CS: f += 4 * grid + 2 * nz * ny * nx; break;  // x, z, T, column-major y; masks
+        case OKS_STRATEGY_CS:  // x, z, T, compact y; mask and its column-major copy; mask words and offsets
+            f += 4 * grid + 2 * nz * ny * nx + nz * nx * ((ny + 63) / 64 + 1) * 8 + 8;
+            break;
         case OKS_STRATEGY_SENSE: f += 3 * grid + 2 * out; break;
         default: break;
     }
