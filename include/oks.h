@@ -159,6 +159,29 @@ typedef struct oks_pattern {
  * where the reference returns None.  Host-only integer logic. */
 int oks_detect_pattern(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx, oks_pattern* out);
 
+/* ----------------------------------------------------- readout placement */
+/* An image-pass acquisition as read_kspace_with_mask reads it: the
+ * AcquisitionHeader fields it uses (io/ismrmrd/acquisition.rs:20-64) and where
+ * its samples start in the data buffer ([active_channels][number_of_samples],
+ * channel-major, complex64), in complex samples. */
+typedef struct oks_acq {
+    uint64_t flags;  /* ISMRMRD acquisition flags (NON_IMAGE_MASK, ACQ_IS_REVERSE) */
+    uint16_t number_of_samples, active_channels, discard_pre, discard_post, center_sample;
+    uint16_t kspace_encode_step_1, kspace_encode_step_2, slice;
+    int64_t data_offset;
+} oks_acq;
+
+/* The encoding fields placement and oversampling removal read
+ * (io/ismrmrd/header.rs: encoded_matrix, recon_matrix.x, ky/kz_limit.center). */
+typedef struct oks_encoding {
+    int32_t encoded_x, encoded_y, encoded_z, recon_x, ky_center, kz_center;
+} oks_encoding;
+
+/* Sizing pass of read_kspace_with_mask (io/ismrmrd/reader.rs:235-287): the
+ * grid shape [nc, nz, ny, nx].  OKS_E_MISSING_FIELD where the reference
+ * errors (no image acquisition, no channels).  Host arithmetic. */
+int oks_place_shape(const oks_acq* acqs, int64_t n, const oks_encoding* enc, int64_t shape[4]);
+
 /* ------------------------------------------------------------- context */
 /* Opaque per-device state: stream, FFT plans, twiddles, workspaces, cached
  * HIP graphs (replaces the reference's per-call FftPlanner construction,
@@ -286,6 +309,20 @@ int oks_walsh_maps_dev(oks_ctx* ctx, const oks_c32* img, int64_t nc, int64_t nz,
  * kernel above 8. */
 int oks_espirit_maps_dev(oks_ctx* ctx, const oks_c32* acs, int64_t nc, int64_t acs_y, int64_t acs_x, int64_t ny,
                          int64_t nx, int64_t kernel, float threshold, int64_t iters, oks_c32* maps, int64_t* kept);
+/* read_kspace_with_mask (io/ismrmrd/reader.rs:225-468) with the strategies'
+ * preprocess closure (recon/strategy.rs:153-161).  Every image acquisition is
+ * whitened by W when it has W's channel count (W: device [w_nc, w_nc] or
+ * NULL) and, with remove_oversampling and a valid encoded_x / recon_x pair,
+ * loses its readout oversampling when it carries encoded_x samples; then it is
+ * placed: DC offsets, centre sample and discards, first writer wins at the
+ * readout centre, reversed readouts flipped.  acqs and mask are host memory,
+ * data (n_data complex samples) and grid (the oks_place_shape shape) device
+ * memory; mask [nz, ny, nx] and placed may be NULL.  OKS_E_INCONSISTENT when
+ * an image acquisition's samples fall outside data.  Navigator phase
+ * correction is not applied (no navigator estimate is formed). */
+int oks_place_acquisitions_dev(oks_ctx* ctx, const oks_acq* acqs, int64_t n, const oks_c32* data, int64_t n_data,
+                               const oks_encoding* enc, const oks_c32* W, int32_t w_nc, int32_t remove_oversampling,
+                               oks_c32* grid, uint8_t* mask, int64_t* placed);
 /* fista_cs_single_coil (recon/cs.rs:59-139) on `planes` independent images:
  * kzf [planes, ny, nx], mask [mask_planes (1 or planes), ny, nx] -> out. */
 int oks_fista_cs_dev(oks_ctx* ctx, const oks_c32* kzf, const uint8_t* mask, int64_t planes, int64_t mask_planes,

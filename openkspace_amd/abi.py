@@ -83,6 +83,30 @@ class GrappaDesc(C.Structure):
         return d
 
 
+# oks_acq: the AcquisitionHeader fields read_kspace_with_mask reads
+# (io/ismrmrd/acquisition.rs) plus the offset of the acquisition's samples.
+ACQ_DTYPE = np.dtype([("flags", "<u8"), ("number_of_samples", "<u2"), ("active_channels", "<u2"),
+                      ("discard_pre", "<u2"), ("discard_post", "<u2"), ("center_sample", "<u2"),
+                      ("kspace_encode_step_1", "<u2"), ("kspace_encode_step_2", "<u2"), ("slice", "<u2"),
+                      ("data_offset", "<i8")])
+
+
+class Acq(C.Structure):
+    _fields_ = [(name, {"<u8": C.c_uint64, "<u2": C.c_uint16, "<i8": C.c_int64}[ACQ_DTYPE.fields[name][0].str])
+                for name in ACQ_DTYPE.names]
+
+
+class Encoding(C.Structure):
+    _fields_ = [("encoded_x", C.c_int32), ("encoded_y", C.c_int32), ("encoded_z", C.c_int32),
+                ("recon_x", C.c_int32), ("ky_center", C.c_int32), ("kz_center", C.c_int32)]
+
+
+ACQ_IS_NOISE_MEASUREMENT = 1 << 18
+ACQ_IS_NAVIGATION_DATA = 1 << 22
+ACQ_IS_PHASECORR_DATA = 1 << 23
+ACQ_IS_REVERSE = 1 << 21
+
+
 class SenseDesc(C.Structure):
     _fields_ = [("remove_oversampling", C.c_int32), ("prewhiten", C.c_int32),
                 ("phase_correct", C.c_int32), ("map_source", C.c_int32),

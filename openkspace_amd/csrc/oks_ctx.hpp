@@ -66,6 +66,7 @@ struct oks_ctx {
     std::vector<uint8_t> h_acq;                    // host staging: acquired rows
     std::vector<uint8_t> h_mask;                   // host staging: per-cell mask
     std::vector<uint8_t> h_batch_mask;             // host staging: a server batch's masks
+    std::vector<unsigned char> h_items;            // host staging: readout placement table
 
     // Instantiated HIP graphs keyed by what their captured launches depend on.
     struct GraphKey {

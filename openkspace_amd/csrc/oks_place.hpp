@@ -1,0 +1,34 @@
+// oks_place.hpp -- argument blocks of the readout placement kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "oks_fft.hpp"
+
+namespace oks {
+
+constexpr int kPlaceReverse = 1;  // ACQ_IS_REVERSE: kept samples time-reversed
+constexpr int kPlaceWhiten = 2;   // acquisition has W's channel count
+constexpr int kPlaceOs = 4;       // readout-oversampling removal applies (ns == encoded_x)
+
+// One placed readout: samples data[src + c * ns + s], destination
+// grid[c][kz][ky][dst, dst + len) from processed sample dpre on.
+struct PlaceItem {
+    long long src;
+    int ns, nc_fit, kz, ky, dst, len, dpre, flags;
+};
+
+struct PlaceArgs {
+    const float2* data;
+    const PlaceItem* items;
+    const float2* W;  // [nc, nc] lower triangular (kPlaceWhiten items)
+    float2* grid;     // [nc, nz, ny, nx], zeroed beforehand
+    int nz, ny, nx;
+    int enc, rec;     // oversampling removal enc -> rec (0 when off)
+    FftPlan p_enc, p_rec;
+};
+
+size_t place_lds(int enc, int wpb);
+hipError_t launch_place(const PlaceArgs& a, long long n_items, int wpb, hipStream_t s);
+
+}  // namespace oks

@@ -68,3 +68,44 @@ def recon_dev(ctx, fn_name, desc, case):
         return o, flags, g
     finally:
         dc.free()
+
+
+def place_shape(acqs, enc):
+    """Sizing pass of read_kspace_with_mask (io/ismrmrd/reader.rs:235-287) on an
+    acquisition table of abi.ACQ_DTYPE: the grid shape (nc, nz, ny, nx), or None
+    where the reference errors (no image acquisition)."""
+    a = np.ascontiguousarray(acqs, dtype=abi.ACQ_DTYPE)
+    e = abi.Encoding(*[int(v) for v in enc])
+    shape = np.zeros(4, np.int64)
+    st = lib().oks_place_shape(a.ctypes.data, len(a), C.byref(e), shape.ctypes.data)
+    if st == abi.OKS_E_MISSING_FIELD:
+        return None
+    check(None, st)
+    return tuple(int(v) for v in shape)
+
+
+def place_acquisitions(ctx, acqs, data, enc, W=None, remove_oversampling=False):
+    """read_kspace_with_mask (io/ismrmrd/reader.rs:225-468) with the strategies'
+    preprocess closure (recon/strategy.rs:153-161) on device-resident samples.
+
+    acqs: host acquisition table (abi.ACQ_DTYPE); data: DeviceArray of complex64
+    samples; enc: (encoded_x, encoded_y, encoded_z, recon_x, ky_center,
+    kz_center); W: DeviceArray [w_nc, w_nc] or None.  Returns (grid DeviceArray
+    [nc, nz, ny, nx], mask bool [nz, ny, nx], placed count)."""
+    a = np.ascontiguousarray(acqs, dtype=abi.ACQ_DTYPE)
+    shape = place_shape(a, enc)
+    if shape is None:
+        raise ValueError("no image acquisitions found")
+    e = abi.Encoding(*[int(v) for v in enc])
+    grid = DeviceArray(ctx, shape, np.complex64)
+    mask = np.zeros(shape[1:], np.uint8)
+    placed = C.c_int64(0)
+    try:
+        check(ctx.ptr, lib().oks_place_acquisitions_dev(
+            ctx.ptr, a.ctypes.data, len(a), data.addr, int(np.prod(data.shape)), C.byref(e),
+            W.addr if W is not None else None, W.shape[0] if W is not None else 0, int(bool(remove_oversampling)),
+            grid.addr, mask.ctypes.data, C.byref(placed)))
+    except Exception:
+        grid.free()
+        raise
+    return grid, mask.astype(bool), int(placed.value)

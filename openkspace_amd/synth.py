@@ -212,3 +212,53 @@ def make_config(name, **over):
     p = dict(CONFIGS[name])
     p.update(over)
     return make_case(**p)
+
+
+def acquisitions(kspace, mask, *, slice_axis=False, ky_center=None, kz_center=None, reverse_every=0,
+                 discard=(0, 0), shuffle_seed=None, noise_scans=0, recon_x=0, seed=SEED_BASE):
+    """The readouts an ISMRMRD file would hold for `kspace` [nc, nz, ny, nx]:
+    one image acquisition per sampled (z, y) row, placed back onto that row by
+    read_kspace_with_mask (io/ismrmrd/reader.rs:331-465).
+
+    Rows land at ky = step_1 + ny/2 - ky_center (and likewise kz, or idx.slice
+    for multi-slice 2-D data).  Every `reverse_every`-th readout carries
+    ACQ_IS_REVERSE with its kept samples stored reversed; `discard` pads each
+    readout with that many extra samples before and after the row, and
+    center_sample marks the row's DC sample (nx/2 + discard_pre).  The sizing
+    pass uses the raw sample count, so the placed grid is max(nx, nx +
+    discard_pre + discard_post) wide with the row at nx'/2 - nx/2.  `noise_scans` noise
+    acquisitions are interleaved (they must be skipped).  Returns
+    (acquisition table of abi.ACQ_DTYPE, samples complex64,
+     encoding (encoded_x, encoded_y, encoded_z, recon_x, ky_center, kz_center)).
+    """
+    from openkspace_amd import abi
+    nc, nz, ny, nx = kspace.shape
+    kyc = ny // 2 if ky_center is None else ky_center
+    kzc = nz // 2 if kz_center is None else kz_center
+    dpre, dpost = discard
+    ns = nx + dpre + dpost
+    rng = np.random.default_rng(seed)
+    rows = [(z, y) for z in range(nz) for y in range(ny) if mask[z, y].any()]
+    if shuffle_seed is not None:
+        np.random.default_rng(shuffle_seed).shuffle(rows)
+    n = len(rows) + noise_scans
+    acq = np.zeros(n, abi.ACQ_DTYPE)
+    data = np.empty(n * nc * ns, np.complex64)
+    noise_at = set(np.linspace(0, n - 1, noise_scans).astype(int).tolist()) if noise_scans else set()
+    off, r = 0, 0
+    for i in range(n):
+        seg = (rng.standard_normal((nc, ns)) + 1j * rng.standard_normal((nc, ns))).astype(np.complex64)
+        if i in noise_at:
+            acq[i] = (abi.ACQ_IS_NOISE_MEASUREMENT, ns, nc, 0, 0, 0, 0, 0, 0, off)
+        else:
+            z, y = rows[r]
+            r += 1
+            rev = reverse_every > 0 and r % reverse_every == 0
+            line = kspace[:, z, y, :]
+            seg[:, dpre:dpre + nx] = line[:, ::-1] if rev else line
+            ky = y - ny // 2 + kyc
+            kz, sl = (0, z) if slice_axis else (z - nz // 2 + kzc, 0)
+            acq[i] = (abi.ACQ_IS_REVERSE if rev else 0, ns, nc, dpre, dpost, nx // 2 + dpre, ky, kz, sl, off)
+        data[off:off + nc * ns] = seg.reshape(-1)
+        off += nc * ns
+    return acq, data, (nx, ny, nz, recon_x, kyc, kzc)

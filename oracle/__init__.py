@@ -54,11 +54,14 @@ def lib():
             getattr(L, f"oko_soft_threshold_{s}").argtypes = [_P, _I64, _I64, C.c_double]
             getattr(L, f"oko_centred_fft2_{s}").argtypes = [_P, _I64, _I64, C.c_int]
             getattr(L, f"oko_fista_cs_{s}").argtypes = [_P, _P, _I64, _I64, _I64, C.c_double, _P]
+            getattr(L, f"oko_place_{s}").argtypes = [_P, _I64, _P, _P, _P, C.c_int, C.c_int, _P, _P]
+            getattr(L, f"oko_place_{s}").restype = _I64
             for n in ("ifft_rss", "grappa", "cs"):
                 getattr(L, f"oko_recon_{n}_{s}").argtypes = [_P, _P, _P, _P, _P]
             getattr(L, f"oko_recon_sense_{s}").argtypes = [_P, _P, _P, _P, _P, _P]
         L.oko_set_threads.argtypes = [C.c_int]
         L.oko_detect_pattern.argtypes = [_P, _I64, _I64, _I64, _P]
+        L.oko_place_shape.argtypes = [_P, _I64, _P, _P, _P]
         L.oko_pattern_detect_1d.argtypes = [_P, _I64, _P]
         L.oko_pf_detect.argtypes = [_P, _I64, _I64, _I64, _I64]
         L.oko_grappa_geom.argtypes = [_I64] * 6 + [_P]
@@ -159,6 +162,31 @@ def os_remove(lines, rec, prec="f32"):
     out = np.empty((nc, rec), dtype=_ct(prec))
     st = getattr(lib(), f"oko_os_remove_{prec}")(a.ctypes.data, nc, enc, rec, out.ctypes.data)
     return out if st == 0 else None
+
+
+def place(acqs, data, enc, W=None, remove_os=False, prec="f32"):
+    """read_kspace_with_mask with the strategies' preprocess closure on an
+    acquisition table (numpy array of openkspace_amd.abi.ACQ_DTYPE records,
+    samples at data[data_offset:], [nc][ns] channel-major).
+    enc = (encoded_x, encoded_y, encoded_z, recon_x, ky_center, kz_center).
+    Returns (grid [nc, nz, ny, nx], mask [nz, ny, nx] bool, placed count),
+    or None where the reference errors (no image acquisition)."""
+    a = np.ascontiguousarray(acqs)
+    assert a.dtype.itemsize == 32
+    d = _c(np.asarray(data).reshape(-1), prec)
+    e = np.ascontiguousarray(enc, dtype=np.int64)
+    shape = np.zeros(4, np.int64)
+    sa = C.c_int(0)
+    if lib().oko_place_shape(a.ctypes.data, len(a), e.ctypes.data, shape.ctypes.data, C.byref(sa)) != 0:
+        return None
+    grid = np.zeros(tuple(shape), dtype=_ct(prec))
+    mask = np.zeros(tuple(shape[1:]), dtype=np.uint8)
+    Wc = _c(W, prec) if W is not None else None
+    n = getattr(lib(), f"oko_place_{prec}")(a.ctypes.data, len(a), d.ctypes.data, e.ctypes.data,
+                                             Wc.ctypes.data if Wc is not None else None,
+                                             Wc.shape[0] if Wc is not None else 0, int(bool(remove_os)),
+                                             grid.ctypes.data, mask.ctypes.data)
+    return grid, mask.astype(bool), int(n)
 
 
 def detect_pattern(mask):

@@ -447,6 +447,15 @@ OKO_EXPORT int oko_get_threads(void) { return n_threads(); }
     OKO_EXPORT int oko_prewhiten_calibrate_##S(const T* noise, int nc, int64_t ns, T* W) {   \
         return prewhiten_calibrate((const cx<T>*)noise, nc, ns, (cx<T>*)W) ? 0 : 1;          \
     }                                                                                        \
+    OKO_EXPORT int64_t oko_place_##S(const Acq* a, int64_t n, const T* data, const int64_t enc[6], \
+                                     const T* W, int w_nc, int remove_os, T* grid, uint8_t* mask) { \
+        int64_t shape[4];                                                                    \
+        bool sa = false;                                                                     \
+        if (!place_shape(a, n, Encoding{enc[0], enc[1], enc[2], enc[3], enc[4], enc[5]}, shape, &sa)) \
+            return -1;                                                                       \
+        return place(a, n, (const cx<T>*)data, Encoding{enc[0], enc[1], enc[2], enc[3], enc[4], enc[5]}, \
+                     (const cx<T>*)W, w_nc, remove_os != 0, shape, sa, (cx<T>*)grid, mask);  \
+    }                                                                                        \
     OKO_EXPORT void oko_prewhiten_apply_##S(const T* W, int nc, T* data, int64_t ns) {       \
         prewhiten_apply((const cx<T>*)W, nc, (cx<T>*)data, ns);                              \
     }                                                                                        \
@@ -527,6 +536,13 @@ OKO_EXPORT int oko_get_threads(void) { return n_threads(); }
     }
 
 OKO_BOTH(DEF_FFT)
+
+OKO_EXPORT int oko_place_shape(const Acq* a, int64_t n, const int64_t enc[6], int64_t shape[4], int* slice_axis) {
+    bool sa = false;
+    if (!place_shape(a, n, Encoding{enc[0], enc[1], enc[2], enc[3], enc[4], enc[5]}, shape, &sa)) return 1;
+    *slice_axis = sa ? 1 : 0;
+    return 0;
+}
 
 OKO_EXPORT int oko_detect_pattern(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx,
                                   int64_t out[5]) {

@@ -330,6 +330,102 @@ void os_remove_line(const cx<T>* in, int64_t enc, int64_t rec, cx<T>* out,
 }
 
 // -------------------------------------------------------------- GRAPPA
+// ------------------------------------------------------------- placement
+// An image-pass acquisition as read_kspace_with_mask reads it: the
+// AcquisitionHeader fields it uses (io/ismrmrd/acquisition.rs) and the
+// offset of its [nc][ns] channel-major samples.
+struct Acq {
+    uint64_t flags;
+    uint16_t ns, nc, dpre, dpost, cs, ky, kz, slice;
+    int64_t offset;
+};
+// NON_IMAGE_MASK and ACQ_IS_REVERSE (io/ismrmrd/acquisition.rs:70-94), bit
+// b of the ISMRMRD flag list at 1 << (b - 1).
+constexpr uint64_t kNonImage = (1ull << 18) | (1ull << 23) | (1ull << 22) | (1ull << 27) | (1ull << 25) |
+                               (1ull << 26) | (1ull << 28);
+constexpr uint64_t kReverse = 1ull << 21;
+
+struct Encoding {
+    int64_t ex, ey, ez, rec_x, ky_center, kz_center;
+};
+
+inline int64_t sat_sub(int64_t a, int64_t b) { return a > b ? a - b : 0; }
+
+// Sizing pass of read_kspace_with_mask (io/ismrmrd/reader.rs:235-287):
+// shape [nc, nz, ny, nx] and whether idx.slice is the third axis.  Returns
+// false where the reference errors (no image acquisition, 0 channels).
+inline bool place_shape(const Acq* a, int64_t n, const Encoding& e, int64_t shape[4], bool* slice_axis) {
+    int64_t max_ky = 0, max_kz = 0, max_slice = 0, max_ns = 0, nc = 0, n_image = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (a[i].flags & kNonImage) continue;
+        ++n_image;
+        max_ky = std::max<int64_t>(max_ky, a[i].ky);
+        max_kz = std::max<int64_t>(max_kz, a[i].kz);
+        max_slice = std::max<int64_t>(max_slice, a[i].slice);
+        max_ns = std::max<int64_t>(max_ns, a[i].ns);
+        nc = std::max<int64_t>(nc, a[i].nc);
+    }
+    if (n_image == 0 || nc == 0) return false;
+    const bool sa = max_kz == 0 && max_slice > 0;
+    shape[0] = nc;
+    shape[1] = sa ? std::max<int64_t>(max_slice + 1, 1) : std::max<int64_t>(std::max(e.ez, max_kz + 1), 1);
+    shape[2] = std::max<int64_t>(std::max(e.ey, max_ky + 1), 1);
+    shape[3] = std::max(e.ex, max_ns);
+    *slice_axis = sa;
+    return true;
+}
+
+// Placement pass (io/ismrmrd/reader.rs:331-465) with the strategies'
+// preprocess closure (recon/strategy.rs:153-161): whitening when the
+// acquisition has W's channel count (recon/prewhiten.rs:118-142), then
+// readout-oversampling removal when valid and the line has encoded_x samples
+// (recon/oversampling.rs:103-154), which leaves rec samples, centre rec / 2
+// and no discards.  grid [nc, nz, ny, nx] and mask [nz, ny, nx] start zeroed.
+// Returns the number of placed acquisitions.
+template <class T>
+int64_t place(const Acq* a, int64_t n, const cx<T>* data, const Encoding& e, const cx<T>* W, int w_nc,
+              bool remove_os, const int64_t shape[4], bool slice_axis, cx<T>* grid, uint8_t* mask) {
+    const int64_t nc = shape[0], nz = shape[1], ny = shape[2], nx = shape[3];
+    const int64_t ky_off = ny / 2 - e.ky_center, kz_off = slice_axis ? 0 : nz / 2 - e.kz_center;
+    const bool os = remove_os && os_valid(e.ex, e.rec_x);
+    Fft<T> fi(os ? (int)e.ex : 1, +1), ff(os ? (int)e.rec_x : 1, -1);
+    std::vector<cx<T>> work, tmp;
+    int64_t placed = 0;
+    for (int64_t q = 0; q < n; ++q) {
+        const Acq& h = a[q];
+        if (h.flags & kNonImage) continue;
+        int64_t ns = h.ns, cs = h.cs, dpre = h.dpre, dpost = h.dpost;
+        const int64_t anc = h.nc;
+        work.assign(data + h.offset, data + h.offset + anc * ns);
+        if (W && anc == w_nc) prewhiten_apply(W, w_nc, work.data(), ns);
+        if (os && ns == e.ex) {
+            tmp.assign((size_t)anc * e.rec_x, cx<T>{});
+            for (int64_t c = 0; c < anc; ++c) os_remove_line(work.data() + c * ns, ns, e.rec_x, tmp.data() + c * e.rec_x, fi, ff);
+            work.swap(tmp);
+            ns = e.rec_x;
+            cs = e.rec_x / 2;
+            dpre = dpost = 0;
+        }
+        const int64_t ky = (int64_t)h.ky + ky_off, kz = (int64_t)(slice_axis ? h.slice : h.kz) + kz_off;
+        if (ky < 0 || ky >= ny || kz < 0 || kz >= nz) continue;
+        const int64_t keep = sat_sub(ns, dpre + dpost);
+        const int64_t dst = (cs > 0 && cs >= dpre) ? sat_sub(nx / 2, cs - dpre) : sat_sub(nx, keep) / 2;
+        const int64_t len = std::min(std::min(keep, sat_sub(ns, dpre)), sat_sub(nx, dst));
+        if (len == 0) continue;
+        uint8_t* mrow = mask + (kz * ny + ky) * nx;
+        if (mrow[dst + len / 2]) continue;  // first writer wins at the readout centre
+        const bool rev = (h.flags & kReverse) != 0;
+        for (int64_t c = 0; c < std::min(anc, nc); ++c) {
+            cx<T>* row = grid + ((c * nz + kz) * ny + ky) * nx + dst;
+            const cx<T>* src = work.data() + c * ns + dpre;
+            for (int64_t i = 0; i < len; ++i) row[i] = src[rev ? len - 1 - i : i];
+        }
+        for (int64_t i = 0; i < len; ++i) mrow[dst + i] = 1;
+        ++placed;
+    }
+    return placed;
+}
+
 struct Pattern {
     int64_t r, acs_start, acs_end, ky_lo, ky_hi;
 };

@@ -38,7 +38,14 @@ STRUCTS = {
     "oks_ifft_rss_desc": abi.IfftRssDesc, "oks_grappa_desc": abi.GrappaDesc, "oks_sense_desc": abi.SenseDesc,
     "oks_cs_desc": abi.CsDesc, "oks_pattern": abi.Pattern, "oks_server_config": abi.ServerConfig,
     "oks_request": abi.Request, "oks_completion": abi.Completion, "oks_server_stats": abi.ServerStats,
+    "oks_acq": abi.Acq, "oks_encoding": abi.Encoding,
 }
+
+
+def test_acquisition_table_dtype_matches_struct():
+    assert abi.ACQ_DTYPE.itemsize == C.sizeof(abi.Acq) == 32
+    for name in abi.ACQ_DTYPE.names:
+        assert abi.ACQ_DTYPE.fields[name][1] == getattr(abi.Acq, name).offset, name
 
 
 def test_struct_layouts_match_header(tmp_path):
