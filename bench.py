@@ -18,7 +18,9 @@ c1 / c2 / c3 / c4: one multi-slice reconstruction call (ifft-rss 16 x 15 x
 640 x 368, GRAPPA 8 x 32 x 384^2 R=4, CS 32 x 8 x 320^2 120 iterations, 3-D
 SENSE 32 x 160 x 256^2 R=3, which falls back to IFFT + RSS) per step.
 c4u: the SENSE unfold + g-factor of the C4 volume at ky = 258 (divisible by
-R = 3), one launch per volume.  hbm_copy: the copy-bandwidth ceiling.
+R = 3), one launch per volume.  c1p: the C1 acquisition's 5888 readouts
+placed into the k-space grid with fused whitening and oversampling removal.
+hbm_copy: the copy-bandwidth ceiling.
 """
 import argparse
 import ctypes as C
@@ -142,12 +144,10 @@ def workload_c4():
 
 WORKLOADS = {"c1": workload_c1, "c2": workload_c2, "c3": workload_c3, "c4": workload_c4}
 
-# HBM bytes the dominant kernels move per algorithmic unit, measured with the
+# HBM bytes the dominant kernels move per algorithmic byte, measured with the
 # FETCH_SIZE / WRITE_SIZE counters scaled by a copy kernel of known size
-# (profiles/hbm_traffic_counters.md): CS 73 B per pixel-iteration (41 algorithmic),
-# ifft-rss 61.9 MB per 15 x 368 x 640 slice (28.73 MB algorithmic).
-# HBM bytes moved per algorithmic byte, from the FETCH_SIZE / WRITE_SIZE
-# counter runs (calibrated on a copy kernel of known size)
+# (profiles/hbm_traffic_counters.md): CS 65.4 B per pixel-iteration (41
+# algorithmic), ifft-rss 60.8 MB per 15 x 368 x 640 slice (28.73 MB algorithmic).
 TRAFFIC_PER_ALGO_BYTE = {"c1": 60.8e6 / 28.73e6, "c3": 65.4 / 41.0, "served": 65.4 / 41.0}
 
 # Stages the served roofline can be taken on: (oks_stage id, bound, unit, peak).
@@ -293,6 +293,107 @@ def run_c4u(a):
                                "sample": f"{done} of {nz} kz planes through the FP32 oracle unfold + g-factor, "
                                          f"single thread, {dt:.1f} s"}
     for d in (da, dm, dr, dg):
+        d.free()
+    ctx.close()
+    print(json.dumps(out))
+
+
+def run_c1p(a):
+    """Readout placement of the C1 acquisition: 16 slices x 368 readouts of 15
+    coils x 640 samples (2x readout oversampling), in file order (one readout
+    after another, channel-major), whitened by the device-calibrated noise
+    whitener and cut to 320 samples by the oversampling removal as they are
+    placed into the [15, 16, 368, 640] grid (read_kspace_with_mask with the
+    strategies' preprocess closure).  Samples and grid resident in HBM; the
+    acquisition table is host memory, as the header stream of a file is."""
+    from openkspace_amd import abi, synth
+    from openkspace_amd._lib import Context, DeviceArray, Event, check, lib
+    case = synth.make_config("c1_ifft_rss")
+    nc, nz, ny, nx = case["kspace"].shape
+    rec = nx // 2
+    acqs, data, enc = synth.acquisitions(case["kspace"], case["mask"], slice_axis=True, recon_x=rec)
+    noise = case["noise"]
+    e = abi.Encoding(*enc)
+    ctx = Context(0)
+    dd = DeviceArray.from_host(ctx, data)
+    dn = DeviceArray.from_host(ctx, noise)
+    dw = DeviceArray(ctx, (nc, nc), np.complex64)
+    L = lib()
+    check(ctx.ptr, L.oks_prewhiten_calibrate_dev(ctx.ptr, dn.addr, nc, noise.shape[1], dw.addr))
+    grid = DeviceArray(ctx, (nc, nz, ny, nx), np.complex64)
+    mask = np.zeros((nz, ny, nx), np.uint8)
+    placed = C.c_int64(0)
+
+    def step():
+        check(ctx.ptr, L.oks_place_acquisitions_dev(ctx.ptr, acqs.ctypes.data, len(acqs), dd.addr, data.size,
+                                                     C.byref(e), dw.addr, nc, 1, grid.addr, mask.ctypes.data,
+                                                     C.byref(placed)))
+
+    for _ in range(a.warmup):
+        step()
+    ctx.synchronize()
+    ctx.set_option("time_stage", abi.STAGE_PLACE)
+    evs = [(Event(ctx), Event(ctx)) for _ in range(a.steps)]
+    t0 = time.perf_counter()
+    for s, ev in evs:
+        s.record()
+        step()
+        ev.record()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    kms, kbytes, nl = ctx.stage_time()
+    ctx.set_option("time_stage", abi.STAGE_NONE)
+    lat = np.array([s.elapsed_ms(ev) for s, ev in evs])
+    assert placed.value == nz * ny
+    avg_s = kms / nl * 1e-3
+    algo = kbytes / nl
+    out = {
+        "metric": METRIC, "value": nz * a.steps / wall, "unit": "slices/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": wall * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64/f32", "data": "synthetic",
+        "config": {"workload": "c1_placement", "slices": nz, "coils": nc, "readouts": len(acqs),
+                   "samples_per_readout": nx, "recon_x": rec, "prewhiten": True, "remove_oversampling": True,
+                   "note": "readouts -> [coil, slice, ky, kx] grid with fused whitening and oversampling "
+                           "removal; the grid zero fill is in the step, not in the kernel time"},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "mean": float(lat.mean())},
+        "roofline": {"bound": "hbm", "achieved": algo / avg_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": algo / avg_s / 1e9 / HBM_PEAK_GBS, "traffic": None, "kernel": "k_place",
+                     "algorithmic_bytes": algo, "avg_launch_ms": avg_s * 1e3},
+    }
+    if not a.no_check or not a.no_cpu:
+        import oracle as O
+        W = dw.download()
+    if not a.no_check:
+        g = grid.download()
+        errs = {}
+        for z in (0, nz // 2):
+            sel = acqs[acqs["slice"] == z].copy()
+            sel["slice"] = 0
+            sub = (enc[0], enc[1], 1, enc[3], enc[4], 0)
+            r32, m32, _ = O.place(sel, data, sub, W=W, remove_os=True, prec="f32")
+            r64 = O.place(sel, data, sub, W=W, remove_os=True, prec="f64")[0]
+            errs[f"slice={z}"] = {
+                "vs_fp32_oracle": float(np.linalg.norm(g[:, z] - r32[:, 0]) / np.linalg.norm(r32)),
+                "vs_fp64_oracle": float(np.linalg.norm(g[:, z] - r64[:, 0]) / np.linalg.norm(r64)),
+                "fp32_oracle_vs_fp64": float(np.linalg.norm(r32 - r64) / np.linalg.norm(r64)),
+                "mask_exact": bool(np.array_equal(mask[z].astype(bool), m32[0]))}
+        out["rel_l2"] = errs
+    if not a.no_cpu:
+        O.set_threads(1)
+        done, t0 = 0, time.perf_counter()
+        while done < nz:
+            sel = acqs[acqs["slice"] == done].copy()
+            sel["slice"] = 0
+            O.place(sel, data, (enc[0], enc[1], 1, enc[3], enc[4], 0), W=W, remove_os=True, prec="f32")
+            done += 1
+            if time.perf_counter() - t0 > a.cpu_seconds:
+                break
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": done / dt, "unit": "slices/s", "cores": 1, "kind": "port",
+                               "sample": f"{done} of {nz} slices placed by the FP32 oracle with whitening and "
+                                         f"oversampling removal, single thread, {dt:.1f} s"}
+    for d in (dd, dn, dw, grid):
         d.free()
     ctx.close()
     print(json.dumps(out))
@@ -513,6 +614,9 @@ def main():
         return
     if a.workload == "c4u":
         run_c4u(a)
+        return
+    if a.workload == "c1p":
+        run_c1p(a)
         return
     if a.workload == "served":
         if a.steps == 50 and "--steps" not in sys.argv:

@@ -218,7 +218,8 @@ enum oks_stage {
     OKS_STAGE_IFFT_RSS = 1,          /* IFFT + crop + RSS tail; bytes: grid read + image write      */
     OKS_STAGE_GRAPPA_CALIBRATE = 2,  /* Gram, rhs, FP64 ridge solve; flops of recon/grappa.rs:156-308 */
     OKS_STAGE_GRAPPA_APPLY = 3,      /* synthesis GEMM; flops 8 x filled points x nc x n_src          */
-    OKS_STAGE_CS_ITERATIONS = 4      /* FISTA iterations; bytes 5*8 + 1 per pixel, plane, iteration   */
+    OKS_STAGE_CS_ITERATIONS = 4,     /* FISTA iterations; bytes 5*8 + 1 per pixel, plane, iteration   */
+    OKS_STAGE_PLACE = 5              /* readout placement; bytes: samples read + placed samples written */
 };
 int oks_ctx_stage_time(oks_ctx* ctx, double* ms, double* units, int64_t* launches);
 

@@ -57,7 +57,7 @@ static void place_dev(oks_ctx* ctx, const oks_acq* acqs, int64_t n, const oks_c3
     const int64_t nc = shape[0], nz = shape[1], ny = shape[2], nx = shape[3];
     if (nz > 65535 || ny > 65535 || nx > 65535) fail(OKS_E_UNSUPPORTED, "grid too large");
     const bool os = remove_os && os_valid(enc->encoded_x, enc->recon_x);
-    if (os && (enc->encoded_x > 4096 || place_lds(enc->encoded_x, 1) > 160 * 1024))
+    if (os && (enc->encoded_x > 4096 || place_lds(enc->encoded_x, enc->recon_x, 1) > 160 * 1024))
         fail(OKS_E_UNSUPPORTED, "readout longer than 4096 samples");
     const int64_t ky_off = ny / 2 - enc->ky_center, kz_off = sa ? 0 : nz / 2 - enc->kz_center;
     std::vector<uint8_t>& mask = ctx->h_mask;
@@ -126,20 +126,33 @@ static void place_dev(oks_ctx* ctx, const oks_acq* acqs, int64_t n, const oks_c3
         a.nz = (int)nz;
         a.ny = (int)ny;
         a.nx = (int)nx;
-        int wpb = 4;
+        // one wavefront per coil, up to 8; the oversampling lines of a
+        // workgroup stay within 64 KB of LDS so that several share a CU
+        int wpb = (int)std::min<int64_t>(8, std::max<int64_t>(4, nc));
+        bool white = false;
+        for (const auto& x : items) white = white || (x.second.flags & kPlaceWhiten);
+        const int ncw = white ? place_ncw(w_nc) : 0;
         if (os) {
             a.enc = enc->encoded_x;
             a.rec = enc->recon_x;
             a.p_enc = ctx->plan(a.enc);
             a.p_rec = ctx->plan(a.rec);
-            while (wpb > 1 && place_lds(a.enc, wpb) > 160 * 1024) --wpb;
+            while (wpb > 1 && place_lds(a.enc, a.rec, wpb) > 64 * 1024) --wpb;
         }
+        // algorithmic bytes: the samples each readout needs (the whole line
+        // for oversampling removal, else the kept ones) and the placed ones
+        double bytes = 0.0;
+        for (const auto& x : items) {
+            const PlaceItem& it = x.second;
+            bytes += 8.0 * it.nc_fit * (((it.flags & kPlaceOs) ? it.ns : it.len) + it.len);
+        }
+        StageTimer timer(ctx, OKS_STAGE_PLACE, bytes);
         size_t b = 0;
         while (b < items.size()) {
             size_t e = b;
             while (e < items.size() && items[e].first == items[b].first) ++e;
             a.items = dtab + b;
-            OKS_HIP(launch_place(a, (long long)(e - b), wpb, ctx->stream));
+            OKS_HIP(launch_place(a, (long long)(e - b), wpb, ncw, ctx->stream));
             b = e;
         }
     }

@@ -28,7 +28,12 @@ struct PlaceArgs {
     FftPlan p_enc, p_rec;
 };
 
-size_t place_lds(int enc, int wpb);
-hipError_t launch_place(const PlaceArgs& a, long long n_items, int wpb, hipStream_t s);
+// Register bound of the whitening step for w_nc channels (8, 16 or 32; 0
+// without whitening; -1 above 32).
+int place_ncw(int w_nc);
+// Dynamic LDS bytes of a launch with `wpb` wavefronts (0 without
+// oversampling removal, enc = 0).
+size_t place_lds(int enc, int rec, int wpb);
+hipError_t launch_place(const PlaceArgs& a, long long n_items, int wpb, int ncw, hipStream_t s);
 
 }  // namespace oks
