@@ -18,7 +18,8 @@ c1 / c2 / c3 / c4: one multi-slice reconstruction call (ifft-rss 16 x 15 x
 640 x 368, GRAPPA 8 x 32 x 384^2 R=4, CS 32 x 8 x 320^2 120 iterations, 3-D
 SENSE 32 x 160 x 256^2 R=3, which falls back to IFFT + RSS) per step.
 c4u: the SENSE unfold + g-factor of the C4 volume at ky = 258 (divisible by
-R = 3), one launch per volume.  c1p: the C1 acquisition's 5888 readouts
+R = 3), one launch per volume.  c1h: C1 with 6/8 partial Fourier along ky,
+reconstructed by homodyne.  c1p: the C1 acquisition's 5888 readouts
 placed into the k-space grid with fused whitening and oversampling removal.
 hbm_copy: the copy-bandwidth ceiling.
 """
@@ -72,6 +73,31 @@ def workload_c1():
                 "remove_oversampling": False, "prewhiten": True, "noise_samples": case["noise"].shape[1]},
         case=case, desc=desc, fn="oks_recon_ifft_rss_dev", oracle="ifft_rss", slices=nz,
         bound="hbm", algo=algo_bytes, kernel="ifft_rss pass sequence (whitener + row + column)",
+        cpu_unit=lambda c, z: _slice_case(c, z), dtype="c64/f32",
+    )
+
+
+def workload_c1h():
+    """C1 acquired with 6/8 partial Fourier along ky (rows below 92 of 368
+    missing): ifft-rss dispatches to the homodyne reconstruction."""
+    from openkspace_amd import abi, synth
+    case = synth.make_config("c1_ifft_rss")
+    m = case["mask"].copy()
+    m[:, :92] = False
+    case["mask"] = np.ascontiguousarray(m)
+    case["kspace"] = np.ascontiguousarray(np.where(m[None], case["kspace"], 0).astype(np.complex64))
+    nc, nz, ny, nx = case["kspace"].shape
+    desc = abi.IfftRssDesc.default(remove_oversampling=0)
+    tx = nx // 2
+    algo_bytes = nz * (nc * ny * nx * 8 + ny * tx * 4)
+    return dict(
+        name="c1h_homodyne",
+        config={"workload": "c1h_homodyne", "slices": nz, "coils": nc, "ky": ny, "kx": nx,
+                "readout_oversampling": 2, "recon_matrix": [ny, tx], "strategy": "IfftRss",
+                "partial_fourier": "rows [92, 368) acquired (6/8), homodyne", "prewhiten": True,
+                "remove_oversampling": False},
+        case=case, desc=desc, fn="oks_recon_ifft_rss_dev", oracle="ifft_rss", slices=nz,
+        bound="hbm", algo=algo_bytes, kernel="homodyne launch sequence (whitening, kx pass, ky pass + demod + RSS)",
         cpu_unit=lambda c, z: _slice_case(c, z), dtype="c64/f32",
     )
 
@@ -142,7 +168,7 @@ def workload_c4():
     )
 
 
-WORKLOADS = {"c1": workload_c1, "c2": workload_c2, "c3": workload_c3, "c4": workload_c4}
+WORKLOADS = {"c1": workload_c1, "c2": workload_c2, "c3": workload_c3, "c4": workload_c4, "c1h": workload_c1h}
 
 # HBM bytes the dominant kernels move per algorithmic byte, measured with the
 # FETCH_SIZE / WRITE_SIZE counters scaled by a copy kernel of known size

@@ -54,7 +54,7 @@ enum oks_status {
 
 /* Non-error fallbacks of the reference, reported as flags beside OKS_OK
  * (recon/prewhiten.rs:49-66, recon/oversampling.rs:51-60,
- *  recon/strategy.rs:349-355,389-395,602-647,829-842, recon/sense.rs:131-136). */
+ *  recon/strategy.rs:163-187,349-355,389-395,602-647,829-842, recon/sense.rs:131-136). */
 enum oks_flags {
     OKS_F_PREWHITEN_SKIPPED = 1u << 0,  /* from_noise_acqs returned None      */
     OKS_F_OS_SKIPPED = 1u << 1,         /* OversamplingRemover::new was None  */
@@ -63,7 +63,8 @@ enum oks_flags {
     OKS_F_SENSE_FALLBACK = 1u << 4,     /* SENSE fell back to IFFT+RSS        */
     OKS_F_SENSE_PIXEL_FAILED = 1u << 5, /* >=1 pixel system not PD -> zero    */
     OKS_F_CS_FALLBACK = 1u << 6,        /* odd dims -> IFFT+RSS               */
-    OKS_F_CROPPED = 1u << 7             /* maybe_crop changed the shape       */
+    OKS_F_CROPPED = 1u << 7,            /* maybe_crop changed the shape       */
+    OKS_F_HOMODYNE = 1u << 8            /* partial Fourier: homodyne path     */
 };
 
 /* FftMode (recon/strategy.rs:72-79). */

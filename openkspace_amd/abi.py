@@ -35,6 +35,7 @@ F_SENSE_FALLBACK = 1 << 4
 F_SENSE_PIXEL_FAILED = 1 << 5
 F_CS_FALLBACK = 1 << 6
 F_CROPPED = 1 << 7
+F_HOMODYNE = 1 << 8
 
 FFT_AUTO, FFT_2D, FFT_3D = 0, 1, 2
 MAPS_WALSH, MAPS_ESPIRIT = 0, 1

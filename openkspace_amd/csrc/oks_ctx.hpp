@@ -67,6 +67,7 @@ struct oks_ctx {
     std::vector<uint8_t> h_mask;                   // host staging: per-cell mask
     std::vector<uint8_t> h_batch_mask;             // host staging: a server batch's masks
     std::vector<unsigned char> h_items;            // host staging: readout placement table
+    std::vector<float> h_weights;                  // host staging: partial-Fourier weights
 
     // Instantiated HIP graphs keyed by what their captured launches depend on.
     struct GraphKey {

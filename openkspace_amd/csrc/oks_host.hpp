@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -107,10 +108,14 @@ inline bool detect_pattern(const uint8_t* mask, int64_t nz, int64_t ny, int64_t 
     return pattern_detect(ky_any(mask, nz, ny, nx), p);
 }
 
-// PartialFourierPlan::detect gate (recon/partial_fourier.rs:55-117).
-inline bool pf_detect(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx, int64_t ky_dc) {
+// PartialFourierPlan::detect (recon/partial_fourier.rs:55-117): the plan
+// when exactly one side of DC is clipped; ramp as detect sets it (:106-107).
+struct PfPlan {
+    int64_t ny, ky_dc, ky_lo, ky_hi, sym_half, ramp;
+};
+inline bool pf_plan_ky(const std::vector<uint8_t>& any, int64_t ky_dc, PfPlan* p) {
+    const int64_t ny = (int64_t)any.size();
     if (ny < 8) return false;
-    std::vector<uint8_t> any = ky_any(mask, nz, ny, nx);
     int64_t lo = -1, hi = -1;
     for (int64_t y = 0; y < ny; ++y)
         if (any[y]) {
@@ -126,7 +131,47 @@ inline bool pf_detect(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx, i
     if (longer == 0) return false;
     const float asym = (float)longer / (float)std::max<int64_t>(shorter, 1);
     if (asym < 1.10f) return false;
-    return shorter >= 4;
+    if (shorter < 4) return false;
+    if (p) *p = PfPlan{ny, ky_dc, lo, hi, shorter, std::min(std::max<int64_t>(shorter / 4, 2), shorter - 1)};
+    return true;
+}
+inline bool pf_plan(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx, int64_t ky_dc, PfPlan* p) {
+    if (ny < 8) return false;
+    return pf_plan_ky(ky_any(mask, nz, ny, nx), ky_dc, p);
+}
+inline bool pf_detect(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx, int64_t ky_dc) {
+    return pf_plan(mask, nz, ny, nx, ky_dc, nullptr);
+}
+
+// Hann value 0.5 - 0.5 cos(2 pi t) in f32 (recon/partial_fourier.rs:128-133,181-182).
+inline float pf_hann(float t) { return 0.5f - 0.5f * std::cos(6.2831855f * t); }
+
+// low_pass_weights (recon/partial_fourier.rs:138-148): Hann window over
+// [dc - sym, dc + sym] clipped to the grid; homodyne_weights (:153-198): 1 on
+// the symmetric band, Hann ramps of half-width `ramp` at |d| = sym rising to 2
+// on the long side and falling to 0 on the short one, 2 on the long tail, 0
+// outside [ky_lo, ky_hi].  f32 as the reference computes them.
+inline void pf_weights(const PfPlan& p, float* low, float* homo) {
+    std::fill(low, low + p.ny, 0.0f);
+    std::fill(homo, homo + p.ny, 0.0f);
+    const int64_t lo = p.ky_dc > p.sym_half ? p.ky_dc - p.sym_half : 0;
+    const int64_t hi = std::min(p.ky_dc + p.sym_half, p.ny - 1), width = hi - lo + 1;
+    for (int64_t ky = lo; ky <= hi; ++ky)
+        low[ky] = width == 1 ? 1.0f : pf_hann((float)(ky - lo) / (float)(width - 1));
+    const int64_t c = p.ky_dc, sym = p.sym_half, r = p.ramp;
+    const bool above_long = (p.ky_hi - c) >= (c - p.ky_lo);
+    for (int64_t ky = p.ky_lo; ky <= p.ky_hi; ++ky) {
+        const int64_t d = ky - c, ad = d < 0 ? -d : d;
+        const bool long_side = (d >= 0 && above_long) || (d <= 0 && !above_long);
+        if (ad <= sym - r) {
+            homo[ky] = 1.0f;
+        } else if (ad <= sym + r) {
+            const float hv = pf_hann((float)(ad - (sym - r)) / (float)(2 * r));
+            homo[ky] = long_side ? 1.0f + hv : 1.0f - hv;
+        } else {
+            homo[ky] = long_side ? 2.0f : 0.0f;
+        }
+    }
 }
 
 }  // namespace oks

@@ -435,7 +435,7 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
     float2* B = F::scratch ? A + n : nullptr;
     const float2* __restrict__ src = a.in + row * n;
     const uint8_t* m = a.mask ? a.mask + (row % a.mask_rows) * n : nullptr;
-    const int half = a.centred ? n / 2 : 0;
+    const int half = a.centred == 2 ? (n + 1) / 2 : a.centred ? n / 2 : 0;
     for (int x = lane; x < n; x += 64) {
         float2 v = src[x];
         if (m && !m[x]) v = make_float2(0.f, 0.f);

@@ -46,6 +46,9 @@ struct RowArgs {
     const uint8_t* mask;  // optional [mask_rows, n] (row % mask_rows), 0 -> zero input
     long long rows, mask_rows;
     int tx, x0;
+    // 0: plain; 1: rotate by ceil(n/2) before and floor(n/2) after
+    // (recon/shift.rs:10-26); 2: floor(n/2) before and ceil(n/2) after
+    // (NumPy's pair, recon/partial_fourier.rs:212-268)
     int centred;
     float scale;
     FftPlan p;

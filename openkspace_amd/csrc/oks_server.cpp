@@ -125,6 +125,9 @@ int64_t footprint(const Req& q) {
             f += 4 * grid + 2 * nz * ny * nx + nz * nx * ((ny + 63) / 64 + 1) * 8 + 8;
             break;
         case OKS_STRATEGY_SENSE: f += 3 * grid + 2 * out; break;
+        case OKS_STRATEGY_IFFT_RSS:  // homodyne: the grid after the reader closure
+            if (q.desc.ifft.partial_fourier && pf_plan_ky(q.ky, ny / 2, nullptr)) f += grid;
+            break;
         default: break;
     }
     return f;

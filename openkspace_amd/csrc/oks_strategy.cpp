@@ -12,6 +12,7 @@
 #include "oks_host.hpp"
 #include "oks_kernels.hpp"
 #include "oks_abi_util.hpp"
+#include "oks_homodyne.hpp"
 #include "oks_pipeline.hpp"
 #include "oks_sense.hpp"
 
@@ -253,20 +254,72 @@ void ifft_rss_grid(oks_ctx* ctx, const float2* k, int64_t nc, int64_t nz, int64_
     }
 }
 
+// homodyne_reconstruct (recon/partial_fourier.rs:287-356) of the grid after
+// the reader closure, written in the crop window w: the kx inverse transform
+// of every coil's slices (NumPy's shift pair, x crop fused), then the ky pass
+// with both weightings, the demodulation and the coil sum in one kernel.
+static void homodyne_dev(oks_ctx* ctx, const oks_kspace* k, const float2* W, int os_mode, int rec, const PfPlan& pf,
+                         const Win w[3], float* out) {
+    const int64_t nc = k->nc, nz = k->nz, ny = k->ny, nx = k->nx;
+    const int tx = (int)w[2].len, ty = (int)w[1].len;
+    const int64_t zlo = w[0].off, zn = w[0].len;
+    if (rows_lds((int)nx) > 160 * 1024) fail(OKS_E_UNSUPPORTED, "row length too large for LDS");
+    const int sx = ny > 384 ? 4 : 8;
+    if (homodyne_cols_lds((int)ny, sx, ty) > 160 * 1024) fail(OKS_E_UNSUPPORTED, "ky length too large for LDS");
+    const float2* g = (W || os_mode) ? prep_grid(ctx, k, W, os_mode, rec, "hd_grid") : (const float2*)k->kspace;
+    float2* T = (float2*)ctx->buf("hd_rows", sizeof(float2) * nc * zn * ny * tx);
+    RowArgs r{};
+    r.mask_rows = 1;
+    r.rows = zn * ny;
+    r.tx = tx;
+    r.x0 = (int)w[2].off;
+    r.centred = 2;
+    r.scale = 1.0f / (float)nx;
+    r.p = ctx->plan((int)nx);
+    for (int64_t c = 0; c < nc; ++c) {
+        r.in = g + (c * nz + zlo) * ny * nx;
+        r.out = T + c * zn * ny * tx;
+        OKS_HIP(launch_rows(r, true, ctx->stream));
+    }
+    // weights staged in the context so they outlive the asynchronous copy
+    ctx->h_weights.assign(2 * ny, 0.0f);
+    pf_weights(pf, ctx->h_weights.data(), ctx->h_weights.data() + ny);
+    float* dw = (float*)ctx->buf("hd_weights", sizeof(float) * 2 * ny);
+    OKS_HIP(hipMemcpyAsync(dw, ctx->h_weights.data(), sizeof(float) * 2 * ny, hipMemcpyHostToDevice, ctx->stream));
+    HdColArgs a{};
+    a.T = T;
+    a.L = dw;
+    a.H = dw + ny;
+    a.out = out;
+    a.nc = (int)nc;
+    a.zn = (int)zn;
+    a.tx = tx;
+    a.y0 = (int)w[1].off;
+    a.ty = ty;
+    a.sx = sx;
+    a.p = ctx->plan((int)ny);
+    OKS_HIP(launch_homodyne_cols(a, ctx->stream));
+}
+
 // IfftRss::reconstruct (recon/strategy.rs:113-223) on device-resident inputs.
 void recon_ifft_rss_dev(oks_ctx* ctx, const oks_ifft_rss_desc* d, const oks_kspace* k, oks_image* img) {
     check_kspace(k);
     if (!d || !img || !img->data) fail(OKS_E_INVALID_ARG, "null descriptor or image");
     uint32_t flags = 0;
     const bool td = three_d_of(d->fft_mode, k);
-    if (!td && d->partial_fourier && pf_detect(k->mask, k->nz, k->ny, k->nx, k->ny / 2))
-        fail(OKS_E_UNSUPPORTED, "partial-Fourier acquisition: homodyne reconstruction is not on this path");
     const float2* W = d->prewhiten ? whitener(ctx, (const float2*)k->noise, k->nc, k->n_noise, &flags, "W") : nullptr;
     int os_mode = 0, rec = 0;
     resolve_os(k, d->remove_oversampling, &os_mode, &rec, &flags);
     Win w[3];
     output_windows(k, d->crop_to_recon_matrix, w, &flags);
-    ifft_rss_grid(ctx, (const float2*)k->kspace, k->nc, k->nz, k->ny, k->nx, W, os_mode, rec, td, w, img->data);
+    // partial Fourier along ky dispatches to homodyne (recon/strategy.rs:163-187)
+    PfPlan pf;
+    if (!td && d->partial_fourier && pf_plan(k->mask, k->nz, k->ny, k->nx, k->ny / 2, &pf)) {
+        homodyne_dev(ctx, k, W, os_mode, rec, pf, w, img->data);
+        flags |= OKS_F_HOMODYNE;
+    } else {
+        ifft_rss_grid(ctx, (const float2*)k->kspace, k->nc, k->nz, k->ny, k->nx, W, os_mode, rec, td, w, img->data);
+    }
     img->nz = w[0].len;
     img->ny = w[1].len;
     img->nx = w[2].len;

@@ -64,6 +64,10 @@ def lib():
         L.oko_place_shape.argtypes = [_P, _I64, _P, _P, _P]
         L.oko_pattern_detect_1d.argtypes = [_P, _I64, _P]
         L.oko_pf_detect.argtypes = [_P, _I64, _I64, _I64, _I64]
+        L.oko_pf_plan.argtypes = [_P, _I64, _I64, _I64, _I64, _P]
+        L.oko_pf_weights.argtypes = [_P, _P, _P]
+        for s in ("f32", "f64"):
+            getattr(L, f"oko_homodyne_{s}").argtypes = [_P, _I64, _I64, _I64, _I64, _P, _P]
         L.oko_grappa_geom.argtypes = [_I64] * 6 + [_P]
         L.oko_grappa_fills.argtypes = [_P, _I64, _I64, _I64, _P, _I64]
         L.oko_grappa_fills.restype = _I64
@@ -206,6 +210,39 @@ def pattern_detect_1d(ky_any):
 def pf_detect(mask, ky_dc):
     m = np.ascontiguousarray(mask, dtype=np.uint8)
     return bool(lib().oko_pf_detect(m.ctypes.data, *m.shape, ky_dc))
+
+
+PF_FIELDS = ("ny", "ky_dc", "ky_lo", "ky_hi", "sym_half", "ramp")
+
+
+def pf_plan(mask, ky_dc):
+    """PartialFourierPlan::detect (recon/partial_fourier.rs:55-117) on a
+    [nz, ny, nx] mask: dict of the plan fields, or None."""
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    out = np.zeros(6, np.int64)
+    if not lib().oko_pf_plan(m.ctypes.data, *m.shape, ky_dc, out.ctypes.data):
+        return None
+    return dict(zip(PF_FIELDS, (int(v) for v in out)))
+
+
+def pf_weights(plan):
+    """(low_pass_weights, homodyne_weights) of a plan, float32 [ny] each
+    (recon/partial_fourier.rs:120-198)."""
+    p = np.array([plan[f] for f in PF_FIELDS], np.int64)
+    lo, hw = np.zeros(plan["ny"], np.float32), np.zeros(plan["ny"], np.float32)
+    lib().oko_pf_weights(p.ctypes.data, lo.ctypes.data, hw.ctypes.data)
+    return lo, hw
+
+
+def homodyne(kspace, plan, prec="f64"):
+    """homodyne_reconstruct (recon/partial_fourier.rs:287-356): [nc, nz, ny, nx]
+    -> [nz, ny, nx] magnitude."""
+    k = _c(kspace, prec)
+    nc, nz, ny, nx = k.shape
+    p = np.array([plan[f] for f in PF_FIELDS], np.int64)
+    out = np.zeros((nz, ny, nx), np.float32 if prec == "f32" else np.float64)
+    getattr(lib(), f"oko_homodyne_{prec}")(k.ctypes.data, nc, nz, ny, nx, p.ctypes.data, out.ctypes.data)
+    return out
 
 
 def grappa_geom(nc, ny_acs, nx, r, kky, kkx):

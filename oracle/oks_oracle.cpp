@@ -71,7 +71,7 @@ bool detect_pattern(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx, Pat
     return pattern_detect(any.data(), ny, p);
 }
 
-bool pf_detect(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx, int64_t ky_dc) {
+bool pf_plan(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx, int64_t ky_dc, PfPlan* p) {
     if (ny < 8) return false;
     std::vector<uint8_t> any(ny, 0);
     for (int64_t z = 0; z < nz; ++z)
@@ -91,7 +91,50 @@ bool pf_detect(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx, int64_t 
     float asym = (float)longer / (float)std::max<int64_t>(shorter, 1);
     if (asym < 1.10f) return false;
     if (shorter < 4) return false;
+    if (p) *p = PfPlan{ny, ky_dc, lo, hi, shorter, std::min(std::max<int64_t>(shorter / 4, 2), shorter - 1)};
     return true;
+}
+
+bool pf_detect(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx, int64_t ky_dc) {
+    return pf_plan(mask, nz, ny, nx, ky_dc, nullptr);
+}
+
+// hann_window (recon/partial_fourier.rs:121-134), f32.
+static std::vector<float> hann(int64_t n) {
+    if (n == 0) return {};
+    if (n == 1) return {1.0f};
+    std::vector<float> w(n);
+    for (int64_t i = 0; i < n; ++i) {
+        const float x = (float)i / (float)(n - 1);
+        w[i] = 0.5f - 0.5f * std::cos(6.2831855f * x);
+    }
+    return w;
+}
+
+void pf_weights(const PfPlan& p, float* low, float* homo) {
+    std::fill(low, low + p.ny, 0.0f);
+    std::fill(homo, homo + p.ny, 0.0f);
+    const int64_t lo = p.ky_dc > p.sym_half ? p.ky_dc - p.sym_half : 0;
+    const int64_t hi = std::min(p.ky_dc + p.sym_half, p.ny - 1);
+    const std::vector<float> h = hann(hi - lo + 1);
+    for (int64_t ky = lo; ky <= hi; ++ky) low[ky] = h[ky - lo];
+    const int64_t c = p.ky_dc, sym = p.sym_half, r = p.ramp;
+    const bool above_is_long = (p.ky_hi - c) >= (c - p.ky_lo);
+    for (int64_t ky = p.ky_lo; ky <= p.ky_hi; ++ky) {
+        const int64_t d = ky - c, ad = d < 0 ? -d : d;
+        const bool long_side = (d >= 0 && above_is_long) || (d <= 0 && !above_is_long);
+        float w;
+        if (ad <= sym - r) {
+            w = 1.0f;
+        } else if (ad <= sym + r) {
+            const float t = (float)(ad - (sym - r)) / (float)(2 * r);
+            const float hv = 0.5f - 0.5f * std::cos(6.2831855f * t);
+            w = long_side ? 1.0f + hv : 1.0f - hv;
+        } else {
+            w = long_side ? 2.0f : 0.0f;
+        }
+        homo[ky] = w;
+    }
 }
 
 int grappa_geom(int64_t nc, int64_t ny_acs, int64_t nx_acs, int64_t r, int64_t kky, int64_t kkx,
@@ -243,8 +286,15 @@ static int recon_ifft_rss(const oks_ifft_rss_desc* d, const oks_kspace* in, T* o
     Grid<T> g = prep<T>(in, d->prewhiten, d->remove_oversampling);
     *flags = g.flags;
     bool td = three_d_of(d->fft_mode, in);
-    if (!td && d->partial_fourier && pf_detect(g.mask.data(), g.nz, g.ny, g.nx, g.ny / 2))
-        return OKS_E_UNSUPPORTED;  // homodyne_reconstruct is not on this path
+    PfPlan pf;
+    if (!td && d->partial_fourier && pf_plan(g.mask.data(), g.nz, g.ny, g.nx, g.ny / 2, &pf)) {
+        // homodyne_reconstruct replaces IFFT + RSS (recon/strategy.rs:163-187)
+        std::vector<T> img(g.nz * g.ny * g.nx);
+        homodyne(g.k.data(), g.nc, g.nz, g.ny, g.nx, pf, img.data());
+        *flags |= OKS_F_HOMODYNE;
+        finish(img, g.nz, g.ny, g.nx, d->crop_to_recon_matrix, in, out, shape, flags);
+        return OKS_OK;
+    }
     ifft_grid(g, td);
     std::vector<T> img;
     rss_grid(g, img);
@@ -513,6 +563,11 @@ OKO_EXPORT int oko_get_threads(void) { return n_threads(); }
         Fft<T> fx((int)nx, s), fy((int)ny, s);                                               \
         centred_fft2((cx<T>*)a, ny, nx, fx, fy);                                             \
     }                                                                                        \
+    OKO_EXPORT void oko_homodyne_##S(const T* k, int64_t nc, int64_t nz, int64_t ny,          \
+                                     int64_t nx, const int64_t* plan, T* out) {              \
+        PfPlan p{plan[0], plan[1], plan[2], plan[3], plan[4], plan[5]};                      \
+        homodyne((const cx<T>*)k, nc, nz, ny, nx, p, out);                                   \
+    }                                                                                        \
     OKO_EXPORT int oko_fista_cs_##S(const T* kzf, const uint8_t* mask, int64_t ny,           \
                                     int64_t nx, int64_t iters, double lambda, T* out) {      \
         return fista_cs((const cx<T>*)kzf, mask, ny, nx, iters, (T)(float)lambda,            \
@@ -557,6 +612,19 @@ OKO_EXPORT int oko_pattern_detect_1d(const uint8_t* ky_any, int64_t ny, int64_t 
     out[0] = p.r; out[1] = p.acs_start; out[2] = p.acs_end; out[3] = p.ky_lo; out[4] = p.ky_hi;
     return 0;
 }
+OKO_EXPORT int oko_pf_plan(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx, int64_t ky_dc,
+                           int64_t* plan) {
+    PfPlan p;
+    if (!pf_plan(mask, nz, ny, nx, ky_dc, &p)) return 0;
+    const int64_t v[6] = {p.ny, p.ky_dc, p.ky_lo, p.ky_hi, p.sym_half, p.ramp};
+    std::copy(v, v + 6, plan);
+    return 1;
+}
+
+OKO_EXPORT void oko_pf_weights(const int64_t* plan, float* low, float* homo) {
+    pf_weights(PfPlan{plan[0], plan[1], plan[2], plan[3], plan[4], plan[5]}, low, homo);
+}
+
 OKO_EXPORT int oko_pf_detect(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx,
                              int64_t ky_dc) {
     return pf_detect(mask, nz, ny, nx, ky_dc) ? 1 : 0;

@@ -437,6 +437,86 @@ bool detect_pattern(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx, Pat
 // PartialFourierPlan::detect gate (recon/partial_fourier.rs:55-117).
 bool pf_detect(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx, int64_t ky_dc);
 
+// PartialFourierPlan (recon/partial_fourier.rs:29-45) as detect builds it
+// (recon/partial_fourier.rs:106-116).
+struct PfPlan {
+    int64_t ny, ky_dc, ky_lo, ky_hi, sym_half, ramp;
+};
+bool pf_plan(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx, int64_t ky_dc, PfPlan* p);
+// low_pass_weights and homodyne_weights (recon/partial_fourier.rs:120-198),
+// computed in f32 as the reference computes them.
+void pf_weights(const PfPlan& p, float* low, float* homo);
+
+// ifft2_plane (recon/partial_fourier.rs:206-269) on [ny, nx]: rotate left by
+// floor(n/2) on both axes, inverse transform along x then y with 1/n after
+// each, rotate left by ceil(n/2) (NumPy's shift pair, unlike recon/shift.rs
+// for odd n).
+template <class T>
+void ifft2_plane(cx<T>* a, int64_t ny, int64_t nx, const Fft<T>& fx, const Fft<T>& fy) {
+    std::vector<cx<T>> buf(std::max(ny, nx)), tmp(std::max(ny, nx));
+    auto col_rotate = [&](int64_t k) {
+        for (int64_t x = 0; x < nx; ++x) {
+            for (int64_t y = 0; y < ny; ++y) buf[y] = a[y * nx + x];
+            for (int64_t y = 0; y < ny; ++y) a[y * nx + x] = buf[(y + k) % ny];
+        }
+    };
+    auto row_rotate = [&](int64_t k) {
+        for (int64_t y = 0; y < ny; ++y) {
+            for (int64_t x = 0; x < nx; ++x) buf[x] = a[y * nx + x];
+            for (int64_t x = 0; x < nx; ++x) a[y * nx + x] = buf[(x + k) % nx];
+        }
+    };
+    col_rotate(ny / 2);
+    row_rotate(nx / 2);
+    const T sx = T(1) / (T)nx, sy = T(1) / (T)ny;
+    for (int64_t y = 0; y < ny; ++y) {
+        for (int64_t x = 0; x < nx; ++x) tmp[x] = a[y * nx + x];
+        fx.run(tmp.data());
+        for (int64_t x = 0; x < nx; ++x) a[y * nx + x] = scale(tmp[x], sx);
+    }
+    for (int64_t x = 0; x < nx; ++x) {
+        for (int64_t y = 0; y < ny; ++y) tmp[y] = a[y * nx + x];
+        fy.run(tmp.data());
+        for (int64_t y = 0; y < ny; ++y) a[y * nx + x] = scale(tmp[y], sy);
+    }
+    col_rotate(ny - ny / 2);
+    row_rotate(nx - nx / 2);
+}
+
+// homodyne_reconstruct (recon/partial_fourier.rs:287-356): k [nc, nz, ny, nx]
+// -> out [nz, ny, nx].  Per slice and coil the H- and L-weighted planes are
+// inverse transformed, the image demodulated by the low-pass phase and the
+// squared real parts summed over coils (ascending), then the square root.
+template <class T>
+void homodyne(const cx<T>* k, int64_t nc, int64_t nz, int64_t ny, int64_t nx, const PfPlan& p, T* out) {
+    std::vector<float> lw(ny), hw(ny);
+    pf_weights(p, lw.data(), hw.data());
+    Fft<T> fx((int)nx, +1), fy((int)ny, +1);
+    const int64_t plane = ny * nx;
+    parallel_for(nz, [&](int64_t z) {
+        std::vector<cx<T>> homo(plane), lowp(plane);
+        std::vector<T> acc(plane, T(0));
+        for (int64_t c = 0; c < nc; ++c) {
+            const cx<T>* src = k + (c * nz + z) * plane;
+            for (int64_t y = 0; y < ny; ++y)
+                for (int64_t x = 0; x < nx; ++x) {
+                    homo[y * nx + x] = scale(src[y * nx + x], (T)hw[y]);
+                    lowp[y * nx + x] = scale(src[y * nx + x], (T)lw[y]);
+                }
+            ifft2_plane(homo.data(), ny, nx, fx, fy);
+            ifft2_plane(lowp.data(), ny, nx, fx, fy);
+            for (int64_t i = 0; i < plane; ++i) {
+                const cx<T> ph = lowp[i];
+                const T nrm = cnorm(ph);
+                const cx<T> dm = nrm > (T)1e-12f ? cx<T>{ph.re / nrm, -ph.im / nrm} : cx<T>{T(1), T(0)};
+                const T m = (homo[i] * dm).re;
+                acc[i] += m * m;
+            }
+        }
+        for (int64_t i = 0; i < plane; ++i) out[z * plane + i] = std::sqrt(acc[i]);
+    });
+}
+
 enum { GR_OK = 0, GR_BAD_CONFIG = 1, GR_ACS_TOO_SMALL = 2, GR_CHOLESKY = 3 };
 
 struct GrappaGeom {

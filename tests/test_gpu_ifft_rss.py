@@ -90,12 +90,16 @@ def test_ifft_rss_3d_and_zcrop(ctx):
     assert_parity(got, ref32, ref64, TOL32, "3d")
 
 
-def test_ifft_rss_partial_fourier_unsupported(ctx):
+def test_ifft_rss_small_even_grid_goes_homodyne(ctx):
+    # a fully sampled ny = 16 has 8 rows below DC and 7 above: the reference's
+    # gate routes it to homodyne (recon/partial_fourier.rs:84-108)
     case = _case(ny=16)
-    from openkspace_amd._lib import OksError
-    with pytest.raises(OksError) as e:
-        recon_dev(ctx, "oks_recon_ifft_rss_dev", abi.IfftRssDesc.default(), case)
-    assert e.value.code == abi.OKS_E_UNSUPPORTED
+    d = abi.IfftRssDesc.default()
+    st, ref32, rflags, _ = O.recon("ifft_rss", d, case, "f32")
+    _, ref64, _, _ = O.recon("ifft_rss", d, case, "f64")
+    got, flags, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", d, case)
+    assert st == 0 and flags == rflags and flags & abi.F_HOMODYNE
+    assert_parity(got, ref32, ref64, TOL32, "homodyne ny=16")
 
 
 def test_ifft_rss_prewhiten_skipped(ctx):

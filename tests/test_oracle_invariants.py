@@ -151,8 +151,8 @@ def test_cs_strategy_odd_fallback():
 def test_pf_gate_flags_small_even_fully_sampled_grid():
     # recon/partial_fourier.rs:84-108: a fully sampled even ny has ny/2 rows
     # below DC and ny/2-1 above; for ny <= 20 the asymmetry is >= 1.10, so the
-    # reference routes it to homodyne (not on this path -> UNSUPPORTED).
+    # reference routes it to homodyne.
     case = _small_case()
-    st, _, _, _ = O.recon("ifft_rss", abi.IfftRssDesc.default(), case, "f32")
-    assert st == abi.OKS_E_UNSUPPORTED
+    st, _, flags, _ = O.recon("ifft_rss", abi.IfftRssDesc.default(), case, "f32")
+    assert st == 0 and flags & abi.F_HOMODYNE
     assert not O.pf_detect(np.ones((1, 368, 4), np.uint8), 184)
