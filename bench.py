@@ -352,7 +352,7 @@ def run_c1p(a):
 
     def step():
         check(ctx.ptr, L.oks_place_acquisitions_dev(ctx.ptr, acqs.ctypes.data, len(acqs), dd.addr, data.size,
-                                                     C.byref(e), dw.addr, nc, 1, grid.addr, mask.ctypes.data,
+                                                     C.byref(e), dw.addr, nc, 0, 1, grid.addr, mask.ctypes.data,
                                                      C.byref(placed)))
 
     for _ in range(a.warmup):

@@ -161,14 +161,16 @@ typedef struct oks_pattern {
 int oks_detect_pattern(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx, oks_pattern* out);
 
 /* ----------------------------------------------------- readout placement */
-/* An image-pass acquisition as read_kspace_with_mask reads it: the
- * AcquisitionHeader fields it uses (io/ismrmrd/acquisition.rs:20-64) and where
- * its samples start in the data buffer ([active_channels][number_of_samples],
- * channel-major, complex64), in complex samples. */
+/* An acquisition as read_kspace_with_mask and read_calibration read it: the
+ * AcquisitionHeader fields they use (io/ismrmrd/acquisition.rs:15-64; slice
+ * and segment are idx.slice and idx.segment) and where its samples start in
+ * the data buffer ([active_channels][number_of_samples], channel-major,
+ * complex64), in complex samples.  40 bytes. */
 typedef struct oks_acq {
-    uint64_t flags;  /* ISMRMRD acquisition flags (NON_IMAGE_MASK, ACQ_IS_REVERSE) */
+    uint64_t flags;  /* ISMRMRD flags (NON_IMAGE_MASK, ACQ_IS_REVERSE, ACQ_IS_PHASECORR_DATA) */
     uint16_t number_of_samples, active_channels, discard_pre, discard_post, center_sample;
-    uint16_t kspace_encode_step_1, kspace_encode_step_2, slice;
+    uint16_t kspace_encode_step_1, kspace_encode_step_2, slice, segment;
+    uint16_t reserved[3];  /* zero */
     int64_t data_offset;
 } oks_acq;
 
@@ -314,17 +316,19 @@ int oks_espirit_maps_dev(oks_ctx* ctx, const oks_c32* acs, int64_t nc, int64_t a
 /* read_kspace_with_mask (io/ismrmrd/reader.rs:225-468) with the strategies'
  * preprocess closure (recon/strategy.rs:153-161).  Every image acquisition is
  * whitened by W when it has W's channel count (W: device [w_nc, w_nc] or
- * NULL) and, with remove_oversampling and a valid encoded_x / recon_x pair,
- * loses its readout oversampling when it carries encoded_x samples; then it is
- * placed: DC offsets, centre sample and discards, first writer wins at the
- * readout centre, reversed readouts flipped.  acqs and mask are host memory,
- * data (n_data complex samples) and grid (the oks_place_shape shape) device
+ * NULL); with phase_correct, multiplied by the navigator correction of its
+ * (slice, segment) that PhaseCorrector::from_phasecorr_acqs builds from the
+ * table's ACQ_IS_PHASECORR_DATA acquisitions (recon/phasecorr.rs:61-169);
+ * with remove_oversampling and a valid encoded_x / recon_x pair, it loses its
+ * readout oversampling when it carries encoded_x samples; then it is placed:
+ * DC offsets, centre sample and discards, first writer wins at the readout
+ * centre, reversed readouts flipped.  acqs and mask are host memory, data
+ * (n_data complex samples) and grid (the oks_place_shape shape) device
  * memory; mask [nz, ny, nx] and placed may be NULL.  OKS_E_INCONSISTENT when
- * an image acquisition's samples fall outside data.  Navigator phase
- * correction is not applied (no navigator estimate is formed). */
+ * an image acquisition's or navigator's samples fall outside data. */
 int oks_place_acquisitions_dev(oks_ctx* ctx, const oks_acq* acqs, int64_t n, const oks_c32* data, int64_t n_data,
-                               const oks_encoding* enc, const oks_c32* W, int32_t w_nc, int32_t remove_oversampling,
-                               oks_c32* grid, uint8_t* mask, int64_t* placed);
+                               const oks_encoding* enc, const oks_c32* W, int32_t w_nc, int32_t phase_correct,
+                               int32_t remove_oversampling, oks_c32* grid, uint8_t* mask, int64_t* placed);
 /* fista_cs_single_coil (recon/cs.rs:59-139) on `planes` independent images:
  * kzf [planes, ny, nx], mask [mask_planes (1 or planes), ny, nx] -> out. */
 int oks_fista_cs_dev(oks_ctx* ctx, const oks_c32* kzf, const uint8_t* mask, int64_t planes, int64_t mask_planes,

@@ -66,7 +66,7 @@ def lib():
     L.oks_rss_dev.argtypes = [_P, _P, _I64, _I64, _P]
     L.oks_crop_dev.argtypes = [_P, _P] + [_I64] * 6 + [_P, _P]
     L.oks_place_shape.argtypes = [_P, _I64, _P, _P]
-    L.oks_place_acquisitions_dev.argtypes = [_P, _P, _I64, _P, _I64, _P, _P, _I32, _I32, _P, _P, _P]
+    L.oks_place_acquisitions_dev.argtypes = [_P, _P, _I64, _P, _I64, _P, _P, _I32, _I32, _I32, _P, _P, _P]
     L.oks_os_remove_dev.argtypes = [_P, _P] + [_I64] * 3 + [_P]
     L.oks_prewhiten_calibrate_dev.argtypes = [_P, _P, _I32, _I64, _P]
     L.oks_prewhiten_apply_dev.argtypes = [_P, _P, _P, _I32, _I64]

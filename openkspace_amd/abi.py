@@ -89,12 +89,22 @@ class GrappaDesc(C.Structure):
 ACQ_DTYPE = np.dtype([("flags", "<u8"), ("number_of_samples", "<u2"), ("active_channels", "<u2"),
                       ("discard_pre", "<u2"), ("discard_post", "<u2"), ("center_sample", "<u2"),
                       ("kspace_encode_step_1", "<u2"), ("kspace_encode_step_2", "<u2"), ("slice", "<u2"),
-                      ("data_offset", "<i8")])
+                      ("segment", "<u2"), ("reserved", "<u2", (3,)), ("data_offset", "<i8")])
 
 
 class Acq(C.Structure):
-    _fields_ = [(name, {"<u8": C.c_uint64, "<u2": C.c_uint16, "<i8": C.c_int64}[ACQ_DTYPE.fields[name][0].str])
-                for name in ACQ_DTYPE.names]
+    _fields_ = [("flags", C.c_uint64)] + \
+               [(name, C.c_uint16) for name in ACQ_DTYPE.names[1:10]] + \
+               [("reserved", C.c_uint16 * 3), ("data_offset", C.c_int64)]
+
+
+def acq_records(n, **fields):
+    """n acquisition records (ACQ_DTYPE), the named fields set (scalars or
+    length-n sequences), the others zero."""
+    a = np.zeros(n, ACQ_DTYPE)
+    for k, v in fields.items():
+        a[k] = v
+    return a
 
 
 class Encoding(C.Structure):

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "../../include/oks.h"
@@ -20,6 +21,9 @@ namespace oks {
 constexpr uint64_t kNonImageMask = (1ull << 18) | (1ull << 23) | (1ull << 22) | (1ull << 27) | (1ull << 25) |
                                    (1ull << 26) | (1ull << 28);
 constexpr uint64_t kReverseFlag = 1ull << 21;
+// ACQ_IS_NOISE_MEASUREMENT and ACQ_IS_PHASECORR_DATA: read_calibration files
+// an acquisition under noise first, else under phasecorr (io/ismrmrd/reader.rs:167-183)
+constexpr uint64_t kNoiseFlag = 1ull << 18, kPhaseCorrFlag = 1ull << 23;
 
 static int64_t sat_sub(int64_t a, int64_t b) { return a > b ? a - b : 0; }
 
@@ -46,9 +50,40 @@ static bool shape_of(const oks_acq* a, int64_t n, const oks_encoding& e, int64_t
     return true;
 }
 
+// PhaseCorrector::from_phasecorr_acqs bins (recon/phasecorr.rs:61-104): the
+// navigators of each (slice, segment) in table order, those whose length
+// differs from the bin's first dropped.  Fills the bin and scan tables and
+// the key -> bin index map; returns the longest bin.
+static int nav_bins(const oks_acq* acqs, int64_t n, int64_t n_data, std::vector<NavBin>& bins,
+                    std::vector<NavScan>& scans, std::map<std::pair<int, int>, int>& index) {
+    std::map<std::pair<int, int>, std::vector<NavScan>> by_key;
+    std::map<std::pair<int, int>, int> ns_of;
+    for (int64_t q = 0; q < n; ++q) {
+        const oks_acq& h = acqs[q];
+        if ((h.flags & kNoiseFlag) || !(h.flags & kPhaseCorrFlag)) continue;
+        const int64_t ns = h.number_of_samples, nc = h.active_channels;
+        if (ns == 0 || nc == 0) continue;
+        if (h.data_offset < 0 || h.data_offset > n_data || nc * ns > n_data - h.data_offset)
+            fail(OKS_E_INCONSISTENT, "navigator samples outside the data buffer");
+        const std::pair<int, int> key{h.slice, h.segment};
+        auto it = ns_of.find(key);
+        if (it == ns_of.end()) it = ns_of.emplace(key, (int)ns).first;
+        if (it->second != (int)ns) continue;
+        by_key[key].push_back(NavScan{(long long)h.data_offset, (int)nc});
+    }
+    int longest = 0;
+    for (auto& kv : by_key) {
+        index[kv.first] = (int)bins.size();
+        bins.push_back(NavBin{(int)scans.size(), (int)kv.second.size(), ns_of[kv.first]});
+        scans.insert(scans.end(), kv.second.begin(), kv.second.end());
+        longest = std::max(longest, ns_of[kv.first]);
+    }
+    return longest;
+}
+
 static void place_dev(oks_ctx* ctx, const oks_acq* acqs, int64_t n, const oks_c32* data, int64_t n_data,
-                      const oks_encoding* enc, const oks_c32* W, int32_t w_nc, int32_t remove_os, oks_c32* grid,
-                      uint8_t* mask_out, int64_t* placed_out) {
+                      const oks_encoding* enc, const oks_c32* W, int32_t w_nc, int32_t phase_correct,
+                      int32_t remove_os, oks_c32* grid, uint8_t* mask_out, int64_t* placed_out) {
     if (!acqs || n < 0 || !data || n_data < 0 || !enc || !grid) fail(OKS_E_INVALID_ARG, "bad placement arguments");
     if (W && (w_nc < 1 || w_nc > 32)) fail(OKS_E_UNSUPPORTED, "whitening supports 1 to 32 channels");
     int64_t shape[4];
@@ -60,6 +95,10 @@ static void place_dev(oks_ctx* ctx, const oks_acq* acqs, int64_t n, const oks_c3
     if (os && (enc->encoded_x > 4096 || place_lds(enc->encoded_x, enc->recon_x, 1) > 160 * 1024))
         fail(OKS_E_UNSUPPORTED, "readout longer than 4096 samples");
     const int64_t ky_off = ny / 2 - enc->ky_center, kz_off = sa ? 0 : nz / 2 - enc->kz_center;
+    std::vector<NavBin> bins;
+    std::vector<NavScan> scans;
+    std::map<std::pair<int, int>, int> bin_of;
+    const int corr_stride = phase_correct ? nav_bins(acqs, n, n_data, bins, scans, bin_of) : 0;
     std::vector<uint8_t>& mask = ctx->h_mask;
     mask.assign((size_t)nz * ny * nx, 0);
     // placed segments per (kz, ky) row: [dst, dst + len) and launch generation
@@ -105,23 +144,54 @@ static void place_dev(oks_ctx* ctx, const oks_acq* acqs, int64_t n, const oks_c3
         it.dpre = (int)dpre;
         it.flags = ((h.flags & kReverseFlag) ? kPlaceReverse : 0) | ((W && anc == w_nc) ? kPlaceWhiten : 0) |
                    (os_here ? kPlaceOs : 0);
+        // the bin of (slice, segment), else of (65535, segment) (recon/phasecorr.rs:145-153)
+        it.corr = -1;
+        it.corr_len = 0;
+        if (!bins.empty()) {
+            auto b = bin_of.find({h.slice, h.segment});
+            if (b == bin_of.end()) b = bin_of.find({65535, h.segment});
+            if (b != bin_of.end()) {
+                it.corr = b->second;
+                it.corr_len = bins[b->second].ns;
+            }
+        }
         items.push_back({gen, it});
         std::memset(mrow + dst, 1, (size_t)len);
     }
     // overlapping placements keep the reference's order: later generations
     // run after earlier ones; segments of one generation never overlap
     std::stable_sort(items.begin(), items.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
-    // staging owned by the context, so it outlives the asynchronous copy
-    ctx->h_items.resize(sizeof(PlaceItem) * items.size());
+    // staging owned by the context, so it outlives the asynchronous copies:
+    // the item table, then the navigator bin and scan tables
+    const size_t items_bytes = sizeof(PlaceItem) * items.size();
+    const size_t bins_off = (items_bytes + 15) & ~(size_t)15;
+    const size_t scans_off = (bins_off + sizeof(NavBin) * bins.size() + 15) & ~(size_t)15;
+    ctx->h_items.assign(scans_off + sizeof(NavScan) * scans.size(), 0);
     PlaceItem* tab = (PlaceItem*)ctx->h_items.data();
     for (size_t i = 0; i < items.size(); ++i) tab[i] = items[i].second;
+    if (!bins.empty()) {
+        std::memcpy(ctx->h_items.data() + bins_off, bins.data(), sizeof(NavBin) * bins.size());
+        std::memcpy(ctx->h_items.data() + scans_off, scans.data(), sizeof(NavScan) * scans.size());
+    }
     OKS_HIP(hipMemsetAsync(grid, 0, sizeof(oks_c32) * (size_t)nc * nz * ny * nx, ctx->stream));
+    float2* dcorr = nullptr;
+    if (!bins.empty() && !items.empty()) {
+        unsigned char* dtabs = (unsigned char*)ctx->buf("place_nav", ctx->h_items.size() - bins_off);
+        OKS_HIP(hipMemcpyAsync(dtabs, ctx->h_items.data() + bins_off, ctx->h_items.size() - bins_off,
+                               hipMemcpyHostToDevice, ctx->stream));
+        dcorr = (float2*)ctx->buf("place_corr", sizeof(float2) * bins.size() * (size_t)corr_stride);
+        OKS_HIP(launch_nav_corr((const float2*)data, (const NavBin*)dtabs,
+                                (const NavScan*)(dtabs + (scans_off - bins_off)), (int)bins.size(), dcorr,
+                                corr_stride, ctx->stream));
+    }
     if (!items.empty()) {
         PlaceItem* dtab = (PlaceItem*)ctx->buf("place_items", sizeof(PlaceItem) * items.size());
         OKS_HIP(hipMemcpyAsync(dtab, tab, sizeof(PlaceItem) * items.size(), hipMemcpyHostToDevice, ctx->stream));
         PlaceArgs a{};
         a.data = (const float2*)data;
         a.W = (const float2*)W;
+        a.corr = dcorr;
+        a.corr_stride = corr_stride;
         a.grid = (float2*)grid;
         a.nz = (int)nz;
         a.ny = (int)ny;
@@ -173,10 +243,10 @@ int oks_place_shape(const oks_acq* acqs, int64_t n, const oks_encoding* enc, int
 }
 
 int oks_place_acquisitions_dev(oks_ctx* ctx, const oks_acq* acqs, int64_t n, const oks_c32* data, int64_t n_data,
-                               const oks_encoding* enc, const oks_c32* W, int32_t w_nc, int32_t remove_oversampling,
-                               oks_c32* grid, uint8_t* mask, int64_t* placed) {
+                               const oks_encoding* enc, const oks_c32* W, int32_t w_nc, int32_t phase_correct,
+                               int32_t remove_oversampling, oks_c32* grid, uint8_t* mask, int64_t* placed) {
     return guarded(ctx, [&] {
-        place_dev(ctx, acqs, n, data, n_data, enc, W, w_nc, remove_oversampling, grid, mask, placed);
+        place_dev(ctx, acqs, n, data, n_data, enc, W, w_nc, phase_correct, remove_oversampling, grid, mask, placed);
     });
 }
 

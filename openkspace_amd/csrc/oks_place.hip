@@ -3,16 +3,19 @@
 // preprocess (recon/strategy.rs:153-161) fused in.  The host decides which
 // readouts land where (oks_place.cpp); one workgroup per placed readout moves
 // the samples:
-//   1. one wavefront per coil: the readout-oversampling chain
-//      (recon/oversampling.rs:122-147) in LDS when it applies, then the kept
-//      samples [dpre, dpre + len), reversed for ACQ_IS_REVERSE lines, into
-//      grid[c][kz][ky][dst, dst + len);
+//   1. one wavefront per coil: the navigator phase correction of the raw
+//      samples (recon/phasecorr.rs:141-169) as they are loaded, the
+//      readout-oversampling chain (recon/oversampling.rs:122-147) in LDS when
+//      it applies, then the kept samples [dpre, dpre + len), reversed for
+//      ACQ_IS_REVERSE lines, into grid[c][kz][ky][dst, dst + len);
 //   2. when the acquisition is whitened: y = W s per sample of the placed
 //      segment, in place (recon/prewhiten.rs:128-141), the coil vector held
 //      in registers.  Whitening mixes coils at one sample and the other
 //      steps act along the readout of each coil, so applying it last gives
-//      the reference's result up to FP32 rounding.  The segment was written
-//      by this workgroup just before, so the re-read is served by the cache.
+//      the reference's result up to FP32 rounding (the phase correction
+//      scales every coil of a sample alike, so it commutes with W as well).
+//      The segment was written by this workgroup just before, so the re-read
+//      is served by the cache.
 // ENC / REC > 0 select compile-time transform lengths (in-place stages,
 // twiddles in LDS, one line of LDS per wavefront); 0 the runtime-length
 // transform with a scratch line.  NCW bounds the whitened channel count.
@@ -37,17 +40,25 @@ struct PlaceLds {
     static constexpr int lines = ENC > 0 ? 1 : 2;
 };
 
+// Raw sample x of a line, times the navigator correction where one applies.
+__device__ __forceinline__ float2 raw(const float2* __restrict__ src, const float2* __restrict__ corr, int corr_len,
+                                      int x) {
+    const float2 v = src[x];
+    return (corr && x < corr_len) ? cmul(v, corr[x]) : v;
+}
+
 // Oversampling chain of one coil line: rotate by ceil(enc/2), IFFT(enc)/enc,
 // rotate by enc/2, keep [(enc-rec)/2, +rec), rotate by ceil(rec/2), FFT(rec),
 // and return the line whose sample (m + rec/2) % rec is processed sample m.
 template <int ENC, int REC>
-__device__ __forceinline__ const float2* os_line(const float2* __restrict__ src, float2* buf, const PlaceArgs& a,
-                                                 const float2* tw_e, const float2* tw_r, int lane) {
+__device__ __forceinline__ const float2* os_line(const float2* __restrict__ src, const float2* __restrict__ corr,
+                                                 int corr_len, float2* buf, const PlaceArgs& a, const float2* tw_e,
+                                                 const float2* tw_r, int lane) {
     if constexpr (ENC > 0) {
         constexpr int half = ENC / 2, crop0 = (ENC - REC) / 2, rh = (REC + 1) / 2;
         constexpr int BPL = (REC + 63) / 64;
 #pragma unroll
-        for (int x = lane; x < ENC; x += 64) buf[(x + half) % ENC] = src[x];
+        for (int x = lane; x < ENC; x += 64) buf[(x + half) % ENC] = raw(src, corr, corr_len, x);
         wave_sync();
         CFft<ENC, true>::run(buf, tw_e, lane);
         float2 v[BPL];
@@ -69,7 +80,7 @@ __device__ __forceinline__ const float2* os_line(const float2* __restrict__ src,
     } else {
         const int enc = a.enc, rec = a.rec, half = enc / 2;
         float2* scr = buf + enc;
-        for (int x = lane; x < enc; x += 64) buf[(x + half) % enc] = src[x];
+        for (int x = lane; x < enc; x += 64) buf[(x + half) % enc] = raw(src, corr, corr_len, x);
         wave_sync();
         float2* res = fft_wave<true>(buf, scr, a.p_enc, lane, 64);
         float2* q = (res == buf) ? scr : buf;
@@ -102,11 +113,12 @@ __global__ void __launch_bounds__(kPlaceMaxThreads) k_place(PlaceArgs a) {
     }
     float2* buf = lds + L::tw + (size_t)w * L::lines * (ENC > 0 ? ENC : a.enc);
     const int rec_half = (ENC > 0 ? REC : a.rec) / 2, rec = ENC > 0 ? REC : a.rec;
+    const float2* corr = it.corr >= 0 ? a.corr + (size_t)it.corr * a.corr_stride : nullptr;
     for (int c = w; c < it.nc_fit; c += wpb) {
         const float2* __restrict__ src = a.data + it.src + (size_t)c * it.ns;
         float2* out = a.grid + (size_t)c * plane + row;
         if (os) {
-            const float2* S = os_line<ENC, REC>(src, buf, a, tw_e, tw_r, lane);
+            const float2* S = os_line<ENC, REC>(src, corr, it.corr_len, buf, a, tw_e, tw_r, lane);
             // processed line: rec samples, natural sample m at S[(m + rec/2) % rec], no discards
             for (int i = lane; i < it.len; i += 64) {
                 const int m = rev ? it.len - 1 - i : i;
@@ -116,7 +128,8 @@ __global__ void __launch_bounds__(kPlaceMaxThreads) k_place(PlaceArgs a) {
             }
             wave_sync();
         } else {
-            for (int i = lane; i < it.len; i += 64) out[i] = src[it.dpre + (rev ? it.len - 1 - i : i)];
+            for (int i = lane; i < it.len; i += 64)
+                out[i] = raw(src, corr, it.corr_len, it.dpre + (rev ? it.len - 1 - i : i));
         }
     }
     if constexpr (NCW > 0) {
@@ -174,7 +187,42 @@ hipError_t go_ncw(const PlaceArgs& a, long long n, int wpb, int ncw, size_t lds,
 // of the common matrix sizes).
 #define OKS_PLACE_PAIRS(X) X(640, 320) X(512, 256) X(320, 160) X(256, 128)
 
+// One workgroup per navigator bin: per sample, the coil sums of the bin's
+// scans (channels ascending) added in table order, the average, and
+// conj(avg) / (|avg|, 0) in num_complex's division order, or 1 where
+// |avg| < 1e-20 (recon/phasecorr.rs:70-121).
+__global__ void __launch_bounds__(256) k_nav_corr(const float2* __restrict__ data, const NavBin* __restrict__ bins,
+                                                  const NavScan* __restrict__ scans, float2* __restrict__ corr,
+                                                  int corr_stride) {
+    const NavBin b = bins[blockIdx.x];
+    const float inv_n = 1.0f / (float)b.count;
+    for (int k = threadIdx.x; k < b.ns; k += blockDim.x) {
+        float2 acc = make_float2(0.f, 0.f);
+        for (int s = 0; s < b.count; ++s) {
+            const NavScan sc = scans[b.first + s];
+            float2 cs = make_float2(0.f, 0.f);
+            for (int c = 0; c < sc.nc; ++c) cs = cadd(cs, data[sc.off + (long long)c * b.ns + k]);
+            acc = cadd(acc, cs);
+        }
+        const float2 avg = make_float2(acc.x * inv_n, acc.y * inv_n);
+        const float mag = hypotf(avg.x, avg.y);
+        float2 v = make_float2(1.f, 0.f);
+        if (!(mag < 1e-20f)) {
+            const float den = mag * mag;
+            v = make_float2((avg.x * mag) / den, (-avg.y * mag) / den);
+        }
+        corr[(size_t)blockIdx.x * corr_stride + k] = v;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_nav_corr(const float2* data, const NavBin* bins, const NavScan* scans, int n_bins, float2* corr,
+                           int corr_stride, hipStream_t s) {
+    if (n_bins <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_nav_corr, dim3((unsigned)n_bins), dim3(256), 0, s, data, bins, scans, corr, corr_stride);
+    return hipGetLastError();
+}
 
 int place_ncw(int w_nc) { return w_nc <= 0 ? 0 : w_nc <= 8 ? 8 : w_nc <= 16 ? 16 : w_nc <= 32 ? 32 : -1; }
 

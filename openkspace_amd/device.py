@@ -84,14 +84,16 @@ def place_shape(acqs, enc):
     return tuple(int(v) for v in shape)
 
 
-def place_acquisitions(ctx, acqs, data, enc, W=None, remove_oversampling=False):
+def place_acquisitions(ctx, acqs, data, enc, W=None, remove_oversampling=False, phase_correct=False):
     """read_kspace_with_mask (io/ismrmrd/reader.rs:225-468) with the strategies'
     preprocess closure (recon/strategy.rs:153-161) on device-resident samples.
 
     acqs: host acquisition table (abi.ACQ_DTYPE); data: DeviceArray of complex64
     samples; enc: (encoded_x, encoded_y, encoded_z, recon_x, ky_center,
-    kz_center); W: DeviceArray [w_nc, w_nc] or None.  Returns (grid DeviceArray
-    [nc, nz, ny, nx], mask bool [nz, ny, nx], placed count)."""
+    kz_center); W: DeviceArray [w_nc, w_nc] or None; phase_correct: build the
+    navigator corrector from the table's ACQ_IS_PHASECORR_DATA acquisitions
+    and apply it.  Returns (grid DeviceArray [nc, nz, ny, nx], mask bool
+    [nz, ny, nx], placed count)."""
     a = np.ascontiguousarray(acqs, dtype=abi.ACQ_DTYPE)
     shape = place_shape(a, enc)
     if shape is None:
@@ -103,8 +105,8 @@ def place_acquisitions(ctx, acqs, data, enc, W=None, remove_oversampling=False):
     try:
         check(ctx.ptr, lib().oks_place_acquisitions_dev(
             ctx.ptr, a.ctypes.data, len(a), data.addr, int(np.prod(data.shape)), C.byref(e),
-            W.addr if W is not None else None, W.shape[0] if W is not None else 0, int(bool(remove_oversampling)),
-            grid.addr, mask.ctypes.data, C.byref(placed)))
+            W.addr if W is not None else None, W.shape[0] if W is not None else 0, int(bool(phase_correct)),
+            int(bool(remove_oversampling)), grid.addr, mask.ctypes.data, C.byref(placed)))
     except Exception:
         grid.free()
         raise

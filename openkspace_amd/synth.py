@@ -215,7 +215,8 @@ def make_config(name, **over):
 
 
 def acquisitions(kspace, mask, *, slice_axis=False, ky_center=None, kz_center=None, reverse_every=0,
-                 discard=(0, 0), shuffle_seed=None, noise_scans=0, recon_x=0, seed=SEED_BASE):
+                 discard=(0, 0), shuffle_seed=None, noise_scans=0, recon_x=0, segments=0, nav_per_bin=2,
+                 nav_fallback=False, seed=SEED_BASE):
     """The readouts an ISMRMRD file would hold for `kspace` [nc, nz, ny, nx]:
     one image acquisition per sampled (z, y) row, placed back onto that row by
     read_kspace_with_mask (io/ismrmrd/reader.rs:331-465).
@@ -227,9 +228,19 @@ def acquisitions(kspace, mask, *, slice_axis=False, ky_center=None, kz_center=No
     center_sample marks the row's DC sample (nx/2 + discard_pre).  The sizing
     pass uses the raw sample count, so the placed grid is max(nx, nx +
     discard_pre + discard_post) wide with the row at nx'/2 - nx/2.  `noise_scans` noise
-    acquisitions are interleaved (they must be skipped).  Returns
-    (acquisition table of abi.ACQ_DTYPE, samples complex64,
-     encoding (encoded_x, encoded_y, encoded_z, recon_x, ky_center, kz_center)).
+    acquisitions are interleaved (they must be skipped).
+
+    With `segments` > 0 the readouts carry a per-echo phase error, as TSE/EPI
+    echo trains do: readout r is segment r % segments, and every raw sample k
+    of a (slice, segment) is multiplied by exp(i phi(k)) with phi linear in k
+    (random offset and slope per bin).  `nav_per_bin` ACQ_IS_PHASECORR_DATA
+    navigators per bin, placed first in the table, measure it: positive random
+    amplitudes per coil and sample times exp(i phi(k)), so the navigator
+    correction of recon/phasecorr.rs removes the error exactly.  With
+    `nav_fallback` the navigators carry slice 65535 (the corrector's
+    per-segment fallback key).  Returns (acquisition table of abi.ACQ_DTYPE,
+    samples complex64, encoding (encoded_x, encoded_y, encoded_z, recon_x,
+    ky_center, kz_center)).
     """
     from openkspace_amd import abi
     nc, nz, ny, nx = kspace.shape
@@ -245,11 +256,20 @@ def acquisitions(kspace, mask, *, slice_axis=False, ky_center=None, kz_center=No
     acq = np.zeros(n, abi.ACQ_DTYPE)
     data = np.empty(n * nc * ns, np.complex64)
     noise_at = set(np.linspace(0, n - 1, noise_scans).astype(int).tolist()) if noise_scans else set()
+    phases = {}
+
+    def phase_error(sl, sg):
+        if (sl, sg) not in phases:
+            a0, b0 = rng.uniform(-np.pi, np.pi), rng.uniform(-2.0, 2.0)
+            phases[(sl, sg)] = a0 + b0 * (np.arange(ns) / ns - 0.5)
+        return phases[(sl, sg)]
+
     off, r = 0, 0
     for i in range(n):
         seg = (rng.standard_normal((nc, ns)) + 1j * rng.standard_normal((nc, ns))).astype(np.complex64)
         if i in noise_at:
-            acq[i] = (abi.ACQ_IS_NOISE_MEASUREMENT, ns, nc, 0, 0, 0, 0, 0, 0, off)
+            acq[i] = abi.acq_records(1, flags=abi.ACQ_IS_NOISE_MEASUREMENT, number_of_samples=ns, active_channels=nc,
+                                     data_offset=off)[0]
         else:
             z, y = rows[r]
             r += 1
@@ -258,7 +278,29 @@ def acquisitions(kspace, mask, *, slice_axis=False, ky_center=None, kz_center=No
             seg[:, dpre:dpre + nx] = line[:, ::-1] if rev else line
             ky = y - ny // 2 + kyc
             kz, sl = (0, z) if slice_axis else (z - nz // 2 + kzc, 0)
-            acq[i] = (abi.ACQ_IS_REVERSE if rev else 0, ns, nc, dpre, dpost, nx // 2 + dpre, ky, kz, sl, off)
+            sg = (r - 1) % segments if segments > 0 else 0
+            if segments > 0:
+                seg = (seg * np.exp(1j * phase_error(sl, sg))).astype(np.complex64)
+            acq[i] = abi.acq_records(1, flags=abi.ACQ_IS_REVERSE if rev else 0, number_of_samples=ns, active_channels=nc,
+                                     discard_pre=dpre, discard_post=dpost, center_sample=nx // 2 + dpre,
+                                     kspace_encode_step_1=ky, kspace_encode_step_2=kz, slice=sl, segment=sg,
+                                     data_offset=off)[0]
         data[off:off + nc * ns] = seg.reshape(-1)
         off += nc * ns
-    return acq, data, (nx, ny, nz, recon_x, kyc, kzc)
+    if segments <= 0:
+        return acq, data, (nx, ny, nz, recon_x, kyc, kzc)
+    # navigators first in the table, data appended behind the readouts
+    keys = sorted(phases)
+    nav = abi.acq_records(len(keys) * nav_per_bin, flags=abi.ACQ_IS_PHASECORR_DATA, number_of_samples=ns,
+                          active_channels=nc)
+    ndata = np.empty(len(nav) * nc * ns, np.complex64)
+    j = 0
+    for sl, sg in keys:
+        for _ in range(nav_per_bin):
+            amp = (0.5 + rng.random((nc, ns))) * (0.5 + rng.random())
+            ndata[j * nc * ns:(j + 1) * nc * ns] = (amp * np.exp(1j * phases[(sl, sg)])).astype(np.complex64).ravel()
+            nav[j]["slice"] = 65535 if nav_fallback else sl
+            nav[j]["segment"] = sg
+            nav[j]["data_offset"] = off + j * nc * ns
+            j += 1
+    return np.concatenate([nav, acq]), np.concatenate([data, ndata]), (nx, ny, nz, recon_x, kyc, kzc)

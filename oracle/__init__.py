@@ -54,8 +54,10 @@ def lib():
             getattr(L, f"oko_soft_threshold_{s}").argtypes = [_P, _I64, _I64, C.c_double]
             getattr(L, f"oko_centred_fft2_{s}").argtypes = [_P, _I64, _I64, C.c_int]
             getattr(L, f"oko_fista_cs_{s}").argtypes = [_P, _P, _I64, _I64, _I64, C.c_double, _P]
-            getattr(L, f"oko_place_{s}").argtypes = [_P, _I64, _P, _P, _P, C.c_int, C.c_int, _P, _P]
+            getattr(L, f"oko_place_{s}").argtypes = [_P, _I64, _P, _P, _P, C.c_int, C.c_int, C.c_int, _P, _P]
             getattr(L, f"oko_place_{s}").restype = _I64
+            getattr(L, f"oko_phase_corrections_{s}").argtypes = [_P, _I64, _P, _P, _P, _I64, _I64]
+            getattr(L, f"oko_phase_corrections_{s}").restype = _I64
             for n in ("ifft_rss", "grappa", "cs"):
                 getattr(L, f"oko_recon_{n}_{s}").argtypes = [_P, _P, _P, _P, _P]
             getattr(L, f"oko_recon_sense_{s}").argtypes = [_P, _P, _P, _P, _P, _P]
@@ -168,7 +170,7 @@ def os_remove(lines, rec, prec="f32"):
     return out if st == 0 else None
 
 
-def place(acqs, data, enc, W=None, remove_os=False, prec="f32"):
+def place(acqs, data, enc, W=None, remove_os=False, prec="f32", phase_correct=False):
     """read_kspace_with_mask with the strategies' preprocess closure on an
     acquisition table (numpy array of openkspace_amd.abi.ACQ_DTYPE records,
     samples at data[data_offset:], [nc][ns] channel-major).
@@ -176,7 +178,7 @@ def place(acqs, data, enc, W=None, remove_os=False, prec="f32"):
     Returns (grid [nc, nz, ny, nx], mask [nz, ny, nx] bool, placed count),
     or None where the reference errors (no image acquisition)."""
     a = np.ascontiguousarray(acqs)
-    assert a.dtype.itemsize == 32
+    assert a.dtype.itemsize == 40
     d = _c(np.asarray(data).reshape(-1), prec)
     e = np.ascontiguousarray(enc, dtype=np.int64)
     shape = np.zeros(4, np.int64)
@@ -188,8 +190,8 @@ def place(acqs, data, enc, W=None, remove_os=False, prec="f32"):
     Wc = _c(W, prec) if W is not None else None
     n = getattr(lib(), f"oko_place_{prec}")(a.ctypes.data, len(a), d.ctypes.data, e.ctypes.data,
                                              Wc.ctypes.data if Wc is not None else None,
-                                             Wc.shape[0] if Wc is not None else 0, int(bool(remove_os)),
-                                             grid.ctypes.data, mask.ctypes.data)
+                                             Wc.shape[0] if Wc is not None else 0, int(bool(phase_correct)),
+                                             int(bool(remove_os)), grid.ctypes.data, mask.ctypes.data)
     return grid, mask.astype(bool), int(n)
 
 
@@ -205,6 +207,18 @@ def pattern_detect_1d(ky_any):
     out = np.zeros(5, dtype=np.int64)
     ok = lib().oko_pattern_detect_1d(m.ctypes.data, m.size, out.ctypes.data) == 0
     return tuple(int(v) for v in out) if ok else None
+
+
+def phase_corrections(acqs, data, prec="f32", max_bins=4096, max_len=8192):
+    """PhaseCorrector::from_phasecorr_acqs (recon/phasecorr.rs:61-129) on an
+    acquisition table: {(slice, segment): correction vector}."""
+    a = np.ascontiguousarray(acqs)
+    d = _c(np.asarray(data).reshape(-1), prec)
+    keys = np.zeros((max_bins, 3), np.int64)
+    vecs = np.zeros((max_bins, max_len), _ct(prec))
+    nb = getattr(lib(), f"oko_phase_corrections_{prec}")(a.ctypes.data, len(a), d.ctypes.data, keys.ctypes.data,
+                                                          vecs.ctypes.data, max_bins, max_len)
+    return {(int(keys[b, 0]), int(keys[b, 1])): vecs[b, :keys[b, 2]].copy() for b in range(min(nb, max_bins))}
 
 
 def pf_detect(mask, ky_dc):

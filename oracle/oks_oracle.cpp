@@ -498,13 +498,34 @@ OKO_EXPORT int oko_get_threads(void) { return n_threads(); }
         return prewhiten_calibrate((const cx<T>*)noise, nc, ns, (cx<T>*)W) ? 0 : 1;          \
     }                                                                                        \
     OKO_EXPORT int64_t oko_place_##S(const Acq* a, int64_t n, const T* data, const int64_t enc[6], \
-                                     const T* W, int w_nc, int remove_os, T* grid, uint8_t* mask) { \
+                                     const T* W, int w_nc, int phase_correct, int remove_os, T* grid, \
+                                     uint8_t* mask) {                                        \
         int64_t shape[4];                                                                    \
         bool sa = false;                                                                     \
         if (!place_shape(a, n, Encoding{enc[0], enc[1], enc[2], enc[3], enc[4], enc[5]}, shape, &sa)) \
             return -1;                                                                       \
+        PhaseCorrector<T> pc;                                                                \
+        if (phase_correct) pc.build(a, n, (const cx<T>*)data);                               \
         return place(a, n, (const cx<T>*)data, Encoding{enc[0], enc[1], enc[2], enc[3], enc[4], enc[5]}, \
-                     (const cx<T>*)W, w_nc, remove_os != 0, shape, sa, (cx<T>*)grid, mask);  \
+                     (const cx<T>*)W, w_nc, phase_correct ? &pc : nullptr, remove_os != 0, shape, sa, \
+                     (cx<T>*)grid, mask);                                                    \
+    }                                                                                        \
+    OKO_EXPORT int64_t oko_phase_corrections_##S(const Acq* a, int64_t n, const T* data,     \
+                                                 int64_t* keys, T* vecs, int64_t max_bins,   \
+                                                 int64_t max_len) {                          \
+        PhaseCorrector<T> pc;                                                                \
+        pc.build(a, n, (const cx<T>*)data);                                                  \
+        int64_t b = 0;                                                                       \
+        for (const auto& kv : pc.corr) {                                                     \
+            if (b >= max_bins) break;                                                        \
+            keys[3 * b] = kv.first.first;                                                    \
+            keys[3 * b + 1] = kv.first.second;                                               \
+            keys[3 * b + 2] = (int64_t)kv.second.size();                                     \
+            for (int64_t k = 0; k < (int64_t)kv.second.size() && k < max_len; ++k)          \
+                ((cx<T>*)vecs)[b * max_len + k] = kv.second[k];                              \
+            ++b;                                                                             \
+        }                                                                                    \
+        return (int64_t)pc.corr.size();                                                      \
     }                                                                                        \
     OKO_EXPORT void oko_prewhiten_apply_##S(const T* W, int nc, T* data, int64_t ns) {       \
         prewhiten_apply((const cx<T>*)W, nc, (cx<T>*)data, ns);                              \

@@ -20,6 +20,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <thread>
 #include <vector>
 
@@ -336,7 +337,7 @@ void os_remove_line(const cx<T>* in, int64_t enc, int64_t rec, cx<T>* out,
 // offset of its [nc][ns] channel-major samples.
 struct Acq {
     uint64_t flags;
-    uint16_t ns, nc, dpre, dpost, cs, ky, kz, slice;
+    uint16_t ns, nc, dpre, dpost, cs, ky, kz, slice, segment, reserved[3];
     int64_t offset;
 };
 // NON_IMAGE_MASK and ACQ_IS_REVERSE (io/ismrmrd/acquisition.rs:70-94), bit
@@ -344,6 +345,73 @@ struct Acq {
 constexpr uint64_t kNonImage = (1ull << 18) | (1ull << 23) | (1ull << 22) | (1ull << 27) | (1ull << 25) |
                                (1ull << 26) | (1ull << 28);
 constexpr uint64_t kReverse = 1ull << 21;
+constexpr uint64_t kNoiseScan = 1ull << 18, kPhaseCorrScan = 1ull << 23;
+
+// PhaseCorrector (recon/phasecorr.rs:46-170).  from_phasecorr_acqs: the
+// navigators are the acquisitions read_calibration files under phasecorr
+// (io/ismrmrd/reader.rs:167-183: ACQ_IS_PHASECORR_DATA and not a noise scan),
+// in table order; each with samples and channels adds its coil sum (channels
+// ascending) to the bin of its (slice, segment), whose length the first scan
+// sets (scans of another length are dropped); a bin's vector is its sum times
+// 1/count, turned into conj(avg) / (|avg|, 0) with num_complex's division,
+// or 1 where |avg| < 1e-20.  apply: the image acquisition's bin, else the bin
+// (65535, segment), multiplies samples [0, min(ns, len)) of every channel.
+template <class T>
+struct PhaseCorrector {
+    std::map<std::pair<int, int>, std::vector<cx<T>>> corr;
+
+    void build(const Acq* a, int64_t n, const cx<T>* data) {
+        std::map<std::pair<int, int>, std::pair<std::vector<cx<T>>, uint32_t>> bins;
+        std::vector<std::pair<int, int>> order;
+        for (int64_t q = 0; q < n; ++q) {
+            const Acq& h = a[q];
+            if ((h.flags & kNoiseScan) || !(h.flags & kPhaseCorrScan)) continue;
+            const int64_t ns = h.ns, nc = h.nc;
+            if (ns == 0 || nc == 0) continue;
+            std::vector<cx<T>> cs(ns, cx<T>{T(0), T(0)});
+            for (int64_t c = 0; c < nc; ++c)
+                for (int64_t k = 0; k < ns; ++k) cs[k] += data[h.offset + c * ns + k];
+            const std::pair<int, int> key{h.slice, h.segment};
+            auto it = bins.find(key);
+            if (it == bins.end()) {
+                it = bins.emplace(key, std::make_pair(std::vector<cx<T>>(ns, cx<T>{T(0), T(0)}), 0u)).first;
+            }
+            if ((int64_t)it->second.first.size() != ns) continue;
+            for (int64_t k = 0; k < ns; ++k) it->second.first[k] += cs[k];
+            it->second.second += 1;
+        }
+        for (auto& kv : bins) {
+            std::vector<cx<T>> v = kv.second.first;
+            const T inv_n = T(1) / (T)kv.second.second;
+            for (cx<T>& x : v) {
+                const cx<T> avg = x * cx<T>{inv_n, T(0)};
+                const T mag = cnorm(avg);
+                if (mag < (T)1e-20f) {
+                    x = cx<T>{T(1), T(0)};
+                } else {
+                    const cx<T> cj = conj(avg);
+                    const T den = mag * mag + T(0) * T(0);
+                    x = cx<T>{(cj.re * mag + cj.im * T(0)) / den, (cj.im * mag - cj.re * T(0)) / den};
+                }
+            }
+            corr.emplace(kv.first, std::move(v));
+        }
+    }
+    const std::vector<cx<T>>* find(int slice, int segment) const {
+        auto it = corr.find({slice, segment});
+        if (it == corr.end()) it = corr.find({65535, segment});
+        return it == corr.end() ? nullptr : &it->second;
+    }
+    // on [nc][ns] samples of acquisition h
+    void apply(const Acq& h, cx<T>* s, int64_t nc, int64_t ns) const {
+        if (corr.empty()) return;
+        const std::vector<cx<T>>* v = find(h.slice, h.segment);
+        if (!v) return;
+        const int64_t m = std::min<int64_t>(ns, (int64_t)v->size());
+        for (int64_t c = 0; c < nc; ++c)
+            for (int64_t k = 0; k < m; ++k) s[c * ns + k] = s[c * ns + k] * (*v)[k];
+    }
+};
 
 struct Encoding {
     int64_t ex, ey, ez, rec_x, ky_center, kz_center;
@@ -377,14 +445,16 @@ inline bool place_shape(const Acq* a, int64_t n, const Encoding& e, int64_t shap
 
 // Placement pass (io/ismrmrd/reader.rs:331-465) with the strategies'
 // preprocess closure (recon/strategy.rs:153-161): whitening when the
-// acquisition has W's channel count (recon/prewhiten.rs:118-142), then
-// readout-oversampling removal when valid and the line has encoded_x samples
+// acquisition has W's channel count (recon/prewhiten.rs:118-142), the
+// navigator phase correction (pc may be null), then readout-oversampling
+// removal when valid and the line has encoded_x samples
 // (recon/oversampling.rs:103-154), which leaves rec samples, centre rec / 2
 // and no discards.  grid [nc, nz, ny, nx] and mask [nz, ny, nx] start zeroed.
 // Returns the number of placed acquisitions.
 template <class T>
 int64_t place(const Acq* a, int64_t n, const cx<T>* data, const Encoding& e, const cx<T>* W, int w_nc,
-              bool remove_os, const int64_t shape[4], bool slice_axis, cx<T>* grid, uint8_t* mask) {
+              const PhaseCorrector<T>* pc, bool remove_os, const int64_t shape[4], bool slice_axis, cx<T>* grid,
+              uint8_t* mask) {
     const int64_t nc = shape[0], nz = shape[1], ny = shape[2], nx = shape[3];
     const int64_t ky_off = ny / 2 - e.ky_center, kz_off = slice_axis ? 0 : nz / 2 - e.kz_center;
     const bool os = remove_os && os_valid(e.ex, e.rec_x);
@@ -398,6 +468,7 @@ int64_t place(const Acq* a, int64_t n, const cx<T>* data, const Encoding& e, con
         const int64_t anc = h.nc;
         work.assign(data + h.offset, data + h.offset + anc * ns);
         if (W && anc == w_nc) prewhiten_apply(W, w_nc, work.data(), ns);
+        if (pc) pc->apply(h, work.data(), anc, ns);
         if (os && ns == e.ex) {
             tmp.assign((size_t)anc * e.rec_x, cx<T>{});
             for (int64_t c = 0; c < anc; ++c) os_remove_line(work.data() + c * ns, ns, e.rec_x, tmp.data() + c * e.rec_x, fi, ff);

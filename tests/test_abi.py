@@ -43,7 +43,7 @@ STRUCTS = {
 
 
 def test_acquisition_table_dtype_matches_struct():
-    assert abi.ACQ_DTYPE.itemsize == C.sizeof(abi.Acq) == 32
+    assert abi.ACQ_DTYPE.itemsize == C.sizeof(abi.Acq) == 40
     for name in abi.ACQ_DTYPE.names:
         assert abi.ACQ_DTYPE.fields[name][1] == getattr(abi.Acq, name).offset, name
 

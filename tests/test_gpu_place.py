@@ -1,6 +1,7 @@
 """GPU parity of the readout placement (read_kspace_with_mask,
 io/ismrmrd/reader.rs:225-468, with the strategies' preprocess closure,
-recon/strategy.rs:153-161) against the oracle's restatement.
+recon/strategy.rs:153-161: whitening, navigator phase correction,
+readout-oversampling removal) against the oracle's restatement.
 
 Bars: without preprocessing the placement moves samples and must be
 bit-exact, mask and placed count included, also where later readouts
@@ -29,11 +30,11 @@ def ctx():
     c.close()
 
 
-def _run(ctx, a, d, e, W=None, os=False):
+def _run(ctx, a, d, e, W=None, os=False, pc=False):
     dd = DeviceArray.from_host(ctx, d)
     dw = DeviceArray.from_host(ctx, W) if W is not None else None
     try:
-        grid, mask, n = place_acquisitions(ctx, a, dd, e, W=dw, remove_oversampling=os)
+        grid, mask, n = place_acquisitions(ctx, a, dd, e, W=dw, remove_oversampling=os, phase_correct=pc)
         ctx.synchronize()
         g = grid.download()
         grid.free()
@@ -45,9 +46,8 @@ def _run(ctx, a, d, e, W=None, os=False):
 
 
 def _acq(ns, nc, ky, cs=0, dpre=0, flags=0, off=0):
-    a = np.zeros(1, abi.ACQ_DTYPE)
-    a[0] = (flags, ns, nc, dpre, 0, cs, ky, 0, 0, off)
-    return a
+    return abi.acq_records(1, flags=flags, number_of_samples=ns, active_channels=nc, discard_pre=dpre,
+                           center_sample=cs, kspace_encode_step_1=ky, data_offset=off)
 
 
 @pytest.mark.parametrize("kw", [dict(), dict(reverse_every=3, shuffle_seed=1, noise_scans=5),
@@ -158,6 +158,40 @@ def test_c1_shaped_whiten_and_oversampling(ctx):
     assert_parity(g, r32, r64, TOL, "place c1-shaped")
 
 
+@pytest.mark.parametrize("kw", [dict(segments=3), dict(segments=2, reverse_every=2, discard=(2, 1), shuffle_seed=1),
+                                dict(segments=4, nav_fallback=True, noise_scans=3)])
+def test_phase_correction(ctx, kw):
+    case = synth.make_case(nc=5, nz=4, ny=32, nx=40, kind="grappa", r=2, acs=8, seed=7, ellipses=3, three_d=True)
+    a, d, e = synth.acquisitions(case["kspace"], case["mask"], **kw)
+    g, m, n = _run(ctx, a, d, e, pc=True)
+    r32, rm, rn = O.place(a, d, e, prec="f32", phase_correct=True)
+    r64 = O.place(a, d, e, prec="f64", phase_correct=True)[0]
+    assert n == rn and np.array_equal(m, rm)
+    assert_parity(g, r32, r64, TOL, f"place phasecorr {kw}")
+
+
+def test_phase_correction_without_navigators_is_exact_copy(ctx):
+    case = synth.make_case(nc=3, nz=2, ny=16, nx=12, kind="full", seed=2, ellipses=2, three_d=True)
+    a, d, e = synth.acquisitions(case["kspace"], case["mask"], reverse_every=2)
+    g, m, n = _run(ctx, a, d, e, pc=True)
+    assert np.array_equal(g, O.place(a, d, e)[0])
+
+
+def test_c1_geometry_navigators_whitening_oversampling(ctx):
+    # multi-slice 2-D, 15 coils, 640 encoded -> 320, 3-echo trains with
+    # navigators per (slice, echo), reversed even echoes, noise scans
+    nc, nz, ny, enc, rec = 15, 2, 368, 640, 320
+    case = synth.make_case(nc=nc, nz=nz, ny=ny, nx=enc, kind="full", seed=23, ellipses=4)
+    a, d, e = synth.acquisitions(case["kspace"], case["mask"], slice_axis=True, recon_x=rec, segments=3,
+                                 reverse_every=2, noise_scans=4)
+    W = O.prewhiten_calibrate(case["noise"], "f32")
+    g, m, n = _run(ctx, a, d, e, W=W, os=True, pc=True)
+    r32, rm, rn = O.place(a, d, e, W=W, remove_os=True, prec="f32", phase_correct=True)
+    r64 = O.place(a, d, e, W=W, remove_os=True, prec="f64", phase_correct=True)[0]
+    assert n == rn == nz * ny and np.array_equal(m, rm)
+    assert_parity(g, r32, r64, TOL, "place c1 navigators")
+
+
 def test_errors(ctx):
     data = np.zeros(64, np.complex64)
     dd = DeviceArray.from_host(ctx, data)
@@ -169,6 +203,12 @@ def test_errors(ctx):
         noise = _acq(16, 1, 0, flags=abi.ACQ_IS_NOISE_MEASUREMENT)
         with pytest.raises(ValueError):
             place_acquisitions(ctx, noise, dd, (16, 4, 1, 0, 2, 0))
+        nav = abi.acq_records(1, flags=abi.ACQ_IS_PHASECORR_DATA, number_of_samples=16, active_channels=4,
+                              data_offset=10)  # 64 samples from offset 10 overrun 64
+        with pytest.raises(OksError) as ei:
+            place_acquisitions(ctx, np.concatenate([nav, _acq(16, 1, 1, cs=8)]), dd, (16, 4, 1, 0, 2, 0),
+                               phase_correct=True)
+        assert ei.value.code == abi.OKS_E_INCONSISTENT
         W = DeviceArray(ctx, (33, 33), np.complex64)
         try:
             with pytest.raises(OksError) as ei:

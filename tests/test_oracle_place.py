@@ -61,9 +61,9 @@ def test_multislice_uses_slice_axis():
 
 
 def _acq(ns, nc, ky, kz=0, sl=0, cs=0, dpre=0, dpost=0, flags=0, off=0):
-    a = np.zeros(1, abi.ACQ_DTYPE)
-    a[0] = (flags, ns, nc, dpre, dpost, cs, ky, kz, sl, off)
-    return a
+    return abi.acq_records(1, flags=flags, number_of_samples=ns, active_channels=nc, discard_pre=dpre,
+                           discard_post=dpost, center_sample=cs, kspace_encode_step_1=ky, kspace_encode_step_2=kz,
+                           slice=sl, data_offset=off)
 
 
 def test_dc_offsets_and_out_of_range():
