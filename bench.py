@@ -21,7 +21,8 @@ c4u: the SENSE unfold + g-factor of the C4 volume at ky = 258 (divisible by
 R = 3), one launch per volume.  c1h: C1 with 6/8 partial Fourier along ky,
 reconstructed by homodyne.  c1p: the C1 acquisition's 5888 readouts
 placed into the k-space grid with fused whitening and oversampling removal.
-hbm_copy: the copy-bandwidth ceiling.
+render: the PNG writers' pixels of a 160 x 256 x 256 volume.  hbm_copy: the
+copy-bandwidth ceiling.
 """
 import argparse
 import ctypes as C
@@ -425,6 +426,75 @@ def run_c1p(a):
     print(json.dumps(out))
 
 
+def run_render(a):
+    """The PNG writers' pixels (percentile window 0.5-99.5, gamma 0.9) of a
+    160 x 256 x 256 magnitude volume resident in HBM: per slice two exact
+    order statistics by radix select, then the byte map; one call per step."""
+    from openkspace_amd._lib import Context, DeviceArray, Event, check, lib
+    nz, ny, nx = 160, 256, 256
+    rng = np.random.default_rng(11)
+    vol = np.abs(rng.standard_normal((nz, ny, nx)) + 1j * rng.standard_normal((nz, ny, nx))).astype(np.float32)
+    ctx = Context(0)
+    dv = DeviceArray.from_host(ctx, vol)
+    do = DeviceArray(ctx, (nz, ny, nx), np.uint8)
+    L = lib()
+
+    def step():
+        check(ctx.ptr, L.oks_render_slices_dev(ctx.ptr, dv.addr, nz, ny, nx, C.c_float(0.5), C.c_float(99.5), 0,
+                                                do.addr, None))
+
+    for _ in range(a.warmup):
+        step()
+    ctx.synchronize()
+    evs = [(Event(ctx), Event(ctx)) for _ in range(a.steps)]
+    t0 = time.perf_counter()
+    for s, e in evs:
+        s.record()
+        step()
+        e.record()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    lat = np.array([s.elapsed_ms(e) for s, e in evs])
+    algo = nz * ny * nx * (4 + 1)  # the volume read once, the bytes written once
+    avg_s = float(lat.mean()) * 1e-3
+    out = {
+        "metric": METRIC, "value": nz * a.steps / wall, "unit": "slices/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": wall * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 -> u8", "data": "synthetic",
+        "config": {"workload": "render_png_pixels", "slices": nz, "ky": ny, "kx": nx, "pct_low": 0.5,
+                   "pct_high": 99.5, "note": "write_png_windowed's pixels for every slice; the PNG container "
+                                             "stays with the host"},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "mean": float(lat.mean())},
+        "roofline": {"bound": "hbm", "achieved": algo / avg_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": algo / avg_s / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_select + k_window (one call)", "algorithmic_bytes": algo,
+                     "avg_launch_ms": avg_s * 1e3},
+    }
+    if not a.no_check or not a.no_cpu:
+        import oracle as O
+    if not a.no_check:
+        got = do.download()
+        ref, _ = O.render_slices(vol[:8])
+        out["bytes_equal_first_8_slices"] = float(np.mean(got[:8] == ref))
+    if not a.no_cpu:
+        O.set_threads(1)
+        done, t0 = 0, time.perf_counter()
+        while done < nz:
+            O.render_slices(vol[done:done + 1])
+            done += 1
+            if time.perf_counter() - t0 > a.cpu_seconds:
+                break
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": done / dt, "unit": "slices/s", "cores": 1, "kind": "port",
+                               "sample": f"{done} of {nz} slices through the oracle's sort-based writer, "
+                                         f"single thread, {dt:.1f} s"}
+    dv.free()
+    do.free()
+    ctx.close()
+    print(json.dumps(out))
+
+
 def _pct(v, q):
     return float(np.percentile(np.asarray(v), q)) if len(v) else None
 
@@ -643,6 +713,9 @@ def main():
         return
     if a.workload == "c1p":
         run_c1p(a)
+        return
+    if a.workload == "render":
+        run_render(a)
         return
     if a.workload == "served":
         if a.steps == 50 and "--steps" not in sys.argv:

@@ -334,6 +334,28 @@ int oks_place_acquisitions_dev(oks_ctx* ctx, const oks_acq* acqs, int64_t n, con
 int oks_fista_cs_dev(oks_ctx* ctx, const oks_c32* kzf, const uint8_t* mask, int64_t planes, int64_t mask_planes,
                      int64_t ny, int64_t nx, int64_t iters, float lambda, oks_c32* out);
 
+/* -------------------------------------------------------- output writers */
+enum oks_render_mode {
+    OKS_RENDER_WINDOWED = 0, /* write_png_windowed (cli/main.rs:1112-1145)             */
+    OKS_RENDER_LINEAR = 1    /* g-factor PNG: write_png_linear on [1, max(P99, 2)]
+                                (cli/main.rs:866-870, 1095-1109)                     */
+};
+/* The 8-bit pixels of the CLI's PNG writers for every slice of vol [nz, ny, nx]
+ * (device): per slice the window from percentile() (cli/main.rs:1147-1153,
+ * exact order statistics of the slice) and byte = round(clamp((v - lo) /
+ * (hi - lo), 0, 1)^0.9 * 255) (no exponent in the linear mode) -> out
+ * [nz, ny, nx] u8 (device).  The PNG container itself stays with the host.
+ * window (host, may be NULL) receives [nz][2] (lo, hi) and makes the call
+ * synchronous.  OKS_E_INVALID_ARG for a window the CLI rejects
+ * (cli/main.rs:625-628).  pct_low/pct_high are ignored in the linear mode. */
+int oks_render_slices_dev(oks_ctx* ctx, const float* vol, int64_t nz, int64_t ny, int64_t nx, float pct_low,
+                          float pct_high, int32_t mode, uint8_t* out, float* window);
+/* write_nifti_volume's header (cli/main.rs:1033-1073): 348 header bytes and the
+ * empty 4-byte extension block, 352 bytes into out; the volume follows as f32
+ * little-endian, x fastest (the [nz, ny, nx] C order).  OKS_E_UNSUPPORTED
+ * beyond i16 dimensions.  Host-only. */
+int oks_nifti_header(int64_t nz, int64_t ny, int64_t nx, uint8_t* out);
+
 /* ------------------------------------------------------------- serving */
 /* No counterpart in the reference, whose CLI reconstructs one file per call
  * (cli/main.rs:678-784).  A request loop with continuous batching: requests

@@ -85,6 +85,8 @@ def lib():
     L.oks_server_drain.argtypes = [_P, _I32]
     L.oks_server_get_stats.argtypes = [_P, _P]
     L.oks_request_footprint.argtypes = [_P, C.POINTER(_I64)]
+    L.oks_render_slices_dev.argtypes = [_P, _P, _I64, _I64, _I64, C.c_float, C.c_float, _I32, _P, _P]
+    L.oks_nifti_header.argtypes = [_I64, _I64, _I64, _P]
     _lib = L
     return L
 

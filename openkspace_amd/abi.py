@@ -112,6 +112,8 @@ class Encoding(C.Structure):
                 ("recon_x", C.c_int32), ("ky_center", C.c_int32), ("kz_center", C.c_int32)]
 
 
+RENDER_WINDOWED, RENDER_LINEAR = 0, 1
+
 ACQ_IS_NOISE_MEASUREMENT = 1 << 18
 ACQ_IS_NAVIGATION_DATA = 1 << 22
 ACQ_IS_PHASECORR_DATA = 1 << 23

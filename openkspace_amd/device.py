@@ -111,3 +111,26 @@ def place_acquisitions(ctx, acqs, data, enc, W=None, remove_oversampling=False, 
         grid.free()
         raise
     return grid, mask.astype(bool), int(placed.value)
+
+
+def render_slices(ctx, vol, pct_low=0.5, pct_high=99.5, mode=0):
+    """The 8-bit PNG pixels of every slice of a device volume (DeviceArray
+    float32 [nz, ny, nx]) as the CLI's writers compute them
+    (cli/main.rs:1095-1153).  Returns (pixels uint8 [nz, ny, nx], windows
+    float32 [nz, 2])."""
+    nz, ny, nx = vol.shape
+    out = DeviceArray(ctx, (nz, ny, nx), np.uint8)
+    win = np.zeros((nz, 2), np.float32)
+    try:
+        check(ctx.ptr, lib().oks_render_slices_dev(ctx.ptr, vol.addr, nz, ny, nx, float(pct_low), float(pct_high),
+                                                    int(mode), out.addr, win.ctypes.data))
+        return out.download(), win
+    finally:
+        out.free()
+
+
+def nifti_header(nz, ny, nx):
+    """write_nifti_volume's 352 leading bytes (cli/main.rs:1033-1073)."""
+    out = np.zeros(352, np.uint8)
+    check(None, lib().oks_nifti_header(nz, ny, nx, out.ctypes.data))
+    return out.tobytes()

@@ -68,6 +68,8 @@ def lib():
         L.oko_pf_detect.argtypes = [_P, _I64, _I64, _I64, _I64]
         L.oko_pf_plan.argtypes = [_P, _I64, _I64, _I64, _I64, _P]
         L.oko_pf_weights.argtypes = [_P, _P, _P]
+        L.oko_render_slices.argtypes = [_P, _I64, _I64, _I64, C.c_float, C.c_float, C.c_int, _P, _P]
+        L.oko_nifti_header.argtypes = [_I64, _I64, _I64, _P]
         for s in ("f32", "f64"):
             getattr(L, f"oko_homodyne_{s}").argtypes = [_P, _I64, _I64, _I64, _I64, _P, _P]
         L.oko_grappa_geom.argtypes = [_I64] * 6 + [_P]
@@ -224,6 +226,23 @@ def phase_corrections(acqs, data, prec="f32", max_bins=4096, max_len=8192):
 def pf_detect(mask, ky_dc):
     m = np.ascontiguousarray(mask, dtype=np.uint8)
     return bool(lib().oko_pf_detect(m.ctypes.data, *m.shape, ky_dc))
+
+
+def render_slices(vol, pct_low=0.5, pct_high=99.5, mode=0):
+    """The PNG writers' pixels of every slice of vol [nz, ny, nx] float32
+    (cli/main.rs:1095-1153): (bytes [nz, ny, nx] uint8, windows [nz, 2])."""
+    v = np.ascontiguousarray(vol, dtype=np.float32)
+    nz, ny, nx = v.shape
+    out = np.zeros(v.shape, np.uint8)
+    win = np.zeros((nz, 2), np.float32)
+    lib().oko_render_slices(v.ctypes.data, nz, ny, nx, pct_low, pct_high, mode, out.ctypes.data, win.ctypes.data)
+    return out, win
+
+
+def nifti_header(nz, ny, nx):
+    """write_nifti_volume's 352 leading bytes (cli/main.rs:1033-1073), or None."""
+    out = np.zeros(352, np.uint8)
+    return out.tobytes() if lib().oko_nifti_header(nz, ny, nx, out.ctypes.data) == 0 else None
 
 
 PF_FIELDS = ("ny", "ky_dc", "ky_lo", "ky_hi", "sym_half", "ramp")

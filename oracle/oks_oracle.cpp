@@ -633,6 +633,72 @@ OKO_EXPORT int oko_pattern_detect_1d(const uint8_t* ky_any, int64_t ny, int64_t 
     out[0] = p.r; out[1] = p.acs_start; out[2] = p.acs_end; out[3] = p.ky_lo; out[4] = p.ky_hi;
     return 0;
 }
+// percentile() (cli/main.rs:1147-1153) on a sorted slice: index
+// round(pct / 100 * (n - 1)) in f32, Rust's saturating `as usize`, capped.
+static float percentile_sorted(const std::vector<float>& s, float pct) {
+    if (s.empty()) return 0.0f;
+    const float kf = std::round((pct / 100.0f) * (float)(s.size() - 1));
+    size_t k = 0;
+    if (kf > 0.0f) k = kf >= 1.8e19f ? SIZE_MAX : (size_t)kf;
+    return s[std::min(k, s.size() - 1)];
+}
+
+// Rust's `f32 as u8`: saturating, NaN to 0.
+static uint8_t sat_u8(float v) {
+    if (!(v == v)) return 0;
+    return (uint8_t)(v < 0.0f ? 0.0f : (v > 255.0f ? 255.0f : v));
+}
+
+// write_png_windowed (cli/main.rs:1112-1145, mode 0) and the g-factor form of
+// write_png_linear (:866-870, 1095-1109, mode 1) for every slice of vol
+// [nz, ny, nx]: out [nz, ny, nx] bytes, window [nz][2].
+OKO_EXPORT void oko_render_slices(const float* vol, int64_t nz, int64_t ny, int64_t nx, float pct_low,
+                                  float pct_high, int mode, uint8_t* out, float* window) {
+    const int64_t n = ny * nx;
+    parallel_for(nz, [&](int64_t z) {
+        const float* s = vol + z * n;
+        std::vector<float> v(s, s + n);
+        std::stable_sort(v.begin(), v.end(), [](float a, float b) { return a < b; });
+        float lo, hi;
+        if (mode == 0) {
+            lo = percentile_sorted(v, pct_low);
+            hi = std::fmax(percentile_sorted(v, pct_high), lo + std::numeric_limits<float>::epsilon());
+        } else {
+            lo = 1.0f;
+            hi = std::fmax(std::fmax(percentile_sorted(v, 99.0f), 2.0f), lo + std::numeric_limits<float>::epsilon());
+        }
+        window[2 * z] = lo;
+        window[2 * z + 1] = hi;
+        for (int64_t i = 0; i < n; ++i) {
+            float norm = (s[i] - lo) / (hi - lo);
+            norm = norm < 0.0f ? 0.0f : (norm > 1.0f ? 1.0f : norm);  // f32::clamp keeps NaN
+            const float b = (mode == 0 ? std::pow(norm, 0.9f) : norm) * 255.0f;
+            out[z * n + i] = sat_u8(std::round(b));
+        }
+    });
+}
+
+// write_nifti_volume's header (cli/main.rs:1033-1073): 348 bytes plus the
+// empty extension block.
+OKO_EXPORT int oko_nifti_header(int64_t nz, int64_t ny, int64_t nx, uint8_t* out) {
+    if (nx > 32767 || ny > 32767 || nz > 32767) return 1;
+    std::memset(out, 0, 352);
+    const int32_t sizeof_hdr = 348;
+    std::memcpy(out, &sizeof_hdr, 4);
+    const int16_t dims[8] = {3, (int16_t)nx, (int16_t)ny, (int16_t)nz, 1, 1, 1, 1};
+    std::memcpy(out + 40, dims, 16);
+    const int16_t datatype = 16, bitpix = 32;
+    std::memcpy(out + 70, &datatype, 2);
+    std::memcpy(out + 72, &bitpix, 2);
+    const float pixdim[8] = {1.0f, 1.0f, 1.0f, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    std::memcpy(out + 76, pixdim, 32);
+    const float vox_offset = 352.0f, scl_slope = 1.0f;
+    std::memcpy(out + 108, &vox_offset, 4);
+    std::memcpy(out + 112, &scl_slope, 4);
+    std::memcpy(out + 344, "n+1", 4);
+    return 0;
+}
+
 OKO_EXPORT int oko_pf_plan(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx, int64_t ky_dc,
                            int64_t* plan) {
     PfPlan p;
