@@ -45,6 +45,7 @@ __global__ void __launch_bounds__(cs_cols_max_threads(F::n)) k_cs_cols(CsColArgs
     extern __shared__ float2 lds[];
     const int n = F::n ? F::n : a.p.n;
     const int SX = a.sx, ls = n + 1;
+    const int lsx = __builtin_ctz((unsigned)SX);  // SX is a power of two
     const int WPB = blockDim.x >> 6;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const float2* tw = stage_twiddles<F>(lds, a.p);
@@ -59,10 +60,10 @@ __global__ void __launch_bounds__(cs_cols_max_threads(F::n)) k_cs_cols(CsColArgs
     const long long coil = (a.plane0 + plane) / a.nz;
     const float2* YC = a.yC + coil * a.base[(long long)a.nz * a.nx];  // this coil's compact data
     for (int i = threadIdx.x; i < n * SX; i += blockDim.x) {
-        const int y = i / SX, col = i - y * SX;
+        const int y = i >> lsx, col = i & (SX - 1);
         float2 v = make_float2(0.f, 0.f);
         if (xs + col < a.nx) v = T[(size_t)y * a.nx + xs + col];
-        A[(size_t)col * ls + (y + half) % n] = v;
+        A[(size_t)col * ls + wrap_n(y + half, n)] = v;
     }
     __syncthreads();
     for (int col = w; col < SX; col += WPB) {
@@ -130,8 +131,8 @@ __global__ void __launch_bounds__(cs_cols_max_threads(F::n)) k_cs_cols(CsColArgs
     }
     __syncthreads();
     for (int i = threadIdx.x; i < n * SX; i += blockDim.x) {
-        const int yo = i / SX, col = i - yo * SX;
-        if (xs + col < a.nx) T[(size_t)yo * a.nx + xs + col] = cscale(A[(size_t)col * ls + (yo + half) % n], a.scale);
+        const int yo = i >> lsx, col = i & (SX - 1);
+        if (xs + col < a.nx) T[(size_t)yo * a.nx + xs + col] = cscale(A[(size_t)col * ls + wrap_n(yo + half, n)], a.scale);
     }
 }
 
@@ -257,8 +258,8 @@ __global__ void __launch_bounds__(256) k_cs_update(CsUpdArgs a) {
     const size_t base = ((size_t)plane * a.ny + 2 * yi) * n;
     const int half = n / 2;
     for (int x = lane; x < n; x += 64) {
-        R0[(x + half) % n] = a.T[base + x];
-        R1[(x + half) % n] = a.T[base + n + x];
+        R0[wrap_n(x + half, n)] = a.T[base + x];
+        R1[wrap_n(x + half, n)] = a.T[base + n + x];
     }
     wave_sync();
     float2* g0 = F::run_inv(R0, S, a.p, tw, lane);
@@ -271,10 +272,10 @@ __global__ void __launch_bounds__(256) k_cs_update(CsUpdArgs a) {
         const int x0 = 2 * j, x1 = 2 * j + 1;
         // v = z - g at the four pixels of block j
         const float2 za = Z[x0], zb = Z[x1], zd = Z[n + x0], ze = Z[n + x1];
-        const float2 va = csub(za, cscale(g0[(x0 + half) % n], s));
-        const float2 vb = csub(zb, cscale(g0[(x1 + half) % n], s));
-        const float2 vd = csub(zd, cscale(g1[(x0 + half) % n], s));
-        const float2 ve = csub(ze, cscale(g1[(x1 + half) % n], s));
+        const float2 va = csub(za, cscale(g0[wrap_n(x0 + half, n)], s));
+        const float2 vb = csub(zb, cscale(g0[wrap_n(x1 + half, n)], s));
+        const float2 vd = csub(zd, cscale(g1[wrap_n(x0 + half, n)], s));
+        const float2 ve = csub(ze, cscale(g1[wrap_n(x1 + half, n)], s));
         const float2 ll = cmul(cadd(cadd(cadd(va, vb), vd), ve), c);
         const float2 hl = soft(cmul(csub(csub(cadd(va, vd), vb), ve), c), lam);
         const float2 lh = soft(cmul(csub(csub(cadd(va, vb), vd), ve), c), lam);
@@ -301,10 +302,10 @@ __global__ void __launch_bounds__(256) k_cs_update(CsUpdArgs a) {
         if (a.write_t) {
             // the new z into the slots this lane read g from: the ifftshift
             // positions of the forward row transform
-            g0[(x0 + half) % n] = z0;
-            g0[(x1 + half) % n] = z1;
-            g1[(x0 + half) % n] = z2;
-            g1[(x1 + half) % n] = z3;
+            g0[wrap_n(x0 + half, n)] = z0;
+            g0[wrap_n(x1 + half, n)] = z1;
+            g1[wrap_n(x0 + half, n)] = z2;
+            g1[wrap_n(x1 + half, n)] = z3;
         }
     }
     if (!a.write_t) return;
@@ -315,8 +316,8 @@ __global__ void __launch_bounds__(256) k_cs_update(CsUpdArgs a) {
     wave_sync();
     float2* To = a.Tn + base;
     for (int x = lane; x < n; x += 64) {
-        To[x] = cscale(f0[(x + half) % n], s);
-        To[n + x] = cscale(f1[(x + half) % n], s);
+        To[x] = cscale(f0[wrap_n(x + half, n)], s);
+        To[n + x] = cscale(f1[wrap_n(x + half, n)], s);
     }
 }
 
@@ -408,6 +409,7 @@ static hipError_t go_cs_cols(const CsColArgs& a, hipStream_t s) {
 
 hipError_t launch_cs_cols(const CsColArgs& a, hipStream_t s) {
     if (cs_cols_lds(a.sx, a.p.n) > 160 * 1024) return hipErrorInvalidValue;
+    if (a.sx < 1 || (a.sx & (a.sx - 1)) != 0) return hipErrorInvalidValue;  // strip index maps use shifts
     switch (a.p.n) {
 #define OKS_CASE(N) \
     case N: return go_cs_cols<CsFixed<N>>(a, s);

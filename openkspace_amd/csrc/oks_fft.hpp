@@ -49,6 +49,9 @@ __device__ __forceinline__ float2 twid(const float2* tw, int idx) {
 // which is what lets a workgroup prefetch its next input while it computes.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
+// v mod n for v in [0, 2n): the centring rotations of the index maps.
+__device__ __forceinline__ int wrap_n(int v, int n) { return v >= n ? v - n : v; }
+
 // Wavefront-scope LDS ordering between Stockham stages.
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -27,6 +27,7 @@ __global__ void __launch_bounds__(kHdThreads) k_hd_cols(HdColArgs a) {
     extern __shared__ float2 lds[];
     const int n = F::n ? F::n : a.p.n;
     const int SX = a.sx, ls = n + 1, WPB = blockDim.x >> 6;
+    const int lsx = __builtin_ctz((unsigned)SX);  // SX is a power of two
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const float2* tw = stage_twiddles<F>(lds, a.p);
     float2* A = lds + tw_slots<F>();                          // [SX][ls], H-weighted
@@ -43,7 +44,7 @@ __global__ void __launch_bounds__(kHdThreads) k_hd_cols(HdColArgs a) {
         __syncthreads();  // the previous coil's lines are consumed; acc initialised
         const float2* T = a.T + ((size_t)c * a.zn + zi) * n * a.tx;
         for (int i = threadIdx.x; i < n * SX; i += blockDim.x) {
-            const int y = i / SX, col = i - y * SX;
+            const int y = i >> lsx, col = i & (SX - 1);
             float2 v = make_float2(0.f, 0.f);
             if (xs + col < a.tx) v = T[(size_t)y * a.tx + xs + col];
             int p = y + hy;
@@ -68,7 +69,7 @@ __global__ void __launch_bounds__(kHdThreads) k_hd_cols(HdColArgs a) {
                 wave_sync();
             }
             for (int y = lane; y < a.ty; y += 64) {
-                int idx = (a.y0 + y + hy) % n;
+                const int idx = wrap_n(a.y0 + y + hy, n);
                 const float2 h = cscale(la[idx], inv), l = cscale(lb[idx], inv);
                 const float nrm = hypotf(l.x, l.y);
                 float m = h.x;
@@ -80,7 +81,7 @@ __global__ void __launch_bounds__(kHdThreads) k_hd_cols(HdColArgs a) {
     }
     __syncthreads();
     for (int i = threadIdx.x; i < a.ty * SX; i += blockDim.x) {
-        const int y = i / SX, col = i - y * SX;
+        const int y = i >> lsx, col = i & (SX - 1);
         if (xs + col < a.tx) a.out[((size_t)zi * a.ty + y) * a.tx + xs + col] = sqrtf(acc[i]);
     }
 }
@@ -120,7 +121,8 @@ size_t homodyne_cols_lds(int n, int sx, int ty) {
 }
 
 hipError_t launch_homodyne_cols(const HdColArgs& a, hipStream_t s) {
-    if (a.sx < 1 || a.nc < 1 || a.ty < 1 || a.tx < 1 || a.y0 < 0 || a.y0 + a.ty > a.p.n) return hipErrorInvalidValue;
+    if (a.sx < 1 || (a.sx & (a.sx - 1)) != 0 || a.nc < 1 || a.ty < 1 || a.tx < 1 || a.y0 < 0 || a.y0 + a.ty > a.p.n)
+        return hipErrorInvalidValue;
     switch (a.p.n) {
 #define OKS_CASE(N) \
     case N: return go<FixedFft<N, true>>(a, s);

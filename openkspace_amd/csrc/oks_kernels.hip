@@ -140,7 +140,7 @@ __device__ __forceinline__ void load_whitened(const RowPwArgs& a, float2* A, int
 #pragma unroll
         for (int j = 0; j < NCMAX; ++j)
             if (j < nc) s[j] = src[(size_t)j * a.in_plane + x];
-        const int pos = (x + half) % n;
+        const int pos = wrap_n(x + half, n);
         if (a.W) {
 #pragma unroll
             for (int i = 0; i < NCMAX; ++i) {
@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(512) k_rows_pw(RowPwArgs a) {
     for (int c = w; c < nc; c += WPB) {
         float2* res = F::run(A + (size_t)c * n, B ? B + (size_t)w * n : nullptr, a.p_enc, tw, lane);
         float2* dstp = a.out + (size_t)c * a.out_plane + row * tx;
-        for (int x = lane; x < tx; x += 64) dstp[x] = cscale(res[(x0 + x + half) % n], a.scale);
+        for (int x = lane; x < tx; x += 64) dstp[x] = cscale(res[wrap_n(x0 + x + half, n)], a.scale);
         wave_sync();
     }
 }
@@ -439,12 +439,12 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
     for (int x = lane; x < n; x += 64) {
         float2 v = src[x];
         if (m && !m[x]) v = make_float2(0.f, 0.f);
-        A[(x + half) % n] = v;
+        A[wrap_n(x + half, n)] = v;
     }
     wave_sync();
     float2* res = F::run(A, B, a.p, tw, lane);
     float2* dst = a.out + row * a.tx;
-    for (int x = lane; x < a.tx; x += 64) dst[x] = cscale(res[(a.x0 + x + half) % n], a.scale);
+    for (int x = lane; x < a.tx; x += 64) dst[x] = cscale(res[wrap_n(a.x0 + x + half, n)], a.scale);
 }
 
 // ---------------------------------------------------------------------------
@@ -459,6 +459,7 @@ __global__ void __launch_bounds__(512) k_cols(ColArgs a) {
     extern __shared__ float2 lds[];
     const int n = F::n ? F::n : a.p.n;
     const int SX = a.sx;
+    const int lsx = __builtin_ctz((unsigned)SX);  // SX is a power of two
     const int WPB = blockDim.x >> 6;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int ls = n + 1;                                        // padded line stride (bank spread)
@@ -490,7 +491,7 @@ __global__ void __launch_bounds__(512) k_cols(ColArgs a) {
         for (int k = 0; k < PF; ++k) {
             const int i = (int)threadIdx.x + k * 512;
             if (i < n * SX) {
-                const int y = i / SX, col = i - y * SX;
+                const int y = i >> lsx, col = i & (SX - 1);
                 pf[k] = xs + col < a.X ? src[(size_t)y * a.X + xs + col] : make_float2(0.f, 0.f);
             }
         }
@@ -504,8 +505,8 @@ __global__ void __launch_bounds__(512) k_cols(ColArgs a) {
             for (int k = 0; k < PF; ++k) {
                 const int i = (int)threadIdx.x + k * 512;
                 if (i < n * SX) {
-                    const int y = i / SX, col = i - y * SX;
-                    A[(size_t)col * ls + (y + half) % n] = pf[k];
+                    const int y = i >> lsx, col = i & (SX - 1);
+                    A[(size_t)col * ls + wrap_n(y + half, n)] = pf[k];
                 }
             }
             lds_barrier();
@@ -513,10 +514,10 @@ __global__ void __launch_bounds__(512) k_cols(ColArgs a) {
         } else {
             const float2* __restrict__ src = a.in + ((size_t)c * a.Z + z) * n * a.X;
             for (int i = threadIdx.x; i < n * SX; i += blockDim.x) {
-                const int y = i / SX, col = i - y * SX;
+                const int y = i >> lsx, col = i & (SX - 1);
                 float2 v = make_float2(0.f, 0.f);
                 if (xs + col < a.X) v = src[(size_t)y * a.X + xs + col];
-                A[(size_t)col * ls + (y + half) % n] = v;
+                A[(size_t)col * ls + wrap_n(y + half, n)] = v;
             }
             __syncthreads();
         }
@@ -526,7 +527,7 @@ __global__ void __launch_bounds__(512) k_cols(ColArgs a) {
             if (a.mode == 1) {
                 float* ac = acc + (size_t)col * ty;
                 for (int yo = lane; yo < ty; yo += 64) {
-                    const float2 v = res[(y0 + yo + half) % n];
+                    const float2 v = res[wrap_n(y0 + yo + half, n)];
                     ac[yo] += v.x * v.x + v.y * v.y;
                 }
             } else if (res != line) {
@@ -538,9 +539,9 @@ __global__ void __launch_bounds__(512) k_cols(ColArgs a) {
         if (a.mode == 0) {
             float2* dst = a.out + ((size_t)c * a.Zo + zi) * ty * a.X;
             for (int i = threadIdx.x; i < ty * SX; i += blockDim.x) {
-                const int yo = i / SX, col = i - yo * SX;
+                const int yo = i >> lsx, col = i & (SX - 1);
                 if (xs + col < a.X)
-                    dst[(size_t)yo * a.X + xs + col] = cscale(A[(size_t)col * ls + (y0 + yo + half) % n], a.scale);
+                    dst[(size_t)yo * a.X + xs + col] = cscale(A[(size_t)col * ls + wrap_n(y0 + yo + half, n)], a.scale);
             }
             lds_barrier();
         }
@@ -550,13 +551,13 @@ __global__ void __launch_bounds__(512) k_cols(ColArgs a) {
         if (a.G == 1) {
             float* dst = a.out_r + (size_t)zi * ty * a.X;
             for (int i = threadIdx.x; i < ty * SX; i += blockDim.x) {
-                const int yo = i / SX, col = i - yo * SX;
+                const int yo = i >> lsx, col = i & (SX - 1);
                 if (xs + col < a.X) dst[(size_t)yo * a.X + xs + col] = sqrtf(acc[(size_t)col * ty + yo]) * a.scale;
             }
         } else {
             float* dst = a.part + ((size_t)g * a.Zo + zi) * ty * a.X;
             for (int i = threadIdx.x; i < ty * SX; i += blockDim.x) {
-                const int yo = i / SX, col = i - yo * SX;
+                const int yo = i >> lsx, col = i & (SX - 1);
                 if (xs + col < a.X) dst[(size_t)yo * a.X + xs + col] = acc[(size_t)col * ty + yo];
             }
         }
@@ -789,6 +790,7 @@ static hipError_t go_cols_dir(const ColArgs& a, hipStream_t s) {
 
 hipError_t launch_cols(const ColArgs& a, bool inverse, hipStream_t s) {
     if (cols_lds(a.sx, a.p.n, a.ty, a.mode) > 160 * 1024) return hipErrorInvalidValue;
+    if (a.sx < 1 || (a.sx & (a.sx - 1)) != 0) return hipErrorInvalidValue;  // strip index maps use shifts
     if (a.G < 1 || a.G > a.C) return hipErrorInvalidValue;
     if (a.mode == 1 && a.G > 1 && !a.part) return hipErrorInvalidValue;
     return inverse ? go_cols_dir<true>(a, s) : go_cols_dir<false>(a, s);
