@@ -54,6 +54,8 @@ def _args():
     p.add_argument("--load", type=float, default=0.5, help="served: Poisson rate as a fraction of served rate")
     p.add_argument("--stage", default="cs_iterations", help="served: stage timed for the roofline")
     p.add_argument("--no-graphs", action="store_true", help="launch CS iterations eagerly (no HIP graph)")
+    p.add_argument("--ctx-option", action="append", default=[], metavar="NAME=VALUE",
+                   help="c1-c4: a launch-shape option of the context (oks_ctx_set_option)")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-check", action="store_true")
@@ -732,6 +734,10 @@ def main():
     if a.no_graphs:
         ctx.set_option("graphs", 0)
         wl["config"]["iteration_loop"] = "eager launches"
+    for opt in a.ctx_option:
+        name, val = opt.split("=")
+        ctx.set_option(name, int(val))
+        wl["config"].setdefault("ctx_options", {})[name] = int(val)
     dc = DevCase(ctx, wl["case"])
     ctx.synchronize()
 
