@@ -170,14 +170,14 @@ __global__ void __launch_bounds__(256) k_zgemm_sbf16(typename L::Args args) {
             cr[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
             ci[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
         }
-    const int K = ld.K();
+    const int K0 = ld.k_begin(), K = ld.k_end();
     const int lr = lane & 15, lk = (lane >> 4) * 8;
     // Step k0's operands are fetched into registers during step k0 - BK's
     // MFMAs; the LDS-only barriers leave those loads in flight.
     float2 va[8], vb[8];
-    ld.fetch_a(0, va);
-    ld.template fetch_b<BN>(0, vb);
-    for (int k0 = 0; k0 < K; k0 += BK) {
+    ld.fetch_a(K0, va);
+    ld.template fetch_b<BN>(K0, vb);
+    for (int k0 = K0; k0 < K; k0 += BK) {
         ld.commit_a(PA, va);
         ld.template commit_b<BN>(PB, vb);
         lds_barrier();
@@ -234,16 +234,19 @@ __global__ void __launch_bounds__(256) k_zgemm_sbf16(typename L::Args args) {
 // The column decompositions are tabulated once per workgroup (element offsets
 // from the slice's ACS origin).  Only the lower triangle of the Gram is ever
 // read by the Cholesky, so tiles strictly above the diagonal are skipped.
+// blockIdx.z = z splits + s: split s sums positions [s kc, (s + 1) kc) only,
+// into its own partial output (k_normal64 adds the partials in FP64), so a
+// slice's tiles spread over `splits` times more workgroups.
 struct CalibLoader {
     static constexpr int kTab = 2048 + 512;  // source columns + target columns
     struct Args {
         const float2* grid;
-        float2* out;  // [nz][M][N]
-        int nc, nz, ny, nx, r, kky, kkx, acs0, n_kx, n_pos, rhs, M, N;
+        float2* out;  // [splits][nz][M][N]
+        int nc, nz, ny, nx, r, kky, kkx, acs0, n_kx, n_pos, rhs, M, N, splits, kc;
         const int* skip;  // per slice, or null
     };
     Args a;
-    int m0, n0, z;
+    int m0, n0, z, s;
     const float2* base;
     const int* src_of;
     const int* tgt_of;
@@ -251,7 +254,8 @@ struct CalibLoader {
         const Blk b = xcd_block();
         m0 = b.x * 64;
         n0 = b.y * (a.rhs ? 32 : 64);
-        z = b.z;
+        z = b.z / a.splits;
+        s = b.z - z * a.splits;
         base = a.grid + ((size_t)z * a.ny + a.acs0) * a.nx;
     }
     __device__ bool active() const { return a.rhs || m0 + 64 > n0; }
@@ -271,7 +275,8 @@ struct CalibLoader {
         src_of = tab;
         tgt_of = tab + 2048;
     }
-    __device__ int K() const { return a.n_pos; }
+    __device__ int k_begin() const { return s * a.kc; }
+    __device__ int k_end() const { return min(a.n_pos, (s + 1) * a.kc); }
     // One thread: eight consecutive positions (contiguous kx0 within an ACS
     // row) of one source column; the commit stores them as one 16-byte write
     // per plane.  Thread t holds row t / 4, positions k0 + 8 (t % 4) + [0, 8).
@@ -315,7 +320,7 @@ struct CalibLoader {
     __device__ void store(int m, int n, float2 v) const {
         m += m0;
         n += n0;
-        if (m < a.M && n < a.N) a.out[((size_t)z * a.M + m) * a.N + n] = v;
+        if (m < a.M && n < a.N) a.out[(((size_t)s * a.nz + z) * a.M + m) * a.N + n] = v;
     }
 };
 
@@ -365,7 +370,8 @@ struct ApplyLoader {
         }
         src_of = tab;
     }
-    __device__ int K() const { return a.n_src; }
+    __device__ int k_begin() const { return 0; }
+    __device__ int k_end() const { return a.n_src; }
     // kx0 (m) fastest across lanes (coalesced), eight consecutive source
     // columns j per thread (thread t: m = t % 64, j = k0 + 8 (t / 64) + [0, 8)),
     // committed as one 16-byte write per plane.
@@ -495,10 +501,14 @@ __global__ void __launch_bounds__(256) k_zgemm64(Zgemm64Args g) {
 __global__ void __launch_bounds__(256) k_normal64(Normal64Args a) {
     __shared__ double red[256];
     const int z = blockIdx.y;
-    const float2* G = a.G32 + (size_t)z * a.n * a.n;
-    double2* H = a.G64 + (size_t)z * a.n * a.n;
+    const size_t nn = (size_t)a.n * a.n, nm = (size_t)a.n * a.m;
+    // split p's partial Gram of slice z
+    const float2* G = a.G32 + (size_t)z * nn;
+    const size_t gs = (size_t)a.nz * nn;
+    double2* H = a.G64 + (size_t)z * nn;
     double s = 0.0;
-    for (int i = threadIdx.x; i < a.n; i += blockDim.x) s += (double)G[(size_t)i * a.n + i].x;
+    for (int i = threadIdx.x; i < a.n; i += blockDim.x)
+        for (int p = 0; p < a.splits; ++p) s += (double)G[p * gs + (size_t)i * a.n + i].x;
     red[threadIdx.x] = s;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
@@ -507,17 +517,28 @@ __global__ void __launch_bounds__(256) k_normal64(Normal64Args a) {
     }
     const double mean = red[0] / fmax((double)a.n, 1.0);
     const double lam = (double)a.ridge * fmax(mean, (double)1.1920929e-07f);
-    const size_t nn = (size_t)a.n * a.n, nm = (size_t)a.n * a.m;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nn; e += stride) {
         const size_t i = e / a.n, j = e - i * a.n;
-        const float2 v = G[e];
-        H[e] = make_double2((double)v.x + (i == j ? lam : 0.0), (double)v.y);
+        double x = 0.0, y = 0.0;
+        for (int p = 0; p < a.splits; ++p) {
+            const float2 v = G[p * gs + e];
+            x += (double)v.x;
+            y += (double)v.y;
+        }
+        H[e] = make_double2(x + (i == j ? lam : 0.0), y);
     }
     const float2* R = a.R32 + (size_t)z * nm;
+    const size_t rs = (size_t)a.nz * nm;
     double2* Rd = a.R64 + (size_t)z * nm;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nm; e += stride)
-        Rd[e] = make_double2(R[e].x, R[e].y);
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nm; e += stride) {
+        double x = 0.0, y = 0.0;
+        for (int p = 0; p < a.splits; ++p) {
+            x += (double)R[p * rs + e].x;
+            y += (double)R[p * rs + e].y;
+        }
+        Rd[e] = make_double2(x, y);
+    }
 }
 
 // Cholesky of the diagonal block G[k0:k0+kn, k0:k0+kn] (lower) and its
@@ -696,13 +717,15 @@ hipError_t launch_grappa_fills(const GrappaFillArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_calib_gemm(const CalibArgs& c, hipStream_t s) {
+    if (c.splits < 1 || c.kc < BK || c.kc % BK != 0 || (long long)(c.splits - 1) * c.kc >= c.n_pos)
+        return hipErrorInvalidValue;  // every split holds at least one position
     CalibLoader::Args a{c.grid, c.out, c.nc, c.nz, c.ny, c.nx, c.r, c.kky, c.kkx, c.acs0, c.n_kx, c.n_pos, c.rhs,
-                        c.M, c.N, nullptr};
+                        c.M, c.N, c.splits, c.kc, nullptr};
     if (c.rhs) {
-        dim3 grid((c.M + 63) / 64, (c.N + 31) / 32, c.nz);
+        dim3 grid((c.M + 63) / 64, (c.N + 31) / 32, c.nz * c.splits);
         hipLaunchKernelGGL((k_zgemm_sbf16<CalibLoader, 32>), grid, dim3(256), 0, s, a);
     } else {
-        dim3 grid((c.M + 63) / 64, (c.N + 63) / 64, c.nz);
+        dim3 grid((c.M + 63) / 64, (c.N + 63) / 64, c.nz * c.splits);
         hipLaunchKernelGGL((k_zgemm_sbf16<CalibLoader, 64>), grid, dim3(256), 0, s, a);
     }
     return hipGetLastError();

@@ -15,8 +15,9 @@ struct GrappaFillArgs {
 
 struct CalibArgs {
     const float2* grid;
-    float2* out;  // [nz][M][N]
+    float2* out;  // [splits][nz][M][N]: partial sums over the position ranges
     int nc, nz, ny, nx, r, kky, kkx, acs0, n_kx, n_pos, rhs, M, N;
+    int splits, kc;  // position ranges [s kc, (s + 1) kc), kc a multiple of 32
 };
 
 struct ApplyArgs {
@@ -42,11 +43,11 @@ struct Zgemm64Args {
 };
 
 struct Normal64Args {
-    const float2* G32;
-    const float2* R32;
+    const float2* G32;  // [splits][nz][n][n] partial Grams
+    const float2* R32;  // [splits][nz][n][m] partial right-hand sides
     double2* G64;
     double2* R64;
-    int n, m;
+    int n, m, nz, splits;
     float ridge;
 };
 

@@ -71,11 +71,20 @@ static void grappa_calibrate_batch(oks_ctx* ctx, const float2* grid, int nc, int
     const double dn = n, dm = m, dp = g.n_pos;
     StageTimer timer(ctx, OKS_STAGE_GRAPPA_CALIBRATE,
                      8.0 * nz * (dp * dn * (dn + 1) / 2 + dp * dn * dm + dn * dn * dn / 3 + dn * dn * dm));
-    float2* G32 = (float2*)ctx->buf("gr_G32", sizeof(float2) * (size_t)nz * n * n);
-    float2* R32 = (float2*)ctx->buf("gr_R32", sizeof(float2) * (size_t)nz * n * m);
-    CalibArgs ca{grid, G32, nc, nz, ny, nx, r, kky, kkx, acs0, g.n_kx_pos, g.n_pos, 0, n, n};
+    // split the position sum so the Gram's lower-triangle tiles fill the
+    // GPU: up to 8 ranges of at least 8 32-position steps, while the tiles
+    // stay under 2048 workgroups
+    const int steps = (g.n_pos + 31) / 32, nb = (n + 63) / 64;
+    const long long tiles = (long long)nz * nb * (nb + 1) / 2;
+    int splits = 1;
+    while (splits < 8 && tiles * splits * 2 <= 2048 && steps / (splits * 2) >= 8) splits *= 2;
+    const int kc = (steps + splits - 1) / splits * 32;
+    splits = (g.n_pos + kc - 1) / kc;
+    float2* G32 = (float2*)ctx->buf("gr_G32", sizeof(float2) * (size_t)splits * nz * n * n);
+    float2* R32 = (float2*)ctx->buf("gr_R32", sizeof(float2) * (size_t)splits * nz * n * m);
+    CalibArgs ca{grid, G32, nc, nz, ny, nx, r, kky, kkx, acs0, g.n_kx_pos, g.n_pos, 0, n, n, splits, kc};
     OKS_HIP(launch_calib_gemm(ca, ctx->stream));
-    CalibArgs cb{grid, R32, nc, nz, ny, nx, r, kky, kkx, acs0, g.n_kx_pos, g.n_pos, 1, n, m};
+    CalibArgs cb{grid, R32, nc, nz, ny, nx, r, kky, kkx, acs0, g.n_kx_pos, g.n_pos, 1, n, m, splits, kc};
     OKS_HIP(launch_calib_gemm(cb, ctx->stream));
     const size_t nn = (size_t)n * n, nm = (size_t)n * m;
     double2* G = (double2*)ctx->buf("gr_G64", sizeof(double2) * nz * nn);
@@ -86,7 +95,7 @@ static void grappa_calibrate_batch(oks_ctx* ctx, const float2* grid, int nc, int
     const int nblk = (n + 63) / 64;
     double2* D = (double2*)ctx->buf("gr_D64", sizeof(double2) * (size_t)nz * nblk * 64 * 64);
     OKS_HIP(hipMemsetAsync(fail, 0, sizeof(int) * nz, ctx->stream));
-    Normal64Args na{G32, R32, G, R, n, m, ridge};
+    Normal64Args na{G32, R32, G, R, n, m, nz, splits, ridge};
     OKS_HIP(launch_normal64(na, nz, ctx->stream));
     long long cnt = 3;
     const long long snn = (long long)nn, snm = (long long)nm, sD = (long long)nblk * 64 * 64;

@@ -118,6 +118,11 @@ int64_t footprint(const Req& q) {
                 const int64_t nt = nc * (p.r - 1), nb = (ns + 63) / 64;
                 // FP32 Gram + rhs, FP64 normal matrix, factor, rhs, Y, X, diagonal inverses, weights
                 f += nz * (ns * ns * (8 + 16 + 16) + ns * nt * (8 + 16 * 3) + nb * 64 * 64 * 16 + nt * ns * 8);
+                // the calibration's split partial sums (at most the split count the tiles allow)
+                const int64_t tiles = nz * nb * (nb + 1) / 2;
+                int64_t sp = 1;
+                while (sp < 8 && tiles * sp * 2 <= 2048) sp *= 2;
+                f += (sp - 1) * nz * (ns * ns + ns * nt) * 8;
             }
             break;
         }
