@@ -256,8 +256,9 @@ void ifft_rss_grid(oks_ctx* ctx, const float2* k, int64_t nc, int64_t nz, int64_
 
 // homodyne_reconstruct (recon/partial_fourier.rs:287-356) of the grid after
 // the reader closure, written in the crop window w: the kx inverse transform
-// of every coil's slices (NumPy's shift pair, x crop fused), then the ky pass
-// with both weightings, the demodulation and the coil sum in one kernel.
+// of every coil's slices (NumPy's shift pair, x crop fused; for even kx the
+// closure is fused into it as well), then the ky pass with both weightings,
+// the demodulation and the coil sum in one kernel.
 static void homodyne_dev(oks_ctx* ctx, const oks_kspace* k, const float2* W, int os_mode, int rec, const PfPlan& pf,
                          const Win w[3], float* out) {
     const int64_t nc = k->nc, nz = k->nz, ny = k->ny, nx = k->nx;
@@ -266,20 +267,44 @@ static void homodyne_dev(oks_ctx* ctx, const oks_kspace* k, const float2* W, int
     if (rows_lds((int)nx) > 160 * 1024) fail(OKS_E_UNSUPPORTED, "row length too large for LDS");
     const int sx = ny > 384 ? 4 : 8;
     if (homodyne_cols_lds((int)ny, sx, ty) > 160 * 1024) fail(OKS_E_UNSUPPORTED, "ky length too large for LDS");
-    const float2* g = (W || os_mode) ? prep_grid(ctx, k, W, os_mode, rec, "hd_grid") : (const float2*)k->kspace;
     float2* T = (float2*)ctx->buf("hd_rows", sizeof(float2) * nc * zn * ny * tx);
-    RowArgs r{};
-    r.mask_rows = 1;
-    r.rows = zn * ny;
-    r.tx = tx;
-    r.x0 = (int)w[2].off;
-    r.centred = 2;
-    r.scale = 1.0f / (float)nx;
-    r.p = ctx->plan((int)nx);
-    for (int64_t c = 0; c < nc; ++c) {
-        r.in = g + (c * nz + zlo) * ny * nx;
-        r.out = T + c * zn * ny * tx;
-        OKS_HIP(launch_rows(r, true, ctx->stream));
+    if (nx % 2 == 0 &&
+        rows_prewhiten_lds((int)nc, (int)nx, os_mode, W != nullptr, ctx->rows_pipe != 0) <= 160 * 1024) {
+        // even kx: NumPy's shift pair is the ifft-rss row pass's, which fuses
+        // the reader closure (whitening, oversampling removal) and the x crop
+        RowPwArgs a{};
+        a.pipe = ctx->rows_pipe;
+        a.W = W;
+        a.nc = (int)nc;
+        a.nx = (int)nx;
+        a.tx = tx;
+        a.x0 = (int)w[2].off;
+        a.scale = 1.0f / (float)nx;
+        a.os_mode = os_mode;
+        a.rec = rec;
+        a.in_plane = nz * ny * nx;
+        a.p_enc = ctx->plan((int)nx);
+        if (os_mode) a.p_rec = ctx->plan(rec);
+        a.in = (const float2*)k->kspace + zlo * ny * nx;
+        a.out = T;
+        a.rows = zn * ny;
+        a.out_plane = zn * ny * tx;
+        OKS_HIP(launch_rows_prewhiten_ifft(a, ctx->stream));
+    } else {
+        const float2* g = (W || os_mode) ? prep_grid(ctx, k, W, os_mode, rec, "hd_grid") : (const float2*)k->kspace;
+        RowArgs r{};
+        r.mask_rows = 1;
+        r.rows = zn * ny;
+        r.tx = tx;
+        r.x0 = (int)w[2].off;
+        r.centred = 2;
+        r.scale = 1.0f / (float)nx;
+        r.p = ctx->plan((int)nx);
+        for (int64_t c = 0; c < nc; ++c) {
+            r.in = g + (c * nz + zlo) * ny * nx;
+            r.out = T + c * zn * ny * tx;
+            OKS_HIP(launch_rows(r, true, ctx->stream));
+        }
     }
     // weights staged in the context so they outlive the asynchronous copy
     ctx->h_weights.assign(2 * ny, 0.0f);
