@@ -22,6 +22,7 @@ namespace oks {
 namespace {
 
 constexpr int kSelectThreads = 1024;
+constexpr int kSelectBatch = 8;
 
 // Order-preserving key of a float: negative values inverted, positive ones
 // with the sign bit set, so unsigned order is numeric order.
@@ -48,11 +49,22 @@ __global__ void __launch_bounds__(kSelectThreads) k_select(RenderArgs a) {
         // keys whose digits above `shift` equal the target's prefix so far
         const uint32_t hi_mask = shift == 24 ? 0u : (0xffffffffu << (shift + 8));
         const uint32_t p0 = prefix[0], p1 = prefix[1];
-        for (long long i = threadIdx.x; i < n; i += blockDim.x) {
-            const uint32_t k = fkey(s[i]);
-            const uint32_t d = (k >> shift) & 0xffu;
-            if ((k & hi_mask) == p0) atomicAdd(&hist[0][d], 1u);
-            if ((k & hi_mask) == p1) atomicAdd(&hist[1][d], 1u);
+        // kSelectBatch loads in flight per thread before their histogram updates
+        for (long long i0 = 0; i0 < n; i0 += (long long)blockDim.x * kSelectBatch) {
+            uint32_t k[kSelectBatch];
+#pragma unroll
+            for (int b = 0; b < kSelectBatch; ++b) {
+                const long long i = i0 + (long long)b * blockDim.x + threadIdx.x;
+                k[b] = i < n ? fkey(s[i]) : 0u;  // key 0 is below every float's key
+            }
+#pragma unroll
+            for (int b = 0; b < kSelectBatch; ++b) {
+                const long long i = i0 + (long long)b * blockDim.x + threadIdx.x;
+                if (i >= n) continue;
+                const uint32_t d = (k[b] >> shift) & 0xffu;
+                if ((k[b] & hi_mask) == p0) atomicAdd(&hist[0][d], 1u);
+                if ((k[b] & hi_mask) == p1) atomicAdd(&hist[1][d], 1u);
+            }
         }
         __syncthreads();
         if (threadIdx.x < 2) {
@@ -77,25 +89,45 @@ __global__ void __launch_bounds__(kSelectThreads) k_select(RenderArgs a) {
     }
 }
 
+__device__ __forceinline__ uint8_t window_byte(float v, float lo, float hi, int mode) {
+    float norm = (v - lo) / (hi - lo);
+    // f32::clamp: NaN stays NaN
+    norm = norm < 0.0f ? 0.0f : (norm > 1.0f ? 1.0f : norm);
+    const float b = (mode == kRenderLinear ? norm : powf(norm, 0.9f)) * 255.0f;
+    const float r = roundf(b);
+    return (b != b) ? (uint8_t)0 : (uint8_t)(r < 0.0f ? 0.0f : (r > 255.0f ? 255.0f : r));
+}
+
+// Four pixels per thread (16-byte loads, 4-byte stores) when a slice holds a
+// multiple of four pixels, so the four share a slice and its window, and the
+// buffers are aligned for it (a.vec4, set by the launcher).
 __global__ void k_window(RenderArgs a) {
     const long long n = (long long)a.ny * a.nx, total = n * a.nz;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (long long)gridDim.x * blockDim.x) {
-        const long long z = i / n;
-        const float lo = a.window[2 * z], hi = a.window[2 * z + 1];
-        float norm = (a.vol[i] - lo) / (hi - lo);
-        // f32::clamp: NaN stays NaN
-        norm = norm < 0.0f ? 0.0f : (norm > 1.0f ? 1.0f : norm);
-        const float v = (a.mode == kRenderLinear ? norm : powf(norm, 0.9f)) * 255.0f;
-        const float r = roundf(v);
-        a.out[i] = (v != v) ? (uint8_t)0 : (uint8_t)(r < 0.0f ? 0.0f : (r > 255.0f ? 255.0f : r));
+    const long long step = (long long)gridDim.x * blockDim.x;
+    if (a.vec4) {
+        const float4* v4 = (const float4*)a.vol;
+        uchar4* o4 = (uchar4*)a.out;
+        for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total / 4; q += step) {
+            const long long z = (4 * q) / n;
+            const float lo = a.window[2 * z], hi = a.window[2 * z + 1];
+            const float4 v = v4[q];
+            o4[q] = make_uchar4(window_byte(v.x, lo, hi, a.mode), window_byte(v.y, lo, hi, a.mode),
+                                window_byte(v.z, lo, hi, a.mode), window_byte(v.w, lo, hi, a.mode));
+        }
+    } else {
+        for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += step) {
+            const long long z = i / n;
+            a.out[i] = window_byte(a.vol[i], a.window[2 * z], a.window[2 * z + 1], a.mode);
+        }
     }
 }
 
 }  // namespace
 
-hipError_t launch_render(const RenderArgs& a, hipStream_t s) {
+hipError_t launch_render(const RenderArgs& args, hipStream_t s) {
+    RenderArgs a = args;
     if (a.nz <= 0 || a.ny <= 0 || a.nx <= 0 || a.nz > 0x7fffffff) return hipErrorInvalidValue;
+    a.vec4 = ((long long)a.ny * a.nx) % 4 == 0 && (uintptr_t)a.vol % 16 == 0 && (uintptr_t)a.out % 4 == 0;
     hipLaunchKernelGGL(k_select, dim3((unsigned)a.nz), dim3(kSelectThreads), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

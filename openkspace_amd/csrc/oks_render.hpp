@@ -16,6 +16,7 @@ struct RenderArgs {
     float* window;     // [nz][2] (lo, hi), written by the select pass
     long long k_lo, k_hi;  // ranks of the two order statistics in a slice
     int nz, ny, nx, mode;
+    int vec4;  // set by launch_render: four pixels per thread
 };
 
 hipError_t launch_render(const RenderArgs& a, hipStream_t s);
