@@ -17,6 +17,7 @@ baseline (the FP32 oracle, one thread, on a bounded sample of the stream).
 c1 / c2 / c3 / c4: one multi-slice reconstruction call (ifft-rss 16 x 15 x
 640 x 368, GRAPPA 8 x 32 x 384^2 R=4, CS 32 x 8 x 320^2 120 iterations, 3-D
 SENSE 32 x 160 x 256^2 R=3, which falls back to IFFT + RSS) per step.
+c1os: C1 with readout-oversampling removal on (the reference-exact form).
 c4u: the SENSE unfold + g-factor of the C4 volume at ky = 258 (divisible by
 R = 3), one launch per volume.  c1h: C1 with 6/8 partial Fourier along ky,
 reconstructed by homodyne.  c1p: the C1 acquisition's 5888 readouts
@@ -78,6 +79,19 @@ def workload_c1():
         bound="hbm", algo=algo_bytes, kernel="ifft_rss pass sequence (whitener + row + column)",
         cpu_unit=lambda c, z: _slice_case(c, z), dtype="c64/f32",
     )
+
+
+def workload_c1os():
+    """C1 with the reference-exact composition: readout-oversampling removal on
+    (IfftRss default), so every line runs the 640 -> 320 chain and is placed
+    back into the 640-wide grid before the 2-D IFFT, RSS and the crop to 320."""
+    w = workload_c1()
+    from openkspace_amd import abi
+    w["desc"] = abi.IfftRssDesc.default(remove_oversampling=1)
+    w["name"] = "c1os_ifft_rss"
+    w["config"] = dict(w["config"], workload="c1os_ifft_rss", remove_oversampling=True)
+    w["kernel"] = "ifft_rss pass sequence (whitener + oversampling-chain row pass + column pass)"
+    return w
 
 
 def workload_c1h():
@@ -171,7 +185,8 @@ def workload_c4():
     )
 
 
-WORKLOADS = {"c1": workload_c1, "c2": workload_c2, "c3": workload_c3, "c4": workload_c4, "c1h": workload_c1h}
+WORKLOADS = {"c1": workload_c1, "c2": workload_c2, "c3": workload_c3, "c4": workload_c4, "c1h": workload_c1h,
+             "c1os": workload_c1os}
 
 # HBM bytes the dominant kernels move per algorithmic byte, measured with the
 # FETCH_SIZE / WRITE_SIZE counters scaled by a copy kernel of known size

@@ -218,3 +218,32 @@ def test_errors(ctx):
             W.free()
     finally:
         dd.free()
+
+
+def test_placement_then_ifft_rss_end_to_end(ctx):
+    # the image pass of a file on the GPU: navigators, noise scans, reversed
+    # echoes and 2x oversampled readouts placed with the closure fused, then
+    # ifft-rss on the placed grid with the closure off (it has run), against
+    # the oracle doing the same two steps
+    from tests.gpu_util import recon_dev
+    nc, nz, ny, enc, rec = 6, 2, 48, 128, 64
+    case = synth.make_case(nc=nc, nz=nz, ny=ny, nx=enc, kind="full", seed=31, ellipses=4)
+    a, d, e = synth.acquisitions(case["kspace"], case["mask"], slice_axis=True, recon_x=rec, segments=2,
+                                 reverse_every=2, noise_scans=3)
+    W = O.prewhiten_calibrate(case["noise"], "f64").astype(np.complex64)
+    g, m, n = _run(ctx, a, d, e, W=W, os=True, pc=True)
+    hdr = dict(encoded_x=enc, encoded_y=ny, encoded_z=1, recon_x=rec, recon_y=ny, recon_z=1, is_3d_encoding=0)
+    desc = abi.IfftRssDesc.default(remove_oversampling=0, prewhiten=0, partial_fourier=0)
+    got, _, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", desc,
+                          dict(kspace=g, mask=m.astype(np.uint8), noise=None, hdr=hdr))
+    refs = []
+    for prec in ("f32", "f64"):
+        rg, rm, _ = O.place(a, d, e, W=W, remove_os=True, prec=prec, phase_correct=True)
+        # the strategy's input tensor is complex64 (the FP64 twin's placed grid
+        # is rounded to it, 1e-7, before its FP64 reconstruction)
+        st, img, _, _ = O.recon("ifft_rss", desc, dict(kspace=np.ascontiguousarray(rg.astype(np.complex64)),
+                                                       mask=rm.astype(np.uint8), noise=None, hdr=hdr), prec)
+        assert st == 0
+        refs.append(img)
+    assert got.shape == refs[0].shape == (nz, ny, rec)
+    assert_parity(got, refs[0], refs[1], TOL, "placement + ifft-rss")

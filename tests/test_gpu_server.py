@@ -215,3 +215,43 @@ def test_admission_budget_queue_and_arrivals(ctx):
             assert e.value.code == abi.OKS_E_INVALID_ARG
     finally:
         sl.free()
+
+
+def test_partial_fourier_requests_served(ctx):
+    # ifft-rss requests acquired at 6/8 partial Fourier take the homodyne path
+    # (recon/strategy.rs:163-187) in a batch as alone
+    desc = abi.IfftRssDesc.default(remove_oversampling=0)
+    slices = []
+    for i in range(3):
+        case = synth.make_case(nc=4, nz=1, ny=64, nx=96, kind="full", seed=2000 + i, ellipses=5)
+        m = case["mask"].copy()
+        m[:, :16] = False
+        case["mask"] = np.ascontiguousarray(m)
+        case["kspace"] = np.ascontiguousarray(np.where(m[None], case["kspace"], 0).astype(np.complex64))
+        slices.append(Slice(ctx, case, desc, i))
+    try:
+        alone = {}
+        with Server(slots=1, max_batch=1) as s:
+            for sl in slices:
+                s.submit(sl.request())
+                c = s.collect(1)[sl.tag]
+                assert c.status == 0 and c.flags & abi.F_HOMODYNE
+                alone[sl.tag] = (sl.image(c), c.flags)
+        mixed = slices * 3
+        with Server(slots=2, max_batch=8) as s:
+            t0 = s.now() + 0.05
+            for j, sl in enumerate(mixed):
+                s.submit(Server.request(sl.desc, sl.ks, sl.out.addr, None, 100 + j, t0))
+            got = s.collect(len(mixed))
+        assert max(c.batch for c in got.values()) > 1
+        for sl in slices:
+            c = [got[100 + j] for j, m in enumerate(mixed) if m is sl][-1]
+            a_img, a_fl = alone[sl.tag]
+            assert c.flags == a_fl and rel(sl.image(c), a_img) <= TOL["ifft_rss"]
+            st32, r32, f32, _ = O.recon("ifft_rss", desc, sl.case, "f32")
+            _, r64, _, _ = O.recon("ifft_rss", desc, sl.case, "f64")
+            assert st32 == 0 and f32 == a_fl
+            assert_parity(a_img, r32, r64, TOL["ifft_rss"], "served homodyne")
+    finally:
+        for sl in slices:
+            sl.free()
