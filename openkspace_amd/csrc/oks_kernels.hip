@@ -300,50 +300,124 @@ __global__ void __launch_bounds__(kRowsPipeThreads) k_rows_pw_pipe(RowPwArgs a) 
     }
 }
 
+// Oversampling chain of one whitened coil line, held at its ifftshift
+// positions in `buf` (recon/oversampling.rs:122-147): IFFT(enc) / enc, keep
+// [(enc-rec)/2, +rec) at the ifftshift positions of a rec-point line,
+// FFT(rec) unscaled.  On return the processed line's natural sample m sits at
+// (m + rec/2) % rec of the returned line.  ENC > 0 transforms in place with
+// compile-time lengths (twiddles tw_e / tw_r in LDS, `scr` unused); ENC == 0
+// ping-pongs with the scratch line `scr`.
+template <int ENC, int REC>
+__device__ __forceinline__ float2* os_chain(float2* buf, float2* scr, const RowPwArgs& a, const float2* tw_e,
+                                            const float2* tw_r, int lane) {
+    if constexpr (ENC > 0) {
+        constexpr int half = ENC / 2, crop0 = (ENC - REC) / 2, rh = (REC + 1) / 2;
+        constexpr int BPL = (REC + 63) / 64;
+        CFft<ENC, true>::run(buf, tw_e, lane);
+        float2 v[BPL];
+        const float inv = 1.0f / (float)ENC;
+#pragma unroll
+        for (int b = 0; b < BPL; ++b) {
+            const int i = lane + 64 * b;
+            if (i < REC) v[b] = cscale(buf[(((i + rh) % REC) + crop0 + half) % ENC], inv);
+        }
+        wave_sync();
+#pragma unroll
+        for (int b = 0; b < BPL; ++b) {
+            const int i = lane + 64 * b;
+            if (i < REC) buf[i] = v[b];
+        }
+        wave_sync();
+        CFft<REC, false>::run(buf, tw_r, lane);
+        return buf;
+    } else {
+        const int nx = a.nx, half = nx / 2, rec = a.rec, crop0 = (nx - rec) / 2, rh = (rec + 1) / 2;
+        float2* res = fft_wave<true>(buf, scr, a.p_enc, lane, 64);
+        float2* q = (res == buf) ? scr : buf;
+        const float inv_nx = 1.0f / (float)nx;
+        for (int i = lane; i < rec; i += 64) q[i] = cscale(res[(((i + rh) % rec) + crop0 + half) % nx], inv_nx);
+        wave_sync();
+        float2* other = (q == buf) ? scr : buf;
+        return fft_wave<false>(q, other, a.p_rec, lane, 64);
+    }
+}
+
+// LDS float2 slots of the oversampling-chain row kernels: twiddles of both
+// fixed lengths, the nc coil lines, and for runtime lengths one scratch line
+// per wavefront (8 wavefronts).
+template <int ENC, int REC>
+struct OsRowLds {
+    static constexpr int tw = ENC > 0 ? ((ENC + 1) & ~1) + ((REC + 1) & ~1) : 0;
+};
+
+template <int ENC, int REC>
+__device__ __forceinline__ void os_stage_twiddles(float2* lds, const RowPwArgs& a) {
+    if constexpr (ENC > 0) {
+        for (int i = threadIdx.x; i < ENC; i += blockDim.x) lds[i] = a.p_enc.tw[i];
+        for (int i = threadIdx.x; i < REC; i += blockDim.x) lds[((ENC + 1) & ~1) + i] = a.p_rec.tw[i];
+    }
+}
+
 // Same row pass with the reference's oversampling-removal chain
 // (recon/oversampling.rs:122-147) and its re-centring into the nx grid
 // (io/ismrmrd/reader.rs:392-399) run on every line first.
-template <int NCMAX>
+template <int NCMAX, int ENC, int REC>
 __global__ void __launch_bounds__(512) k_rows_pw_os(RowPwArgs a) {
     extern __shared__ float2 lds[];
-    const int nx = a.nx, nc = a.nc;
+    using L = OsRowLds<ENC, REC>;
+    const int nx = ENC > 0 ? ENC : a.nx, nc = a.nc;
     const int WPB = blockDim.x >> 6;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    float2* A = lds;                       // nc * nx
-    float2* B = lds + (size_t)nc * nx;     // WPB * nx scratch
+    os_stage_twiddles<ENC, REC>(lds, a);
+    const float2* tw_e = lds;
+    const float2* tw_r = lds + (ENC > 0 ? ((ENC + 1) & ~1) : 0);
+    float2* A = lds + L::tw;                                // nc * nx
+    float2* B = ENC > 0 ? nullptr : A + (size_t)nc * nx;    // WPB * nx scratch (runtime lengths)
     const size_t row = blockIdx.x;
     load_whitened<NCMAX>(a, A, nx, row);
     __syncthreads();
-    const int half = nx / 2;
+    const int half = nx / 2, nh = (nx + 1) / 2;
+    const int rec = ENC > 0 ? REC : a.rec, dst = half - rec / 2;
     const int tx = a.tx, x0 = a.x0;
     for (int c = w; c < nc; c += WPB) {
         float2* buf = A + (size_t)c * nx;
-        float2* scr = B + (size_t)w * nx;
-        float2* res = fft_wave<true>(buf, scr, a.p_enc, lane, 64);
-        // res holds b = IFFT_nx(ifftshift(line)); img[x] = b[(x+nx/2)%nx] / nx
-        const int rec = a.rec, crop0 = (nx - rec) / 2, rh = (rec + 1) / 2;
-        float2* q = (res == buf) ? scr : buf;
-        const float inv_nx = 1.0f / (float)nx;
-        for (int i = lane; i < rec; i += 64) {
-            const int m = (i + rh) % rec;  // ifftshift of the cropped line
-            q[i] = cscale(res[(m + crop0 + half) % nx], inv_nx);
+        float2* scr = B ? B + (size_t)w * nx : nullptr;
+        float2* S = os_chain<ENC, REC>(buf, scr, a, tw_e, tw_r, lane);
+        // the processed line re-centred at [dst, dst + rec) of a zero grid
+        // line, written at its ifftshift positions for the kx IFFT
+        float2* res;
+        if constexpr (ENC > 0) {
+            constexpr int BPE = (ENC + 63) / 64;
+            float2 v[BPE];
+#pragma unroll
+            for (int b = 0; b < BPE; ++b) {
+                const int i = lane + 64 * b;
+                if (i < ENC) {
+                    const int m = wrap_n(i + nh, ENC) - dst;
+                    v[b] = (m >= 0 && m < REC) ? S[wrap_n(m + REC / 2, REC)] : make_float2(0.f, 0.f);
+                }
+            }
+            wave_sync();
+#pragma unroll
+            for (int b = 0; b < BPE; ++b) {
+                const int i = lane + 64 * b;
+                if (i < ENC) buf[i] = v[b];
+            }
+            wave_sync();
+            CFft<ENC, true>::run(buf, tw_e, lane);
+            res = buf;
+        } else {
+            float2* K = (S == buf) ? scr : buf;
+            for (int i = lane; i < nx; i += 64) {
+                const int m = (i + nh) % nx - dst;
+                K[i] = (m >= 0 && m < rec) ? S[(m + rec / 2) % rec] : make_float2(0.f, 0.f);
+            }
+            wave_sync();
+            float2* other2 = (K == buf) ? scr : buf;
+            res = fft_wave<true>(K, other2, a.p_enc, lane, 64);
         }
-        wave_sync();
-        float2* other = (q == buf) ? scr : buf;
-        float2* S = fft_wave<false>(q, other, a.p_rec, lane, 64);
-        float2* K = (S == buf) ? scr : buf;
-        const int dst = half - rec / 2;
-        const int nh = (nx + 1) / 2;
-        for (int i = lane; i < nx; i += 64) {
-            const int xk = (i + nh) % nx;  // ifftshift of the re-centred grid line
-            const int m = xk - dst;
-            K[i] = (m >= 0 && m < rec) ? S[(m + rec / 2) % rec] : make_float2(0.f, 0.f);
-        }
-        wave_sync();
-        float2* other2 = (K == buf) ? scr : buf;
-        res = fft_wave<true>(K, other2, a.p_enc, lane, 64);
         float2* dstp = a.out + (size_t)c * a.out_plane + row * tx;
-        for (int x = lane; x < tx; x += 64) dstp[x] = cscale(res[(x0 + x + half) % nx], a.scale);
+        for (int x = lane; x < tx; x += 64) dstp[x] = cscale(res[wrap_n(x0 + x + half, nx)], a.scale);
         wave_sync();
     }
 }
@@ -353,18 +427,22 @@ __global__ void __launch_bounds__(512) k_rows_pw_os(RowPwArgs a) {
 // (recon/oversampling.rs:122-147) and lands re-centred at
 // [nx/2 - rec/2, nx/2 + rec/2) of a zeroed grid row (io/ismrmrd/reader.rs:392-399).
 // Rows that were never acquired (all-zero mask row, `acq[row] == 0`) stay zero.
-template <int NCMAX>
+template <int NCMAX, int ENC, int REC>
 __global__ void __launch_bounds__(512) k_prep_os(RowPwArgs a, const uint8_t* acq) {
     extern __shared__ float2 lds[];
-    const int nx = a.nx, nc = a.nc;
+    using L = OsRowLds<ENC, REC>;
+    const int nx = ENC > 0 ? ENC : a.nx, nc = a.nc;
     const int WPB = blockDim.x >> 6;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    float2* A = lds;
-    float2* B = lds + (size_t)nc * nx;
+    os_stage_twiddles<ENC, REC>(lds, a);
+    const float2* tw_e = lds;
+    const float2* tw_r = lds + (ENC > 0 ? ((ENC + 1) & ~1) : 0);
+    float2* A = lds + L::tw;
+    float2* B = ENC > 0 ? nullptr : A + (size_t)nc * nx;
     const size_t row = blockIdx.x;
     load_whitened<NCMAX>(a, A, nx, row);
     __syncthreads();
-    const int half = nx / 2, rec = a.rec, dst = half - rec / 2;
+    const int half = nx / 2, rec = ENC > 0 ? REC : a.rec, dst = half - rec / 2;
     const bool on = acq[row] != 0;
     for (int c = w; c < nc; c += WPB) {
         float2* outp = a.out + (size_t)c * a.out_plane + row * nx;
@@ -373,18 +451,11 @@ __global__ void __launch_bounds__(512) k_prep_os(RowPwArgs a, const uint8_t* acq
             continue;
         }
         float2* buf = A + (size_t)c * nx;
-        float2* scr = B + (size_t)w * nx;
-        float2* res = fft_wave<true>(buf, scr, a.p_enc, lane, 64);
-        const int crop0 = (nx - rec) / 2, rh = (rec + 1) / 2;
-        float2* q = (res == buf) ? scr : buf;
-        const float inv_nx = 1.0f / (float)nx;
-        for (int i = lane; i < rec; i += 64) q[i] = cscale(res[(((i + rh) % rec) + crop0 + half) % nx], inv_nx);
-        wave_sync();
-        float2* other = (q == buf) ? scr : buf;
-        float2* S = fft_wave<false>(q, other, a.p_rec, lane, 64);
+        float2* scr = B ? B + (size_t)w * nx : nullptr;
+        const float2* S = os_chain<ENC, REC>(buf, scr, a, tw_e, tw_r, lane);
         for (int x = lane; x < nx; x += 64) {
             const int m = x - dst;
-            outp[x] = (m >= 0 && m < rec) ? S[(m + rec / 2) % rec] : make_float2(0.f, 0.f);
+            outp[x] = (m >= 0 && m < rec) ? S[wrap_n(m + rec / 2, rec)] : make_float2(0.f, 0.f);
         }
         wave_sync();
     }
@@ -655,6 +726,26 @@ static bool is_fixed(int n) {
 
 static size_t tw_lds_slots(int n) { return is_fixed(n) ? (size_t)((n + 1) & ~1) : 0; }
 
+// Oversampling pairs (enc, rec = enc / 2) the chain kernels are built for with
+// compile-time lengths; other pairs take the runtime-length transform.
+#define OKS_OS_PAIRS(X) X(640, 320) X(512, 256) X(320, 160) X(256, 128)
+
+static bool os_pair(int enc, int rec) {
+    switch (enc) {
+#define OKS_CASE(E, R) \
+    case E: return rec == R;
+        OKS_OS_PAIRS(OKS_CASE)
+#undef OKS_CASE
+        default: return false;
+    }
+}
+
+static size_t os_rows_lds(int nc, int nx, int rec) {
+    const bool fixed = os_pair(nx, rec);
+    const size_t tw = fixed ? (size_t)((nx + 1) & ~1) + (size_t)((rec + 1) & ~1) : 0;
+    return (tw + (size_t)nc * nx + (fixed ? 0 : (size_t)8 * nx)) * sizeof(float2);
+}
+
 static bool pipe_size(int n) {
     switch (n) {
 #define OKS_CASE(N) case N:
@@ -667,8 +758,9 @@ static bool pipe_size(int n) {
 static bool rows_pipe_used(bool pipe, int os_mode, int nx) { return pipe && !os_mode && pipe_size(nx); }
 static bool rows_pipe_used(const RowPwArgs& a) { return rows_pipe_used(a.pipe != 0, a.os_mode, a.nx); }
 
-size_t rows_prewhiten_lds(int nc, int nx, int os_mode, bool white, bool pipe) {
-    if (os_mode || !is_fixed(nx)) return ((size_t)nc * nx + (size_t)8 * nx) * sizeof(float2);
+size_t rows_prewhiten_lds(int nc, int nx, int os_mode, int rec, bool white, bool pipe) {
+    if (os_mode) return os_rows_lds(nc, nx, rec);
+    if (!is_fixed(nx)) return ((size_t)nc * nx + (size_t)8 * nx) * sizeof(float2);
     // pipelined kernel: twiddles, W^T when whitening, coils
     if (rows_pipe_used(pipe, os_mode, nx))
         return (tw_lds_slots(nx) + (white ? (size_t)nc * nc : 0) + (size_t)nc * nx) * sizeof(float2);
@@ -694,12 +786,27 @@ static hipError_t go_rows_pw_pipe(const RowPwArgs& a, size_t lds, hipStream_t s)
     return hipGetLastError();
 }
 
+template <int NCMAX, int ENC, int REC>
+static hipError_t go_rows_pw_os(const RowPwArgs& a, size_t lds, hipStream_t s) {
+    OKS_ATTR(hipFuncSetAttribute((const void*)k_rows_pw_os<NCMAX, ENC, REC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)lds));
+    hipLaunchKernelGGL((k_rows_pw_os<NCMAX, ENC, REC>), dim3((unsigned)a.rows), dim3(512), lds, s, a);
+    return hipGetLastError();
+}
+
 template <int NCMAX>
 static hipError_t go_rows_pw_n(const RowPwArgs& a, size_t lds, hipStream_t s) {
     if (a.os_mode) {
-        OKS_ATTR(hipFuncSetAttribute((const void*)k_rows_pw_os<NCMAX>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_rows_pw_os<NCMAX>, dim3((unsigned)a.rows), dim3(512), lds, s, a);
-        return hipGetLastError();
+        if (os_pair(a.nx, a.rec)) {
+            switch (a.nx) {
+#define OKS_CASE(E, R) \
+    case E: return go_rows_pw_os<NCMAX, E, R>(a, lds, s);
+                OKS_OS_PAIRS(OKS_CASE)
+#undef OKS_CASE
+                default: break;
+            }
+        }
+        return go_rows_pw_os<NCMAX, 0, 0>(a, lds, s);
     }
     if (rows_pipe_used(a)) {
         switch (a.nx) {
@@ -720,7 +827,7 @@ static hipError_t go_rows_pw_n(const RowPwArgs& a, size_t lds, hipStream_t s) {
 }
 
 hipError_t launch_rows_prewhiten_ifft(const RowPwArgs& a, hipStream_t s) {
-    const size_t lds = rows_prewhiten_lds(a.nc, a.nx, a.os_mode, a.W != nullptr, a.pipe != 0);
+    const size_t lds = rows_prewhiten_lds(a.nc, a.nx, a.os_mode, a.rec, a.W != nullptr, a.pipe != 0);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     if (a.nc <= 16) return go_rows_pw_n<16>(a, lds, s);
     if (a.nc <= 32) return go_rows_pw_n<32>(a, lds, s);
@@ -796,19 +903,34 @@ hipError_t launch_cols(const ColArgs& a, bool inverse, hipStream_t s) {
     return inverse ? go_cols_dir<true>(a, s) : go_cols_dir<false>(a, s);
 }
 
-hipError_t launch_prep_os(const RowPwArgs& a, const uint8_t* acq, hipStream_t s) {
-    const size_t lds = ((size_t)a.nc * a.nx + (size_t)8 * a.nx) * sizeof(float2);
-    if (lds > 160 * 1024) return hipErrorInvalidValue;
-    if (a.nc <= 16) {
-        OKS_ATTR(hipFuncSetAttribute((const void*)k_prep_os<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_prep_os<16>, dim3((unsigned)a.rows), dim3(512), lds, s, a, acq);
-    } else if (a.nc <= 32) {
-        OKS_ATTR(hipFuncSetAttribute((const void*)k_prep_os<32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_prep_os<32>, dim3((unsigned)a.rows), dim3(512), lds, s, a, acq);
-    } else {
-        return hipErrorInvalidValue;
-    }
+template <int NCMAX, int ENC, int REC>
+static hipError_t go_prep_os(const RowPwArgs& a, const uint8_t* acq, size_t lds, hipStream_t s) {
+    OKS_ATTR(hipFuncSetAttribute((const void*)k_prep_os<NCMAX, ENC, REC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)lds));
+    hipLaunchKernelGGL((k_prep_os<NCMAX, ENC, REC>), dim3((unsigned)a.rows), dim3(512), lds, s, a, acq);
     return hipGetLastError();
+}
+
+template <int NCMAX>
+static hipError_t go_prep_os_n(const RowPwArgs& a, const uint8_t* acq, size_t lds, hipStream_t s) {
+    if (os_pair(a.nx, a.rec)) {
+        switch (a.nx) {
+#define OKS_CASE(E, R) \
+    case E: return go_prep_os<NCMAX, E, R>(a, acq, lds, s);
+            OKS_OS_PAIRS(OKS_CASE)
+#undef OKS_CASE
+            default: break;
+        }
+    }
+    return go_prep_os<NCMAX, 0, 0>(a, acq, lds, s);
+}
+
+hipError_t launch_prep_os(const RowPwArgs& a, const uint8_t* acq, hipStream_t s) {
+    const size_t lds = os_rows_lds(a.nc, a.nx, a.rec);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    if (a.nc <= 16) return go_prep_os_n<16>(a, acq, lds, s);
+    if (a.nc <= 32) return go_prep_os_n<32>(a, acq, lds, s);
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_os_remove(const float2* in, float2* out, long long rows, int enc, int rec, const FftPlan& p_enc,

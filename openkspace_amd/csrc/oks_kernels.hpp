@@ -74,7 +74,7 @@ struct ColArgs {
 
 hipError_t launch_whitener(const float2* noise, int nc, int ns, float2* W, int* status, hipStream_t s);
 hipError_t launch_whitener_batch(const WhitenArgs& a, int B, hipStream_t s);
-size_t rows_prewhiten_lds(int nc, int nx, int os_mode, bool white, bool pipe);
+size_t rows_prewhiten_lds(int nc, int nx, int os_mode, int rec, bool white, bool pipe);
 hipError_t launch_rows_prewhiten_ifft(const RowPwArgs& a, hipStream_t s);
 hipError_t launch_rows(const RowArgs& a, bool inverse, hipStream_t s);
 size_t cols_lds(int sx, int n, int ty, int mode);

@@ -217,7 +217,7 @@ void ifft_rss_grid(oks_ctx* ctx, const float2* k, int64_t nc, int64_t nz, int64_
     a.in_plane = nz * ny * nx;
     a.p_enc = ctx->plan((int)nx);
     if (os_mode) a.p_rec = ctx->plan(rec);
-    if (rows_prewhiten_lds((int)nc, (int)nx, os_mode, W != nullptr, ctx->rows_pipe != 0) > 160 * 1024)
+    if (rows_prewhiten_lds((int)nc, (int)nx, os_mode, rec, W != nullptr, ctx->rows_pipe != 0) > 160 * 1024)
         fail(OKS_E_UNSUPPORTED, "readout too long for the fused row pass");
     if (!three_d) {
         // slices actually written: the z window of the crop
@@ -269,7 +269,7 @@ static void homodyne_dev(oks_ctx* ctx, const oks_kspace* k, const float2* W, int
     if (homodyne_cols_lds((int)ny, sx, ty) > 160 * 1024) fail(OKS_E_UNSUPPORTED, "ky length too large for LDS");
     float2* T = (float2*)ctx->buf("hd_rows", sizeof(float2) * nc * zn * ny * tx);
     if (nx % 2 == 0 &&
-        rows_prewhiten_lds((int)nc, (int)nx, os_mode, W != nullptr, ctx->rows_pipe != 0) <= 160 * 1024) {
+        rows_prewhiten_lds((int)nc, (int)nx, os_mode, rec, W != nullptr, ctx->rows_pipe != 0) <= 160 * 1024) {
         // even kx: NumPy's shift pair is the ifft-rss row pass's, which fuses
         // the reader closure (whitening, oversampling removal) and the x crop
         RowPwArgs a{};
