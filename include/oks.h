@@ -203,7 +203,9 @@ void* oks_ctx_stream(oks_ctx* ctx);
  * speed only, never results.  "rows_pipe" (0/1, default 1) selects the
  * pipelined IFFT+RSS row pass, which whitens after the kx IFFT instead of
  * before it (the same linear map, FP32 rounding in another order);
- * "cols_pipe" (0/1, default 1) lets column passes prefetch the next coil. */
+ * "cols_pipe" (0/1, default 1) lets column passes prefetch the next coil;
+ * "rows_waves" (0 = automatic, or 8..16) sets the wavefronts per workgroup of
+ * the one-row-per-workgroup row passes. */
 int oks_ctx_set_option(oks_ctx* ctx, const char* name, int64_t value);
 int oks_ctx_synchronize(oks_ctx* ctx);
 /* Flags raised on the device by the *_dev calls since the last take

@@ -269,6 +269,9 @@ int oks_ctx_set_option(oks_ctx* ctx, const char* name, int64_t value) {
         } else if (n == "rows_pipe" || n == "cols_pipe") {
             if (value != 0 && value != 1) fail(OKS_E_INVALID_ARG, n + " must be 0 or 1");
             (n == "rows_pipe" ? ctx->rows_pipe : ctx->cols_pipe) = (int)value;
+        } else if (n == "rows_waves") {
+            if (value != 0 && (value < 8 || value > 16)) fail(OKS_E_INVALID_ARG, "rows_waves must be 0 or 8..16");
+            ctx->rows_waves = (int)value;
         } else if (n == "zchunk") {
             if (value < 1) fail(OKS_E_INVALID_ARG, "zchunk must be >= 1");
             ctx->zchunk = (int)value;

@@ -7,6 +7,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "oks_fft.hpp"
 #include "oks_kernels.hpp"
@@ -130,6 +133,12 @@ __global__ void __launch_bounds__(256) k_whitener(WhitenArgs wa) {
 // recon/prewhiten.rs:128-141), written to LDS at its ifftshift position, then
 // one wavefront per coil runs the kx IFFT and stores the fftshifted, x-cropped
 // line (recon/fft.rs:19-29 + recon/strategy.rs:227-253).
+// Largest workgroup of the one-row-per-workgroup passes for a compile-time
+// length n (0: runtime length).  Runtime lengths keep 8 wavefronts, one
+// scratch line each; 368 keeps 8 because its transform needs more than the
+// 128 registers a 1024-thread workgroup leaves.
+constexpr int rows_pw_bound(int n) { return (n == 0 || n == 368) ? 512 : 1024; }
+
 template <int NCMAX>
 __device__ __forceinline__ void load_whitened(const RowPwArgs& a, float2* A, int n, size_t row) {
     const int nc = a.nc;
@@ -160,7 +169,7 @@ __device__ __forceinline__ void load_whitened(const RowPwArgs& a, float2* A, int
 }
 
 template <int NCMAX, class F>
-__global__ void __launch_bounds__(512) k_rows_pw(RowPwArgs a) {
+__global__ void __launch_bounds__(rows_pw_bound(F::n)) k_rows_pw(RowPwArgs a) {
     extern __shared__ float2 lds[];
     const int n = F::n ? F::n : a.nx;
     const int nc = a.nc;
@@ -362,7 +371,7 @@ __device__ __forceinline__ void os_stage_twiddles(float2* lds, const RowPwArgs& 
 // (recon/oversampling.rs:122-147) and its re-centring into the nx grid
 // (io/ismrmrd/reader.rs:392-399) run on every line first.
 template <int NCMAX, int ENC, int REC>
-__global__ void __launch_bounds__(512) k_rows_pw_os(RowPwArgs a) {
+__global__ void __launch_bounds__(rows_pw_bound(ENC)) k_rows_pw_os(RowPwArgs a) {
     extern __shared__ float2 lds[];
     using L = OsRowLds<ENC, REC>;
     const int nx = ENC > 0 ? ENC : a.nx, nc = a.nc;
@@ -428,7 +437,7 @@ __global__ void __launch_bounds__(512) k_rows_pw_os(RowPwArgs a) {
 // [nx/2 - rec/2, nx/2 + rec/2) of a zeroed grid row (io/ismrmrd/reader.rs:392-399).
 // Rows that were never acquired (all-zero mask row, `acq[row] == 0`) stay zero.
 template <int NCMAX, int ENC, int REC>
-__global__ void __launch_bounds__(512) k_prep_os(RowPwArgs a, const uint8_t* acq) {
+__global__ void __launch_bounds__(rows_pw_bound(ENC)) k_prep_os(RowPwArgs a, const uint8_t* acq) {
     extern __shared__ float2 lds[];
     using L = OsRowLds<ENC, REC>;
     const int nx = ENC > 0 ? ENC : a.nx, nc = a.nc;
@@ -767,10 +776,44 @@ size_t rows_prewhiten_lds(int nc, int nx, int os_mode, int rec, bool white, bool
     return (tw_lds_slots(nx) + (size_t)nc * nx) * sizeof(float2);
 }
 
+// Workgroup width of the one-row-per-workgroup passes: a.waves wavefronts when
+// set; otherwise, from 8 up to one wavefront per coil (two rounds past 16
+// coils, so the transform phase runs in full rounds), the width that keeps the
+// most wavefronts resident per CU by HIP's occupancy calculator (registers,
+// LDS, the wider on ties).  At 15 coils of 640 samples (80 registers, 80 KiB
+// of LDS) that is 12: two workgroups per CU, where 15 would leave one.
+static int rows_pw_threads(const void* fn, const RowPwArgs& a, int n, size_t lds) {
+    if (rows_pw_bound(n) <= 512) return 512;
+    if (a.waves > 0) return 64 * std::min(16, a.waves);
+    const int cap = std::max(8, std::min(16, a.nc <= 16 ? a.nc : (a.nc + 1) / 2));
+    static std::mutex mu;
+    static std::map<std::tuple<const void*, size_t, int>, int> memo;
+    const auto key = std::make_tuple(fn, lds, cap);
+    {
+        std::lock_guard<std::mutex> g(mu);
+        const auto it = memo.find(key);
+        if (it != memo.end()) return it->second;
+    }
+    int best = 8, best_res = -1;
+    for (int w = 8; w <= cap; ++w) {
+        int blocks = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, 64 * w, lds) != hipSuccess) blocks = 0;
+        if (blocks * w >= best_res) {
+            best_res = blocks * w;
+            best = w;
+        }
+    }
+    if (best_res <= 0) best = 8;  // calculator unavailable: the narrowest width
+    std::lock_guard<std::mutex> g(mu);
+    memo[key] = 64 * best;
+    return 64 * best;
+}
+
 template <int NCMAX, class F>
 static hipError_t go_rows_pw(const RowPwArgs& a, size_t lds, hipStream_t s) {
     OKS_ATTR(hipFuncSetAttribute((const void*)k_rows_pw<NCMAX, F>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_rows_pw<NCMAX, F>), dim3((unsigned)a.rows), dim3(512), lds, s, a);
+    const int threads = rows_pw_threads((const void*)k_rows_pw<NCMAX, F>, a, F::n, lds);
+    hipLaunchKernelGGL((k_rows_pw<NCMAX, F>), dim3((unsigned)a.rows), dim3(threads), lds, s, a);
     return hipGetLastError();
 }
 
@@ -790,7 +833,8 @@ template <int NCMAX, int ENC, int REC>
 static hipError_t go_rows_pw_os(const RowPwArgs& a, size_t lds, hipStream_t s) {
     OKS_ATTR(hipFuncSetAttribute((const void*)k_rows_pw_os<NCMAX, ENC, REC>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)lds));
-    hipLaunchKernelGGL((k_rows_pw_os<NCMAX, ENC, REC>), dim3((unsigned)a.rows), dim3(512), lds, s, a);
+    const int threads = rows_pw_threads((const void*)k_rows_pw_os<NCMAX, ENC, REC>, a, ENC, lds);
+    hipLaunchKernelGGL((k_rows_pw_os<NCMAX, ENC, REC>), dim3((unsigned)a.rows), dim3(threads), lds, s, a);
     return hipGetLastError();
 }
 
@@ -907,7 +951,8 @@ template <int NCMAX, int ENC, int REC>
 static hipError_t go_prep_os(const RowPwArgs& a, const uint8_t* acq, size_t lds, hipStream_t s) {
     OKS_ATTR(hipFuncSetAttribute((const void*)k_prep_os<NCMAX, ENC, REC>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)lds));
-    hipLaunchKernelGGL((k_prep_os<NCMAX, ENC, REC>), dim3((unsigned)a.rows), dim3(512), lds, s, a, acq);
+    const int threads = rows_pw_threads((const void*)k_prep_os<NCMAX, ENC, REC>, a, ENC, lds);
+    hipLaunchKernelGGL((k_prep_os<NCMAX, ENC, REC>), dim3((unsigned)a.rows), dim3(threads), lds, s, a, acq);
     return hipGetLastError();
 }
 

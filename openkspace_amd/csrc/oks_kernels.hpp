@@ -35,6 +35,7 @@ struct RowPwArgs {
     float scale;
     int os_mode, rec;   // reference oversampling removal to `rec` samples
     int pipe;           // fixed lengths: pipelined persistent kernel (1) or one row per workgroup (0)
+    int waves;          // one row per workgroup: wavefronts per workgroup (0 = automatic, 8..16)
     FftPlan p_enc;      // n = nx, inverse
     FftPlan p_rec;      // n = rec, forward (os_mode)
 };

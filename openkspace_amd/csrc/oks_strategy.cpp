@@ -162,6 +162,7 @@ float2* prep_grid(oks_ctx* ctx, const oks_kspace* k, const float2* W, int os_mod
         a.nx = (int)k->nx;
         a.rec = rec;
         a.os_mode = 1;
+        a.waves = ctx->rows_waves;
         a.p_enc = ctx->plan((int)k->nx);
         a.p_rec = ctx->plan(rec);
         OKS_HIP(launch_prep_os(a, acq, ctx->stream));
@@ -206,6 +207,7 @@ void ifft_rss_grid(oks_ctx* ctx, const float2* k, int64_t nc, int64_t nz, int64_
                      (double)nc * zread * ny * nx * 8.0 + (double)w[0].len * ty * tx * 4.0);
     RowPwArgs a{};
     a.pipe = ctx->rows_pipe;
+    a.waves = ctx->rows_waves;
     a.W = W;
     a.nc = (int)nc;
     a.nx = (int)nx;
@@ -274,6 +276,7 @@ static void homodyne_dev(oks_ctx* ctx, const oks_kspace* k, const float2* W, int
         // the reader closure (whitening, oversampling removal) and the x crop
         RowPwArgs a{};
         a.pipe = ctx->rows_pipe;
+        a.waves = ctx->rows_waves;
         a.W = W;
         a.nc = (int)nc;
         a.nx = (int)nx;

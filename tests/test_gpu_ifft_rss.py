@@ -155,3 +155,34 @@ def test_ifft_rss_pass_forms(ctx, rows_pipe, cols_pipe, nc, nx, prewhiten):
         ctx.set_option("cols_pipe", 1)
     assert st == 0 and flags == rflags and got.shape == ref32.shape
     assert_parity(got, ref32, ref64, TOL32, f"rows_pipe={rows_pipe} cols_pipe={cols_pipe} nc={nc} nx={nx}")
+
+
+@pytest.mark.parametrize("nc,nx,remove_os", [(15, 640, 0), (15, 640, 1), (5, 256, 1), (20, 320, 0), (9, 368, 0)])
+def test_ifft_rss_row_pass_widths(ctx, nc, nx, remove_os):
+    # the wavefronts per workgroup of the one-row-per-workgroup row passes
+    # (automatic, or 8 .. 16) change the launch only: every width gives the
+    # same bits, and those meet the parity bar
+    case = _case(nc=nc, nz=2, ny=24, nx=nx, os_x=2)
+    d = abi.IfftRssDesc.default(remove_oversampling=remove_os)
+    st, ref32, rflags, _ = O.recon("ifft_rss", d, case, "f32")
+    _, ref64, _, _ = O.recon("ifft_rss", d, case, "f64")
+    ctx.set_option("rows_pipe", 0)
+    outs = {}
+    try:
+        for w in (0, 8, 11, 16):
+            ctx.set_option("rows_waves", w)
+            outs[w], flags, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", d, case)
+            assert flags == rflags
+    finally:
+        ctx.set_option("rows_waves", 0)
+        ctx.set_option("rows_pipe", 1)
+    for w, got in outs.items():
+        assert np.array_equal(got, outs[8]), f"rows_waves={w}"
+    assert st == 0
+    assert_parity(outs[0], ref32, ref64, TOL32, f"nc={nc} nx={nx} os={remove_os}")
+
+
+def test_rows_waves_option_range(ctx):
+    for bad in (-1, 1, 7, 17):
+        with pytest.raises(Exception):
+            ctx.set_option("rows_waves", bad)
