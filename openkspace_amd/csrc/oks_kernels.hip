@@ -26,100 +26,161 @@ namespace oks {
 // ---------------------------------------------------------------------------
 // Noise whitener, FP64 (recon/prewhiten.rs:48-111), one workgroup per
 // request b = blockIdx.x.  Psi = (1/max(N-1,1)) sum_s n_s n_s^H, L = chol(Psi),
-// W = L^-1, all in double in LDS; W is stored as complex64.  Where the
-// reference returns None (fewer samples than channels, or a covariance that is
-// not positive definite: recon/prewhiten.rs:49-66,160-164) W = I and the
-// request's status is raised.  256 threads, nc <= 32.
-__global__ void __launch_bounds__(256) k_whitener(WhitenArgs wa) {
-    __shared__ double2 psi[32 * 32];
-    __shared__ double2 L[32 * 32];
-    __shared__ double2 Li[32 * 32];
-    __shared__ float2 chunk[32 * 256];  // noise samples staged 256 at a time
-    __shared__ int bad;
+// W = L^-1, all in double; W is stored as complex64.  Where the reference
+// returns None (fewer samples than channels, or a covariance that is not
+// positive definite: recon/prewhiten.rs:49-66,160-164) W = I and the
+// request's status is raised.  512 threads, nc <= 32.
+//
+//   covariance : lower-triangle pairs (i, j) x S sample ranges spread over
+//                the workgroup, four independent FP64 accumulators per item;
+//                noise staged 256 samples at a time at a row stride of 257
+//                samples, so that the rows start in different LDS banks;
+//   Cholesky   : wavefront 0, lane j for row j, left-looking by columns
+//                (recon/prewhiten.rs:148-179): step i forms every lane's
+//                s_j = Psi_ji - sum_{k<i} L_jk conj(L_ik) at once, the
+//                diagonal comes from lane i by a shuffle, and only a
+//                wavefront barrier separates the steps;
+//   inverse    : wavefront 0, lane c forward-substitutes column c of L^-1.
+constexpr int kWhitenThreads = 512;
+
+__global__ void __launch_bounds__(kWhitenThreads) k_whitener(WhitenArgs wa) {
+    constexpr int kStride = 257;
+    constexpr int T = kWhitenThreads;
+    __shared__ double2 L[32 * 32];  // Psi's lower triangle, then the factor, [i nc + k]
+    __shared__ __align__(16) float2 chunk[32 * kStride];
+    __shared__ double dinv[32];
     const int t = threadIdx.x, b = blockIdx.x, nc = wa.nc;
     const float2* __restrict__ noise = wa.noise[b];
     const int ns = (noise && wa.ns[b] >= nc) ? wa.ns[b] : 0;
     float2* __restrict__ W = wa.W + (size_t)b * nc * nc;
     const double denom = fmax((double)ns - 1.0, 1.0);
-    double accr[4] = {0.0, 0.0, 0.0, 0.0}, acci[4] = {0.0, 0.0, 0.0, 0.0};
+    // item it = s * npairs + p, p = i (i + 1) / 2 + j; thread t takes t and t + T
+    const int npairs = nc * (nc + 1) / 2;
+    const int S = npairs >= T ? 1 : min(8, T / npairs);
+    const int items = npairs * S;
+    int pi[2], pj[2], ps[2];
+    double acc[2][4][2] = {};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int it = t + k * T;
+        const int p = it % npairs;
+        int i = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
+        while (i > 0 && i * (i + 1) / 2 > p) --i;
+        while ((i + 1) * (i + 2) / 2 <= p) ++i;
+        pi[k] = i;
+        pj[k] = p - i * (i + 1) / 2;
+        ps[k] = it / npairs;
+    }
     for (int s0 = 0; s0 < ns; s0 += 256) {
         const int cs = min(256, ns - s0);
         __syncthreads();
-        for (int e = t; e < nc * 256; e += blockDim.x) {
-            const int c = e / 256, q = e % 256;
-            chunk[e] = q < cs ? noise[(size_t)c * ns + s0 + q] : make_float2(0.f, 0.f);
+        for (int e = t; e < nc * 256; e += T) {
+            const int c = e >> 8, q = e & 255;
+            chunk[c * kStride + q] = q < cs ? noise[(size_t)c * ns + s0 + q] : make_float2(0.f, 0.f);
         }
         __syncthreads();
-        for (int k = 0; k < 4; ++k) {
-            const int e = t + k * 256;
-            if (e >= nc * nc) break;
-            const int i = e / nc, j = e % nc;
-            if (j > i) continue;
-            const float2* a = chunk + i * 256;
-            const float2* b = chunk + j * 256;
-            for (int q = 0; q < cs; ++q) {
-                const double ar = a[q].x, ai = a[q].y, br = b[q].x, bi = b[q].y;
-                accr[k] += ar * br + ai * bi;  // a * conj(b)
-                acci[k] += ai * br - ar * bi;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (t + k * T < items) {
+                const float2* x = chunk + pi[k] * kStride;
+                const float2* y = chunk + pj[k] * kStride;
+                const int q0 = ps[k] * cs / S, q1 = (ps[k] + 1) * cs / S;
+                int q = q0;
+                for (; q + 4 <= q1; q += 4) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const double ar = x[q + u].x, ai = x[q + u].y, br = y[q + u].x, bi = y[q + u].y;
+                        acc[k][u][0] += ar * br + ai * bi;  // x * conj(y)
+                        acc[k][u][1] += ai * br - ar * bi;
+                    }
+                }
+                for (; q < q1; ++q) {
+                    const double ar = x[q].x, ai = x[q].y, br = y[q].x, bi = y[q].y;
+                    acc[k][0][0] += ar * br + ai * bi;
+                    acc[k][0][1] += ai * br - ar * bi;
+                }
             }
         }
     }
-    for (int k = 0; k < 4; ++k) {
-        const int e = t + k * 256;
-        if (e >= nc * nc) break;
-        psi[e] = make_double2(accr[k] / denom, acci[k] / denom);
-        L[e] = make_double2(0.0, 0.0);
-        Li[e] = make_double2(0.0, 0.0);
-    }
-    if (t == 0) bad = ns == 0;
+    // partial sums through the (now free) staging buffer, then Psi
+    double2* part = reinterpret_cast<double2*>(chunk);
     __syncthreads();
-    // Row-wise Cholesky (recon/prewhiten.rs:148-179) on the lower triangle.
-    for (int i = 0; i < nc; ++i) {
-        if (t == 0) {
-            double dr = psi[i * nc + i].x;
-            for (int k = 0; k < i; ++k) {
-                double2 l = L[i * nc + k];
-                dr -= l.x * l.x + l.y * l.y;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int it = t + k * T;
+        if (it < items)
+            part[it] = make_double2((acc[k][0][0] + acc[k][1][0]) + (acc[k][2][0] + acc[k][3][0]),
+                                    (acc[k][0][1] + acc[k][1][1]) + (acc[k][2][1] + acc[k][3][1]));
+    }
+    __syncthreads();
+    for (int e = t; e < nc * nc; e += T) {
+        const int i = e / nc, j = e - (e / nc) * nc;
+        double sr = 0.0, si = 0.0;
+        if (j <= i) {
+            const int p = i * (i + 1) / 2 + j;
+            for (int q = 0; q < S; ++q) {
+                sr += part[q * npairs + p].x;
+                si += part[q * npairs + p].y;
             }
-            if (!(dr > 0.0) || !isfinite(dr)) bad = 1;
-            L[i * nc + i] = make_double2(sqrt(fmax(dr, 0.0)), 0.0);
         }
-        __syncthreads();
-        if (bad) break;
-        const double inv = 1.0 / L[i * nc + i].x;
-        for (int j = i + 1 + t; j < nc; j += blockDim.x) {
-            double sr = psi[j * nc + i].x, si = psi[j * nc + i].y;
-            for (int k = 0; k < i; ++k) {
-                const double2 a = L[j * nc + k], b = L[i * nc + k];  // a * conj(b)
-                sr -= a.x * b.x + a.y * b.y;
-                si -= a.y * b.x - a.x * b.y;
+        L[e] = make_double2(sr / denom, si / denom);
+    }
+    __syncthreads();
+    if (t >= 64) return;
+    const int lane = t;
+    bool bad = ns == 0;  // uniform over the wavefront
+    if (!bad) {
+        for (int i = 0; i < nc; ++i) {
+            double sr = 0.0, si = 0.0;
+            if (lane >= i && lane < nc) {
+                sr = L[lane * nc + i].x;
+                si = L[lane * nc + i].y;
+#pragma unroll 4
+                for (int k = 0; k < i; ++k) {
+                    const double2 r = L[lane * nc + k], q = L[i * nc + k];  // L_jk conj(L_ik)
+                    sr -= r.x * q.x + r.y * q.y;
+                    si -= r.y * q.x - r.x * q.y;
+                }
             }
-            L[j * nc + i] = make_double2(sr * inv, si * inv);
+            const double dr = __shfl(sr, i);
+            if (!(dr > 0.0) || !isfinite(dr)) {
+                bad = true;
+                break;
+            }
+            const double d = sqrt(dr), inv = 1.0 / d;
+            if (lane == i) {
+                L[i * nc + i] = make_double2(d, 0.0);
+                dinv[i] = inv;
+            } else if (lane > i && lane < nc) {
+                L[lane * nc + i] = make_double2(sr * inv, si * inv);
+            }
+            wave_sync();
         }
-        __syncthreads();
     }
     if (!bad) {
-        // W = L^-1 by forward substitution, one column per thread.
-        for (int col = t; col < nc; col += blockDim.x) {
-            for (int i = col; i < nc; ++i) {
-                double sr = (i == col) ? 1.0 : 0.0, si = 0.0;
-                for (int j = col; j < i; ++j) {
-                    const double2 a = L[i * nc + j], b = Li[j * nc + col];
-                    sr -= a.x * b.x - a.y * b.y;
-                    si -= a.x * b.y + a.y * b.x;
+        // lane c: column c of L^-1 into Li (its own column only, so no
+        // ordering between lanes is needed), then to W as complex64
+        double2* Li = part;
+        if (lane < nc) {
+            for (int i = lane; i < nc; ++i) {
+                double sr = (i == lane) ? 1.0 : 0.0, si = 0.0;
+#pragma unroll 4
+                for (int k = lane; k < i; ++k) {
+                    const double2 q = L[i * nc + k], x = Li[k * nc + lane];
+                    sr -= q.x * x.x - q.y * x.y;
+                    si -= q.x * x.y + q.y * x.x;
                 }
-                const double d = L[i * nc + i].x;
-                Li[i * nc + col] = make_double2(sr / d, si / d);
+                Li[i * nc + lane] = make_double2(sr * dinv[i], si * dinv[i]);
+            }
+            for (int i = 0; i < nc; ++i) {
+                const double2 v = i < lane ? make_double2(0.0, 0.0) : Li[i * nc + lane];
+                W[i * nc + lane] = make_float2((float)v.x, (float)v.y);
             }
         }
+    } else {
+        for (int e = lane; e < nc * nc; e += 64) W[e] = make_float2(e / nc == e % nc ? 1.0f : 0.0f, 0.0f);
     }
-    __syncthreads();
-    for (int e = t; e < nc * nc; e += blockDim.x) {
-        const int i = e / nc, j = e % nc;
-        double2 v = bad ? make_double2(i == j ? 1.0 : 0.0, 0.0) : Li[e];
-        W[e] = make_float2((float)v.x, (float)v.y);
-    }
-    if (t == 0) {
+    if (lane == 0) {
         if (wa.per_request)
             wa.status[b] = bad;
         else if (bad)
@@ -717,7 +778,7 @@ hipError_t launch_whitener(const float2* noise, int nc, int ns, float2* W, int* 
 
 hipError_t launch_whitener_batch(const WhitenArgs& a, int B, hipStream_t s) {
     if (a.nc < 1 || a.nc > 32 || B < 1 || B > OKS_MAX_BATCH) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_whitener, dim3((unsigned)B), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_whitener, dim3((unsigned)B), dim3(kWhitenThreads), 0, s, a);
     return hipGetLastError();
 }
 
