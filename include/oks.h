@@ -205,7 +205,10 @@ void* oks_ctx_stream(oks_ctx* ctx);
  * before it (the same linear map, FP32 rounding in another order);
  * "cols_pipe" (0/1, default 1) lets column passes prefetch the next coil;
  * "rows_waves" (0 = automatic, or 8..16) sets the wavefronts per workgroup of
- * the one-row-per-workgroup row passes. */
+ * the one-row-per-workgroup row passes; "cols_sx" (0 = automatic, or a power
+ * of two up to 16) caps the columns per workgroup of the column passes and
+ * "cols_threads" (0 = 512, or 256) sets their workgroup size (prefetching
+ * needs 512). */
 int oks_ctx_set_option(oks_ctx* ctx, const char* name, int64_t value);
 int oks_ctx_synchronize(oks_ctx* ctx);
 /* Flags raised on the device by the *_dev calls since the last take

@@ -269,6 +269,13 @@ int oks_ctx_set_option(oks_ctx* ctx, const char* name, int64_t value) {
         } else if (n == "rows_pipe" || n == "cols_pipe") {
             if (value != 0 && value != 1) fail(OKS_E_INVALID_ARG, n + " must be 0 or 1");
             (n == "rows_pipe" ? ctx->rows_pipe : ctx->cols_pipe) = (int)value;
+        } else if (n == "cols_sx") {
+            if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
+                fail(OKS_E_INVALID_ARG, "cols_sx must be 0, 1, 2, 4, 8 or 16");
+            ctx->cols_sx = (int)value;
+        } else if (n == "cols_threads") {
+            if (value != 0 && value != 256 && value != 512) fail(OKS_E_INVALID_ARG, "cols_threads must be 0, 256 or 512");
+            ctx->cols_threads = (int)value;
         } else if (n == "rows_waves") {
             if (value != 0 && (value < 8 || value > 16)) fail(OKS_E_INVALID_ARG, "rows_waves must be 0 or 8..16");
             ctx->rows_waves = (int)value;

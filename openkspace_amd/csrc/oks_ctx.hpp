@@ -62,6 +62,8 @@ struct oks_ctx {
     int cs_threads = 512;                          // CS column-pass workgroup size
     int rows_pipe = 1;                             // ifft-rss row pass: pipelined persistent kernel
     int rows_waves = 0;                            // one-row-per-workgroup row passes: wavefronts (0 = automatic)
+    int cols_sx = 0;                               // column passes: columns per workgroup cap (0 = automatic)
+    int cols_threads = 0;                          // column passes: workgroup size (0 = 512, or 256)
     int cols_pipe = 1;                             // column passes: prefetch the next coil
     long long cs_chunk = 0;                        // CS planes per iteration-loop chunk (0 = all)
     std::vector<uint8_t> h_acq;                    // host staging: acquired rows

@@ -971,10 +971,12 @@ size_t cols_lds(int sx, int n, int ty, int mode) {
            (mode == 1 ? (size_t)sx * ty * sizeof(float) : 0);
 }
 
+static int cols_threads(const ColArgs& a) { return a.threads == 256 ? 256 : 512; }
+
 template <class F, bool PIPE>
 static hipError_t go_cols_k(const ColArgs& a, size_t lds, unsigned grid, hipStream_t s) {
     OKS_ATTR(hipFuncSetAttribute((const void*)k_cols<F, PIPE>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_cols<F, PIPE>), dim3(grid), dim3(512), lds, s, a);
+    hipLaunchKernelGGL((k_cols<F, PIPE>), dim3(grid), dim3(PIPE ? 512 : cols_threads(a)), lds, s, a);
     return hipGetLastError();
 }
 
@@ -985,7 +987,7 @@ static hipError_t go_cols(const ColArgs& a, hipStream_t s) {
     const unsigned grid = (unsigned)(a.G * a.Zo * strips);
     if (a.sx > 16) return hipErrorInvalidValue;
     // the smallest coil group has C / G coils
-    if (a.pipe && F::n != 0 && a.C / a.G >= 2) return go_cols_k<F, true>(a, lds, grid, s);
+    if (a.pipe && cols_threads(a) == 512 && F::n != 0 && a.C / a.G >= 2) return go_cols_k<F, true>(a, lds, grid, s);
     return go_cols_k<F, false>(a, lds, grid, s);
 }
 
@@ -1005,6 +1007,7 @@ hipError_t launch_cols(const ColArgs& a, bool inverse, hipStream_t s) {
     if (a.sx < 1 || (a.sx & (a.sx - 1)) != 0) return hipErrorInvalidValue;  // strip index maps use shifts
     if (a.G < 1 || a.G > a.C) return hipErrorInvalidValue;
     if (a.mode == 1 && a.G > 1 && !a.part) return hipErrorInvalidValue;
+    if (a.threads != 0 && a.threads != 256 && a.threads != 512) return hipErrorInvalidValue;
     return inverse ? go_cols_dir<true>(a, s) : go_cols_dir<false>(a, s);
 }
 

@@ -69,7 +69,8 @@ struct ColArgs {
     int y0, ty;         // centred output window along n
     int centred;
     float scale;
-    int pipe;           // prefetch the next coil's strip when a workgroup loops over coils
+    int pipe;           // prefetch the next coil's strip when a workgroup loops over coils (512 threads)
+    int threads;        // workgroup size: 0 = 512, or 256
     FftPlan p;
 };
 

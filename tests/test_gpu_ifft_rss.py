@@ -186,3 +186,31 @@ def test_rows_waves_option_range(ctx):
     for bad in (-1, 1, 7, 17):
         with pytest.raises(Exception):
             ctx.set_option("rows_waves", bad)
+
+
+def test_ifft_rss_column_pass_shapes(ctx):
+    # columns per workgroup and workgroup size of the column passes change the
+    # launch only: the same bits for every shape
+    case = _case(nc=7, nz=2, ny=40, nx=128, os_x=2)
+    d = abi.IfftRssDesc.default(remove_oversampling=0)
+    st, ref32, rflags, _ = O.recon("ifft_rss", d, case, "f32")
+    _, ref64, _, _ = O.recon("ifft_rss", d, case, "f64")
+    outs = {}
+    try:
+        for sx, th, pipe in [(0, 0, 1), (0, 0, 0), (8, 256, 1), (4, 256, 0), (2, 512, 1), (16, 256, 0)]:
+            ctx.set_option("cols_sx", sx)
+            ctx.set_option("cols_threads", th)
+            ctx.set_option("cols_pipe", pipe)
+            outs[(sx, th, pipe)], flags, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", d, case)
+            assert flags == rflags
+    finally:
+        ctx.set_option("cols_sx", 0)
+        ctx.set_option("cols_threads", 0)
+        ctx.set_option("cols_pipe", 1)
+    base = outs[(0, 0, 1)]
+    for k, got in outs.items():
+        assert np.array_equal(got, base), f"cols shape {k}"
+    assert_parity(base, ref32, ref64, TOL32, "column shapes")
+    for name, bad in (("cols_sx", 3), ("cols_sx", 32), ("cols_threads", 128)):
+        with pytest.raises(Exception):
+            ctx.set_option(name, bad)
