@@ -558,7 +558,9 @@ __global__ void __launch_bounds__(kPotrfThreads) k_potrf64_diag(Potrf64Args a) {
     __shared__ double2 L[64][65];
     __shared__ double2 I[64][65];
     __shared__ double2 colb[2][64];
+    __shared__ double2 piv[2];  // (l_jj, 1 / l_jj) of the column in colb[j & 1]
     __shared__ double dinv[64];
+    __shared__ int badf;
     const int z = blockIdx.x;
     if (a.skip[z]) return;
     const int n = a.n, k0 = a.k0, kn = a.kn, t = threadIdx.x;
@@ -579,16 +581,24 @@ __global__ void __launch_bounds__(kPotrfThreads) k_potrf64_diag(Potrf64Args a) {
         I[e / 64][e % 64] = make_double2(0.0, 0.0);
     }
     if (t < 64) dinv[t] = 0.0;
+    // The pivot of each column (square root and reciprocal of its diagonal
+    // entry) is formed once, by the thread that holds that entry, when it
+    // hands the column over, rather than by every thread after the barrier.
+    auto pivot = [&](double d, int slot) {
+        if (!(d > 0.0) || !isfinite(d)) badf = 1;
+        const double ljj = sqrt(fmax(d, 1e-300));
+        piv[slot] = make_double2(ljj, 1.0 / ljj);
+    };
+    if (t == 0) badf = 0;
     if (tc == 0)
 #pragma unroll
         for (int ia = 0; ia < 2; ++ia) colb[0][ti + 32 * ia] = r[ia][0];
     __syncthreads();
-    int bad = 0;
+    if (t == 0 && kn > 0) pivot(r[0][0].x, 0);
+    __syncthreads();
     for (int j = 0; j < kn; ++j) {
         const double2* col = colb[j & 1];
-        const double d = col[j].x;
-        bad |= (!(d > 0.0) || !isfinite(d));
-        const double ljj = sqrt(fmax(d, 1e-300)), inv = 1.0 / ljj;
+        const double ljj = piv[j & 1].x, inv = piv[j & 1].y;
         // scaled column j: L[i][j] = col[i] / ljj (i > j), L[j][j] = ljj
         if (t < 64) {
             if (t > j && t < kn) L[t][j] = make_double2(col[t].x * inv, col[t].y * inv);
@@ -618,12 +628,15 @@ __global__ void __launch_bounds__(kPotrfThreads) k_potrf64_diag(Potrf64Args a) {
             const int jb = jn >> 5;
 #pragma unroll
             for (int ib = 0; ib < 2; ++ib)
-                if (ib == jb)
+                if (ib == jb) {
 #pragma unroll
                     for (int ia = 0; ia < 2; ++ia) colb[jn & 1][ti + 32 * ia] = r[ia][ib];
+                    if (ti == tc) pivot(r[ib][ib].x, jn & 1);  // row jn: ia == jb
+                }
         }
         __syncthreads();
     }
+    const int bad = badf;
     // Level 0: wavefront b inverts the 16 x 16 block at 16 b; four lanes own
     // one column and split its dot products.
     const int w = t >> 6, lane = t & 63;
