@@ -152,8 +152,8 @@ def workload_c3():
     nc, nz, ny, nx = case["kspace"].shape
     iters = 120
     desc = abi.CsDesc.default(iters=iters, lambda_=0.01)
-    # per coil-slice-iteration: read z, x, y and the mask, write x and z
-    algo_bytes = nc * nz * iters * (5 * ny * nx * 8 + ny * nx)
+    # per coil-slice-iteration: read x_k, x_{k-1}, y and the mask, write x_{k+1}
+    algo_bytes = nc * nz * iters * (4 * ny * nx * 8 + ny * nx)
     return dict(
         name="c3_cs",
         config={"workload": "c3_cs", "slices": nz, "coils": nc, "ky": ny, "kx": nx, "mask": "variable-density R=6, "
@@ -190,9 +190,9 @@ WORKLOADS = {"c1": workload_c1, "c2": workload_c2, "c3": workload_c3, "c4": work
 
 # HBM bytes the dominant kernels move per algorithmic byte, measured with the
 # FETCH_SIZE / WRITE_SIZE counters scaled by a copy kernel of known size
-# (profiles/hbm_traffic_counters.md): CS 65.4 B per pixel-iteration (41
+# (profiles/hbm_traffic_counters.md): CS 57.5 B per pixel-iteration (33
 # algorithmic), ifft-rss 60.8 MB per 15 x 368 x 640 slice (28.73 MB algorithmic).
-TRAFFIC_PER_ALGO_BYTE = {"c1": 60.8e6 / 28.73e6, "c3": 65.4 / 41.0, "served": 65.4 / 41.0}
+TRAFFIC_PER_ALGO_BYTE = {"c1": 60.8e6 / 28.73e6, "c3": 57.5 / 33.0, "served": 57.5 / 33.0}
 
 # Stages the served roofline can be taken on: (oks_stage id, bound, unit, peak).
 STAGES = {

@@ -13,6 +13,10 @@
 //                   row pair), the FISTA momentum z = x+ + a (x+ - x)
 //                   (recon/cs.rs:114-136), and the forward row transform of the
 //                   new z into T for the next iteration.
+// Only the iterates are kept in memory, in two buffers that swap roles every
+// iteration: z_k is formed again from x_k and x_{k-1} (the same expression
+// the previous update used), and x_{k+1} overwrites x_{k-1}.  Per pixel that
+// is 8 bytes less written than keeping x and z.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -266,12 +270,20 @@ __global__ void __launch_bounds__(256) k_cs_update(CsUpdArgs a) {
     float2* g1 = F::run_inv(R1, S ? S + n : nullptr, a.p, tw, lane);
     const float s = a.scale, lam = a.lambda, alpha = a.alpha;
     const float2 c = make_float2(0.5f, 0.f);
-    float2* X = a.x + base;
-    float2* Z = a.z + base;
+    const float2* XC = a.xc + base;
+    float2* XO = a.xo + base;
+    const float2 ap = make_float2(a.alpha_prev, 0.f);
     for (int j = lane; j < n / 2; j += 64) {
         const int x0 = 2 * j, x1 = 2 * j + 1;
+        // z_k formed from the two latest iterates, as the previous update
+        // formed it (only x is kept in memory)
+        const float2 xa = XC[x0], xb = XC[x1], xd = XC[n + x0], xe = XC[n + x1];
+        const float2 oa = XO[x0], ob = XO[x1], od = XO[n + x0], oe = XO[n + x1];
+        const float2 za = cadd(xa, cmul(ap, csub(xa, oa)));
+        const float2 zb = cadd(xb, cmul(ap, csub(xb, ob)));
+        const float2 zd = cadd(xd, cmul(ap, csub(xd, od)));
+        const float2 ze = cadd(xe, cmul(ap, csub(xe, oe)));
         // v = z - g at the four pixels of block j
-        const float2 za = Z[x0], zb = Z[x1], zd = Z[n + x0], ze = Z[n + x1];
         const float2 va = csub(za, cscale(g0[wrap_n(x0 + half, n)], s));
         const float2 vb = csub(zb, cscale(g0[wrap_n(x1 + half, n)], s));
         const float2 vd = csub(zd, cscale(g1[wrap_n(x0 + half, n)], s));
@@ -285,21 +297,16 @@ __global__ void __launch_bounds__(256) k_cs_update(CsUpdArgs a) {
         const float2 nb = cmul(csub(csub(cadd(ll, lh), hl), hh), c);
         const float2 nd = cmul(csub(csub(cadd(ll, hl), lh), hh), c);
         const float2 ne = cmul(cadd(csub(csub(ll, hl), lh), hh), c);
-        const float2 xa = X[x0], xb = X[x1], xd = X[n + x0], xe = X[n + x1];
-        X[x0] = na;
-        X[x1] = nb;
-        X[n + x0] = nd;
-        X[n + x1] = ne;
-        const float2 al = make_float2(alpha, 0.f);
-        const float2 z0 = cadd(na, cmul(al, csub(na, xa)));
-        const float2 z1 = cadd(nb, cmul(al, csub(nb, xb)));
-        const float2 z2 = cadd(nd, cmul(al, csub(nd, xd)));
-        const float2 z3 = cadd(ne, cmul(al, csub(ne, xe)));
-        Z[x0] = z0;
-        Z[x1] = z1;
-        Z[n + x0] = z2;
-        Z[n + x1] = z3;
+        XO[x0] = na;
+        XO[x1] = nb;
+        XO[n + x0] = nd;
+        XO[n + x1] = ne;
         if (a.write_t) {
+            const float2 al = make_float2(alpha, 0.f);
+            const float2 z0 = cadd(na, cmul(al, csub(na, xa)));
+            const float2 z1 = cadd(nb, cmul(al, csub(nb, xb)));
+            const float2 z2 = cadd(nd, cmul(al, csub(nd, xd)));
+            const float2 z3 = cadd(ne, cmul(al, csub(ne, xe)));
             // the new z into the slots this lane read g from: the ifftshift
             // positions of the forward row transform
             g0[wrap_n(x0 + half, n)] = z0;

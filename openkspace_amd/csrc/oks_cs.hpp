@@ -24,13 +24,15 @@ struct CsColArgs {
 struct CsUpdArgs {
     const float2* T;  // [planes, ny, nx]
     float2* Tn;       // row transform of the new z (may alias T), when write_t
-    float2* x;
-    float2* z;
+    const float2* xc; // x_k
+    float2* xo;       // x_{k-1} in, x_{k+1} out
     int write_t;
     long long pairs;  // planes * ny / 2
     int ny;
     float scale;      // 1/sqrt(nx)
-    float lambda, alpha;
+    float lambda;
+    float alpha_prev; // z_k = x_k + alpha_prev (x_k - x_{k-1})
+    float alpha;      // z_{k+1} = x_{k+1} + alpha (x_{k+1} - x_k)
     FftPlan p;        // n = nx
 };
 

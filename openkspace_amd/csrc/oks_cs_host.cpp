@@ -109,11 +109,15 @@ static void cs_iterations(oks_ctx* ctx, const CsProblem& P, const std::vector<fl
         ca.plane0 = p0;
         ua.T = T;
         ua.Tn = T;
-        ua.x = x;
-        ua.z = z;
         ua.pairs = np * (P.ny / 2);
+        // x and z both hold x_0 here.  Iteration it writes x_{it+1} over
+        // x_{it-1}; the roles start so that the last write lands in x.
         for (int it = 0; it < P.iters; ++it) {
+            const bool into_x = (P.iters - 1 - it) % 2 == 0;
+            ua.xo = into_x ? x : z;
+            ua.xc = into_x ? z : x;
             OKS_HIP(launch_cs_cols(ca, ctx->stream));
+            ua.alpha_prev = it > 0 ? alpha[it - 1] : 0.0f;
             ua.alpha = alpha[it];
             ua.write_t = it + 1 < P.iters;
             OKS_HIP(launch_cs_update(ua, ctx->stream));
@@ -136,9 +140,9 @@ static void run_fista(oks_ctx* ctx, const CsProblem& P) {
     OKS_HIP(launch_copy2(P.T, P.x, P.z, P.planes * P.ny * P.nx, ctx->stream));
     if (P.iters <= 0) return;
     const std::vector<float> alpha = fista_alphas(P.iters);
-    // algorithmic bytes per pixel, plane and iteration: read z, x, y and the
-    // mask, write x and z
-    const double it_bytes = (double)P.planes * P.iters * ((double)P.ny * P.nx * (5.0 * 8.0) + (double)P.ny * P.nx);
+    // algorithmic bytes per pixel, plane and iteration: read x_k, x_{k-1}, y
+    // and the mask, write x_{k+1}
+    const double it_bytes = (double)P.planes * P.iters * ((double)P.ny * P.nx * (4.0 * 8.0) + (double)P.ny * P.nx);
     if (!ctx->use_graphs) {
         StageTimer timer(ctx, OKS_STAGE_CS_ITERATIONS, it_bytes);
         cs_iterations(ctx, P, alpha);
