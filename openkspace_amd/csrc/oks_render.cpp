@@ -49,6 +49,8 @@ static void render_dev(oks_ctx* ctx, const float* vol, int64_t nz, int64_t ny, i
         fail(OKS_E_INVALID_ARG, "unknown render mode");
     }
     a.window = (float*)ctx->buf("render_window", sizeof(float) * 2 * (size_t)nz);
+    a.hist = (unsigned*)ctx->buf("render_hist", sizeof(unsigned) * 512 * (size_t)nz);
+    a.sel = (unsigned*)ctx->buf("render_sel", sizeof(unsigned) * 4 * (size_t)nz);
     OKS_HIP(launch_render(a, ctx->stream));
     if (window) {
         OKS_HIP(hipMemcpyAsync(window, a.window, sizeof(float) * 2 * (size_t)nz, hipMemcpyDeviceToHost, ctx->stream));

@@ -3,8 +3,9 @@ against the oracle.
 
 Bars: the windows are selected elements of each slice and must be bit-equal;
 the bytes are equal everywhere except where the value before rounding lies
-within 1e-3 of a half-integer (the device's powf and the host's differ by an
-ulp), and there by at most one.
+within 1e-3 of a half-integer (the device forms norm^0.9 with the hardware
+log2 / exp2 and divides by a per-slice reciprocal: a few ulp from the host's
+powf and division), and there by at most one.
 """
 import numpy as np
 import pytest
@@ -85,3 +86,16 @@ def test_invalid_windows(ctx):
             assert e.value.code == abi.OKS_E_INVALID_ARG
     finally:
         dv.free()
+
+
+@pytest.mark.parametrize("shape", [(2, 101, 99), (3, 256, 256)])
+def test_select_split_over_workgroups(ctx, shape):
+    # slices of 8192 pixels or more are split over several workgroups per
+    # digit pass (101 x 99 also leaves the float4 path: 9999 is odd); heavy
+    # ties and negative values exercise the per-digit counts and the carry of
+    # the remaining rank from pass to pass
+    rng = np.random.default_rng(shape[1])
+    v = np.round(rng.standard_normal(shape) * 4.0).astype(np.float32) / 4.0
+    _check(ctx, v, label=f"ties {shape}")
+    _check(ctx, v, 0.0, 100.0, label=f"ties {shape} full range")
+    _check(ctx, np.abs(v) + 1.0, mode=abi.RENDER_LINEAR, label=f"linear {shape}")
