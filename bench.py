@@ -359,6 +359,9 @@ def run_c1p(a):
     noise = case["noise"]
     e = abi.Encoding(*enc)
     ctx = Context(0)
+    for opt in a.ctx_option:
+        name, val = opt.split("=")
+        ctx.set_option(name, int(val))
     dd = DeviceArray.from_host(ctx, data)
     dn = DeviceArray.from_host(ctx, noise)
     dw = DeviceArray(ctx, (nc, nc), np.complex64)
