@@ -723,7 +723,8 @@ def cpu_baseline(wl, seconds):
 def main():
     a = _args()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        print(json.dumps({"error": "the benchmark runs on one GPU only (the path has no multi-GPU form); "
+                                    "--gpus must be 1"}))
         sys.exit(2)
     if a.workload == "hbm_copy":
         run_hbm_copy(a)
