@@ -141,6 +141,20 @@ def test_grappa_oversampled_grid(ctx):
     assert_parity(got, r32, r64, TOL, "grappa os")
 
 
+@pytest.mark.parametrize("nc,nx", [(12, 256), (5, 320)])
+def test_grappa_oversampled_grid_compiled_pairs(ctx, nc, nx):
+    # readouts of 256 and 320 samples take the oversampling chain with
+    # compile-time transform pairs (256 -> 128, 320 -> 160); 12 coils give a
+    # workgroup wider than 8 wavefronts
+    case = _gcase(nc=nc, nx=nx, os_x=2, r=2, acs=16, ny=40)
+    d = abi.GrappaDesc.default()
+    st, r32, rflags, _ = O.recon("grappa", d, case, "f32")
+    _, r64, _, _ = O.recon("grappa", d, case, "f64")
+    got, flags, _ = recon_dev(ctx, "oks_recon_grappa_dev", d, case)
+    assert st == 0 and flags == rflags and not flags & abi.F_OS_SKIPPED
+    assert_parity(got, r32, r64, TOL, f"grappa os nc={nc} nx={nx}")
+
+
 def test_grappa_c2_shape(ctx):
     # The C2 slice shape (32 coils, 384 x 384, R=4, 24 ACS rows, kernel 4x5,
     # ridge 1e-3) on 2 of the 8 slices: every slice is calibrated and
