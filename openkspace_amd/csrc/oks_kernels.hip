@@ -451,32 +451,63 @@ __global__ void __launch_bounds__(rows_pw_bound(ENC)) k_rows_pw_os(RowPwArgs a) 
     const int tx = a.tx, x0 = a.x0;
     for (int c = w; c < nc; c += WPB) {
         float2* buf = A + (size_t)c * nx;
-        float2* scr = B ? B + (size_t)w * nx : nullptr;
-        float2* S = os_chain<ENC, REC>(buf, scr, a, tw_e, tw_r, lane);
-        // the processed line re-centred at [dst, dst + rec) of a zero grid
-        // line, written at its ifftshift positions for the kx IFFT
-        float2* res;
+        float2* dstp = a.out + (size_t)c * a.out_plane + row * tx;
         if constexpr (ENC > 0) {
-            constexpr int BPE = (ENC + 63) / 64;
-            float2 v[BPE];
+            // With enc = 2 rec the kx IFFT of the processed line, re-centred
+            // into the enc-wide row, is its 2x band-limited interpolation:
+            // output X (natural index), d = X - rec, is
+            //   d = 2t     : rec * y[t + rec/2], y the cropped image line;
+            //   d = 2t + 1 : the centred IFFT(rec) of P_k exp(2 pi i k / enc)
+            //                at t + rec/2, P the processed line (k in [-rec/2, rec/2)),
+            // both times the row scale.  One rec-point transform replaces the
+            // enc-point one, and y is kept from before the forward transform.
+            static_assert(ENC == 2 * REC, "compiled oversampling pairs halve the readout");
+            constexpr int crop0 = (ENC - REC) / 2, rh = REC / 2;
+            constexpr int BPL = (REC + 63) / 64;
+            CFft<ENC, true>::run(buf, tw_e, lane);
+            float2 v[BPL], y[BPL];
+            const float inv = 1.0f / (float)ENC;
 #pragma unroll
-            for (int b = 0; b < BPE; ++b) {
+            for (int b = 0; b < BPL; ++b) {
                 const int i = lane + 64 * b;
-                if (i < ENC) {
-                    const int m = wrap_n(i + nh, ENC) - dst;
-                    v[b] = (m >= 0 && m < REC) ? S[wrap_n(m + REC / 2, REC)] : make_float2(0.f, 0.f);
+                if (i < REC) {
+                    v[b] = cscale(buf[wrap_n(wrap_n(i + rh, REC) + crop0 + half, ENC)], inv);  // ifftshift of the crop
+                    y[b] = cscale(buf[wrap_n(i + crop0 + half, ENC)], inv);                   // the crop, natural order
                 }
             }
             wave_sync();
 #pragma unroll
-            for (int b = 0; b < BPE; ++b) {
+            for (int b = 0; b < BPL; ++b) {
                 const int i = lane + 64 * b;
-                if (i < ENC) buf[i] = v[b];
+                if (i < REC) {
+                    buf[i] = v[b];
+                    buf[REC + i] = y[b];
+                }
             }
             wave_sync();
-            CFft<ENC, true>::run(buf, tw_e, lane);
-            res = buf;
+            CFft<REC, false>::run(buf, tw_r, lane);  // buf[s] = P[(s + rec/2) % rec]
+            for (int s2 = lane; s2 < REC; s2 += 64)
+                buf[s2] = cmul(buf[s2], tw_e[s2 < REC / 2 ? s2 : s2 - REC + ENC]);  // exp(2 pi i k / enc)
+            wave_sync();
+            CFft<REC, true>::run(buf, tw_r, lane);
+            const float sc_even = a.scale * (float)REC;
+            for (int x = lane; x < tx; x += 64) {
+                const int d = x0 + x - REC;  // in [-rec, rec)
+                float2 o;
+                if ((d & 1) == 0) {
+                    o = cscale(buf[REC + (d >> 1) + REC / 2], sc_even);
+                } else {
+                    const int u = ((d - 1) >> 1) + REC / 2;
+                    o = cscale(buf[wrap_n(u + REC / 2, REC)], a.scale);
+                }
+                dstp[x] = o;
+            }
+            wave_sync();
         } else {
+            float2* scr = B + (size_t)w * nx;
+            float2* S = os_chain<0, 0>(buf, scr, a, tw_e, tw_r, lane);
+            // the processed line re-centred at [dst, dst + rec) of a zero grid
+            // line, written at its ifftshift positions for the kx IFFT
             float2* K = (S == buf) ? scr : buf;
             for (int i = lane; i < nx; i += 64) {
                 const int m = (i + nh) % nx - dst;
@@ -484,11 +515,10 @@ __global__ void __launch_bounds__(rows_pw_bound(ENC)) k_rows_pw_os(RowPwArgs a) 
             }
             wave_sync();
             float2* other2 = (K == buf) ? scr : buf;
-            res = fft_wave<true>(K, other2, a.p_enc, lane, 64);
+            float2* res = fft_wave<true>(K, other2, a.p_enc, lane, 64);
+            for (int x = lane; x < tx; x += 64) dstp[x] = cscale(res[wrap_n(x0 + x + half, nx)], a.scale);
+            wave_sync();
         }
-        float2* dstp = a.out + (size_t)c * a.out_plane + row * tx;
-        for (int x = lane; x < tx; x += 64) dstp[x] = cscale(res[wrap_n(x0 + x + half, nx)], a.scale);
-        wave_sync();
     }
 }
 
