@@ -163,3 +163,32 @@ def test_cs_c3_shape(ctx):
     got, flags, _ = recon_dev(ctx, "oks_recon_cs_dev", d, case)
     assert st == 0 and flags == rflags and got.shape == (2, 320, 320)
     assert_parity(got, r32, r64, TOL, "C3 cs", iterative=True)
+
+
+@pytest.mark.parametrize("iters", [1, 2, 3, 7])
+def test_fista_iteration_count_parity(ctx, iters):
+    # the iterates live in two buffers that swap roles every iteration; the
+    # last iterate must reach the output for odd and even counts alike
+    case = synth.make_case(nc=1, nz=1, ny=32, nx=48, kind="cs", seed=40 + iters, ellipses=5)
+    kzf, mask = case["kspace"][0, 0], case["mask"][0]
+    got = gpu_fista(ctx, kzf, mask, iters, 0.01)[0]
+    _, r32 = O.fista_cs(kzf, mask, iters, 0.01, "f32")
+    _, r64 = O.fista_cs(kzf, mask, iters, 0.01, "f64")
+    assert_parity(got, r32, r64, TOL, f"fista {iters}it", iterative=True)
+
+
+@pytest.mark.parametrize("iters", [6, 7])
+def test_cs_chunked_planes_match(ctx, iters):
+    # planes processed in chunks (each with its own buffer rotation) give the
+    # same bits as one chunk
+    case = _cs_case(nc=3, nz=3)
+    d = abi.CsDesc.default(iters=iters)
+    ctx.set_option("graphs", 0)
+    try:
+        whole, _, _ = recon_dev(ctx, "oks_recon_cs_dev", d, case)
+        ctx.set_option("cs_chunk", 2)
+        chunked, _, _ = recon_dev(ctx, "oks_recon_cs_dev", d, case)
+    finally:
+        ctx.set_option("cs_chunk", 0)
+        ctx.set_option("graphs", 1)
+    assert np.array_equal(chunked, whole)
