@@ -82,6 +82,15 @@ void* oks_ctx::buf(const std::string& name, size_t bytes) {
     return b.p;
 }
 
+hipStream_t oks_ctx::aux_stream() {
+    if (!aux) {
+        OKS_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
+        OKS_HIP(hipEventCreateWithFlags(&aux_fork, hipEventDisableTiming));
+        OKS_HIP(hipEventCreateWithFlags(&aux_join, hipEventDisableTiming));
+    }
+    return aux;
+}
+
 hipEvent_t oks_ctx::take_event() {
     if (!event_pool.empty()) {
         hipEvent_t e = event_pool.back();
@@ -155,6 +164,11 @@ void oks_ctx::release() {
     timed_pending.clear();
     for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
     event_pool.clear();
+    if (aux_fork) (void)hipEventDestroy(aux_fork);
+    if (aux_join) (void)hipEventDestroy(aux_join);
+    if (aux) (void)hipStreamDestroy(aux);
+    aux_fork = aux_join = nullptr;
+    aux = nullptr;
     if (own_stream && stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
 }

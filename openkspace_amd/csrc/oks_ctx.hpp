@@ -49,6 +49,8 @@ struct oks_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    hipStream_t aux = nullptr;                     // side stream for independent work (default priority)
+    hipEvent_t aux_fork = nullptr, aux_join = nullptr;
     std::string err;
     std::recursive_mutex mu;                       // one ctx = one logical owner; calls serialise
     std::map<int, float2*> tw;                     // n -> twiddle table on device
@@ -128,6 +130,7 @@ struct oks_ctx {
     std::vector<hipEvent_t> event_pool;
     double timed_ms = 0.0, timed_units = 0.0;
     long long timed_launches = 0;
+    hipStream_t aux_stream();  // created on first use, with its two events
     hipEvent_t take_event();
     void timed_collect();  // call once the stream has synchronised
 

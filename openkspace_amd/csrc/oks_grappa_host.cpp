@@ -53,9 +53,9 @@ static int grappa_geom(int64_t nc, int64_t ny_acs, int64_t nx, int64_t r, int64_
 
 static void zg(oks_ctx* ctx, int batch, const double2* A, long long sA, int lda, int opA, const double2* B,
                long long sB, int ldb, int opB, double2* C, long long sC, int ldc, int M, int N, int K, double alpha,
-               double beta, const int* skip, int lower = 0) {
+               double beta, const int* skip, int lower = 0, hipStream_t s = nullptr) {
     Zgemm64Args g{A, B, C, sA, sB, sC, M, N, K, lda, ldb, ldc, opA, opB, alpha, beta, skip, lower};
-    OKS_HIP(launch_zgemm64(g, batch, ctx->stream));
+    OKS_HIP(launch_zgemm64(g, batch, s ? s : ctx->stream));
 }
 
 // Calibrate the GRAPPA weights of every slice of `grid` from the ACS rows
@@ -99,6 +99,11 @@ static void grappa_calibrate_batch(oks_ctx* ctx, const float2* grid, int nc, int
     OKS_HIP(launch_normal64(na, nz, ctx->stream));
     long long cnt = 3;
     const long long snn = (long long)nn, snm = (long long)nm, sD = (long long)nblk * 64 * 64;
+    // The right-hand-side updates of block k need only Dk and the panel
+    // L[r0:, k0:k0+kn]; they run on the side stream while the main stream
+    // goes on with the trailing update and the next diagonal block, and the
+    // two streams join before the backward substitution.
+    hipStream_t s2 = ctx->aux_stream();
     for (int kb = 0; kb < nblk; ++kb) {
         const int k0 = kb * 64, kn = std::min(64, n - k0), r0 = k0 + kn, rn = n - r0;
         Potrf64Args pa{G, L, D, fail, fail, status, n, k0, kn, kb, nblk};
@@ -108,18 +113,24 @@ static void grappa_calibrate_batch(oks_ctx* ctx, const float2* grid, int nc, int
             // L[r0:, k0:k0+kn] = G[r0:, k0:k0+kn] Dk^H
             zg(ctx, nz, G + (size_t)r0 * n + k0, snn, n, 0, Dk, sD, 64, 2, L + (size_t)r0 * n + k0, snn, n, rn, kn,
                kn, 1.0, 0.0, fail);
+        }
+        OKS_HIP(hipEventRecord(ctx->aux_fork, ctx->stream));
+        OKS_HIP(hipStreamWaitEvent(s2, ctx->aux_fork, 0));
+        if (rn > 0) {
             // G[r0:, r0:] -= L[r0:, k0:k0+kn] L[r0:, k0:k0+kn]^H
             zg(ctx, nz, L + (size_t)r0 * n + k0, snn, n, 0, L + (size_t)r0 * n + k0, snn, n, 2,
                G + (size_t)r0 * n + r0, snn, n, rn, rn, kn, -1.0, 1.0, fail, 1);
         }
-        // Y_k = Dk R_k ; R[r0:] -= L[r0:, k0:k0+kn] Y_k
+        // Y_k = Dk R_k ; R[r0:] -= L[r0:, k0:k0+kn] Y_k   (side stream)
         zg(ctx, nz, Dk, sD, 64, 0, R + (size_t)k0 * m, snm, m, 0, Y + (size_t)k0 * m, snm, m, kn, m, kn, 1.0, 0.0,
-           fail);
+           fail, 0, s2);
         if (rn > 0)
             zg(ctx, nz, L + (size_t)r0 * n + k0, snn, n, 0, Y + (size_t)k0 * m, snm, m, 0, R + (size_t)r0 * m, snm, m,
-               rn, m, kn, -1.0, 1.0, fail);
+               rn, m, kn, -1.0, 1.0, fail, 0, s2);
         cnt += 5;
     }
+    OKS_HIP(hipEventRecord(ctx->aux_join, s2));
+    OKS_HIP(hipStreamWaitEvent(ctx->stream, ctx->aux_join, 0));
     for (int kb = nblk - 1; kb >= 0; --kb) {
         const int k0 = kb * 64, kn = std::min(64, n - k0);
         const double2* Dk = D + (size_t)kb * 64 * 64;
