@@ -183,13 +183,16 @@ _SERVE_KIND = {"ifft_rss": "full", "grappa": "grappa", "cs": "cs"}
 def serve_pool(per_combo=3, seed=SERVE_SEED):
     """Distinct single-slice acquisitions for every (shape, strategy) pair of
     the served stream: {(shape_index, strategy): [case, ...]}."""
-    pool = {}
-    for si, (ny, nx, nc, os_x) in enumerate(SERVE_SHAPES):
-        for mi, (name, _) in enumerate(SERVE_MIX):
-            pool[(si, name)] = [make_case(nc=nc, nz=1, ny=ny, nx=nx, os_x=os_x, kind=_SERVE_KIND[name], r=4,
-                                          acs=24, seed=seed + 1000 * si + 100 * mi + j)
-                                for j in range(per_combo)]
-    return pool
+    return {(si, name): [serve_pool_case(si, name, j, seed) for j in range(per_combo)]
+            for si in range(len(SERVE_SHAPES)) for name, _ in SERVE_MIX}
+
+
+def serve_pool_case(si, name, j, seed=SERVE_SEED):
+    """Acquisition j of the served stream's (shape index si, strategy name) pair."""
+    ny, nx, nc, os_x = SERVE_SHAPES[si]
+    mi = [m[0] for m in SERVE_MIX].index(name)
+    return make_case(nc=nc, nz=1, ny=ny, nx=nx, os_x=os_x, kind=_SERVE_KIND[name], r=4, acs=24,
+                     seed=seed + 1000 * si + 100 * mi + j)
 
 
 def serve_requests(n, seed=SERVE_SEED):
