@@ -103,6 +103,9 @@ struct oks_ctx {
     void graph_store(const std::string& name, const GraphKey& key, hipGraph_t g, hipGraphExec_t e) {
         auto& v = graphs[name];
         if (v.size() >= kGraphsPerName) {
+            // The _dev entry points return before their work completes, so the
+            // evicted exec may still be queued on this stream: let it finish.
+            OKS_HIP(hipStreamSynchronize(stream));
             std::unique_lock<std::shared_mutex> lk(oks::device_alloc_mutex());
             if (v.front().exec) (void)hipGraphExecDestroy(v.front().exec);
             if (v.front().graph) (void)hipGraphDestroy(v.front().graph);
