@@ -1,6 +1,7 @@
 """Build openkspace_amd/liboks_hip.so for gfx950 with hipcc (in-tree).
 
     python -m openkspace_amd.build        # or: from openkspace_amd.build import build; build()
+    python -m openkspace_amd.build --incremental   # skip objects newer than their sources
 
 Every translation unit is compiled with --offload-arch=gfx950 and packed-FP32
 instructions disabled; objects go to openkspace_amd/_build/.
@@ -22,8 +23,16 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-parameter",
 DEVICE = ["--offload-arch=gfx950", "-munsafe-fp-atomics"]
 
 
-def _compile(src):
+def _headers():
+    return glob.glob(os.path.join(CSRC, "*.hpp")) + glob.glob(os.path.join(HERE, "..", "include", "*.h"))
+
+
+def _compile(src, force=True):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    if not force and os.path.exists(obj):
+        newest = max(os.path.getmtime(f) for f in [src] + _headers())
+        if os.path.getmtime(obj) >= newest:
+            return obj
     cmd = [HIPCC] + COMMON + DEVICE + ["-c", src, "-o", obj]
     if src.endswith(".cpp"):
         cmd = [HIPCC, "-x", "hip"] + COMMON + DEVICE + ["-c", src, "-o", obj]
@@ -33,11 +42,13 @@ def _compile(src):
     return obj
 
 
-def build(verbose=False):
+def build(verbose=False, force=True):
+    """Compile every source (force=False: only those older than their object
+    or than any header) and link the library."""
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda s: _compile(s, force), srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -50,5 +61,5 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, force="--incremental" not in sys.argv)
     sys.exit(0)

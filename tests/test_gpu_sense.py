@@ -303,3 +303,40 @@ def test_unfold_volume_errors(ctx):
     assert st == abi.OKS_E_SENSE_INDIVISIBLE
     st, _, _ = gpu_unfold_volume(ctx, maps, maps, 0, 1e-4)
     assert st == abi.OKS_E_SENSE_BAD_ACCEL
+
+
+@pytest.mark.parametrize("nc,ny", [(4, 256), (32, 50)])
+def test_sense_c4_full_kz_fallback(ctx, nc, ny):
+    # The C4 volume at its full 160 kz partitions and 256 readout samples
+    # (the fused fallback's kz = 160 column transform), with 4 coils at the full
+    # 256 ky, and with C4's 32 coils at an indivisible 50 ky; both fall back
+    # to the 3-D IFFT + RSS (recon/strategy.rs:631-647).
+    case = synth.make_config("c4_sense3d", nc=nc, ny=ny, acs=32 if ny > 64 else 16)
+    assert case["kspace"].shape == (nc, 160, ny, 256)
+    d = abi.SenseDesc.default()
+    O.set_threads(16)
+    try:
+        st, r32, rflags, _ = O.recon("sense", d, case, "f32")
+        _, r64, _, _ = O.recon("sense", d, case, "f64")
+    finally:
+        O.set_threads(1)
+    got, flags, _ = recon_dev(ctx, "oks_recon_sense_dev", d, case)
+    assert st == 0 and flags == rflags and flags & abi.F_SENSE_FALLBACK
+    assert got.shape == (160, ny, 256)
+    assert_parity(got, r32, r64, TOL, f"C4 sense fallback nc={nc} ny={ny} nz=160")
+
+
+def test_espirit_c4_kernel6_32_coils(ctx):
+    # C4's ESPIRiT parameters (kernel 6, threshold 0.02, 30 iterations) on 32
+    # coils: calibration width nc * 6^2 = 1152, at a reduced 48 x 40 image
+    case = synth.make_case(nc=32, nz=1, ny=48, nx=40, kind="full", seed=31, ellipses=5)
+    acs = case["kspace"][:, 0, 14:34, :]
+    got, kept = gpu_espirit(ctx, acs, 48, 40, 6, 0.02, 30)
+    O.set_threads(16)
+    try:
+        r32, k32 = O.espirit_maps(acs, 48, 40, 6, 0.02, 30, "f32")
+        r64, k64 = O.espirit_maps(acs, 48, 40, 6, 0.02, 30, "f64")
+    finally:
+        O.set_threads(1)
+    assert kept == k64
+    assert_parity(got, r32, r64, TOL, f"espirit k=6 nc=32 ({kept} kernels)")
