@@ -252,6 +252,15 @@ int oks_event_destroy(oks_ctx* ctx, void* ev);
 int oks_recon_ifft_rss(oks_ctx* ctx, const oks_ifft_rss_desc* d, const oks_kspace* k, oks_image* img);
 int oks_recon_ifft_rss_dev(oks_ctx* ctx, const oks_ifft_rss_desc* d, const oks_kspace* k, oks_image* img);
     /* IfftRss::reconstruct, recon/strategy.rs:113-223 */
+/* No counterpart in the reference: oks_recon_ifft_rss_dev for callers that
+ * cannot allocate device memory (HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and flags; every
+ * workspace comes from a static device array of OKS_NOALLOC_SCRATCH_BYTES per
+ * device instead of the context's allocations, and a call whose workspaces do
+ * not fit fails with OKS_E_UNSUPPORTED.  Calls that overlap on one device
+ * share that array and race. */
+#define OKS_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int oks_recon_ifft_rss_dev_noalloc(oks_ctx* ctx, const oks_ifft_rss_desc* d, const oks_kspace* k, oks_image* img);
 int oks_recon_grappa(oks_ctx* ctx, const oks_grappa_desc* d, const oks_kspace* k, oks_image* img);
 int oks_recon_grappa_dev(oks_ctx* ctx, const oks_grappa_desc* d, const oks_kspace* k, oks_image* img);
     /* GrappaRss::reconstruct, recon/strategy.rs:306-453 */

@@ -42,7 +42,7 @@ def lib():
     L.oks_output_shape.restype = None
     L.oks_detect_pattern.argtypes = [_P, _I64, _I64, _I64, _P]
     for name in ("ifft_rss", "grappa", "sense", "cs"):
-        for suf in ("", "_dev"):
+        for suf in ("", "_dev", "_dev_noalloc"):
             fn = getattr(L, f"oks_recon_{name}{suf}", None)
             if fn is not None:
                 fn.argtypes = [_P, _P, _P, _P]

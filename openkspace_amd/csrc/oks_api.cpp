@@ -44,14 +44,30 @@ oks::FftPlan oks_ctx::plan(int n) {
             push(p);
         }
     P.nst = s;
-    auto it = tw.find(n);
-    if (it == tw.end()) {
+    auto ht = h_tw.find(n);
+    if (ht == h_tw.end()) {
         std::vector<float2> h(std::max(n, 1));
         const long double two_pi = 6.283185307179586476925286766559L;
         for (int k = 0; k < n; ++k) {
             long double a = two_pi * (long double)k / (long double)n;
             h[k] = make_float2((float)cosl(a), (float)sinl(a));
         }
+        ht = h_tw.emplace(n, std::move(h)).first;
+    }
+    const std::vector<float2>& h = ht->second;
+    if (arena.base) {
+        // static-scratch mode: the table is copied into the arena once per call
+        auto at = arena.tw.find(n);
+        if (at == arena.tw.end()) {
+            float2* d = (float2*)arena_take("tw:" + std::to_string(n), sizeof(float2) * h.size());
+            OKS_HIP(hipMemcpyAsync(d, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice, stream));
+            at = arena.tw.emplace(n, d).first;
+        }
+        P.tw = at->second;
+        return P;
+    }
+    auto it = tw.find(n);
+    if (it == tw.end()) {
         float2* d = nullptr;
         {
             std::unique_lock<std::shared_mutex> lk(device_alloc_mutex());
@@ -65,7 +81,21 @@ oks::FftPlan oks_ctx::plan(int n) {
     return P;
 }
 
+void* oks_ctx::arena_take(const std::string& name, size_t bytes) {
+    auto it = arena.named.find(name);
+    if (it != arena.named.end() && it->second.second >= bytes) return arena.base + it->second.first;
+    const size_t off = (arena.used + 255) & ~size_t(255);
+    if (off > arena.cap || bytes > arena.cap - off)
+        fail(OKS_E_UNSUPPORTED, "input needs more device scratch than the no-allocation entry point's static " +
+                                    std::to_string(arena.cap >> 20) + " MiB (" + std::to_string(off + bytes) +
+                                    " bytes at '" + name + "')");
+    arena.used = off + bytes;
+    arena.named[name] = {off, bytes};
+    return arena.base + off;
+}
+
 void* oks_ctx::buf(const std::string& name, size_t bytes) {
+    if (arena.base) return arena_take(name, bytes);
     DevBuf& b = ws[name];
     if (bytes > b.cap) {
         std::unique_lock<std::shared_mutex> lk(device_alloc_mutex());
@@ -218,6 +248,7 @@ int oks_ctx_create(int device, oks_ctx** out) {
         OKS_HIP(hipMalloc(&c->d_status, 8 * sizeof(int)));
         OKS_HIP(hipMemsetAsync(c->d_status, 0, 8 * sizeof(int), c->stream));
         OKS_HIP(hipStreamSynchronize(c->stream));
+        noalloc_setup(device);
         return OKS_OK;
     });
     if (st != OKS_OK) {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <shared_mutex>
@@ -36,6 +37,11 @@ inline void fail(int code, const std::string& m) { throw Status(code, m); }
 // a HIP graph (contexts of one server run on concurrent threads).  Captures
 // hold this lock shared; allocations and releases hold it exclusively.
 std::shared_mutex& device_alloc_mutex();
+
+// Resolves this device's copy of the static scratch array used by the
+// entry point that allocates nothing; called once per device at context
+// creation, with the device current.
+void noalloc_setup(int device);
 
 // Grow-only device buffer.
 struct DevBuf {
@@ -137,6 +143,19 @@ struct oks_ctx {
     hipEvent_t take_event();
     void timed_collect();  // call once the stream has synchronised
 
+    // Static-scratch mode (oks_recon_ifft_rss_dev_noalloc): while `arena.base`
+    // is set, buf() and plan() carve their memory from [base, base + cap) and
+    // never allocate; a request that does not fit fails with OKS_E_UNSUPPORTED.
+    struct Arena {
+        char* base = nullptr;
+        size_t cap = 0, used = 0;
+        std::map<std::string, std::pair<size_t, size_t>> named;  // name -> (offset, bytes)
+        std::map<int, float2*> tw;                               // n -> twiddle table in the arena
+    } arena;
+    size_t arena_room() const { return arena.base ? arena.cap - std::min(arena.cap, (arena.used + 255) & ~size_t(255)) : 0; }
+    void* arena_take(const std::string& name, size_t bytes);
+    std::map<int, std::vector<float2>> h_tw;       // host twiddle tables, exp(+2 pi i k / n) in float
+
     oks::FftPlan plan(int n);
     void* buf(const std::string& name, size_t bytes);
     void release();
@@ -152,7 +171,7 @@ struct StageTimer {
     oks_ctx::TimedLaunch t;
     StageTimer(oks_ctx* ctx, int stage, double units, const int* dev_count = nullptr, int n_count = 0,
                double per_count = 0.0)
-        : c(ctx), on(ctx->timed_stage == stage && stage != 0) {
+        : c(ctx), on(ctx->timed_stage == stage && stage != 0 && !ctx->arena.base) {
         if (!on) return;
         t.units = units;
         t.dev_count = dev_count;

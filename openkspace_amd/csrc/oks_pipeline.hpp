@@ -25,6 +25,7 @@ void resolve_os(const oks_kspace* k, int enabled, int* os_mode, int* rec, uint32
 float2* prep_grid(oks_ctx* ctx, const oks_kspace* k, const float2* W, int os_mode, int rec, const char* slot);
 std::vector<uint8_t> os_mask(const oks_kspace* k, int rec);
 void output_windows(const oks_kspace* k, int crop, Win w[3], uint32_t* flags);
+void recon_ifft_rss_dev(oks_ctx* ctx, const oks_ifft_rss_desc* d, const oks_kspace* k, oks_image* img);
 void ifft_rss_grid(oks_ctx* ctx, const float2* k, int64_t nc, int64_t nz, int64_t ny, int64_t nx, const float2* W,
                    int os_mode, int rec, bool three_d, const Win w[3], float* out);
 

@@ -226,9 +226,21 @@ void ifft_rss_grid(oks_ctx* ctx, const float2* k, int64_t nc, int64_t nz, int64_
     if (!three_d) {
         // slices actually written: the z window of the crop
         const int64_t zlo = w[0].off, zn = w[0].len;
-        const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(zn, ctx->zchunk > 0 ? ctx->zchunk : zn));
-        float2* tmp = (float2*)ctx->buf("rss_tmp", sizeof(float2) * nc * chunk * ny * tx);
+        int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(zn, ctx->zchunk > 0 ? ctx->zchunk : zn));
         const int strips = (tx + 15) / 16;
+        if (ctx->arena.base) {
+            // static scratch: the largest chunk whose intermediate and RSS
+            // partials fit what is left (slices are independent, so the chunk
+            // only changes the coil grouping of the RSS sums)
+            ctx->plan((int)ny);
+            auto need = [&](int64_t cz) {
+                const int G = rss_groups((int)nc, cz * strips);
+                const size_t t = ((sizeof(float2) * nc * cz * ny * tx) + 255) & ~size_t(255);
+                return t + (G > 1 ? sizeof(float) * G * cz * ty * tx : 0);
+            };
+            while (chunk > 1 && need(chunk) > ctx->arena_room()) --chunk;
+        }
+        float2* tmp = (float2*)ctx->buf("rss_tmp", sizeof(float2) * nc * chunk * ny * tx);
         for (int64_t z = 0; z < zn; z += chunk) {
             const int64_t cz = std::min(chunk, zn - z);
             a.in = k + (zlo + z) * ny * nx;
