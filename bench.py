@@ -8,11 +8,14 @@ oversampling), 384^2 x 32}, strategies 50 % ifft-rss / 30 % GRAPPA R=4 / 20 %
 CS (60 iterations), k-space resident in HBM.  A step is one pass of the
 stream (all requests arriving at once) through the continuous-batching
 server; `value` is served slices/s.  The line also carries p50/p99 latency
-under Poisson arrivals at half the served rate, the plain GPU path (the same
-requests one at a time, one launch chain per slice) with its slices/s and
-latency, the alone-vs-served and oracle rel-L2 per strategy and shape, the
-roofline of the dominant stage (HIP events on the slot streams) and the CPU
-baseline (the FP32 oracle, one thread, on a bounded sample of the stream).
+under Poisson arrivals at 0.25, 0.5 and 0.75 of the served rate, the plain
+GPU path (the same requests one at a time, one launch chain per slice, k-space
+resident) and the plain host-buffer path (the blocking entry points, copies
+included) with their slices/s, per-strategy served and plain rates, the
+alone-vs-served and oracle rel-L2 per strategy and shape, the CS roofline over
+the step (41 algorithmic bytes per pixel-iteration), every request's bytes
+over the step, and the CPU baselines (the FP32 oracle on one thread and on
+all usable host threads, on bounded samples of the stream, with the CPU model).
 
 c1 / c2 / c3 / c4: one multi-slice reconstruction call (ifft-rss 16 x 15 x
 640 x 368, GRAPPA 8 x 32 x 384^2 R=4, CS 32 x 8 x 320^2 120 iterations, 3-D
@@ -52,12 +55,16 @@ def _args():
     p.add_argument("--slots", type=int, default=4, help="served: batches in flight")
     p.add_argument("--max-batch", type=int, default=32, help="served: requests per batch")
     p.add_argument("--pool", type=int, default=3, help="served: distinct slices per shape and strategy")
-    p.add_argument("--load", type=float, default=0.5, help="served: Poisson rate as a fraction of served rate")
+    p.add_argument("--host-requests", type=int, default=128, help="served: requests timed on the host-buffer path")
+    p.add_argument("--loads", default="0.25,0.5,0.75",
+                   help="served: Poisson rates as fractions of the served rate (latency sweep; 0.5 is the headline)")
     p.add_argument("--stage", default="cs_iterations", help="served: stage timed for the roofline")
     p.add_argument("--no-graphs", action="store_true", help="launch CS iterations eagerly (no HIP graph)")
     p.add_argument("--ctx-option", action="append", default=[], metavar="NAME=VALUE",
                    help="c1-c4: a launch-shape option of the context (oks_ctx_set_option)")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--cpu-threads", type=int, default=16, help="threads of the all-cores CPU baseline (capped by the "
+                   "host's usable cores)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-check", action="store_true")
     return p.parse_args()
@@ -152,8 +159,8 @@ def workload_c3():
     nc, nz, ny, nx = case["kspace"].shape
     iters = 120
     desc = abi.CsDesc.default(iters=iters, lambda_=0.01)
-    # per coil-slice-iteration: read x_k, x_{k-1}, y and the mask, write x_{k+1}
-    algo_bytes = nc * nz * iters * (4 * ny * nx * 8 + ny * nx)
+    # per coil-slice-iteration: read z, x, y and the mask, write x and z
+    algo_bytes = nc * nz * iters * CS_BYTES_PER_PIXEL_ITER * ny * nx
     return dict(
         name="c3_cs",
         config={"workload": "c3_cs", "slices": nz, "coils": nc, "ky": ny, "kx": nx, "mask": "variable-density R=6, "
@@ -192,7 +199,10 @@ WORKLOADS = {"c1": workload_c1, "c2": workload_c2, "c3": workload_c3, "c4": work
 # FETCH_SIZE / WRITE_SIZE counters scaled by a copy kernel of known size
 # (profiles/hbm_traffic_counters.md): CS 57.5 B per pixel-iteration (33
 # algorithmic), ifft-rss 60.8 MB per 15 x 368 x 640 slice (28.73 MB algorithmic).
-TRAFFIC_PER_ALGO_BYTE = {"c1": 60.8e6 / 28.73e6, "c3": 57.5 / 33.0, "served": 57.5 / 33.0}
+TRAFFIC_PER_ALGO_BYTE = {"c1": 60.8e6 / 28.73e6, "c3": 57.5 / 41.0, "served": 57.5 / 41.0}
+# FISTA's algorithmic bytes per coil-pixel and iteration: read z, x, the
+# measured k-space and the mask, write x and z (recon/cs.rs:99-136)
+CS_BYTES_PER_PIXEL_ITER = 5 * 8 + 1
 
 # Stages the served roofline can be taken on: (oks_stage id, bound, unit, peak).
 STAGES = {
@@ -521,7 +531,7 @@ def _pct(v, q):
 
 def run_served(a):
     from openkspace_amd import abi, synth
-    from openkspace_amd._lib import Context, DeviceArray
+    from openkspace_amd._lib import Context, DeviceArray, check, lib
     from openkspace_amd.server import Server
 
     ctx = Context(0)
@@ -550,14 +560,16 @@ def run_served(a):
     outs = [DeviceArray(ctx, (max_out,), np.float32) for _ in spec]
     N = len(spec)
 
-    def submit_all(srv, arrivals):
-        for i, (key, j) in enumerate(spec):
+    def submit_all(srv, arrivals, idx):
+        for n, i in enumerate(idx):
+            key, j = spec[i]
             d = dev[key][j]
-            srv.submit(Server.request(descs[key[1]], d["ks"], outs[i].addr, None, i, arrivals[i]))
+            srv.submit(Server.request(descs[key[1]], d["ks"], outs[i].addr, None, i, arrivals[n]))
 
-    def one_pass(srv, arrivals):
-        submit_all(srv, arrivals)
-        got = srv.collect(N)
+    def one_pass(srv, arrivals, idx=None):
+        idx = list(range(N)) if idx is None else idx
+        submit_all(srv, arrivals, idx)
+        got = srv.collect(len(idx))
         bad = [c for c in got.values() if c.status != 0]
         if bad:
             kinds = sorted({(spec[int(c.tag)][0][1], synth.SERVE_SHAPES[spec[int(c.tag)][0][0]][:3], int(c.batch))
@@ -585,13 +597,26 @@ def run_served(a):
     for i, (key, j) in enumerate(spec):
         check_idx.setdefault(key, i)
     served_img = {key: (outs[i].download(), got[i]) for key, i in check_idx.items()}
-    # latency under Poisson arrivals at a fraction of the served rate
-    rate = a.load * value
-    t0 = srv.now() + 0.01
-    arr = t0 + synth.poisson_arrivals(N, rate)
-    got = one_pass(srv, list(arr))
-    p_lat = [(c.t_done - c.t_arrival) * 1e3 for c in got.values()]
-    p_thr = N / (max(c.t_done for c in got.values()) - t0)
+    # latency under Poisson arrivals at fractions of the served rate
+    sweep = []
+    for ld in [float(x) for x in a.loads.split(",") if x]:
+        rate = ld * value
+        t0 = srv.now() + 0.01
+        arr = t0 + synth.poisson_arrivals(N, rate)
+        got = one_pass(srv, list(arr))
+        p_lat = [(c.t_done - c.t_arrival) * 1e3 for c in got.values()]
+        sweep.append({"load": ld, "rate_per_s": rate, "p50": _pct(p_lat, 50), "p99": _pct(p_lat, 99),
+                      "mean": float(np.mean(p_lat)),
+                      "achieved_slices_per_s": N / (max(c.t_done for c in got.values()) - t0)})
+    # served passes of one strategy's requests at a time (for the per-strategy ratios)
+    by_strat = {}
+    for i, (key, _) in enumerate(spec):
+        by_strat.setdefault(key[1], []).append(i)
+    served_by = {}
+    for name, idx in by_strat.items():
+        t0 = srv.now() + 0.005
+        g = one_pass(srv, [t0] * len(idx), idx)
+        served_by[name] = len(idx) / (max(c.t_done for c in g.values()) - t0)
     stats = srv.stats()
     srv.close()
     # the plain GPU path: one request per launch chain, one at a time
@@ -602,14 +627,56 @@ def run_served(a):
     plain_wall = max(c.t_done for c in gotp.values()) - t0
     plain_value = N / plain_wall
     plain_service = [(c.t_done - c.t_start) * 1e3 for c in gotp.values()]
+    plain_by = {name: len(idx) / sum(gotp[i].t_done - gotp[i].t_start for i in idx)
+                for name, idx in by_strat.items()}
     plain.close()
     plain_img = {key: outs[i].download() for key, i in check_idx.items()}
+    # the plain host-buffer path: the blocking oks_recon_* entry points, k-space
+    # copied in and the image out per call, one request at a time (what a
+    # ReconStrategy impl calling the library per slice does), on a sample
+    host_fn = {"ifft_rss": "oks_recon_ifft_rss", "grappa": "oks_recon_grappa", "cs": "oks_recon_cs"}
+    host_n = min(N, a.host_requests)
+    hbufs = {}
+    for key, j in spec[:host_n]:
+        case = dev[key][j]["case"]
+        hbufs.setdefault((key, j), (np.ascontiguousarray(case["kspace"], np.complex64),
+                                    np.ascontiguousarray(case["noise"], np.complex64),
+                                    dev[key][j]["m"], np.zeros(max_out, np.float32)))
+
+    def host_call(key, j):
+        k, nz_, m, o = hbufs[(key, j)]
+        ks = abi.make_kspace_struct(k, m, nz_, dev[key][j]["case"]["hdr"])
+        img = abi.Image(o.ctypes.data, None, 0, 0, 0, 0)
+        check(ctx.ptr, getattr(lib(), host_fn[key[1]])(ctx.ptr, C.byref(descs[key[1]]), C.byref(ks),
+                                                        C.byref(img)))
+
+    for key, j in spec[:min(host_n, 32)]:
+        host_call(key, j)  # warm-up: plans, workspaces, graphs
+    t0h = time.perf_counter()
+    for key, j in spec[:host_n]:
+        host_call(key, j)
+    host_value = host_n / (time.perf_counter() - t0h)
 
     d_ms = st1.stage_ms - st0.stage_ms
     d_units = st1.stage_units - st0.stage_units
     d_n = st1.stage_launches - st0.stage_launches
     avg_s = d_ms * 1e-3 / max(d_n, 1)
     achieved = (d_units / max(d_n, 1)) / avg_s / scale if d_n else None
+    # algorithmic HBM bytes of one pass of the stream: CS 41 B per
+    # pixel-iteration (read z, x, y and the mask, write x and z), ifft-rss and
+    # GRAPPA their k-space read once and their image written once
+    cs_iters = descs["cs"].iters
+    step_cs_bytes, step_bytes = 0.0, 0.0
+    for key, j in spec:
+        ny, nx, nc, os_x = synth.SERVE_SHAPES[key[0]]
+        img_b = ny * (nx // os_x) * 4
+        if key[1] == "cs":
+            b = CS_BYTES_PER_PIXEL_ITER * ny * nx * nc * cs_iters + img_b
+            step_cs_bytes += b
+        else:
+            b = nc * ny * nx * 8 + img_b
+        step_bytes += b
+    step_s = sum(walls) / a.steps
     shapes = [f"{ny}x{nx}x{nc}" + (" (2x readout oversampling)" if os_x > 1 else "")
               for ny, nx, nc, os_x in synth.SERVE_SHAPES]
     out = {
@@ -622,24 +689,44 @@ def run_served(a):
                    "max_batch": a.max_batch, "distinct_slices_per_shape_and_strategy": a.pool,
                    "remove_oversampling": False, "prewhiten": True, "arrivals": "all at once per step",
                    "cs_iterations_as_hip_graphs": not a.no_graphs},
-        "latency_ms": {"p50": _pct(p_lat, 50), "p99": _pct(p_lat, 99), "mean": float(np.mean(p_lat)),
-                       "load": f"Poisson, {rate:.0f} requests/s ({a.load:g} x served rate)",
-                       "achieved_slices_per_s": p_thr},
+        "latency_ms": dict(next((w for w in sweep if w["load"] == 0.5), sweep[0] if sweep else {}),
+                           arrivals="Poisson"),
+        "latency_sweep_ms": sweep,
         "saturation_latency_ms": {"p50": _pct(sat_lat, 50), "p99": _pct(sat_lat, 99)},
         "plain_gpu": {"value": plain_value, "unit": "slices/s",
                       "service_latency_ms": {"p50": _pct(plain_service, 50), "p99": _pct(plain_service, 99)},
-                      "path": "same requests one at a time (batch of one, one slot)"},
+                      "path": "same requests one at a time (batch of one, one slot), k-space resident in HBM"},
         "served_vs_plain": value / plain_value,
+        "plain_host_api": {"value": host_value, "unit": "slices/s", "requests": host_n,
+                           "path": "oks_recon_* with host buffers, one request at a time (k-space H2D and image "
+                                   "D2H inside every call)"},
+        "served_vs_plain_host_api": value / host_value,
+        "per_strategy_slices_per_s": {name: {"served": served_by[name], "plain": plain_by[name],
+                                             "served_vs_plain": served_by[name] / plain_by[name]}
+                                      for name in by_strat},
         "batches": {"mean_requests": float(np.mean(batch_sizes)), "count": int(st1.batches - st0.batches),
                     "peak_inflight_bytes": int(stats.peak_inflight_bytes), "hbm_budget": int(stats.hbm_budget)},
     }
     units_per_launch = d_units / max(d_n, 1)
-    tr = TRAFFIC_PER_ALGO_BYTE.get("served") if a.stage == "cs_iterations" else None
-    out["roofline"] = {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
-                       "frac": (achieved / peak) if achieved else None,
-                       "traffic": units_per_launch * tr if tr else None,
-                       "kernel": f"stage {a.stage} on the slot streams", "launches": int(d_n),
-                       "avg_launch_ms": avg_s * 1e3, "algorithmic_units_per_launch": d_units / max(d_n, 1)}
+    tr = TRAFFIC_PER_ALGO_BYTE.get("served")
+    cs_gbs = step_cs_bytes / step_s / 1e9
+    out["roofline"] = {"bound": "hbm", "achieved": cs_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": cs_gbs / HBM_PEAK_GBS, "traffic": step_cs_bytes * tr if tr else None,
+                       "kernel": "CS iteration kernels (k_cs_cols + k_cs_update), whole step",
+                       "basis": "the CS requests' algorithmic bytes per pass of the stream (41 B per "
+                                "pixel-iteration) over ms_per_step; the other strategies' work shares the step, "
+                                "so this is a lower bound on the CS kernels' own rate",
+                       "algorithmic_bytes_per_step": step_cs_bytes, "ms_per_step": step_s * 1e3}
+    out["step_hbm"] = {"achieved": step_bytes / step_s / 1e9, "unit": "GB/s",
+                       "frac": step_bytes / step_s / 1e9 / HBM_PEAK_GBS,
+                       "basis": "every request's algorithmic bytes per pass (CS 41 B per pixel-iteration, "
+                                "ifft-rss and GRAPPA k-space read once + image written once) over ms_per_step"}
+    if d_n:
+        out["stage_events"] = {"stage": a.stage, "bound": bound, "unit": unit,
+                               "achieved": achieved, "frac": achieved / peak, "launches": int(d_n),
+                               "avg_launch_ms": avg_s * 1e3, "algorithmic_units_per_launch": units_per_launch,
+                               "basis": "HIP events around each launch of the stage on the slot streams; "
+                                        "launches overlap on 4 slots, so the durations include contention"}
     rl = {}
     for key, i in check_idx.items():
         c = served_img[key][1]
@@ -666,19 +753,10 @@ def run_served(a):
                              "fp32_oracle_vs_fp64": float(np.linalg.norm(r32 - r64) / np.linalg.norm(r64))})
     out["rel_l2"] = rl
     if not a.no_cpu:
-        import oracle as O
-        O.set_threads(1)
-        done, t0c = 0, time.perf_counter()
-        for key, j in spec:
-            st, _, _, _ = O.recon(key[1], descs[key[1]], dev[key][j]["case"], "f32")
-            assert st == 0
-            done += 1
-            if time.perf_counter() - t0c > a.cpu_seconds:
-                break
-        dt = time.perf_counter() - t0c
-        out["cpu_baseline"] = {"value": done / dt, "unit": "slices/s", "cores": 1, "kind": "port",
-                               "sample": f"first {done} requests of the served stream through the FP32 oracle, "
-                                         f"single thread, {dt:.1f} s"}
+        host = cpu_host()
+        jobs = [(key, dev[key][j]["case"]) for key, j in spec]
+        out["cpu_baseline"] = cpu_stream_rate(jobs, descs, 1, host)
+        out["cpu_baseline_all_cores"] = cpu_stream_rate(jobs, descs, a.cpu_threads, host)
     for lst in dev.values():
         for d in lst:
             for b in d["bufs"]:
@@ -689,35 +767,99 @@ def run_served(a):
     print(json.dumps(out))
 
 
-def cpu_baseline(wl, seconds):
+def cpu_host():
+    """The host CPU the baselines ran on: model name, logical CPUs, usable CPUs."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
+
+
+def cpu_stream_rate(spec, descs, threads, host):
+    """Slices/s of the FP32 oracle on the served stream's mix, on `threads` host
+    threads (the oracle splits each request over coils and rows): one request
+    of every (shape, strategy) pair is timed, CS at 2 and 4 iterations with the
+    per-iteration cost extrapolated to the served count, and the per-pair times
+    are weighted by how often the stream holds each pair."""
     import oracle as O
+    from openkspace_amd import abi
+    n = max(1, min(threads, host["usable_cpus"] or 1))
+    O.set_threads(n)
+    counts = {}
+    for key, case in spec:
+        counts.setdefault(key, [0, case])[0] += 1
+    total, t_all = 0.0, time.perf_counter()
+    for key, (cnt, case) in counts.items():
+        name, d = key[1], descs[key[1]]
+        if name == "cs":
+            tt = []
+            for it in (2, 4):
+                dd = abi.CsDesc.default(remove_oversampling=d.remove_oversampling, iters=it, lambda_=d.lambda_)
+                t0 = time.perf_counter()
+                assert O.recon(name, dd, case, "f32")[0] == 0
+                tt.append(time.perf_counter() - t0)
+            slope = (tt[1] - tt[0]) / 2.0
+            t = max(tt[0] - 2 * slope, 0.0) + d.iters * slope
+        else:
+            t0 = time.perf_counter()
+            assert O.recon(name, d, case, "f32")[0] == 0
+            t = time.perf_counter() - t0
+        total += cnt * t
     O.set_threads(1)
+    dt = time.perf_counter() - t_all
+    n_req = sum(c for c, _ in counts.values())
+    return {"value": n_req / total, "unit": "slices/s", "cores": n, "kind": "port",
+            "sample": f"one request of each of the {len(counts)} (shape, strategy) pairs through the FP32 oracle "
+                      f"on {n} thread(s), CS timed at 2 and 4 iterations and extrapolated to its "
+                      f"{descs['cs'].iters}, weighted by the pairs' counts in the {n_req}-request stream; "
+                      f"{dt:.1f} s of CPU work", "host": host}
+
+
+def cpu_baseline(wl, seconds, threads=1, host=None):
+    """The FP32 oracle on the workload's slices, on `threads` host threads (the
+    oracle splits each call over slices, coils and rows), over a bounded
+    sample: slices/s."""
+    import oracle as O
+    host = host or cpu_host()
+    n = max(1, min(threads, host["usable_cpus"] or 1))
+    O.set_threads(n)
+    how = "single thread (the reference's execution model)" if n == 1 else f"{n} threads"
     case = wl["case"]
-    if wl["cpu_unit"] is None:
-        # one 3-D volume cannot be cut into slices: time the oracle on the
-        # same volume with kz reduced to 8 partitions, per slice
-        from openkspace_amd import synth
-        sub = synth.make_config(wl["name"], nz=8)
-        t0 = time.perf_counter()
-        st, _, _, _ = O.recon(wl["oracle"], wl["desc"], sub, "f32")
+    try:
+        if wl["cpu_unit"] is None:
+            # one 3-D volume cannot be cut into slices: time the oracle on the
+            # same volume with kz reduced to 8 partitions, per slice
+            from openkspace_amd import synth
+            sub = synth.make_config(wl["name"], nz=8)
+            t0 = time.perf_counter()
+            st, _, _, _ = O.recon(wl["oracle"], wl["desc"], sub, "f32")
+            dt = time.perf_counter() - t0
+            assert st == 0
+            return {"value": 8 / dt, "unit": "slices/s", "cores": n, "kind": "port",
+                    "sample": f"the same acquisition with 8 kz partitions through the FP32 oracle, {how}, "
+                              f"{dt:.1f} s", "host": host}
+        nz = case["kspace"].shape[1]
+        done, t0 = 0, time.perf_counter()
+        while done < nz:
+            sub = wl["cpu_unit"](case, done)
+            st, _, _, _ = O.recon(wl["oracle"], wl["desc"], sub, "f32")
+            assert st == 0
+            done += 1
+            if time.perf_counter() - t0 > seconds:
+                break
         dt = time.perf_counter() - t0
-        assert st == 0
-        return {"value": 8 / dt, "unit": "slices/s", "cores": 1, "kind": "port",
-                "sample": f"the same acquisition with 8 kz partitions through the FP32 oracle, single thread, "
-                          f"{dt:.1f} s"}
-    nz = case["kspace"].shape[1]
-    done, t0 = 0, time.perf_counter()
-    while done < nz:
-        sub = wl["cpu_unit"](case, done)
-        st, _, _, _ = O.recon(wl["oracle"], wl["desc"], sub, "f32")
-        assert st == 0
-        done += 1
-        if time.perf_counter() - t0 > seconds:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "slices/s", "cores": 1, "kind": "port",
-            "sample": f"{done} of {nz} slices of {wl['name']} through the FP32 oracle, single thread, "
-                      f"{dt:.1f} s"}
+        return {"value": done / dt, "unit": "slices/s", "cores": n, "kind": "port",
+                "sample": f"{done} of {nz} slices of {wl['name']} through the FP32 oracle, {how}, {dt:.1f} s",
+                "host": host}
+    finally:
+        O.set_threads(1)
 
 
 def main():
@@ -809,6 +951,7 @@ def main():
                          "fp32_oracle_vs_fp64": float(np.linalg.norm(r32 - r64) / np.linalg.norm(r64))}
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, a.cpu_seconds)
+        out["cpu_baseline_all_cores"] = cpu_baseline(wl, a.cpu_seconds, a.cpu_threads)
     ctx.close()
     print(json.dumps(out))
 

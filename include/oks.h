@@ -210,6 +210,25 @@ void* oks_ctx_stream(oks_ctx* ctx);
  * "cols_threads" (0 = 512, or 256) sets their workgroup size (prefetching
  * needs 512). */
 int oks_ctx_set_option(oks_ctx* ctx, const char* name, int64_t value);
+/* Launch shapes from the autotuner (python -m openkspace_amd.autotune, no
+ * counterpart in the reference, whose FftPlanner plans per call,
+ * recon/fft.rs:65-66): at context creation the library reads
+ * $OKS_TUNE_FILE, or tuning/<arch>.json beside the library, a table of
+ * (op, shape key) -> "cs_sx", "cs_threads", "cols_sx", "cols_threads",
+ * "rows_waves".  Ops and keys: "ifft_rss" [nc, ny, nx, tx, nz if 3-D else 1]
+ * for every IFFT + RSS tail, "cs" [ny, nx] for the FISTA passes.  None of
+ * these options changes the computed bits; options set explicitly win, and
+ * option "tuning" = 0 ignores the table.  oks_tune_lookup reads the table
+ * for an architecture ("gfx950") without a device: OKS_OK and *value, or
+ * OKS_E_MISSING_FIELD when the table, the entry or the option is absent. */
+int oks_tune_lookup(const char* arch, const char* op, const int64_t* key, int32_t nkey, const char* name,
+                    int64_t* value);
+/* Current value of an option: the launch options above, "tuning",
+ * "tuned_calls" (calls that took launch shapes from the table) and
+ * "tuning_entries" (entries of this context's table, -1 without one). */
+int oks_ctx_get_option(oks_ctx* ctx, const char* name, int64_t* value);
+/* The device's architecture name without feature suffixes ("gfx950"). */
+int oks_ctx_arch(oks_ctx* ctx, char* buf, int32_t cap);
 int oks_ctx_synchronize(oks_ctx* ctx);
 /* Flags raised on the device by the *_dev calls since the last take
  * (whitener not PD, SENSE pixel failure, GRAPPA calibration failure);

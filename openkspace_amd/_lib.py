@@ -57,6 +57,9 @@ def lib():
     L.oks_grappa_calibrate_dev.argtypes = [_P, _P] + [_I64] * 6 + [C.c_float, _P]
     L.oks_grappa_synthesize_dev.argtypes = [_P, _P, _P] + [_I64] * 7
     L.oks_ctx_set_option.argtypes = [_P, C.c_char_p, _I64]
+    L.oks_ctx_get_option.argtypes = [_P, C.c_char_p, C.POINTER(_I64)]
+    L.oks_ctx_arch.argtypes = [_P, C.c_char_p, _I32]
+    L.oks_tune_lookup.argtypes = [C.c_char_p, C.c_char_p, _P, _I32, C.c_char_p, C.POINTER(_I64)]
     L.oks_fista_cs_dev.argtypes = [_P, _P, _P] + [_I64] * 5 + [C.c_float, _P]
     L.oks_sense_unfold_dev.argtypes = [_P, _P, _P] + [_I64] * 4 + [C.c_float, _P, _P]
     L.oks_sense_unfold_volume_dev.argtypes = [_P, _P, _P] + [_I64] * 5 + [C.c_float, _P, _P]
@@ -89,6 +92,20 @@ def lib():
     L.oks_nifti_header.argtypes = [_I64, _I64, _I64, _P]
     _lib = L
     return L
+
+
+def tune_lookup(arch, op, key, name):
+    """The tuning table's value of option `name` for (op, key) on `arch`, or
+    None when the table, the entry or the option is absent."""
+    import numpy as np
+    k = np.ascontiguousarray(key, dtype=np.int64)
+    v = _I64(0)
+    st = lib().oks_tune_lookup(arch.encode(), op.encode(), k.ctypes.data, len(k), name.encode(), C.byref(v))
+    if st == 3:  # OKS_E_MISSING_FIELD
+        return None
+    if st != 0:
+        raise OksError(st, "malformed tuning table")
+    return int(v.value)
 
 
 def check(ctx, st):
@@ -130,6 +147,16 @@ class Context:
 
     def set_option(self, name, value):
         check(self.ptr, lib().oks_ctx_set_option(self.ptr, name.encode(), int(value)))
+
+    def get_option(self, name):
+        v = _I64(0)
+        check(self.ptr, lib().oks_ctx_get_option(self.ptr, name.encode(), C.byref(v)))
+        return int(v.value)
+
+    def arch(self):
+        buf = C.create_string_buffer(64)
+        check(self.ptr, lib().oks_ctx_arch(self.ptr, buf, 64))
+        return buf.value.decode()
 
     def stage_time(self):
         """(ms, units, launches) of the stage selected by set_option("time_stage", ...)."""

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -15,6 +16,7 @@
 #include "oks_ctx.hpp"
 #include "oks_host.hpp"
 #include "oks_kernels.hpp"
+#include "oks_tune.hpp"
 
 using namespace oks;
 
@@ -230,7 +232,91 @@ int guarded(oks_ctx* ctx, F f) {
 
 }  // namespace
 
+// Launch-shape options a tuning table may set: none changes the computed bits.
+static const char* const kLaunchOptions[] = {"cs_sx",        "cs_threads", "cols_sx",   "cols_threads",
+                                             "rows_waves",   "cs_cols_map", "cs_upd_map"};
+
+int* oks::launch_option(oks_ctx* c, const std::string& n) {
+    if (n == "cs_sx") return &c->cs_sx;
+    if (n == "cs_threads") return &c->cs_threads;
+    if (n == "cols_sx") return &c->cols_sx;
+    if (n == "cols_threads") return &c->cols_threads;
+    if (n == "rows_waves") return &c->rows_waves;
+    if (n == "cs_cols_map") return &c->cs_cols_map;
+    if (n == "cs_upd_map") return &c->cs_upd_map;
+    return nullptr;
+}
+
+uint32_t oks::launch_option_bit(const std::string& n) {
+    for (uint32_t i = 0; i < sizeof(kLaunchOptions) / sizeof(kLaunchOptions[0]); ++i)
+        if (n == kLaunchOptions[i]) return 1u << i;
+    return 0;
+}
+
+bool oks::launch_option_valid(const std::string& n, int64_t v, std::string* why) {
+    bool ok = false;
+    if (n == "cs_sx") {
+        ok = v == 0 || v == 4 || v == 8 || v == 16 || v == 32;
+        *why = "cs_sx must be 0, 4, 8, 16 or 32";
+    } else if (n == "cs_threads") {
+        ok = v == 128 || v == 256 || v == 512 || v == 1024;
+        *why = "cs_threads must be 128, 256, 512 or 1024";
+    } else if (n == "cols_sx") {
+        ok = v == 0 || v == 1 || v == 2 || v == 4 || v == 8 || v == 16;
+        *why = "cols_sx must be 0, 1, 2, 4, 8 or 16";
+    } else if (n == "cols_threads") {
+        ok = v == 0 || v == 256 || v == 512;
+        *why = "cols_threads must be 0, 256 or 512";
+    } else if (n == "rows_waves") {
+        ok = v == 0 || (v >= 8 && v <= 16);
+        *why = "rows_waves must be 0 or 8..16";
+    } else if (n == "cs_cols_map" || n == "cs_upd_map") {
+        ok = v >= 0 && v <= 2;
+        *why = n + " must be 0 (identity), 1 (XOR swizzle) or 2 (padding)";
+    } else {
+        *why = "unknown launch option " + n;
+    }
+    return ok;
+}
+
+oks::TuneScope::TuneScope(oks_ctx* ctx, const char* op, std::initializer_list<int64_t> key) : c(ctx) {
+    for (int i = 0; i < kTuned; ++i) saved[i] = *launch_option(c, kLaunchOptions[i]);
+    if (!c->use_tuning || !c->tune) return;
+    const TuneEntry* e = c->tune->find(op, std::vector<int64_t>(key));
+    if (!e) return;
+    for (int i = 0; i < kTuned; ++i) {
+        if (c->opt_user & (1u << i)) continue;
+        auto it = e->opts.find(kLaunchOptions[i]);
+        std::string why;
+        if (it != e->opts.end() && launch_option_valid(it->first, it->second, &why))
+            *launch_option(c, kLaunchOptions[i]) = (int)it->second;
+    }
+    applied = true;
+    ++c->tuned_calls;
+}
+
+oks::TuneScope::~TuneScope() {
+    for (int i = 0; i < kTuned; ++i) *launch_option(c, kLaunchOptions[i]) = saved[i];
+}
+
 extern "C" {
+
+int oks_tune_lookup(const char* arch, const char* op, const int64_t* key, int32_t nkey, const char* name,
+                    int64_t* value) {
+    if (!arch || !op || (!key && nkey > 0) || nkey < 0 || !name || !value) return OKS_E_INVALID_ARG;
+    try {
+        const TuneTable* t = tune_table_for(arch);
+        if (!t) return OKS_E_MISSING_FIELD;
+        const TuneEntry* e = t->find(op, std::vector<int64_t>(key, key + nkey));
+        if (!e) return OKS_E_MISSING_FIELD;
+        auto it = e->opts.find(name);
+        if (it == e->opts.end()) return OKS_E_MISSING_FIELD;
+        *value = it->second;
+        return OKS_OK;
+    } catch (const std::exception&) {
+        return OKS_E_INCONSISTENT;
+    }
+}
 
 int oks_ctx_create(int device, oks_ctx** out) {
     if (!out) return OKS_E_INVALID_ARG;
@@ -249,6 +335,11 @@ int oks_ctx_create(int device, oks_ctx** out) {
         OKS_HIP(hipMemsetAsync(c->d_status, 0, 8 * sizeof(int), c->stream));
         OKS_HIP(hipStreamSynchronize(c->stream));
         noalloc_setup(device);
+        hipDeviceProp_t prop{};
+        OKS_HIP(hipGetDeviceProperties(&prop, device));
+        std::string arch = prop.gcnArchName;
+        arch = arch.substr(0, arch.find(':'));
+        c->tune = tune_table_for(arch);
         return OKS_OK;
     });
     if (st != OKS_OK) {
@@ -300,30 +391,20 @@ int oks_ctx_set_option(oks_ctx* ctx, const char* name, int64_t value) {
             ctx->timed_stage = (int)value;
             ctx->timed_ms = ctx->timed_units = 0.0;
             ctx->timed_launches = 0;
-        } else if (n == "cs_sx") {
-            if (value != 0 && value != 4 && value != 8 && value != 16 && value != 32)
-                fail(OKS_E_INVALID_ARG, "cs_sx must be 0, 4, 8, 16 or 32");
-            ctx->cs_sx = (int)value;
-        } else if (n == "cs_threads") {
-            if (value != 128 && value != 256 && value != 512 && value != 1024)
-                fail(OKS_E_INVALID_ARG, "cs_threads must be 128, 256, 512 or 1024");
-            ctx->cs_threads = (int)value;
         } else if (n == "cs_chunk") {
             if (value < 0) fail(OKS_E_INVALID_ARG, "cs_chunk must be >= 0");
             ctx->cs_chunk = value;
         } else if (n == "rows_pipe" || n == "cols_pipe") {
             if (value != 0 && value != 1) fail(OKS_E_INVALID_ARG, n + " must be 0 or 1");
             (n == "rows_pipe" ? ctx->rows_pipe : ctx->cols_pipe) = (int)value;
-        } else if (n == "cols_sx") {
-            if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
-                fail(OKS_E_INVALID_ARG, "cols_sx must be 0, 1, 2, 4, 8 or 16");
-            ctx->cols_sx = (int)value;
-        } else if (n == "cols_threads") {
-            if (value != 0 && value != 256 && value != 512) fail(OKS_E_INVALID_ARG, "cols_threads must be 0, 256 or 512");
-            ctx->cols_threads = (int)value;
-        } else if (n == "rows_waves") {
-            if (value != 0 && (value < 8 || value > 16)) fail(OKS_E_INVALID_ARG, "rows_waves must be 0 or 8..16");
-            ctx->rows_waves = (int)value;
+        } else if (n == "tuning") {
+            if (value != 0 && value != 1) fail(OKS_E_INVALID_ARG, "tuning must be 0 or 1");
+            ctx->use_tuning = value != 0;
+        } else if (int* f = launch_option(ctx, n)) {
+            std::string why;
+            if (!launch_option_valid(n, value, &why)) fail(OKS_E_INVALID_ARG, why);
+            *f = (int)value;
+            ctx->opt_user |= launch_option_bit(n);
         } else if (n == "zchunk") {
             if (value < 1) fail(OKS_E_INVALID_ARG, "zchunk must be >= 1");
             ctx->zchunk = (int)value;
@@ -335,6 +416,37 @@ int oks_ctx_set_option(oks_ctx* ctx, const char* name, int64_t value) {
 }
 
 void* oks_ctx_stream(oks_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int oks_ctx_get_option(oks_ctx* ctx, const char* name, int64_t* value) {
+    return guarded(ctx, [&] {
+        const std::string n = name ? name : "";
+        if (!value) fail(OKS_E_INVALID_ARG, "null value");
+        if (n == "tuning") {
+            *value = ctx->use_tuning ? 1 : 0;
+        } else if (n == "tuned_calls") {
+            *value = ctx->tuned_calls;
+        } else if (n == "tuning_entries") {
+            *value = ctx->tune ? (int64_t)ctx->tune->entries.size() : -1;
+        } else if (int* f = launch_option(ctx, n)) {
+            *value = *f;
+        } else {
+            fail(OKS_E_INVALID_ARG, "unknown option " + n);
+        }
+        return OKS_OK;
+    });
+}
+
+int oks_ctx_arch(oks_ctx* ctx, char* buf, int32_t cap) {
+    return guarded(ctx, [&] {
+        if (!buf || cap < 1) fail(OKS_E_INVALID_ARG, "null buffer");
+        hipDeviceProp_t prop{};
+        OKS_HIP(hipGetDeviceProperties(&prop, ctx->device));
+        std::string arch = prop.gcnArchName;
+        arch = arch.substr(0, arch.find(':'));
+        std::snprintf(buf, (size_t)cap, "%s", arch.c_str());
+        return OKS_OK;
+    });
+}
 
 int oks_ctx_synchronize(oks_ctx* ctx) {
     return guarded(ctx, [&] {

@@ -48,7 +48,7 @@ template <class F>
 __global__ void __launch_bounds__(cs_cols_max_threads(F::n)) k_cs_cols(CsColArgs a) {
     extern __shared__ float2 lds[];
     const int n = F::n ? F::n : a.p.n;
-    const int SX = a.sx, ls = n + 1;
+    const int SX = a.sx, ls = (F::n ? lds_span<F::map>(F::n) : n) + 1;
     const int lsx = __builtin_ctz((unsigned)SX);  // SX is a power of two
     const int WPB = blockDim.x >> 6;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(cs_cols_max_threads(F::n)) k_cs_cols(CsColArgs
         const int y = i >> lsx, col = i & (SX - 1);
         float2 v = make_float2(0.f, 0.f);
         if (xs + col < a.nx) v = T[(size_t)y * a.nx + xs + col];
-        A[(size_t)col * ls + wrap_n(y + half, n)] = v;
+        A[(size_t)col * ls + F::ix(wrap_n(y + half, n))] = v;
     }
     __syncthreads();
     for (int col = w; col < SX; col += WPB) {
@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(cs_cols_max_threads(F::n)) k_cs_cols(CsColArgs
         }
         float2* res = F::run(line, B ? B + (size_t)col * ls : nullptr, a.p_fwd, tw, lane);
         if (res != line) {
-            for (int i = lane; i < n; i += 64) line[i] = res[i];
+            for (int i = lane; i < n; i += 64) line[F::ix(i)] = res[F::ix(i)];
             wave_sync();
         }
         if constexpr (F::n != 0) {
@@ -108,8 +108,8 @@ __global__ void __launch_bounds__(cs_cols_max_threads(F::n)) k_cs_cols(CsColArgs
             for (int k = 0; k < PER; ++k) {
                 const int i = lane + 64 * k;
                 if (i < n) {
-                    const float2 v = cscale(line[i], a.scale);
-                    line[i] = mv[k] ? csub(v, yv[k]) : make_float2(0.f, 0.f);
+                    const float2 v = cscale(line[F::ix(i)], a.scale);
+                    line[F::ix(i)] = mv[k] ? csub(v, yv[k]) : make_float2(0.f, 0.f);
                 }
             }
         } else {
@@ -120,8 +120,8 @@ __global__ void __launch_bounds__(cs_cols_max_threads(F::n)) k_cs_cols(CsColArgs
                 const int r = run + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(word >> 32),
                                                                    __builtin_amdgcn_mbcnt_lo((unsigned)word, 0u));
                 if (i < n) {
-                    const float2 v = cscale(line[i], a.scale);
-                    line[i] = ((word >> lane) & 1ull) ? csub(v, ycol[r]) : make_float2(0.f, 0.f);
+                    const float2 v = cscale(line[F::ix(i)], a.scale);
+                    line[F::ix(i)] = ((word >> lane) & 1ull) ? csub(v, ycol[r]) : make_float2(0.f, 0.f);
                 }
                 run += __popcll(word);
             }
@@ -129,14 +129,15 @@ __global__ void __launch_bounds__(cs_cols_max_threads(F::n)) k_cs_cols(CsColArgs
         wave_sync();
         res = F::run_inv(line, B ? B + (size_t)col * ls : nullptr, a.p_inv, tw, lane);
         if (res != line) {
-            for (int i = lane; i < n; i += 64) line[i] = res[i];
+            for (int i = lane; i < n; i += 64) line[F::ix(i)] = res[F::ix(i)];
         }
         wave_sync();
     }
     __syncthreads();
     for (int i = threadIdx.x; i < n * SX; i += blockDim.x) {
         const int yo = i >> lsx, col = i & (SX - 1);
-        if (xs + col < a.nx) T[(size_t)yo * a.nx + xs + col] = cscale(A[(size_t)col * ls + wrap_n(yo + half, n)], a.scale);
+        if (xs + col < a.nx)
+            T[(size_t)yo * a.nx + xs + col] = cscale(A[(size_t)col * ls + F::ix(wrap_n(yo + half, n))], a.scale);
     }
 }
 
@@ -253,17 +254,18 @@ __global__ void __launch_bounds__(256) k_cs_update(CsUpdArgs a) {
     __syncthreads();
     const long long pair = (long long)blockIdx.x * (blockDim.x >> 6) + w;
     if (pair >= a.pairs) return;
-    float2* R0 = lds + tw_slots<F>() + (size_t)w * (F::scratch ? 4 : 2) * n;
-    float2* R1 = R0 + n;
-    float2* S = F::scratch ? R1 + n : nullptr;
+    const int span = F::n ? lds_span<F::map>(F::n) : n;
+    float2* R0 = lds + tw_slots<F>() + (size_t)w * (F::scratch ? 4 : 2) * span;
+    float2* R1 = R0 + span;
+    float2* S = F::scratch ? R1 + span : nullptr;
     const int hp = a.ny / 2;
     const long long plane = pair / hp;
     const int yi = (int)(pair - plane * hp);
     const size_t base = ((size_t)plane * a.ny + 2 * yi) * n;
     const int half = n / 2;
     for (int x = lane; x < n; x += 64) {
-        R0[wrap_n(x + half, n)] = a.T[base + x];
-        R1[wrap_n(x + half, n)] = a.T[base + n + x];
+        R0[F::ix(wrap_n(x + half, n))] = a.T[base + x];
+        R1[F::ix(wrap_n(x + half, n))] = a.T[base + n + x];
     }
     wave_sync();
     float2* g0 = F::run_inv(R0, S, a.p, tw, lane);
@@ -284,10 +286,11 @@ __global__ void __launch_bounds__(256) k_cs_update(CsUpdArgs a) {
         const float2 zd = cadd(xd, cmul(ap, csub(xd, od)));
         const float2 ze = cadd(xe, cmul(ap, csub(xe, oe)));
         // v = z - g at the four pixels of block j
-        const float2 va = csub(za, cscale(g0[wrap_n(x0 + half, n)], s));
-        const float2 vb = csub(zb, cscale(g0[wrap_n(x1 + half, n)], s));
-        const float2 vd = csub(zd, cscale(g1[wrap_n(x0 + half, n)], s));
-        const float2 ve = csub(ze, cscale(g1[wrap_n(x1 + half, n)], s));
+        const int s0 = F::ix(wrap_n(x0 + half, n)), s1 = F::ix(wrap_n(x1 + half, n));
+        const float2 va = csub(za, cscale(g0[s0], s));
+        const float2 vb = csub(zb, cscale(g0[s1], s));
+        const float2 vd = csub(zd, cscale(g1[s0], s));
+        const float2 ve = csub(ze, cscale(g1[s1], s));
         const float2 ll = cmul(cadd(cadd(cadd(va, vb), vd), ve), c);
         const float2 hl = soft(cmul(csub(csub(cadd(va, vd), vb), ve), c), lam);
         const float2 lh = soft(cmul(csub(csub(cadd(va, vb), vd), ve), c), lam);
@@ -309,10 +312,10 @@ __global__ void __launch_bounds__(256) k_cs_update(CsUpdArgs a) {
             const float2 z3 = cadd(ne, cmul(al, csub(ne, xe)));
             // the new z into the slots this lane read g from: the ifftshift
             // positions of the forward row transform
-            g0[wrap_n(x0 + half, n)] = z0;
-            g0[wrap_n(x1 + half, n)] = z1;
-            g1[wrap_n(x0 + half, n)] = z2;
-            g1[wrap_n(x1 + half, n)] = z3;
+            g0[s0] = z0;
+            g0[s1] = z1;
+            g1[s0] = z2;
+            g1[s1] = z3;
         }
     }
     if (!a.write_t) return;
@@ -323,8 +326,8 @@ __global__ void __launch_bounds__(256) k_cs_update(CsUpdArgs a) {
     wave_sync();
     float2* To = a.Tn + base;
     for (int x = lane; x < n; x += 64) {
-        To[x] = cscale(f0[wrap_n(x + half, n)], s);
-        To[n + x] = cscale(f1[wrap_n(x + half, n)], s);
+        To[x] = cscale(f0[F::ix(wrap_n(x + half, n))], s);
+        To[n + x] = cscale(f1[F::ix(wrap_n(x + half, n))], s);
     }
 }
 
@@ -369,24 +372,29 @@ static bool cs_fixed(int n) {
     }
 }
 
-template <int N>
+template <int N, int MAP>
 struct CsFixed {
     static constexpr int n = N;
+    static constexpr int map = MAP;
     static constexpr bool scratch = false;
     static constexpr bool lds_tw = true;
+    // every line access goes through the LDS slot map
+    __device__ __forceinline__ static int ix(int i) { return lds_slot<N, MAP>(i); }
     __device__ __forceinline__ static float2* run(float2* x, float2*, const FftPlan&, const float2* tw, int lane) {
-        CFft<N, false>::run(x, tw, lane);
+        CFft<N, false, MAP>::run(x, tw, lane);
         return x;
     }
     __device__ __forceinline__ static float2* run_inv(float2* x, float2*, const FftPlan&, const float2* tw, int lane) {
-        CFft<N, true>::run(x, tw, lane);
+        CFft<N, true, MAP>::run(x, tw, lane);
         return x;
     }
 };
 struct CsDyn {
     static constexpr int n = 0;
+    static constexpr int map = 0;
     static constexpr bool scratch = true;
     static constexpr bool lds_tw = false;
+    __device__ __forceinline__ static int ix(int i) { return i; }
     __device__ __forceinline__ static float2* run(float2* x, float2* s, const FftPlan& P, const float2*, int lane) {
         return fft_wave<false>(x, s, P, lane, 64);
     }
@@ -397,15 +405,18 @@ struct CsDyn {
 
 static size_t tw_slots_n(int n) { return cs_fixed(n) ? (size_t)((n + 1) & ~1) : 0; }
 
-size_t cs_cols_lds(int sx, int n) {
-    const size_t lines = (size_t)(cs_fixed(n) ? 1 : 2) * sx * (n + 1);
+static int cs_span(int n, int map) { return cs_fixed(n) && map == 2 ? n + n / 16 : n; }
+size_t cs_cols_lds(int sx, int n, int map) {
+    const size_t lines = (size_t)(cs_fixed(n) ? 1 : 2) * sx * (cs_span(n, map) + 1);
     return (tw_slots_n(n) + lines) * sizeof(float2);
 }
-size_t cs_upd_lds(int n) { return (tw_slots_n(n) + (size_t)4 * (cs_fixed(n) ? 2 : 4) * n) * sizeof(float2); }
+size_t cs_upd_lds(int n, int map) {
+    return (tw_slots_n(n) + (size_t)4 * (cs_fixed(n) ? 2 : 4) * cs_span(n, map)) * sizeof(float2);
+}
 
 template <class F>
 static hipError_t go_cs_cols(const CsColArgs& a, hipStream_t s) {
-    const size_t lds = cs_cols_lds(a.sx, a.p.n);
+    const size_t lds = cs_cols_lds(a.sx, a.p.n, F::map);
     const unsigned grid = (unsigned)(a.planes * ((a.nx + a.sx - 1) / a.sx));
     int threads = (a.threads == 1024 || a.threads == 256 || a.threads == 128) ? a.threads : 512;
     threads = std::min(threads, cs_cols_max_threads(F::n));
@@ -414,12 +425,22 @@ static hipError_t go_cs_cols(const CsColArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <int N>
+static hipError_t go_cs_cols_map(const CsColArgs& a, hipStream_t s) {
+    switch (a.map) {
+        case 1: return go_cs_cols<CsFixed<N, 1>>(a, s);
+        case 2: return go_cs_cols<CsFixed<N, 2>>(a, s);
+        default: return go_cs_cols<CsFixed<N, 0>>(a, s);
+    }
+}
+
 hipError_t launch_cs_cols(const CsColArgs& a, hipStream_t s) {
-    if (cs_cols_lds(a.sx, a.p.n) > 160 * 1024) return hipErrorInvalidValue;
+    if (cs_cols_lds(a.sx, a.p.n, a.map) > 160 * 1024) return hipErrorInvalidValue;
     if (a.sx < 1 || (a.sx & (a.sx - 1)) != 0) return hipErrorInvalidValue;  // strip index maps use shifts
+    if (a.map < 0 || a.map > 2) return hipErrorInvalidValue;
     switch (a.p.n) {
 #define OKS_CASE(N) \
-    case N: return go_cs_cols<CsFixed<N>>(a, s);
+    case N: return go_cs_cols_map<N>(a, s);
         OKS_CS_SIZES(OKS_CASE)
 #undef OKS_CASE
         default: return go_cs_cols<CsDyn>(a, s);
@@ -428,18 +449,28 @@ hipError_t launch_cs_cols(const CsColArgs& a, hipStream_t s) {
 
 template <class F>
 static hipError_t go_cs_upd(const CsUpdArgs& a, hipStream_t s) {
-    const size_t lds = cs_upd_lds(a.p.n);
+    const size_t lds = cs_upd_lds(a.p.n, F::map);
     const unsigned grid = (unsigned)((a.pairs + 3) / 4);
     OKS_ATTR2(hipFuncSetAttribute((const void*)k_cs_update<F>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_cs_update<F>, dim3(grid), dim3(256), lds, s, a);
     return hipGetLastError();
 }
 
+template <int N>
+static hipError_t go_cs_upd_map(const CsUpdArgs& a, hipStream_t s) {
+    switch (a.map) {
+        case 1: return go_cs_upd<CsFixed<N, 1>>(a, s);
+        case 2: return go_cs_upd<CsFixed<N, 2>>(a, s);
+        default: return go_cs_upd<CsFixed<N, 0>>(a, s);
+    }
+}
+
 hipError_t launch_cs_update(const CsUpdArgs& a, hipStream_t s) {
-    if (cs_upd_lds(a.p.n) > 160 * 1024) return hipErrorInvalidValue;
+    if (cs_upd_lds(a.p.n, a.map) > 160 * 1024) return hipErrorInvalidValue;
+    if (a.map < 0 || a.map > 2) return hipErrorInvalidValue;
     switch (a.p.n) {
 #define OKS_CASE(N) \
-    case N: return go_cs_upd<CsFixed<N>>(a, s);
+    case N: return go_cs_upd_map<N>(a, s);
         OKS_CS_SIZES(OKS_CASE)
 #undef OKS_CASE
         default: return go_cs_upd<CsDyn>(a, s);

@@ -18,6 +18,7 @@ struct CsColArgs {
     int nz, nx, sx;
     int threads;                // workgroup size: 1024, 512 (default), 256 or 128
     float scale;                // 1/sqrt(ny)
+    int map;                    // LDS slot map of the lines (lds_slot: 0 identity, 1 XOR, 2 padding)
     FftPlan p, p_fwd, p_inv;    // n = ny
 };
 
@@ -33,11 +34,12 @@ struct CsUpdArgs {
     float lambda;
     float alpha_prev; // z_k = x_k + alpha_prev (x_k - x_{k-1})
     float alpha;      // z_{k+1} = x_{k+1} + alpha (x_{k+1} - x_k)
+    int map;          // LDS slot map of the row lines (lds_slot)
     FftPlan p;        // n = nx
 };
 
-size_t cs_cols_lds(int sx, int n);
-size_t cs_upd_lds(int n);
+size_t cs_cols_lds(int sx, int n, int map);
+size_t cs_upd_lds(int n, int map);
 hipError_t launch_cs_cols(const CsColArgs& a, hipStream_t s);
 hipError_t launch_cs_update(const CsUpdArgs& a, hipStream_t s);
 hipError_t launch_rss_crop(const float2* x, int nc, int nz, int ny, int nx, int z0, int tz, int y0, int ty, int x0,

@@ -23,7 +23,7 @@ namespace oks {
 // workgroups still fit in one CU's LDS, unless the context fixes it.
 static int cs_sx(const oks_ctx* ctx, int ny) {
     int sx = ctx->cs_sx > 0 ? ctx->cs_sx : 16;
-    while (sx > 1 && cs_cols_lds(sx, ny) > (ctx->cs_sx > 0 ? 160 : 80) * 1024) sx /= 2;
+    while (sx > 1 && cs_cols_lds(sx, ny, ctx->cs_cols_map) > (ctx->cs_sx > 0 ? 160 : 80) * 1024) sx /= 2;
     return sx;
 }
 
@@ -88,6 +88,7 @@ static void cs_iterations(oks_ctx* ctx, const CsProblem& P, const std::vector<fl
     ca.nx = P.nx;
     ca.sx = cs_sx(ctx, P.ny);
     ca.threads = ctx->cs_threads;
+    ca.map = ctx->cs_cols_map;
     ca.scale = sy_;
     ca.p = ctx->plan(P.ny);
     ca.p_fwd = ca.p;
@@ -96,6 +97,7 @@ static void cs_iterations(oks_ctx* ctx, const CsProblem& P, const std::vector<fl
     ua.ny = P.ny;
     ua.scale = sx_;
     ua.lambda = P.lambda;
+    ua.map = ctx->cs_upd_map;
     ua.p = ctx->plan(P.nx);
     const long long chunk = cs_chunk(ctx, P.planes);
     for (long long p0 = 0; p0 < P.planes; p0 += chunk) {
@@ -127,8 +129,10 @@ static void cs_iterations(oks_ctx* ctx, const CsProblem& P, const std::vector<fl
 
 // x = z = F^-1(mask y) (recon/cs.rs:84-97), then the iterations; result in P.x.
 static void run_fista(oks_ctx* ctx, const CsProblem& P) {
+    TuneScope tuned(ctx, "cs", {P.ny, P.nx});
     const float sx_ = 1.0f / std::sqrt((float)P.nx), sy_ = 1.0f / std::sqrt((float)P.ny);
-    if (cs_cols_lds(cs_sx(ctx, P.ny), P.ny) > 160 * 1024 || cs_upd_lds(P.nx) > 160 * 1024)
+    if (cs_cols_lds(cs_sx(ctx, P.ny), P.ny, ctx->cs_cols_map) > 160 * 1024 ||
+        cs_upd_lds(P.nx, ctx->cs_upd_map) > 160 * 1024)
         fail(OKS_E_UNSUPPORTED, "image too large for the CS passes");
     // plans and attributes exist before any capture (no allocation inside a graph)
     ctx->plan(P.ny);
@@ -140,9 +144,9 @@ static void run_fista(oks_ctx* ctx, const CsProblem& P) {
     OKS_HIP(launch_copy2(P.T, P.x, P.z, P.planes * P.ny * P.nx, ctx->stream));
     if (P.iters <= 0) return;
     const std::vector<float> alpha = fista_alphas(P.iters);
-    // algorithmic bytes per pixel, plane and iteration: read x_k, x_{k-1}, y
-    // and the mask, write x_{k+1}
-    const double it_bytes = (double)P.planes * P.iters * ((double)P.ny * P.nx * (4.0 * 8.0) + (double)P.ny * P.nx);
+    // algorithmic bytes per pixel, plane and iteration of recon/cs.rs:99-136:
+    // read z, x, y and the mask, write x and z
+    const double it_bytes = (double)P.planes * P.iters * ((double)P.ny * P.nx * (5.0 * 8.0) + (double)P.ny * P.nx);
     if (!ctx->use_graphs) {
         StageTimer timer(ctx, OKS_STAGE_CS_ITERATIONS, it_bytes);
         cs_iterations(ctx, P, alpha);
@@ -165,7 +169,8 @@ static void run_fista(oks_ctx* ctx, const CsProblem& P) {
     uint32_t lb;
     std::memcpy(&lb, &P.lambda, 4);
     key.i[5] = lb;
-    key.i[6] = ((long long)ctx->cs_sx << 16) | ctx->cs_threads;
+    key.i[6] = ((long long)ctx->cs_upd_map << 40) | ((long long)ctx->cs_cols_map << 32) |
+               ((long long)ctx->cs_sx << 16) | ctx->cs_threads;
     key.i[7] = ctx->cs_chunk;
     hipGraphExec_t exec = ctx->graph_for("cs", key);
     if (!exec) {

@@ -3,8 +3,9 @@
 
 #include <hip/hip_runtime.h>
 
-#include <cstdint>
 #include <algorithm>
+#include <cstdint>
+#include <initializer_list>
 #include <map>
 #include <mutex>
 #include <shared_mutex>
@@ -14,6 +15,7 @@
 
 #include "../../include/oks.h"
 #include "oks_fft.hpp"
+#include "oks_tune.hpp"
 
 namespace oks {
 
@@ -68,12 +70,18 @@ struct oks_ctx {
     bool kspace_is_scratch = false;                // input grid may be modified in place (server batches)
     int cs_sx = 0;                                 // CS column-pass strip width (0 = automatic)
     int cs_threads = 512;                          // CS column-pass workgroup size
+    int cs_cols_map = 0;                           // CS column pass: LDS slot map of the lines (lds_slot)
+    int cs_upd_map = 0;                            // CS update pass: LDS slot map of the row lines
     int rows_pipe = 1;                             // ifft-rss row pass: pipelined persistent kernel
     int rows_waves = 0;                            // one-row-per-workgroup row passes: wavefronts (0 = automatic)
     int cols_sx = 0;                               // column passes: columns per workgroup cap (0 = automatic)
     int cols_threads = 0;                          // column passes: workgroup size (0 = 512, or 256)
     int cols_pipe = 1;                             // column passes: prefetch the next coil
     long long cs_chunk = 0;                        // CS planes per iteration-loop chunk (0 = all)
+    const oks::TuneTable* tune = nullptr;          // autotuner's launch-shape table for this device, if any
+    bool use_tuning = true;                        // option "tuning"
+    uint32_t opt_user = 0;                         // launch options set explicitly (win over the table)
+    int64_t tuned_calls = 0;                       // calls that took launch shapes from the table
     std::vector<uint8_t> h_acq;                    // host staging: acquired rows
     std::vector<uint8_t> h_mask;                   // host staging: per-cell mask
     std::vector<uint8_t> h_batch_mask;             // host staging: a server batch's masks
@@ -163,6 +171,24 @@ struct oks_ctx {
 };
 
 namespace oks {
+
+// Launch-shape options a tuning table may set (see oks_tune.hpp).
+int* launch_option(oks_ctx* c, const std::string& name);
+uint32_t launch_option_bit(const std::string& name);
+bool launch_option_valid(const std::string& name, int64_t v, std::string* why);
+
+// Applies the tuning table's options for (op, key) to the context for one
+// call, except those set explicitly, and restores the previous values after.
+struct TuneScope {
+    static constexpr int kTuned = 7;
+    oks_ctx* c;
+    int saved[kTuned];
+    bool applied = false;
+    TuneScope(oks_ctx* ctx, const char* op, std::initializer_list<int64_t> key);
+    ~TuneScope();
+    TuneScope(const TuneScope&) = delete;
+    TuneScope& operator=(const TuneScope&) = delete;
+};
 
 // RAII bracket of one launch (or launch sequence) of stage `stage`.
 struct StageTimer {

@@ -236,9 +236,34 @@ constexpr int cfft_nst(int n) {
 }
 constexpr bool cfft_coded(int r) { return r == 2 || r == 3 || r == 4 || r == 5 || r == 8; }
 
-template <int N, bool INV>
+// LDS slot maps of a compile-time line of length N, chosen per kernel
+// (every map gives the same bits; they move where the Stockham stages'
+// stride-R accesses fall among the LDS banks):
+//   0  identity;
+//   1  XOR swizzle: the low five bits XORed with bits 3..7, a permutation of
+//      every aligned block of 32 (used only when N is a multiple of 32, else
+//      identity); nearly conflict-free, but the index can no longer be folded
+//      into the instructions' constant offsets;
+//   2  padding: one spare slot after every 16, so the line spans
+//      N + N / 16 slots; keeps most offsets constant.
+template <int MAP>
+constexpr int lds_span(int n) { return MAP == 2 ? n + n / 16 : n; }
+template <int N, int MAP>
+__device__ __forceinline__ int lds_slot(int i) {
+    if constexpr (MAP == 1 && N >= 64 && N % 32 == 0) {
+        return i ^ ((i >> 3) & 31);
+    } else if constexpr (MAP == 2) {
+        return i + (i >> 4);
+    } else {
+        return i;
+    }
+}
+
+template <int N, bool INV, int MAP = 0>
 struct CFft {
     static constexpr int NST = cfft_nst(N);
+    // LDS slot of line index i
+    __device__ __forceinline__ static int sl(int i) { return lds_slot<N, MAP>(i); }
 
     template <int S, int NS>
     __device__ __forceinline__ static void stage(float2* __restrict__ x, const float2* __restrict__ tw, int lane) {
@@ -253,7 +278,7 @@ struct CFft {
                 const int j = lane + 64 * b;
                 if (j < NR) {
 #pragma unroll
-                    for (int r = 0; r < R; ++r) v[b][r] = x[j + r * NR];
+                    for (int r = 0; r < R; ++r) v[b][r] = x[sl(j + r * NR)];
                 }
             }
             wave_sync();
@@ -269,7 +294,7 @@ struct CFft {
                     dft<R, INV>(v[b]);
                     const int o = (j - k) * R + k;
 #pragma unroll
-                    for (int r = 0; r < R; ++r) x[o + r * NS] = v[b][r];
+                    for (int r = 0; r < R; ++r) x[sl(o + r * NS)] = v[b][r];
                 }
             }
             wave_sync();
@@ -283,11 +308,11 @@ struct CFft {
             constexpr int LPJ = 64 / NR;
             constexpr int PPL = (H + 1 + LPJ - 1) / LPJ;  // output pairs per lane
             const int j = lane % NR, p0 = lane / NR;
-            float2 x0 = x[j];
+            float2 x0 = x[sl(j)];
             float2 t[H], u[H];
 #pragma unroll
             for (int q = 1; q <= H; ++q) {
-                const float2 a = x[j + q * NR], b = x[j + (R - q) * NR];
+                const float2 a = x[sl(j + q * NR)], b = x[sl(j + (R - q) * NR)];
                 t[q - 1] = cadd(a, b);
                 u[q - 1] = csub(a, b);
             }
@@ -318,10 +343,10 @@ struct CFft {
             for (int m = 0; m < PPL; ++m) {
                 const int p = p0 + m * LPJ;
                 if (p == 0) {
-                    x[j * R] = ya[m];
+                    x[sl(j * R)] = ya[m];
                 } else if (p <= H) {
-                    x[j * R + p] = ya[m];
-                    x[j * R + R - p] = yb[m];
+                    x[sl(j * R + p)] = ya[m];
+                    x[sl(j * R + R - p)] = yb[m];
                 }
             }
             wave_sync();
@@ -335,11 +360,11 @@ struct CFft {
                     const int j = i / R, rr = i - (i / R) * R;
                     const int k = j % NS;
                     const int step = (k * TWS + rr * NR) % N;
-                    float2 a = x[j];
+                    float2 a = x[sl(j)];
                     int e = step;
 #pragma unroll 2
                     for (int q = 1; q < R; ++q) {
-                        a = cadd(a, cmul(x[j + q * NR], twid<INV>(tw, e)));
+                        a = cadd(a, cmul(x[sl(j + q * NR)], twid<INV>(tw, e)));
                         e += step;
                         if (e >= N) e -= N;
                     }
@@ -353,7 +378,7 @@ struct CFft {
                 if (i < N) {
                     const int j = i / R, rr = i - (i / R) * R;
                     const int k = j % NS;
-                    x[(j - k) * R + k + rr * NS] = acc[m];
+                    x[sl((j - k) * R + k + rr * NS)] = acc[m];
                 }
             }
             wave_sync();

@@ -202,6 +202,7 @@ static int rss_groups(int nc, long long wgs_per_group) {
 // one chunk is still cache-resident when the column pass reads it.
 void ifft_rss_grid(oks_ctx* ctx, const float2* k, int64_t nc, int64_t nz, int64_t ny, int64_t nx, const float2* W,
                    int os_mode, int rec, bool three_d, const Win w[3], float* out) {
+    TuneScope tuned(ctx, "ifft_rss", {nc, ny, nx, (int64_t)w[2].len, three_d ? nz : 1});
     const int tx = (int)w[2].len, ty = (int)w[1].len;
     // algorithmic bytes: the k-space slices read once, the image written once
     const int64_t zread = three_d ? nz : w[0].len;
