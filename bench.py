@@ -619,6 +619,18 @@ def run_served(a):
         served_by[name] = len(idx) / (max(c.t_done for c in g.values()) - t0)
     stats = srv.stats()
     srv.close()
+    # the same stream with the resident calibration cache on: every request
+    # of a pass repeats one of the pool's slices, so after the warm-up pass
+    # GRAPPA's weights come from HBM instead of a fresh calibration (a
+    # series whose reference scan is shared); reported beside, not as, value
+    csrv = Server(slots=a.slots, max_batch=a.max_batch, graphs=not a.no_graphs, calib_cache_bytes=4 << 30)
+    one_pass(csrv, [csrv.now() + 0.005] * N)
+    c0 = csrv.stats()
+    t0 = csrv.now() + 0.005
+    gc = one_pass(csrv, [t0] * N)
+    c1 = csrv.stats()
+    cache_value = N / (max(c.t_done for c in gc.values()) - t0)
+    csrv.close()
     # the plain GPU path: one request per launch chain, one at a time
     plain = Server(slots=1, max_batch=1, graphs=not a.no_graphs)
     one_pass(plain, [plain.now() + 0.005] * N)  # warm-up pass: plans, workspaces, graphs
@@ -697,6 +709,12 @@ def run_served(a):
                       "service_latency_ms": {"p50": _pct(plain_service, 50), "p99": _pct(plain_service, 99)},
                       "path": "same requests one at a time (batch of one, one slot), k-space resident in HBM"},
         "served_vs_plain": value / plain_value,
+        "served_with_calib_cache": {"value": cache_value, "unit": "slices/s",
+                                    "calib_hits": int(c1.calib_hits - c0.calib_hits),
+                                    "calib_misses": int(c1.calib_misses - c0.calib_misses),
+                                    "path": "the same stream with GRAPPA weights kept resident across passes "
+                                            "(4 GiB cache keyed by a hash of the whitened ACS rows); not the "
+                                            "headline, which calibrates every request as the reference does"},
         "plain_host_api": {"value": host_value, "unit": "slices/s", "requests": host_n,
                            "path": "oks_recon_* with host buffers, one request at a time (k-space H2D and image "
                                    "D2H inside every call)"},

@@ -418,6 +418,10 @@ typedef struct oks_server_config {
     int64_t hbm_budget;   /* bytes of batch working set in flight; 0 = half the free HBM */
     int32_t time_stage;   /* oks_stage timed on every slot (stats.stage_*), 0 = none    */
     int32_t no_graphs;    /* 1: launch iteration loops eagerly instead of as HIP graphs */
+    int32_t reserved0;
+    int64_t calib_cache_bytes; /* resident GRAPPA weights / SENSE maps shared by the slots,
+                                  keyed by a hash of the calibration data; 0 = off (every
+                                  request calibrates, as recon/strategy.rs:366-397 does) */
 } oks_server_config;
 
 typedef struct oks_request {
@@ -445,6 +449,8 @@ typedef struct oks_server_stats {
     int64_t slot_workspace_bytes[OKS_SERVER_MAX_SLOTS]; /* device workspace held per slot */
     double stage_ms, stage_units;     /* config.time_stage totals over all slots */
     int64_t stage_launches;
+    int64_t calib_hits, calib_misses, calib_bytes; /* calibration cache: slices/volumes served
+                                                     from it, calibrated afresh, bytes held */
 } oks_server_stats;
 
 void oks_server_default(oks_server_config* c);

@@ -172,7 +172,7 @@ STRATEGY_OF = {IfftRssDesc: (STRATEGY_IFFT_RSS, "ifft-rss", "ifft_rss"),
 class ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("max_batch", C.c_int32),
                 ("max_pending", C.c_int32), ("hbm_budget", C.c_int64), ("time_stage", C.c_int32),
-                ("no_graphs", C.c_int32)]
+                ("no_graphs", C.c_int32), ("reserved0", C.c_int32), ("calib_cache_bytes", C.c_int64)]
 
 
 class Request(C.Structure):
@@ -192,7 +192,8 @@ class ServerStats(C.Structure):
                 ("batches", C.c_int64), ("pending", C.c_int64), ("inflight_bytes", C.c_int64),
                 ("peak_inflight_bytes", C.c_int64), ("hbm_budget", C.c_int64),
                 ("workspace_releases", C.c_int64), ("slot_workspace_bytes", C.c_int64 * SERVER_MAX_SLOTS),
-                ("stage_ms", C.c_double), ("stage_units", C.c_double), ("stage_launches", C.c_int64)]
+                ("stage_ms", C.c_double), ("stage_units", C.c_double), ("stage_launches", C.c_int64),
+                ("calib_hits", C.c_int64), ("calib_misses", C.c_int64), ("calib_bytes", C.c_int64)]
 
 
 def crop_target(shape, recon):

@@ -175,6 +175,8 @@ void oks_ctx::release_workspace() {
 
 void oks_ctx::release() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (calib_cache == own_cache.get()) calib_cache = nullptr;
+    own_cache.reset();
     std::unique_lock<std::shared_mutex> lk(device_alloc_mutex());
     for (auto& kv : graphs)
         for (auto& e : kv.second) {
@@ -234,7 +236,7 @@ int guarded(oks_ctx* ctx, F f) {
 
 // Launch-shape options a tuning table may set: none changes the computed bits.
 static const char* const kLaunchOptions[] = {"cs_sx",        "cs_threads", "cols_sx",   "cols_threads",
-                                             "rows_waves",   "cs_cols_map", "cs_upd_map"};
+                                             "rows_waves",   "cs_cols_map", "cs_upd_map", "cs_pipe"};
 
 int* oks::launch_option(oks_ctx* c, const std::string& n) {
     if (n == "cs_sx") return &c->cs_sx;
@@ -244,6 +246,7 @@ int* oks::launch_option(oks_ctx* c, const std::string& n) {
     if (n == "rows_waves") return &c->rows_waves;
     if (n == "cs_cols_map") return &c->cs_cols_map;
     if (n == "cs_upd_map") return &c->cs_upd_map;
+    if (n == "cs_pipe") return &c->cs_pipe;
     return nullptr;
 }
 
@@ -270,6 +273,9 @@ bool oks::launch_option_valid(const std::string& n, int64_t v, std::string* why)
     } else if (n == "rows_waves") {
         ok = v == 0 || (v >= 8 && v <= 16);
         *why = "rows_waves must be 0 or 8..16";
+    } else if (n == "cs_pipe") {
+        ok = v == 0 || v == 1;
+        *why = "cs_pipe must be 0 or 1";
     } else if (n == "cs_cols_map" || n == "cs_upd_map") {
         ok = v >= 0 && v <= 2;
         *why = n + " must be 0 (identity), 1 (XOR swizzle) or 2 (padding)";
@@ -366,6 +372,7 @@ const char* oks_last_error(const oks_ctx* ctx) { return ctx ? ctx->err.c_str() :
 int oks_ctx_set_stream(oks_ctx* ctx, void* stream) {
     return guarded(ctx, [&] {
         OKS_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->calib_cache) ctx->calib_cache->publish(ctx->stream);
         std::unique_lock<std::shared_mutex> lk(device_alloc_mutex());
         if (ctx->own_stream && ctx->stream) OKS_HIP(hipStreamDestroy(ctx->stream));
         if (stream) {
@@ -397,6 +404,12 @@ int oks_ctx_set_option(oks_ctx* ctx, const char* name, int64_t value) {
         } else if (n == "rows_pipe" || n == "cols_pipe") {
             if (value != 0 && value != 1) fail(OKS_E_INVALID_ARG, n + " must be 0 or 1");
             (n == "rows_pipe" ? ctx->rows_pipe : ctx->cols_pipe) = (int)value;
+        } else if (n == "calib_cache_mb") {
+            if (value < 0) fail(OKS_E_INVALID_ARG, "calib_cache_mb must be >= 0");
+            OKS_HIP(hipStreamSynchronize(ctx->stream));
+            if (ctx->calib_cache == ctx->own_cache.get()) ctx->calib_cache = nullptr;
+            ctx->own_cache.reset(value > 0 ? new CalibCache(value << 20) : nullptr);
+            if (value > 0) ctx->calib_cache = ctx->own_cache.get();
         } else if (n == "tuning") {
             if (value != 0 && value != 1) fail(OKS_E_INVALID_ARG, "tuning must be 0 or 1");
             ctx->use_tuning = value != 0;
@@ -425,6 +438,8 @@ int oks_ctx_get_option(oks_ctx* ctx, const char* name, int64_t* value) {
             *value = ctx->use_tuning ? 1 : 0;
         } else if (n == "tuned_calls") {
             *value = ctx->tuned_calls;
+        } else if (n == "calib_hits" || n == "calib_misses") {
+            *value = !ctx->calib_cache ? 0 : n == "calib_hits" ? ctx->calib_cache->hits() : ctx->calib_cache->misses();
         } else if (n == "tuning_entries") {
             *value = ctx->tune ? (int64_t)ctx->tune->entries.size() : -1;
         } else if (int* f = launch_option(ctx, n)) {

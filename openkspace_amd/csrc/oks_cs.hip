@@ -141,6 +141,114 @@ __global__ void __launch_bounds__(cs_cols_max_threads(F::n)) k_cs_cols(CsColArgs
     }
 }
 
+// Persistent, pipelined form of k_cs_cols for compile-time lengths: each
+// workgroup (512 threads, strips of 16 columns, two columns per wavefront)
+// walks the strips blockIdx.x, blockIdx.x + gridDim.x, ...; while it
+// transforms one strip, the next strip's samples are already on their way
+// into registers (the barriers between stages order LDS only, so those
+// global loads stay in flight).  Per strip: the prefetched samples go into
+// LDS, the wavefront's mask words and measured samples are fetched, then the
+// next strip's prefetch is issued (after them, so waiting for the measured
+// samples does not wait for the prefetch), the columns are transformed, and
+// the strip is stored.  Same arithmetic, in the same order, as k_cs_cols.
+constexpr int kCsPipeThreads = 512, kCsPipeSX = 16;
+
+template <class F>
+__global__ void __launch_bounds__(kCsPipeThreads) k_cs_cols_pipe(CsColArgs a) {
+    extern __shared__ float2 lds[];
+    constexpr int n = F::n, SX = kCsPipeSX, NT = kCsPipeThreads, WPB = NT / 64, CPW = SX / WPB;
+    constexpr int KP = (n * SX + NT - 1) / NT;  // prefetched samples per thread
+    constexpr int PER = (n + 63) / 64;
+    constexpr int ls = lds_span<F::map>(n) + 1;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const float2* tw = stage_twiddles<F>(lds, a.p);
+    float2* A = lds + tw_slots<F>();  // SX * ls
+    const int strips = (a.nx + SX - 1) / SX;
+    const long long total = a.planes * strips;
+    const int half = n / 2;
+    float2 pf[KP];
+    auto prefetch = [&](long long st) {
+        const long long plane = st / strips;
+        const int xs = (int)(st - plane * strips) * SX;
+        const float2* T = a.T + (size_t)plane * n * a.nx;
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            const int i = (int)threadIdx.x + k * NT;
+            const int y = i / SX, col = i % SX;
+            pf[k] = (i < n * SX && xs + col < a.nx) ? T[(size_t)y * a.nx + xs + col] : make_float2(0.f, 0.f);
+        }
+    };
+    long long st = blockIdx.x;
+    if (st < total) prefetch(st);
+    for (; st < total; st += gridDim.x) {
+        const long long plane = st / strips;
+        const int xs = (int)(st - plane * strips) * SX;
+        const int z = (int)((a.plane0 + plane) % a.nz);
+        const long long coil = (a.plane0 + plane) / a.nz;
+        float2* T = a.T + (size_t)plane * n * a.nx;
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            const int i = (int)threadIdx.x + k * NT;
+            const int y = i / SX, col = i % SX;
+            if (i < n * SX) A[(size_t)col * ls + F::ix(wrap_n(y + half, n))] = pf[k];
+        }
+        // this wavefront's mask words and measured samples
+        const float2* YC = a.yC + coil * a.base[(long long)a.nz * a.nx];
+        unsigned long long wd[CPW][PER];
+        float2 yv[CPW][PER];
+#pragma unroll
+        for (int c = 0; c < CPW; ++c) {
+            const int xc = min(xs + w + c * WPB, a.nx - 1);
+            const long long zx = (long long)z * a.nx + __builtin_amdgcn_readfirstlane(xc);
+            const unsigned long long* wcol = a.mB + zx * a.kw;
+#pragma unroll
+            for (int k = 0; k < PER; ++k) wd[c][k] = wcol[k];
+        }
+#pragma unroll
+        for (int c = 0; c < CPW; ++c) {
+            const int xc = min(xs + w + c * WPB, a.nx - 1);
+            const long long zx = (long long)z * a.nx + __builtin_amdgcn_readfirstlane(xc);
+            const float2* ycol = YC + a.base[zx];
+            int run = 0;
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const unsigned long long word = wd[c][k];
+                const int r = run + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(word >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((unsigned)word, 0u));
+                yv[c][k] = ((word >> lane) & 1ull) ? ycol[r] : make_float2(0.f, 0.f);
+                run += __popcll(word);
+            }
+        }
+        if (st + gridDim.x < total) prefetch(st + gridDim.x);
+        lds_barrier();
+#pragma unroll
+        for (int c = 0; c < CPW; ++c) {
+            const int col = w + c * WPB;
+            if (xs + col >= a.nx) break;
+            float2* line = A + (size_t)col * ls;
+            F::run(line, nullptr, a.p_fwd, tw, lane);
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const int i = lane + 64 * k;
+                if (i < n) {
+                    const float2 v = cscale(line[F::ix(i)], a.scale);
+                    line[F::ix(i)] = ((wd[c][k] >> lane) & 1ull) ? csub(v, yv[c][k]) : make_float2(0.f, 0.f);
+                }
+            }
+            wave_sync();
+            F::run_inv(line, nullptr, a.p_inv, tw, lane);
+            wave_sync();
+        }
+        lds_barrier();
+        for (int i = threadIdx.x; i < n * SX; i += NT) {
+            const int yo = i / SX, col = i % SX;
+            if (xs + col < a.nx)
+                T[(size_t)yo * a.nx + xs + col] = cscale(A[(size_t)col * ls + F::ix(wrap_n(yo + half, n))], a.scale);
+        }
+        lds_barrier();
+    }
+}
+
 // Mask bits in the column pass's lane order, from the column-major mask
 // [nz, nx, ny]: word k of column (z, x) has bit l set when the centred index
 // yo = (l + 64 k + ceil(ny/2)) mod ny is sampled (l + 64 k < ny).  cnt[(z, x)]
@@ -425,8 +533,33 @@ static hipError_t go_cs_cols(const CsColArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Persistent grid: as many workgroups as are resident at once on the 256
+// CUs (occupancy calculator), never more than there are strips.
+template <class F>
+static hipError_t go_cs_cols_pipe(const CsColArgs& a, hipStream_t s) {
+    const size_t lds = cs_cols_lds(kCsPipeSX, a.p.n, F::map);
+    OKS_ATTR2(hipFuncSetAttribute((const void*)k_cs_cols_pipe<F>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds));
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_cs_cols_pipe<F>, kCsPipeThreads, lds) !=
+            hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    const long long strips = a.planes * ((a.nx + kCsPipeSX - 1) / kCsPipeSX);
+    const long long grid = std::min<long long>(strips, 256LL * per_cu);
+    hipLaunchKernelGGL(k_cs_cols_pipe<F>, dim3((unsigned)grid), dim3(kCsPipeThreads), lds, s, a);
+    return hipGetLastError();
+}
+
 template <int N>
 static hipError_t go_cs_cols_map(const CsColArgs& a, hipStream_t s) {
+    if (a.pipe && N != 368 && a.sx == kCsPipeSX) {
+        switch (a.map) {
+            case 1: return go_cs_cols_pipe<CsFixed<N, 1>>(a, s);
+            case 2: return go_cs_cols_pipe<CsFixed<N, 2>>(a, s);
+            default: return go_cs_cols_pipe<CsFixed<N, 0>>(a, s);
+        }
+    }
     switch (a.map) {
         case 1: return go_cs_cols<CsFixed<N, 1>>(a, s);
         case 2: return go_cs_cols<CsFixed<N, 2>>(a, s);

@@ -19,6 +19,7 @@ struct CsColArgs {
     int threads;                // workgroup size: 1024, 512 (default), 256 or 128
     float scale;                // 1/sqrt(ny)
     int map;                    // LDS slot map of the lines (lds_slot: 0 identity, 1 XOR, 2 padding)
+    int pipe;                   // persistent prefetching form (compile-time lengths, 16-column strips)
     FftPlan p, p_fwd, p_inv;    // n = ny
 };
 

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <initializer_list>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <shared_mutex>
 #include <stdexcept>
@@ -15,6 +16,7 @@
 
 #include "../../include/oks.h"
 #include "oks_fft.hpp"
+#include "oks_calib.hpp"
 #include "oks_tune.hpp"
 
 namespace oks {
@@ -72,6 +74,7 @@ struct oks_ctx {
     int cs_threads = 512;                          // CS column-pass workgroup size
     int cs_cols_map = 0;                           // CS column pass: LDS slot map of the lines (lds_slot)
     int cs_upd_map = 0;                            // CS update pass: LDS slot map of the row lines
+    int cs_pipe = 0;                               // CS column pass: persistent prefetching form
     int rows_pipe = 1;                             // ifft-rss row pass: pipelined persistent kernel
     int rows_waves = 0;                            // one-row-per-workgroup row passes: wavefronts (0 = automatic)
     int cols_sx = 0;                               // column passes: columns per workgroup cap (0 = automatic)
@@ -82,6 +85,9 @@ struct oks_ctx {
     bool use_tuning = true;                        // option "tuning"
     uint32_t opt_user = 0;                         // launch options set explicitly (win over the table)
     int64_t tuned_calls = 0;                       // calls that took launch shapes from the table
+    oks::CalibCache* calib_cache = nullptr;        // resident calibration results (a server's, or own_cache)
+    std::unique_ptr<oks::CalibCache> own_cache;    // option "calib_cache_mb" on a plain context
+    std::vector<unsigned long long> h_hash;        // host staging: calibration hashes
     std::vector<uint8_t> h_acq;                    // host staging: acquired rows
     std::vector<uint8_t> h_mask;                   // host staging: per-cell mask
     std::vector<uint8_t> h_batch_mask;             // host staging: a server batch's masks
@@ -180,7 +186,7 @@ bool launch_option_valid(const std::string& name, int64_t v, std::string* why);
 // Applies the tuning table's options for (op, key) to the context for one
 // call, except those set explicitly, and restores the previous values after.
 struct TuneScope {
-    static constexpr int kTuned = 7;
+    static constexpr int kTuned = 8;
     oks_ctx* c;
     int saved[kTuned];
     bool applied = false;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "../../include/oks.h"
@@ -51,6 +52,21 @@ static int grappa_geom(int64_t nc, int64_t ny_acs, int64_t nx, int64_t r, int64_
     return OKS_OK;
 }
 
+// Position ranges the calibration GEMMs split each slice's sum into (the
+// partials are added in FP64), and the positions per range: up to 8 ranges
+// of at least 8 32-position steps, while the Gram's lower-triangle tiles stay
+// under 2048 workgroups.  The ranges set the FP32 summation order, so the
+// weights' bits depend on them.
+static int calib_splits(const GrGeom& g, int nz, int* kc) {
+    const int n = g.n_src;
+    const int steps = (g.n_pos + 31) / 32, nb = (n + 63) / 64;
+    const long long tiles = (long long)nz * nb * (nb + 1) / 2;
+    int splits = 1;
+    while (splits < 8 && tiles * splits * 2 <= 2048 && steps / (splits * 2) >= 8) splits *= 2;
+    *kc = (steps + splits - 1) / splits * 32;
+    return (g.n_pos + *kc - 1) / *kc;
+}
+
 static void zg(oks_ctx* ctx, int batch, const double2* A, long long sA, int lda, int opA, const double2* B,
                long long sB, int ldb, int opB, double2* C, long long sC, int ldc, int M, int N, int K, double alpha,
                double beta, const int* skip, int lower = 0, hipStream_t s = nullptr) {
@@ -74,12 +90,8 @@ static void grappa_calibrate_batch(oks_ctx* ctx, const float2* grid, int nc, int
     // split the position sum so the Gram's lower-triangle tiles fill the
     // GPU: up to 8 ranges of at least 8 32-position steps, while the tiles
     // stay under 2048 workgroups
-    const int steps = (g.n_pos + 31) / 32, nb = (n + 63) / 64;
-    const long long tiles = (long long)nz * nb * (nb + 1) / 2;
-    int splits = 1;
-    while (splits < 8 && tiles * splits * 2 <= 2048 && steps / (splits * 2) >= 8) splits *= 2;
-    const int kc = (steps + splits - 1) / splits * 32;
-    splits = (g.n_pos + kc - 1) / kc;
+    int kc = 0;
+    const int splits = calib_splits(g, nz, &kc);
     float2* G32 = (float2*)ctx->buf("gr_G32", sizeof(float2) * (size_t)splits * nz * n * n);
     float2* R32 = (float2*)ctx->buf("gr_R32", sizeof(float2) * (size_t)splits * nz * n * m);
     CalibArgs ca{grid, G32, nc, nz, ny, nx, r, kky, kkx, acs0, g.n_kx_pos, g.n_pos, 0, n, n, splits, kc};
@@ -175,12 +187,46 @@ static void grappa_volume(oks_ctx* ctx, float2* grid, const oks_kspace* k, const
         return;
     }
     if (g.n_src > 2048 || g.n_tgt > 512) fail(OKS_E_UNSUPPORTED, "GRAPPA kernel too large (n_src > 2048)");
-    float2* w = (float2*)ctx->buf("gr_w", sizeof(float2) * (size_t)nz * (r - 1) * nc * g.n_src);
+    const long long wz = (long long)(r - 1) * nc * g.n_src;
+    float2* w = (float2*)ctx->buf("gr_w", sizeof(float2) * (size_t)nz * wz);
     int* fail = (int*)ctx->buf("gr_fail", sizeof(int) * nz);
-    grappa_calibrate_batch(ctx, grid, nc, nz, ny, nx, (int)p.acs_start, ny_acs, r, kky, kkx, ridge, g, w, fail,
-                           ctx->d_status + 2, nullptr);
+    CalibCache* cache = ctx->calib_cache;
+    std::vector<uint64_t> keys;
+    bool cached = false;
+    if (cache) {
+        // per-slice keys: the ACS rows of the (whitened) grid and every
+        // parameter the weights depend on
+        auto* dh = (unsigned long long*)ctx->buf("gr_hash", sizeof(unsigned long long) * nz);
+        OKS_HIP(launch_calib_hash(grid, nc, nz, ny, nx, (int)p.acs_start, ny_acs, true, dh, ctx->stream));
+        ctx->h_hash.assign(nz, 0);
+        OKS_HIP(hipMemcpyAsync(ctx->h_hash.data(), dh, sizeof(unsigned long long) * nz, hipMemcpyDeviceToHost,
+                               ctx->stream));
+        OKS_HIP(hipStreamSynchronize(ctx->stream));
+        uint32_t rb;
+        std::memcpy(&rb, &ridge, 4);
+        int kc = 0;
+        const int splits = calib_splits(g, nz, &kc);  // the summation order is part of the result
+        for (int z = 0; z < nz; ++z)
+            keys.push_back(calib_key(ctx->h_hash[z], {1, nc, nx, ny_acs, r, kky, kkx, (int64_t)rb, splits, kc}));
+        cached = std::all_of(keys.begin(), keys.end(), [&](uint64_t k) { return cache->contains(k, ctx->stream); });
+        for (int z = 0; cached && z < nz; ++z)
+            cached = cache->fetch(keys[z], {w + (size_t)z * wz, sizeof(float2) * (size_t)wz},
+                                  {fail + z, sizeof(int)}, ctx->stream);
+        if (!cached) cache->count_miss(nz);
+    }
+    if (!cached) {
+        grappa_calibrate_batch(ctx, grid, nc, nz, ny, nx, (int)p.acs_start, ny_acs, r, kky, kkx, ridge, g, w, fail,
+                               ctx->d_status + 2, nullptr);
+        if (cache)
+            for (int z = 0; z < nz; ++z)
+                cache->store(keys[z], {w + (size_t)z * wz, sizeof(float2) * (size_t)wz}, {fail + z, sizeof(int)},
+                             ctx->stream);
+    } else {
+        // the sticky flag the calibration raises for a failed slice
+        OKS_HIP(launch_flag_any(fail, nz, ctx->d_status + 2, ctx->stream));
+    }
     ctx->last_calib_fail = fail;
-    grappa_apply_batch(ctx, grid, nc, nz, ny, nx, r, kky, kkx, w, (long long)(r - 1) * nc * g.n_src, fail, g);
+    grappa_apply_batch(ctx, grid, nc, nz, ny, nx, r, kky, kkx, w, wz, fail, g);
 }
 
 void recon_grappa_dev(oks_ctx* ctx, const oks_grappa_desc* d, const oks_kspace* k, oks_image* img) {

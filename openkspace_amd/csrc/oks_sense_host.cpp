@@ -3,6 +3,8 @@
 // entry points.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include <algorithm>
 #include <cmath>
 
@@ -93,7 +95,28 @@ void recon_sense_dev(oks_ctx* ctx, const oks_sense_desc* d, const oks_kspace* k,
         const long long planes = (long long)nc * nz;
         const size_t tot = (size_t)planes * ny * nx;
         float2* maps = (float2*)ctx->buf("se_maps", sizeof(float2) * tot);
-        if (d->map_source == OKS_MAPS_ESPIRIT) {
+        // resident maps: keyed by the ACS rows of the (whitened, kz-transformed)
+        // grid, which are all the map estimators read, and their parameters
+        CalibCache* cache = ctx->calib_cache;
+        uint64_t key = 0;
+        bool hit = false;
+        if (cache) {
+            auto* dh = (unsigned long long*)ctx->buf("se_hash", sizeof(unsigned long long));
+            OKS_HIP(launch_calib_hash(grid, nc, nz, ny, nx, (int)p.acs_start, (int)(p.acs_end - p.acs_start), false,
+                                      dh, ctx->stream));
+            unsigned long long h = 0;
+            OKS_HIP(hipMemcpyAsync(&h, dh, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+            OKS_HIP(hipStreamSynchronize(ctx->stream));
+            uint32_t tb;
+            std::memcpy(&tb, &d->espirit_threshold, 4);
+            key = calib_key(h, {2, d->map_source, nc, nz, ny, nx, p.acs_start, p.acs_end, d->walsh_window,
+                                d->walsh_iters, d->espirit_kernel, (int64_t)tb, d->espirit_iters, td ? 1 : 0});
+            hit = cache->contains(key, ctx->stream) && cache->fetch(key, {maps, sizeof(float2) * tot}, {nullptr, 0}, ctx->stream);
+            if (!hit && !cache->contains(key, ctx->stream)) cache->count_miss(1);
+        }
+        if (hit) {
+            // maps from the cache
+        } else if (d->map_source == OKS_MAPS_ESPIRIT) {
             // the contiguous ACS block [nc, acs_len, nx] of every slice (recon/strategy.rs:673-688)
             const EspAcs ea{grid + (size_t)p.acs_start * nx, (long long)ny * nx, (long long)nz * ny * nx, nx,
                             (int)(p.acs_end - p.acs_start), nx};
@@ -106,6 +129,7 @@ void recon_sense_dev(oks_ctx* ctx, const oks_sense_desc* d, const oks_kspace* k,
             WalshArgs wa{low, maps, nc, nz, ny, nx, d->walsh_window, d->walsh_iters};
             OKS_HIP(launch_walsh(wa, ctx->stream));
         }
+        if (cache && !hit) cache->store(key, {maps, sizeof(float2) * tot}, {nullptr, 0}, ctx->stream);
         ifft2_planes(ctx, grid, planes, ny, nx);  // aliased coil images
         float* full = (float*)ctx->buf("se_img", sizeof(float) * (size_t)nz * ny * nx);
         float* gfull = (d->compute_gfactor && img->gfactor) ? (float*)ctx->buf("se_g", sizeof(float) * (size_t)nz * ny * nx)

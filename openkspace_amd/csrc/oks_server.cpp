@@ -17,6 +17,8 @@
 // batch mates.  Batches on different slots overlap on the device.
 #include <hip/hip_runtime.h>
 
+#include <memory>
+
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -177,6 +179,7 @@ struct oks_server {
     uint64_t seq = 0;
     int64_t inflight = 0;
     oks_server_stats st{};
+    std::unique_ptr<oks::CalibCache> cache;  // shared by the slots' contexts
     std::string err;
 
     void worker(int slot);
@@ -282,6 +285,7 @@ void oks_server::run_batch(int slot, std::vector<Req*>& batch, std::vector<oks_c
         }
         OKS_HIP(hipMemcpyAsync(h_status, ctx->d_status, sizeof(h_status), hipMemcpyDeviceToHost, ctx->stream));
         OKS_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->calib_cache) ctx->calib_cache->publish(ctx->stream);  // its copies are complete
         OKS_HIP(hipMemsetAsync(ctx->d_status, 0, sizeof(h_status), ctx->stream));
         ctx->timed_collect();
         for (int b = 0; b < B; ++b) {
@@ -418,6 +422,7 @@ int oks_server_create(const oks_server_config* cfg, oks_server** out) {
     *out = nullptr;
     if (cfg->slots < 1 || cfg->slots > OKS_SERVER_MAX_SLOTS || cfg->max_batch < 1 ||
         cfg->max_batch > OKS_SERVER_MAX_BATCH || cfg->hbm_budget < 0 || cfg->max_pending < 0 ||
+        cfg->calib_cache_bytes < 0 ||
         cfg->time_stage < OKS_STAGE_NONE || cfg->time_stage > OKS_STAGE_CS_ITERATIONS)
         return OKS_E_INVALID_ARG;
     oks_server* s = new (std::nothrow) oks_server();
@@ -447,6 +452,10 @@ int oks_server_create(const oks_server_config* cfg, oks_server** out) {
         }
         c->timed_stage = cfg->time_stage;
         c->use_graphs = cfg->no_graphs ? 0 : 1;
+        if (cfg->calib_cache_bytes > 0) {
+            if (!s->cache) s->cache.reset(new oks::CalibCache(cfg->calib_cache_bytes));
+            c->calib_cache = s->cache.get();
+        }
         s->ctxs.push_back(c);
     }
     for (int i = 0; i < cfg->slots; ++i) s->workers.emplace_back([s, i] { s->worker(i); });
@@ -464,6 +473,10 @@ void oks_server_destroy(oks_server* s) {
     for (auto& t : s->workers) t.join();
     for (Req* q : s->pending) delete q;
     for (oks_ctx* c : s->ctxs) oks_ctx_destroy(c);
+    if (s->cache) {
+        (void)hipSetDevice(s->cfg.device);
+        s->cache.reset();
+    }
     delete s;
 }
 
@@ -569,6 +582,11 @@ int oks_server_get_stats(oks_server* s, oks_server_stats* out) {
     out->inflight_bytes = s->inflight;
     out->pending = (int64_t)s->pending.size();
     out->hbm_budget = s->cfg.hbm_budget;
+    if (s->cache) {
+        out->calib_hits = s->cache->hits();
+        out->calib_misses = s->cache->misses();
+        out->calib_bytes = s->cache->bytes();
+    }
     return OKS_OK;
 }
 

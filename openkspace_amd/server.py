@@ -14,13 +14,14 @@ _P = C.c_void_p
 
 class Server:
     def __init__(self, device=0, slots=4, max_batch=32, hbm_budget=0, max_pending=0, time_stage=0,
-                 graphs=True):
+                 graphs=True, calib_cache_bytes=0):
         cfg = abi.ServerConfig()
         lib().oks_server_default(C.byref(cfg))
         cfg.device, cfg.slots, cfg.max_batch = device, slots, max_batch
         cfg.hbm_budget, cfg.max_pending = int(hbm_budget), int(max_pending)
         cfg.time_stage = int(time_stage)
         cfg.no_graphs = 0 if graphs else 1
+        cfg.calib_cache_bytes = int(calib_cache_bytes)
         self.ptr = _P()
         st = lib().oks_server_create(C.byref(cfg), C.byref(self.ptr))
         if st != 0:

@@ -192,3 +192,27 @@ def test_cs_chunked_planes_match(ctx, iters):
         ctx.set_option("cs_chunk", 0)
         ctx.set_option("graphs", 1)
     assert np.array_equal(chunked, whole)
+
+
+@pytest.mark.parametrize("ny,nx", [(256, 256), (320, 320), (384, 384), (368, 640), (640, 368), (160, 96)])
+def test_launch_forms_bit_identical(ny, nx):
+    # The persistent prefetching column pass, the LDS slot maps and the strip
+    # widths only move time: every form gives the default's bits.
+    from openkspace_amd._lib import Context
+    case = synth.make_case(nc=3, nz=2, ny=ny, nx=nx, kind="cs", seed=ny + nx, ellipses=4)
+    d = abi.CsDesc.default(iters=7)
+    forms = [{}, {"cs_pipe": 1}, {"cs_pipe": 1, "cs_cols_map": 1}, {"cs_cols_map": 2, "cs_upd_map": 1},
+             {"cs_upd_map": 2}, {"cs_sx": 8}, {"cs_threads": 1024}]
+    imgs = []
+    for opts in forms:
+        c = Context(0)
+        try:
+            c.set_option("tuning", 0)
+            for k, v in opts.items():
+                c.set_option(k, v)
+            got, _, _ = recon_dev(c, "oks_recon_cs_dev", d, case)
+            imgs.append(got)
+        finally:
+            c.close()
+    for opts, img in zip(forms[1:], imgs[1:]):
+        assert np.array_equal(img, imgs[0]), opts
