@@ -109,10 +109,49 @@ struct Planes {
         *(u32x4*)(p + 2 * ps + o) = (u32x4){wih[0], wih[1], wih[2], wih[3]};
         *(u32x4*)(p + 3 * ps + o) = (u32x4){wil[0], wil[1], wil[2], wil[3]};
     }
+    // The same from operands split in advance (split_bf16): only the 16-bit
+    // halves are regrouped into the four planes.
+    __device__ __forceinline__ void put8p(int row, int k8, const uint2 (&v)[8]) const {
+        unsigned int wrh[4], wrl[4], wih[4], wil[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint2 a = v[2 * q], b = v[2 * q + 1];
+            wrh[q] = (a.x & 0xffffu) | (b.x << 16);
+            wih[q] = (a.x >> 16) | (b.x & 0xffff0000u);
+            wrl[q] = (a.y & 0xffffu) | (b.y << 16);
+            wil[q] = (a.y >> 16) | (b.y & 0xffff0000u);
+        }
+        const int o = row * BKP + k8;
+        const int ps = rows * BKP;
+        *(u32x4*)(p + o) = (u32x4){wrh[0], wrh[1], wrh[2], wrh[3]};
+        *(u32x4*)(p + ps + o) = (u32x4){wrl[0], wrl[1], wrl[2], wrl[3]};
+        *(u32x4*)(p + 2 * ps + o) = (u32x4){wih[0], wih[1], wih[2], wih[3]};
+        *(u32x4*)(p + 3 * ps + o) = (u32x4){wil[0], wil[1], wil[2], wil[3]};
+    }
     __device__ __forceinline__ bf16x8 get(int plane, int row, int k8) const {
         return *(const bf16x8*)(p + plane * rows * BKP + row * BKP + k8);
     }
 };
+
+// x split into bf16 hi and lo exactly as Planes::put8 does:
+// {re_hi | im_hi << 16, re_lo | im_lo << 16}.
+__device__ __forceinline__ uint2 split_bf16(float2 x) {
+    const __bf16 a = (__bf16)x.x, b = (__bf16)x.y;
+    const __bf16 al = (__bf16)(x.x - (float)a), bl = (__bf16)(x.y - (float)b);
+    return make_uint2((unsigned)__builtin_bit_cast(unsigned short, a) |
+                          ((unsigned)__builtin_bit_cast(unsigned short, b) << 16),
+                      (unsigned)__builtin_bit_cast(unsigned short, al) |
+                          ((unsigned)__builtin_bit_cast(unsigned short, bl) << 16));
+}
+// The split of conj(x): the sign bits of both imaginary halves flip (bf16
+// rounding is symmetric, so hi and lo of -y are -hi and -lo).
+__device__ __forceinline__ uint2 conj_split(uint2 v) { return make_uint2(v.x ^ 0x80000000u, v.y ^ 0x80000000u); }
+
+__global__ void __launch_bounds__(256) k_split_bf16(const float2* __restrict__ in, uint2* __restrict__ out,
+                                                    long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        out[i] = split_bf16(in[i]);
+}
 
 __device__ __forceinline__ bf16x8 neg8(bf16x8 v) {
     u32x4 u = __builtin_bit_cast(u32x4, v);
@@ -174,7 +213,7 @@ __global__ void __launch_bounds__(256) k_zgemm_sbf16(typename L::Args args) {
     const int lr = lane & 15, lk = (lane >> 4) * 8;
     // Step k0's operands are fetched into registers during step k0 - BK's
     // MFMAs; the LDS-only barriers leave those loads in flight.
-    float2 va[8], vb[8];
+    uint2 va[8], vb[8];
     ld.fetch_a(K0, va);
     ld.template fetch_b<BN>(K0, vb);
     for (int k0 = K0; k0 < K; k0 += BK) {
@@ -240,14 +279,14 @@ __global__ void __launch_bounds__(256) k_zgemm_sbf16(typename L::Args args) {
 struct CalibLoader {
     static constexpr int kTab = 2048 + 512;  // source columns + target columns
     struct Args {
-        const float2* grid;
+        const uint2* gs;  // the grid split into bf16 hi/lo (split_bf16)
         float2* out;  // [splits][nz][M][N]
         int nc, nz, ny, nx, r, kky, kkx, acs0, n_kx, n_pos, rhs, M, N, splits, kc;
         const int* skip;  // per slice, or null
     };
     Args a;
     int m0, n0, z, s;
-    const float2* base;
+    const uint2* base;
     const int* src_of;
     const int* tgt_of;
     __device__ CalibLoader(const Args& x) : a(x) {
@@ -256,7 +295,7 @@ struct CalibLoader {
         n0 = b.y * (a.rhs ? 32 : 64);
         z = b.z / a.splits;
         s = b.z - z * a.splits;
-        base = a.grid + ((size_t)z * a.ny + a.acs0) * a.nx;
+        base = a.gs + ((size_t)z * a.ny + a.acs0) * a.nx;
     }
     __device__ bool active() const { return a.rhs || m0 + 64 > n0; }
     __device__ void setup(int* tab) {
@@ -280,7 +319,7 @@ struct CalibLoader {
     // One thread: eight consecutive positions (contiguous kx0 within an ACS
     // row) of one source column; the commit stores them as one 16-byte write
     // per plane.  Thread t holds row t / 4, positions k0 + 8 (t % 4) + [0, 8).
-    __device__ void fetch8(const int* of, int col_base, int lim, int rows, int k0, float2 (&v)[8], bool conj) const {
+    __device__ void fetch8(const int* of, int col_base, int lim, int rows, int k0, uint2 (&v)[8], bool conj) const {
         const int t = threadIdx.x;
         if (t >= rows * 4) return;
         const int m = t >> 2, k8 = (t & 3) * 8;
@@ -291,10 +330,10 @@ struct CalibLoader {
         const int off = cin ? of[col] : 0;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            float2 x = make_float2(0.f, 0.f);
+            uint2 x = make_uint2(0u, 0u);
             if (cin && p0 + e < a.n_pos) {
                 x = base[ky0 * a.nx + kx0 + off];
-                if (conj) x.y = -x.y;
+                if (conj) x = conj_split(x);
             }
             v[e] = x;
             if (++kx0 == a.n_kx) {
@@ -303,18 +342,18 @@ struct CalibLoader {
             }
         }
     }
-    __device__ static void commit8(const Planes& P, int rows, const float2 (&v)[8]) {
+    __device__ static void commit8(const Planes& P, int rows, const uint2 (&v)[8]) {
         const int t = threadIdx.x;
-        if (t < rows * 4) P.put8(t >> 2, (t & 3) * 8, v);
+        if (t < rows * 4) P.put8p(t >> 2, (t & 3) * 8, v);
     }
-    __device__ void fetch_a(int k0, float2 (&v)[8]) const { fetch8(src_of, m0, a.M, 64, k0, v, true); }
-    __device__ void commit_a(const Planes& P, const float2 (&v)[8]) const { commit8(P, 64, v); }
+    __device__ void fetch_a(int k0, uint2 (&v)[8]) const { fetch8(src_of, m0, a.M, 64, k0, v, true); }
+    __device__ void commit_a(const Planes& P, const uint2 (&v)[8]) const { commit8(P, 64, v); }
     template <int BN>
-    __device__ void fetch_b(int k0, float2 (&v)[8]) const {
+    __device__ void fetch_b(int k0, uint2 (&v)[8]) const {
         fetch8(a.rhs ? tgt_of : src_of, n0, a.N, BN, k0, v, false);
     }
     template <int BN>
-    __device__ void commit_b(const Planes& P, const float2 (&v)[8]) const {
+    __device__ void commit_b(const Planes& P, const uint2 (&v)[8]) const {
         commit8(P, BN, v);
     }
     __device__ void store(int m, int n, float2 v) const {
@@ -331,7 +370,9 @@ struct ApplyLoader {
     static constexpr int kTab = 2048;
     struct Args {
         float2* grid;
-        const float2* w;     // [nz][r-1][nc][n_src] (slice stride wz; 0 = one kernel for all slices)
+        const uint2* gs;     // the grid split into bf16 hi/lo (sources: sampled rows only)
+        const uint2* ws;     // the weights split into bf16 hi/lo, [nz][r-1][nc][n_src]
+                             // (slice stride wz; 0 = one kernel for all slices)
         const int2* list;    // [nz][r-1][ny]
         const int* cnt;      // [nz][r-1]
         const int* skip;     // [nz] calibration failed -> leave slice undersampled
@@ -341,7 +382,7 @@ struct ApplyLoader {
     Args a;
     int z, d, ty, ky0, m0;
     bool act;
-    const float2* base;
+    const uint2* base;
     const int* src_of;
     __device__ ApplyLoader(const Args& x) : a(x) {
         const Blk b = xcd_block();
@@ -357,7 +398,7 @@ struct ApplyLoader {
             ky0 = e.y;
         }
         m0 = b.x * 64;
-        base = a.grid + ((size_t)z * a.ny + ky0) * a.nx + m0;
+        base = a.gs + ((size_t)z * a.ny + ky0) * a.nx + m0;
     }
     __device__ bool active() const { return act; }
     __device__ void setup(int* tab) {
@@ -375,35 +416,35 @@ struct ApplyLoader {
     // kx0 (m) fastest across lanes (coalesced), eight consecutive source
     // columns j per thread (thread t: m = t % 64, j = k0 + 8 (t / 64) + [0, 8)),
     // committed as one 16-byte write per plane.
-    __device__ void fetch_a(int k0, float2 (&v)[8]) const {
+    __device__ void fetch_a(int k0, uint2 (&v)[8]) const {
         const int m = threadIdx.x & 63, k8 = (threadIdx.x >> 6) * 8;
         const bool min_ = m0 + m < a.n_kx;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int j = k0 + k8 + e;
-            v[e] = (min_ && j < a.n_src) ? base[src_of[j] + m] : make_float2(0.f, 0.f);
+            v[e] = (min_ && j < a.n_src) ? base[src_of[j] + m] : make_uint2(0u, 0u);
         }
     }
-    __device__ void commit_a(const Planes& P, const float2 (&v)[8]) const {
-        P.put8(threadIdx.x & 63, (threadIdx.x >> 6) * 8, v);
+    __device__ void commit_a(const Planes& P, const uint2 (&v)[8]) const {
+        P.put8p(threadIdx.x & 63, (threadIdx.x >> 6) * 8, v);
     }
     // Row n of the weights (one target coil), eight consecutive j per thread.
     template <int BN>
-    __device__ void fetch_b(int k0, float2 (&v)[8]) const {
+    __device__ void fetch_b(int k0, uint2 (&v)[8]) const {
         const int t = threadIdx.x;
         if (t >= BN * 4) return;
         const int n = t >> 2, k8 = (t & 3) * 8;
-        const float2* wd = a.w + (size_t)z * a.wz + (size_t)(d - 1) * a.nc * a.n_src + (size_t)n * a.n_src;
+        const uint2* wd = a.ws + (size_t)z * a.wz + (size_t)(d - 1) * a.nc * a.n_src + (size_t)n * a.n_src;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int j = k0 + k8 + e;
-            v[e] = (n < a.nc && j < a.n_src) ? wd[j] : make_float2(0.f, 0.f);
+            v[e] = (n < a.nc && j < a.n_src) ? wd[j] : make_uint2(0u, 0u);
         }
     }
     template <int BN>
-    __device__ void commit_b(const Planes& P, const float2 (&v)[8]) const {
+    __device__ void commit_b(const Planes& P, const uint2 (&v)[8]) const {
         const int t = threadIdx.x;
-        if (t < BN * 4) P.put8(t >> 2, (t & 3) * 8, v);
+        if (t < BN * 4) P.put8p(t >> 2, (t & 3) * 8, v);
     }
     __device__ void store(int m, int n, float2 v) const {
         m += m0;
@@ -721,6 +762,13 @@ __global__ void k_weights_out(const double2* X, float2* w, int nz, int nc, int r
 }
 
 // ------------------------------------------------------------ launchers
+hipError_t launch_split_bf16(const float2* in, uint2* out, long long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const long long grid = std::min<long long>((n + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_split_bf16, dim3((unsigned)grid), dim3(256), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_grappa_fills(const GrappaFillArgs& a, hipStream_t s) {
     if (a.r < 2 || a.r > 9) return hipErrorInvalidValue;
     const unsigned rows = (unsigned)a.nz * a.ny;
@@ -732,7 +780,7 @@ hipError_t launch_grappa_fills(const GrappaFillArgs& a, hipStream_t s) {
 hipError_t launch_calib_gemm(const CalibArgs& c, hipStream_t s) {
     if (c.splits < 1 || c.kc < BK || c.kc % BK != 0 || (long long)(c.splits - 1) * c.kc >= c.n_pos)
         return hipErrorInvalidValue;  // every split holds at least one position
-    CalibLoader::Args a{c.grid, c.out, c.nc, c.nz, c.ny, c.nx, c.r, c.kky, c.kkx, c.acs0, c.n_kx, c.n_pos, c.rhs,
+    CalibLoader::Args a{c.gs, c.out, c.nc, c.nz, c.ny, c.nx, c.r, c.kky, c.kkx, c.acs0, c.n_kx, c.n_pos, c.rhs,
                         c.M, c.N, c.splits, c.kc, nullptr};
     if (c.rhs) {
         dim3 grid((c.M + 63) / 64, (c.N + 31) / 32, c.nz * c.splits);
@@ -746,7 +794,7 @@ hipError_t launch_calib_gemm(const CalibArgs& c, hipStream_t s) {
 
 hipError_t launch_apply_gemm(const ApplyArgs& c, hipStream_t s) {
     if (c.nc > 32) return hipErrorInvalidValue;
-    ApplyLoader::Args a{c.grid, c.w, c.list, c.cnt, c.skip, c.nc, c.nz, c.ny, c.nx, c.r, c.kky, c.kkx, c.n_kx, c.n_src,
+    ApplyLoader::Args a{c.grid, c.gs, c.ws, c.list, c.cnt, c.skip, c.nc, c.nz, c.ny, c.nx, c.r, c.kky, c.kkx, c.n_kx, c.n_src,
                         c.wz};
     dim3 grid((c.n_kx + 63) / 64, c.ny, c.nz * (c.r - 1));
     hipLaunchKernelGGL((k_zgemm_sbf16<ApplyLoader, 32>), grid, dim3(256), 0, s, a);

@@ -15,6 +15,7 @@ struct GrappaFillArgs {
 
 struct CalibArgs {
     const float2* grid;
+    const uint2* gs;  // grid split into bf16 hi/lo (split_bf16), same indexing
     float2* out;  // [splits][nz][M][N]: partial sums over the position ranges
     int nc, nz, ny, nx, r, kky, kkx, acs0, n_kx, n_pos, rhs, M, N;
     int splits, kc;  // position ranges [s kc, (s + 1) kc), kc a multiple of 32
@@ -22,7 +23,8 @@ struct CalibArgs {
 
 struct ApplyArgs {
     float2* grid;
-    const float2* w;
+    const uint2* gs;     // grid split into bf16 hi/lo (its sampled rows are the sources)
+    const uint2* ws;     // weights split into bf16 hi/lo
     const int2* list;
     const int* cnt;
     const int* skip;
@@ -61,6 +63,9 @@ struct Potrf64Args {
     int n, k0, kn, kb, nblk;
 };
 
+// out[i] = split_bf16(in[i]): {re_hi | im_hi << 16, re_lo | im_lo << 16},
+// hi = bf16(x), lo = bf16(x - hi), both round-to-nearest-even.
+hipError_t launch_split_bf16(const float2* in, uint2* out, long long n, hipStream_t s);
 hipError_t launch_grappa_fills(const GrappaFillArgs& a, hipStream_t s);
 hipError_t launch_calib_gemm(const CalibArgs& a, hipStream_t s);
 hipError_t launch_apply_gemm(const ApplyArgs& a, hipStream_t s);

@@ -77,9 +77,17 @@ static void zg(oks_ctx* ctx, int batch, const double2* A, long long sA, int lda,
 // Calibrate the GRAPPA weights of every slice of `grid` from the ACS rows
 // [acs0, acs0 + ny_acs): w[z][d-1][ch][j], FP32.  Slices whose normal matrix
 // is not positive definite are marked in `fail[z]`.
-static void grappa_calibrate_batch(oks_ctx* ctx, const float2* grid, int nc, int nz, int ny, int nx, int acs0,
-                                   int ny_acs, int r, int kky, int kkx, float ridge, const GrGeom& g, float2* w,
-                                   int* fail, int* status, long long* launches) {
+// The source samples split into bf16 hi/lo once per call (every GEMM that
+// reads them only loads): `elems` complex samples of `src`, into workspace `slot`.
+static const uint2* split_source(oks_ctx* ctx, const float2* src, size_t elems, const char* slot) {
+    uint2* gs = (uint2*)ctx->buf(slot, sizeof(uint2) * elems);
+    OKS_HIP(launch_split_bf16(src, gs, (long long)elems, ctx->stream));
+    return gs;
+}
+
+static void grappa_calibrate_batch(oks_ctx* ctx, const float2* grid, const uint2* gs, int nc, int nz, int ny, int nx,
+                                   int acs0, int ny_acs, int r, int kky, int kkx, float ridge, const GrGeom& g,
+                                   float2* w, int* fail, int* status, long long* launches) {
     (void)ny_acs;
     const int n = g.n_src, m = g.n_tgt;
     // flops of GrappaKernel::calibrate per slice: Gram (Hermitian half), A^H B,
@@ -94,9 +102,10 @@ static void grappa_calibrate_batch(oks_ctx* ctx, const float2* grid, int nc, int
     const int splits = calib_splits(g, nz, &kc);
     float2* G32 = (float2*)ctx->buf("gr_G32", sizeof(float2) * (size_t)splits * nz * n * n);
     float2* R32 = (float2*)ctx->buf("gr_R32", sizeof(float2) * (size_t)splits * nz * n * m);
-    CalibArgs ca{grid, G32, nc, nz, ny, nx, r, kky, kkx, acs0, g.n_kx_pos, g.n_pos, 0, n, n, splits, kc};
+    if (!gs) gs = split_source(ctx, grid, (size_t)nc * nz * ny * nx, "gr_split");
+    CalibArgs ca{grid, gs, G32, nc, nz, ny, nx, r, kky, kkx, acs0, g.n_kx_pos, g.n_pos, 0, n, n, splits, kc};
     OKS_HIP(launch_calib_gemm(ca, ctx->stream));
-    CalibArgs cb{grid, R32, nc, nz, ny, nx, r, kky, kkx, acs0, g.n_kx_pos, g.n_pos, 1, n, m, splits, kc};
+    CalibArgs cb{grid, gs, R32, nc, nz, ny, nx, r, kky, kkx, acs0, g.n_kx_pos, g.n_pos, 1, n, m, splits, kc};
     OKS_HIP(launch_calib_gemm(cb, ctx->stream));
     const size_t nn = (size_t)n * n, nm = (size_t)n * m;
     double2* G = (double2*)ctx->buf("gr_G64", sizeof(double2) * nz * nn);
@@ -159,14 +168,17 @@ static void grappa_calibrate_batch(oks_ctx* ctx, const float2* grid, int nc, int
 }
 
 // Fill the missing rows of every slice of `grid` in place.
-static void grappa_apply_batch(oks_ctx* ctx, float2* grid, int nc, int nz, int ny, int nx, int r, int kky, int kkx,
-                               const float2* w, long long wz, const int* skip, const GrGeom& g) {
+static void grappa_apply_batch(oks_ctx* ctx, float2* grid, const uint2* gs, int nc, int nz, int ny, int nx, int r,
+                               int kky, int kkx, const float2* w, long long wz, const int* skip, const GrGeom& g) {
+    if (!gs) gs = split_source(ctx, grid, (size_t)nc * nz * ny * nx, "gr_split");
+    // the weights split once (one copy per slice, or one for every slice when wz = 0)
+    const uint2* ws = split_source(ctx, w, (size_t)(wz ? wz * nz : (long long)(r - 1) * nc * g.n_src), "gr_wsplit");
     int2* list = (int2*)ctx->buf("gr_list", sizeof(int2) * (size_t)nz * (r - 1) * ny);
     int* cnt = (int*)ctx->buf("gr_cnt", sizeof(int) * (size_t)nz * (r - 1));
     unsigned char* sampled = (unsigned char*)ctx->buf("gr_sampled", (size_t)nz * ny);
     GrappaFillArgs fa{grid, list, cnt, sampled, nz, ny, nx, r, kky};
     OKS_HIP(launch_grappa_fills(fa, ctx->stream));
-    ApplyArgs aa{grid, w, list, cnt, skip, nc, nz, ny, nx, r, kky, kkx, nx - kkx + 1, g.n_src, wz};
+    ApplyArgs aa{grid, gs, ws, list, cnt, skip, nc, nz, ny, nx, r, kky, kkx, nx - kkx + 1, g.n_src, wz};
     // flops: 8 per complex MAC, nc x n_src MACs per filled point, (nx-kkx+1)
     // points per filled row; the filled rows are counted on the device
     StageTimer timer(ctx, OKS_STAGE_GRAPPA_APPLY, 0.0, cnt, nz * (r - 1), 8.0 * (nx - kkx + 1) * nc * g.n_src);
@@ -214,8 +226,11 @@ static void grappa_volume(oks_ctx* ctx, float2* grid, const oks_kspace* k, const
                                   {fail + z, sizeof(int)}, ctx->stream);
         if (!cached) cache->count_miss(nz);
     }
+    // the sources (sampled rows) are never written by the synthesis, so one
+    // split serves the calibration and the synthesis
+    const uint2* gs = split_source(ctx, grid, (size_t)nc * nz * ny * nx, "gr_split");
     if (!cached) {
-        grappa_calibrate_batch(ctx, grid, nc, nz, ny, nx, (int)p.acs_start, ny_acs, r, kky, kkx, ridge, g, w, fail,
+        grappa_calibrate_batch(ctx, grid, gs, nc, nz, ny, nx, (int)p.acs_start, ny_acs, r, kky, kkx, ridge, g, w, fail,
                                ctx->d_status + 2, nullptr);
         if (cache)
             for (int z = 0; z < nz; ++z)
@@ -226,7 +241,7 @@ static void grappa_volume(oks_ctx* ctx, float2* grid, const oks_kspace* k, const
         OKS_HIP(launch_flag_any(fail, nz, ctx->d_status + 2, ctx->stream));
     }
     ctx->last_calib_fail = fail;
-    grappa_apply_batch(ctx, grid, nc, nz, ny, nx, r, kky, kkx, w, wz, fail, g);
+    grappa_apply_batch(ctx, grid, gs, nc, nz, ny, nx, r, kky, kkx, w, wz, fail, g);
 }
 
 void recon_grappa_dev(oks_ctx* ctx, const oks_grappa_desc* d, const oks_kspace* k, oks_image* img) {
@@ -277,7 +292,7 @@ int oks_grappa_calibrate_dev(oks_ctx* ctx, const oks_c32* acs, int64_t nc, int64
         if (st != OKS_OK) fail(st, why);
         if (g.n_src > 2048 || g.n_tgt > 512) fail(OKS_E_UNSUPPORTED, "GRAPPA kernel too large (n_src > 2048)");
         int* f = (int*)ctx->buf("grc_fail", 2 * sizeof(int));
-        grappa_calibrate_batch(ctx, (const float2*)acs, (int)nc, 1, (int)ny_acs, (int)nx, 0, (int)ny_acs, (int)r,
+        grappa_calibrate_batch(ctx, (const float2*)acs, nullptr, (int)nc, 1, (int)ny_acs, (int)nx, 0, (int)ny_acs, (int)r,
                                (int)kky, (int)kkx, ridge, g, (float2*)weights, f, f + 1, nullptr);
         int h = 0;
         OKS_HIP(hipMemcpyAsync(&h, f, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -294,7 +309,7 @@ int oks_grappa_synthesize_dev(oks_ctx* ctx, const oks_c32* weights, oks_c32* ksp
         GrGeom g{};
         g.n_src = (int)(nc * kky * kkx);
         if (g.n_src > 2048) fail(OKS_E_UNSUPPORTED, "GRAPPA kernel too large (n_src > 2048)");
-        grappa_apply_batch(ctx, (float2*)kspace, (int)nc, (int)nz, (int)ny, (int)nx, (int)r, (int)kky, (int)kkx,
+        grappa_apply_batch(ctx, (float2*)kspace, nullptr, (int)nc, (int)nz, (int)ny, (int)nx, (int)r, (int)kky, (int)kkx,
                            (const float2*)weights, 0, nullptr, g);
     });
 }
