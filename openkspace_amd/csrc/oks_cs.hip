@@ -39,9 +39,8 @@ namespace oks {
 // Largest workgroup a column kernel is built for: 512 for the lengths whose
 // transform needs more than the 128 registers per lane a 1024-thread
 // workgroup leaves (they would spill to scratch), 256 for 368, whose kernel
-// needs about 190 registers: at two wavefronts per SIMD, 256-thread
-// workgroups let two strips share a CU (one loading while the other
-// transforms) where a 512-thread workgroup would hold the CU alone.
+// needs about 160 registers: 256-thread workgroups let several strips share
+// a CU (one loading while another transforms).
 constexpr int cs_cols_max_threads(int n) { return n == 368 ? 256 : n == 640 ? 512 : 1024; }
 
 template <class F>
@@ -86,7 +85,7 @@ __global__ void __launch_bounds__(cs_cols_max_threads(F::n)) k_cs_cols(CsColArgs
         constexpr int PER = F::n ? (F::n + 63) / 64 : 1;
         float2 yv[PER];
         bool mv[PER];
-        if constexpr (F::n != 0) {
+        auto fetch_y = [&] {
             int run = 0;
 #pragma unroll
             for (int k = 0; k < PER; ++k) {
@@ -97,7 +96,8 @@ __global__ void __launch_bounds__(cs_cols_max_threads(F::n)) k_cs_cols(CsColArgs
                 yv[k] = mv[k] ? ycol[r] : make_float2(0.f, 0.f);
                 run += __popcll(word);
             }
-        }
+        };
+        if constexpr (F::n != 0) fetch_y();
         float2* res = F::run(line, B ? B + (size_t)col * ls : nullptr, a.p_fwd, tw, lane);
         if (res != line) {
             for (int i = lane; i < n; i += 64) line[F::ix(i)] = res[F::ix(i)];

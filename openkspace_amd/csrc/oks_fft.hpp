@@ -316,12 +316,17 @@ struct CFft {
                 t[q - 1] = cadd(a, b);
                 u[q - 1] = csub(a, b);
             }
-            float2 ya[PPL], yb[PPL];
+            // every read of the line is done: each output pair is written as
+            // soon as it is formed (no output registers are held)
+            wave_sync();
 #pragma unroll
             for (int m = 0; m < PPL; ++m) {
                 const int p = p0 + m * LPJ;
-                float2 A = x0, Bs = make_float2(0.f, 0.f);
+                // one pair's twiddle loads at a time: hoisting every pair's
+                // loads would hold them all in registers
+                __builtin_amdgcn_sched_barrier(0);
                 if (p <= H) {
+                    float2 A = x0, Bs = make_float2(0.f, 0.f);
                     int e = 0;
 #pragma unroll
                     for (int q = 1; q <= H; ++q) {
@@ -333,20 +338,13 @@ struct CFft {
                         Bs.x += u[q - 1].x * w.y;
                         Bs.y += u[q - 1].y * w.y;
                     }
-                }
-                const float2 iB = mul_si<INV>(Bs);
-                ya[m] = cadd(A, iB);
-                yb[m] = csub(A, iB);
-            }
-            wave_sync();
-#pragma unroll
-            for (int m = 0; m < PPL; ++m) {
-                const int p = p0 + m * LPJ;
-                if (p == 0) {
-                    x[sl(j * R)] = ya[m];
-                } else if (p <= H) {
-                    x[sl(j * R + p)] = ya[m];
-                    x[sl(j * R + R - p)] = yb[m];
+                    const float2 iB = mul_si<INV>(Bs);
+                    if (p == 0) {
+                        x[sl(j * R)] = cadd(A, iB);
+                    } else {
+                        x[sl(j * R + p)] = cadd(A, iB);
+                        x[sl(j * R + R - p)] = csub(A, iB);
+                    }
                 }
             }
             wave_sync();

@@ -122,3 +122,19 @@ def test_os_remove_none_cases(ctx, enc, rec):
     dl, do = DeviceArray.from_host(ctx, lines), DeviceArray(ctx, (2, max(rec, 1)), np.complex64)
     assert lib().oks_os_remove_dev(ctx.ptr, dl.addr, 2, enc, rec, do.addr) == abi.OKS_E_UNSUPPORTED
     assert O.os_remove(lines, rec, "f32") is None
+
+
+@pytest.mark.parametrize("fn,desc", [("oks_recon_ifft_rss_dev", abi.IfftRssDesc.default()),
+                                     ("oks_recon_grappa_dev", abi.GrappaDesc.default()),
+                                     ("oks_recon_sense_dev", abi.SenseDesc.default()),
+                                     ("oks_recon_cs_dev", abi.CsDesc.default(iters=2))])
+def test_more_than_32_channels_refused(ctx, fn, desc):
+    # A documented limit of this backend (the reference has none): grids of
+    # more than 32 channels are refused with OKS_E_UNSUPPORTED before any work.
+    from openkspace_amd import synth
+    from openkspace_amd._lib import OksError
+    from tests.gpu_util import recon_dev
+    case = synth.make_case(nc=33, nz=1, ny=32, nx=32, kind="full", seed=3, ellipses=3)
+    with pytest.raises(OksError) as e:
+        recon_dev(ctx, fn, desc, case)
+    assert e.value.code == abi.OKS_E_UNSUPPORTED and "32" in str(e.value)
