@@ -40,6 +40,7 @@ IFFT_GROUPS = [("rows_waves", [0, 8, 10, 12, 14, 16]),
                ("cols_threads", [0, 256, 512])]
 CS_GROUPS = [(("cs_sx", "cs_threads"), list(itertools.product([4, 8, 16, 32], [256, 512, 1024]))),
              ("cs_pipe", [0, 1]),
+             ("cs_coop", [0, 1]),
              ("cs_cols_map", [0, 1, 2]),
              ("cs_upd_map", [0, 1, 2])]
 

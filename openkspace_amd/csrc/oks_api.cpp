@@ -236,7 +236,8 @@ int guarded(oks_ctx* ctx, F f) {
 
 // Launch-shape options a tuning table may set: none changes the computed bits.
 static const char* const kLaunchOptions[] = {"cs_sx",        "cs_threads", "cols_sx",   "cols_threads",
-                                             "rows_waves",   "cs_cols_map", "cs_upd_map", "cs_pipe"};
+                                             "rows_waves",   "cs_cols_map", "cs_upd_map", "cs_pipe",
+                                             "cs_coop"};
 
 int* oks::launch_option(oks_ctx* c, const std::string& n) {
     if (n == "cs_sx") return &c->cs_sx;
@@ -247,6 +248,7 @@ int* oks::launch_option(oks_ctx* c, const std::string& n) {
     if (n == "cs_cols_map") return &c->cs_cols_map;
     if (n == "cs_upd_map") return &c->cs_upd_map;
     if (n == "cs_pipe") return &c->cs_pipe;
+    if (n == "cs_coop") return &c->cs_coop;
     return nullptr;
 }
 
@@ -273,9 +275,9 @@ bool oks::launch_option_valid(const std::string& n, int64_t v, std::string* why)
     } else if (n == "rows_waves") {
         ok = v == 0 || (v >= 8 && v <= 16);
         *why = "rows_waves must be 0 or 8..16";
-    } else if (n == "cs_pipe") {
+    } else if (n == "cs_pipe" || n == "cs_coop") {
         ok = v == 0 || v == 1;
-        *why = "cs_pipe must be 0 or 1";
+        *why = n + " must be 0 or 1";
     } else if (n == "cs_cols_map" || n == "cs_upd_map") {
         ok = v >= 0 && v <= 2;
         *why = n + " must be 0 (identity), 1 (XOR swizzle) or 2 (padding)";

@@ -141,6 +141,77 @@ __global__ void __launch_bounds__(cs_cols_max_threads(F::n)) k_cs_cols(CsColArgs
     }
 }
 
+// Cooperative form of k_cs_cols (lengths whose radices are all coded): a
+// 1024-thread workgroup per strip of 16 columns, wavefront w fetching column
+// w's measured samples and forming its residual as in k_cs_cols, while every
+// transform stage spreads the strip's 16 columns of butterflies over all
+// threads (CoopFft).  Same arithmetic, in the same order, as k_cs_cols.
+constexpr int kCsCoopThreads = 1024, kCsCoopSX = 16;
+
+template <class F>
+__global__ void __launch_bounds__(kCsCoopThreads) k_cs_cols_coop(CsColArgs a) {
+    extern __shared__ float2 lds[];
+    constexpr int n = F::n, SX = kCsCoopSX, NT = kCsCoopThreads;
+    constexpr int ls = n + 1;
+    constexpr int PER = (n + 63) / 64;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const float2* tw = stage_twiddles<F>(lds, a.p);
+    float2* A = lds + tw_slots<F>();  // SX * ls
+    const int strips = (a.nx + SX - 1) / SX;
+    const long long plane = blockIdx.x / strips;
+    const int xs = (int)(blockIdx.x - plane * strips) * SX;
+    const int z = (int)((a.plane0 + plane) % a.nz);
+    const int half = n / 2;
+    float2* T = a.T + (size_t)plane * n * a.nx;
+    const long long coil = (a.plane0 + plane) / a.nz;
+    const float2* YC = a.yC + coil * a.base[(long long)a.nz * a.nx];
+    for (int i = threadIdx.x; i < n * SX; i += NT) {
+        const int y = i / SX, col = i % SX;
+        float2 v = make_float2(0.f, 0.f);
+        if (xs + col < a.nx) v = T[(size_t)y * a.nx + xs + col];
+        A[(size_t)col * ls + wrap_n(y + half, n)] = v;
+    }
+    // wavefront w: column w's mask words and measured samples
+    const bool cin = xs + w < a.nx;
+    const long long zx = (long long)z * a.nx + __builtin_amdgcn_readfirstlane(cin ? xs + w : 0);
+    const unsigned long long* wcol = a.mB + zx * a.kw;
+    const float2* ycol = YC + a.base[zx];
+    float2 yv[PER];
+    bool mv[PER];
+    {
+        int run = 0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const unsigned long long word = cin ? wcol[k] : 0ull;
+            const int r = run + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(word >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((unsigned)word, 0u));
+            mv[k] = (word >> lane) & 1ull;
+            yv[k] = mv[k] ? ycol[r] : make_float2(0.f, 0.f);
+            run += __popcll(word);
+        }
+    }
+    lds_barrier();
+    CoopFft<n, false, NT, SX>::run(A, ls, tw, threadIdx.x);
+    if (cin) {
+        float2* line = A + (size_t)w * ls;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = lane + 64 * k;
+            if (i < n) {
+                const float2 v = cscale(line[i], a.scale);
+                line[i] = mv[k] ? csub(v, yv[k]) : make_float2(0.f, 0.f);
+            }
+        }
+    }
+    lds_barrier();
+    CoopFft<n, true, NT, SX>::run(A, ls, tw, threadIdx.x);
+    for (int i = threadIdx.x; i < n * SX; i += NT) {
+        const int yo = i / SX, col = i % SX;
+        if (xs + col < a.nx)
+            T[(size_t)yo * a.nx + xs + col] = cscale(A[(size_t)col * ls + wrap_n(yo + half, n)], a.scale);
+    }
+}
+
 // Persistent, pipelined form of k_cs_cols for compile-time lengths: each
 // workgroup (512 threads, strips of 16 columns, two columns per wavefront)
 // walks the strips blockIdx.x, blockIdx.x + gridDim.x, ...; while it
@@ -552,7 +623,23 @@ static hipError_t go_cs_cols_pipe(const CsColArgs& a, hipStream_t s) {
 }
 
 template <int N>
+static hipError_t go_cs_cols_coop(const CsColArgs& a, hipStream_t s) {
+    if constexpr (cfft_all_coded(N)) {
+        using F = CsFixed<N, 0>;
+        const size_t lds = cs_cols_lds(kCsCoopSX, N, 0);
+        OKS_ATTR2(hipFuncSetAttribute((const void*)k_cs_cols_coop<F>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+        const unsigned grid = (unsigned)(a.planes * ((a.nx + kCsCoopSX - 1) / kCsCoopSX));
+        hipLaunchKernelGGL(k_cs_cols_coop<F>, dim3(grid), dim3(kCsCoopThreads), lds, s, a);
+        return hipGetLastError();
+    } else {
+        return hipErrorInvalidValue;
+    }
+}
+
+template <int N>
 static hipError_t go_cs_cols_map(const CsColArgs& a, hipStream_t s) {
+    if (a.coop && cfft_all_coded(N) && a.map == 0 && a.sx == kCsCoopSX) return go_cs_cols_coop<N>(a, s);
     if (a.pipe && N != 368 && a.sx == kCsPipeSX) {
         switch (a.map) {
             case 1: return go_cs_cols_pipe<CsFixed<N, 1>>(a, s);

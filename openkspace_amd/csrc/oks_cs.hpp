@@ -20,6 +20,7 @@ struct CsColArgs {
     float scale;                // 1/sqrt(ny)
     int map;                    // LDS slot map of the lines (lds_slot: 0 identity, 1 XOR, 2 padding)
     int pipe;                   // persistent prefetching form (compile-time lengths, 16-column strips)
+    int coop;                   // workgroup-cooperative transforms (coded radices, 16-column strips)
     FftPlan p, p_fwd, p_inv;    // n = ny
 };
 

@@ -90,6 +90,7 @@ static void cs_iterations(oks_ctx* ctx, const CsProblem& P, const std::vector<fl
     ca.threads = ctx->cs_threads;
     ca.map = ctx->cs_cols_map;
     ca.pipe = ctx->cs_pipe;
+    ca.coop = ctx->cs_coop;
     ca.scale = sy_;
     ca.p = ctx->plan(P.ny);
     ca.p_fwd = ca.p;
@@ -170,7 +171,7 @@ static void run_fista(oks_ctx* ctx, const CsProblem& P) {
     uint32_t lb;
     std::memcpy(&lb, &P.lambda, 4);
     key.i[5] = lb;
-    key.i[6] = ((long long)ctx->cs_pipe << 48) | ((long long)ctx->cs_upd_map << 40) | ((long long)ctx->cs_cols_map << 32) |
+    key.i[6] = ((long long)ctx->cs_coop << 52) | ((long long)ctx->cs_pipe << 48) | ((long long)ctx->cs_upd_map << 40) | ((long long)ctx->cs_cols_map << 32) |
                ((long long)ctx->cs_sx << 16) | ctx->cs_threads;
     key.i[7] = ctx->cs_chunk;
     hipGraphExec_t exec = ctx->graph_for("cs", key);

@@ -75,6 +75,7 @@ struct oks_ctx {
     int cs_cols_map = 0;                           // CS column pass: LDS slot map of the lines (lds_slot)
     int cs_upd_map = 0;                            // CS update pass: LDS slot map of the row lines
     int cs_pipe = 0;                               // CS column pass: persistent prefetching form
+    int cs_coop = 0;                               // CS column pass: workgroup-cooperative transforms
     int rows_pipe = 1;                             // ifft-rss row pass: pipelined persistent kernel
     int rows_waves = 0;                            // one-row-per-workgroup row passes: wavefronts (0 = automatic)
     int cols_sx = 0;                               // column passes: columns per workgroup cap (0 = automatic)
@@ -186,7 +187,7 @@ bool launch_option_valid(const std::string& name, int64_t v, std::string* why);
 // Applies the tuning table's options for (op, key) to the context for one
 // call, except those set explicitly, and restores the previous values after.
 struct TuneScope {
-    static constexpr int kTuned = 8;
+    static constexpr int kTuned = 9;
     oks_ctx* c;
     int saved[kTuned];
     bool applied = false;

@@ -389,6 +389,73 @@ struct CFft {
     }
 };
 
+// Every radix of N's compile-time sequence has a coded butterfly (no prime
+// stage).
+constexpr bool cfft_all_coded(int n) {
+    for (int s = 0; s < cfft_nst(n); ++s)
+        if (!cfft_coded(cfft_radix(n, s))) return false;
+    return n > 1;
+}
+
+// Workgroup-cooperative form of CFft<N, INV> on COLS lines at x + c * ls:
+// each Stockham stage spreads the COLS * N / R butterflies of all the lines
+// over the NT threads, where the one-wavefront-per-line transform leaves
+// lanes idle in stages with fewer than 64 butterflies per line (40 of 64 in
+// the radix-8 stages of 320).  Same butterflies, twiddles and arithmetic as
+// CFft, so the same bits; LDS-only barriers between the reads and the writes
+// of every in-place stage.  Threads past the last butterfly (whole
+// wavefronts, when COLS * N / R is a multiple of 64) only take the barriers.
+template <int N, bool INV, int NT, int COLS>
+struct CoopFft {
+    static_assert(cfft_all_coded(N), "cooperative transform needs coded radices");
+    static constexpr int NST = cfft_nst(N);
+
+    template <int S, int NS>
+    __device__ __forceinline__ static void stage(float2* __restrict__ x, int ls, const float2* __restrict__ tw,
+                                                 int tid) {
+        constexpr int R = cfft_radix(N, S);
+        constexpr int NR = N / R;
+        constexpr int TWS = N / (NS * R);
+        constexpr int TOT = COLS * NR;
+        constexpr int PB = (TOT + NT - 1) / NT;
+        float2 v[PB][R];
+#pragma unroll
+        for (int b = 0; b < PB; ++b) {
+            const int idx = tid + b * NT;
+            if (idx < TOT) {
+                const int col = idx / NR, j = idx - col * NR;
+                const float2* line = x + col * ls;
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[b][r] = line[j + r * NR];
+            }
+        }
+        lds_barrier();
+#pragma unroll
+        for (int b = 0; b < PB; ++b) {
+            const int idx = tid + b * NT;
+            if (idx < TOT) {
+                const int col = idx / NR, j = idx - col * NR;
+                float2* line = x + col * ls;
+                const int k = j % NS;
+                if constexpr (NS > 1) {
+#pragma unroll
+                    for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], twid<INV>(tw, r * k * TWS));
+                }
+                dft<R, INV>(v[b]);
+                const int o = (j - k) * R + k;
+#pragma unroll
+                for (int r = 0; r < R; ++r) line[o + r * NS] = v[b][r];
+            }
+        }
+        lds_barrier();
+        if constexpr (S + 1 < NST) stage<S + 1, NS * R>(x, ls, tw, tid);
+    }
+
+    __device__ __forceinline__ static void run(float2* x, int ls, const float2* tw, int tid) {
+        stage<0, 1>(x, ls, tw, tid);
+    }
+};
+
 // Policies the kernels are written against: a fixed length transforms in
 // place; the runtime-length fallback ping-pongs with a scratch line.
 // `tw` is the workgroup's LDS copy of the twiddle table for fixed lengths

@@ -201,7 +201,7 @@ def test_launch_forms_bit_identical(ny, nx):
     from openkspace_amd._lib import Context
     case = synth.make_case(nc=3, nz=2, ny=ny, nx=nx, kind="cs", seed=ny + nx, ellipses=4)
     d = abi.CsDesc.default(iters=7)
-    forms = [{}, {"cs_pipe": 1}, {"cs_pipe": 1, "cs_cols_map": 1}, {"cs_cols_map": 2, "cs_upd_map": 1},
+    forms = [{}, {"cs_coop": 1}, {"cs_pipe": 1}, {"cs_pipe": 1, "cs_cols_map": 1}, {"cs_cols_map": 2, "cs_upd_map": 1},
              {"cs_upd_map": 2}, {"cs_sx": 8}, {"cs_threads": 1024}]
     imgs = []
     for opts in forms:

@@ -59,7 +59,8 @@ def test_malformed_and_missing_tables(tmp_path):
 def test_committed_table():
     t = json.load(open(TABLE))
     assert t["arch"] == "gfx950" and t["entries"]
-    allowed = {"cs_sx", "cs_threads", "cols_sx", "cols_threads", "rows_waves", "cs_cols_map", "cs_upd_map", "cs_pipe"}
+    allowed = {"cs_sx", "cs_threads", "cols_sx", "cols_threads", "rows_waves", "cs_cols_map", "cs_upd_map", "cs_pipe",
+               "cs_coop"}
     for e in t["entries"]:
         assert e["op"] in ("ifft_rss", "cs") and set(e["opts"]) <= allowed
         assert e["ms"] <= e["default_ms"]
