@@ -23,6 +23,7 @@
 
 #include "oks_cs.hpp"
 #include "oks_fft.hpp"
+#include "oks_regfft.hpp"
 
 namespace oks {
 
@@ -318,6 +319,206 @@ __global__ void __launch_bounds__(kCsPipeThreads) k_cs_cols_pipe(CsColArgs a) {
         }
         lds_barrier();
     }
+}
+
+// Register four-step column pass (compile-time lengths N = N1 N2).  With
+// slot i = N2 n1 + n2 on input and k = k1 + N1 k2 on output,
+//   X[k1 + N1 k2] = sum_n2 W_N2^{n2 k2} W_N^{n2 k1} sum_n1 x[N2 n1 + n2] W_N1^{n1 k1}
+// and the inverse with m = m2 + N2 m1,
+//   x[m2 + N2 m1] = sum_k1 W_N1^{-m1 k1} W_N^{-m2 k1} sum_k2 X[k1 + N1 k2] W_N2^{-m2 k2}.
+// One workgroup per (plane, strip of SX columns), three phases with one LDS
+// exchange between them:
+//   A  task (column, n2): the N1 samples of the column read from T (the
+//      ifftshift folded into the rows read), N1-point DFT in registers,
+//      twiddles, into LDS row k1;
+//   B  task (column, k1): LDS row k1, N2-point DFT, the masked residual
+//      r = mask ? (F z - y) : 0 (recon/cs.rs:101-111) on slots k1 + N1 k2,
+//      N2-point inverse DFT, twiddles, back into the same row;
+//   C  task (column, m2): the N1 entries m2 of every row, N1-point inverse
+//      DFT, into T (the fftshift folded into the rows written).
+// The forward transform's second step, the residual and the inverse's first
+// step never leave registers.  Sampling bits and the measured samples come in
+// task order (k_cs_taskbits, k_cs_compact_task): tb[(z nx + x) N1 + k1] =
+// {bit k2 set when slot k1 + N1 k2 is sampled, sampled slots of the column's
+// tasks before k1}.
+template <int N>
+constexpr int cs_reg_threads() { return RegGeom<N, 16>::NT; }
+
+// PIPE: persistent, workgroup b walks the strips b, b + gridDim.x, ...; the
+// next strip's phase-A samples and sampling words are fetched into registers
+// right after this strip's phase A has consumed them, so their latency hides
+// behind phases B and C (the barriers order LDS only).  Otherwise one strip
+// per workgroup.  Both forms compute the same bits.
+template <int N, bool PIPE>
+__global__ void __launch_bounds__(cs_reg_threads<N>()) k_cs_cols_reg(CsColArgs a) {
+    using G = RegGeom<N, 16>;
+    constexpr int N1 = G::N1, N2 = G::N2, SX = G::SX, N2P = G::N2P, CS = G::CS, NT = G::NT;
+    constexpr int half = N / 2;
+    extern __shared__ float2 lds[];
+    float2* tw = lds;  // exp(+2 pi i e / N)
+    float2* L = lds + G::TW;
+    const int t = threadIdx.x;
+    for (int i = t; i < N; i += NT) tw[i] = a.p.tw[i];
+    const int strips = (a.nx + SX - 1) / SX;
+    const long long total = a.planes * strips;
+    const int col0 = t % SX, grp0 = t / SX;
+    float2 va[N1];
+    uint2 tb = make_uint2(0u, 0u);
+    // strip st: x[N2 n1 + n2] for n1 < N1 of task (col, n2 = grp), and the
+    // sampling word of task (col, k1 = grp).  The row offsets come from an
+    // opaque copy of nx, so that the persistent form forms them again in
+    // every strip instead of holding them across the loop.
+    auto fetch = [&](long long st, int col, int grp) {
+        const long long plane = st / strips;
+        const int xs = (int)(st - plane * strips) * SX;
+        const bool cin = xs + col < a.nx;
+        const long long zx = (long long)((a.plane0 + plane) % a.nz) * a.nx + (cin ? xs + col : 0);
+        tb = (grp < N1 && cin) ? a.tb[zx * N1 + grp] : make_uint2(0u, 0u);
+        const float2* T = a.T + (size_t)plane * N * a.nx + xs + col;
+        int nxo = a.nx;
+        asm volatile("" : "+s"(nxo));
+        if (grp < N2) {
+#pragma unroll
+            for (int n1 = 0; n1 < N1; ++n1) {
+                const int y = wrap_n(N2 * n1 + grp + half, N);
+                va[n1] = cin ? T[(unsigned)(y * nxo)] : make_float2(0.f, 0.f);
+            }
+        }
+    };
+    auto strip = [&](long long st, int col, int grp) {
+        float2* Lc = L + col * CS;
+        const long long plane = st / strips;
+        const int xs = (int)(st - plane * strips) * SX;
+        const bool cin = xs + col < a.nx;
+        const int z = (int)((a.plane0 + plane) % a.nz);
+        const long long coil = (a.plane0 + plane) / a.nz;
+        const long long zx = (long long)z * a.nx + (cin ? xs + col : 0);
+        const uint2 tbc = tb;
+        // A: N1-point DFT, output k1 times W_N^{-n2 k1} into row k1
+        // (n2 k1 <= (N1 - 1)(N2 - 1) < N)
+        if (grp < N2) {
+            const int n2 = grp;
+            rdft_emit<N1, false>(va, [&](int k1, float2 v) {
+                Lc[k1 * N2P + n2] = k1 == 0 ? v : cmul(v, twid<false>(tw, n2 * k1));
+            });
+        }
+        if constexpr (PIPE) {
+            __builtin_amdgcn_sched_barrier(0);
+            if (st + gridDim.x < total) fetch(st + gridDim.x, col, grp);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        lds_barrier();
+        // B: N2-point DFT, masked residual on slots k1 + N1 k2, N2-point
+        // inverse DFT, twiddles W_N^{m2 k1}
+        if (grp < N1) {
+            const int k1 = grp;
+            float2* row = Lc + k1 * N2P;
+            float2 v[N2];
+#pragma unroll
+            for (int j = 0; j < N2; ++j) v[j] = row[j];
+            // the measured samples of this task's sampled slots, in slot order
+            float2 yv[N2];
+            const float2* YC = a.yC + coil * a.base[(long long)a.nz * a.nx] + a.base[zx] + tbc.y;
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) {
+                const int r = __builtin_popcount(tbc.x & ((1u << k2) - 1u));
+                yv[k2] = ((tbc.x >> k2) & 1u) ? YC[r] : make_float2(0.f, 0.f);
+            }
+            rdft<N2, false>(v);
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2)
+                v[k2] = ((tbc.x >> k2) & 1u) ? csub(cscale(v[k2], a.scale), yv[k2]) : make_float2(0.f, 0.f);
+            rdft<N2, true>(v);
+#pragma unroll
+            for (int m2 = 1; m2 < N2; ++m2) v[m2] = cmul(v[m2], twid<true>(tw, m2 * k1));
+#pragma unroll
+            for (int j = 0; j < N2; ++j) row[j] = v[j];
+        }
+        lds_barrier();
+        // C: N1-point inverse DFT of entry m2 of every row, into T
+        if (grp < N2 && cin) {
+            const int m2 = grp;
+            float2* T = a.T + (size_t)plane * N * a.nx + xs + col;
+            int nxo = a.nx;
+            asm volatile("" : "+s"(nxo));
+            float2 v[N1];
+#pragma unroll
+            for (int k1 = 0; k1 < N1; ++k1) v[k1] = Lc[k1 * N2P + m2];
+            rdft_emit<N1, true>(v, [&](int m1, float2 r) {
+                const int yo = wrap_n(m2 + N2 * m1 + half, N);
+                T[(unsigned)(yo * nxo)] = cscale(r, a.scale);
+            });
+        }
+    };
+    long long st = blockIdx.x;
+    if (st < total) fetch(st, col0, grp0);
+    lds_barrier();  // the twiddle table
+    if constexpr (PIPE) {
+        for (; st < total; st += gridDim.x) {
+            // task index and column, opaque per strip: every address derived
+            // from them is formed again in each strip
+            int grp = grp0, col = col0;
+            asm volatile("" : "+v"(grp), "+v"(col));
+            strip(st, col, grp);
+            lds_barrier();  // C's reads before the next strip's A writes
+        }
+    } else {
+        if (st < total) strip(st, col0, grp0);
+    }
+}
+
+// Sampling words of the register column pass, from the column-major mask
+// [nz, nx, ny]: for column (z, x) and k1 < N1, bit k2 of tb[(z nx + x) N1 +
+// k1].x is set when slot k1 + N1 k2 is sampled (slot l holds the centred
+// index (l + ceil(ny/2)) mod ny), and tb[...].y counts the sampled slots of
+// the tasks k1' < k1.  One wavefront per column.
+template <int N>
+__global__ void __launch_bounds__(256) k_cs_taskbits(const unsigned char* __restrict__ mT, uint2* __restrict__ tb,
+                                                     long long cols) {
+    constexpr int N1 = reg_n1(N), N2 = N / N1, hup = (N + 1) / 2;
+    const long long c = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (c >= cols) return;
+    const int lane = threadIdx.x & 63;
+    const unsigned char* m = mT + c * N;
+    unsigned w = 0;
+    if (lane < N1) {
+#pragma unroll 4
+        for (int k2 = 0; k2 < N2; ++k2)
+            if (m[wrap_n(lane + N1 * k2 + hup, N)]) w |= 1u << k2;
+    }
+    const int cnt = __builtin_popcount(w);
+    int incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(incl, o);
+        if (lane >= o) incl += u;
+    }
+    if (lane < N1) tb[c * N1 + lane] = make_uint2(w, (unsigned)(incl - cnt));
+}
+
+// The measured k-space at the sampled positions only, in the register column
+// pass's task order: column (z, x) of plane p = c nz + z holds, from
+// yC[c S + base[(z, x)]], task k1's sampled slots k1 + N1 k2 in increasing k2,
+// tasks in increasing k1.  yT is the column-major copy [planes, nx, ny].  One
+// wavefront per (plane, column).
+template <int N>
+__global__ void __launch_bounds__(256) k_cs_compact_task(const float2* __restrict__ yT, const uint2* __restrict__ tb,
+                                                         const long long* __restrict__ base, float2* __restrict__ yC,
+                                                         long long planes, int nz, int nx) {
+    constexpr int N1 = reg_n1(N), N2 = N / N1, hup = (N + 1) / 2;
+    const long long id = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);  // p nx + x
+    if (id >= planes * nx) return;
+    const int lane = threadIdx.x & 63;
+    if (lane >= N1) return;
+    const long long p = id / nx;
+    const int x = (int)(id - p * nx), z = (int)(p % nz);
+    const long long c = p / nz, zx = (long long)z * nx + x;
+    const uint2 e = tb[zx * N1 + lane];
+    float2* dst = yC + c * base[(long long)nz * nx] + base[zx] + e.y;
+    const float2* src = yT + id * N;
+    int r = 0;
+    for (int k2 = 0; k2 < N2; ++k2)
+        if ((e.x >> k2) & 1u) dst[r++] = src[wrap_n(lane + N1 * k2 + hup, N)];
 }
 
 // Mask bits in the column pass's lane order, from the column-major mask
@@ -637,8 +838,34 @@ static hipError_t go_cs_cols_coop(const CsColArgs& a, hipStream_t s) {
     }
 }
 
+template <int N, bool PIPE>
+static hipError_t go_cs_cols_reg_form(const CsColArgs& a, hipStream_t s) {
+    using G = RegGeom<N, 16>;
+    const void* k = (const void*)k_cs_cols_reg<N, PIPE>;
+    OKS_ATTR2(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)G::lds_bytes));
+    const long long strips = a.planes * ((a.nx + G::SX - 1) / G::SX);
+    long long grid = strips;
+    if (PIPE) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, G::NT, G::lds_bytes) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        grid = std::min<long long>(strips, 256LL * per_cu);
+    }
+    if (grid < 1 || grid > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_cs_cols_reg<N, PIPE>), dim3((unsigned)grid), dim3(G::NT), G::lds_bytes, s, a);
+    return hipGetLastError();
+}
+
+template <int N>
+static hipError_t go_cs_cols_reg(const CsColArgs& a, hipStream_t s) {
+    if (!a.tb) return hipErrorInvalidValue;
+    // (the persistent form of 640 would spill: it runs one strip per workgroup)
+    return a.pipe && N != 640 ? go_cs_cols_reg_form<N, true>(a, s) : go_cs_cols_reg_form<N, false>(a, s);
+}
+
 template <int N>
 static hipError_t go_cs_cols_map(const CsColArgs& a, hipStream_t s) {
+    if (a.reg) return go_cs_cols_reg<N>(a, s);
     if (a.coop && cfft_all_coded(N) && a.map == 0 && a.sx == kCsCoopSX) return go_cs_cols_coop<N>(a, s);
     if (a.pipe && N != 368 && a.sx == kCsPipeSX) {
         switch (a.map) {
@@ -654,8 +881,12 @@ static hipError_t go_cs_cols_map(const CsColArgs& a, hipStream_t s) {
     }
 }
 
+bool cs_reg_ok(int n) { return cs_fixed(n) && reg_fft_ok(n); }
+int cs_reg_n1(int n) { return reg_n1(n); }
+
 hipError_t launch_cs_cols(const CsColArgs& a, hipStream_t s) {
-    if (cs_cols_lds(a.sx, a.p.n, a.map) > 160 * 1024) return hipErrorInvalidValue;
+    if (a.reg && !cs_reg_ok(a.p.n)) return hipErrorInvalidValue;
+    if (!a.reg && cs_cols_lds(a.sx, a.p.n, a.map) > 160 * 1024) return hipErrorInvalidValue;
     if (a.sx < 1 || (a.sx & (a.sx - 1)) != 0) return hipErrorInvalidValue;  // strip index maps use shifts
     if (a.map < 0 || a.map > 2) return hipErrorInvalidValue;
     switch (a.p.n) {
@@ -749,6 +980,43 @@ hipError_t launch_cs_compact(const float2* yT, const unsigned long long* mB, con
     if (grid > 0x7fffffffLL || kw != (ny + 63) / 64) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_cs_compact, dim3((unsigned)grid), dim3(256), 0, s, yT, mB, base, yC, planes, nz, ny, nx, kw);
     return hipGetLastError();
+}
+
+template <int N>
+static hipError_t go_cs_task(const float2* yT, const unsigned char* mT, uint2* tb, const long long* base, float2* yC,
+                             long long planes, int nz, int nx, hipStream_t s) {
+    const long long cols = (long long)nz * nx;
+    if (yT == nullptr) {
+        hipLaunchKernelGGL(k_cs_taskbits<N>, dim3((unsigned)((cols + 3) / 4)), dim3(256), 0, s, mT, tb, cols);
+    } else {
+        hipLaunchKernelGGL(k_cs_compact_task<N>, dim3((unsigned)((planes * nx + 3) / 4)), dim3(256), 0, s, yT, tb,
+                           base, yC, planes, nz, nx);
+    }
+    return hipGetLastError();
+}
+
+static hipError_t cs_task(const float2* yT, const unsigned char* mT, uint2* tb, const long long* base, float2* yC,
+                          long long planes, int nz, int ny, int nx, hipStream_t s) {
+    if (!cs_reg_ok(ny) || nz < 1 || nx < 1 || (long long)nz * nx > 0x7fffffffLL * 4 ||
+        planes * nx > 0x7fffffffLL * 4)
+        return hipErrorInvalidValue;
+    switch (ny) {
+#define OKS_CASE(N) \
+    case N: return go_cs_task<N>(yT, mT, tb, base, yC, planes, nz, nx, s);
+        OKS_CS_SIZES(OKS_CASE)
+#undef OKS_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_cs_taskbits(const unsigned char* mT, uint2* tb, int nz, int ny, int nx, hipStream_t s) {
+    return cs_task(nullptr, mT, tb, nullptr, nullptr, 0, nz, ny, nx, s);
+}
+
+hipError_t launch_cs_compact_task(const float2* yT, const uint2* tb, const long long* base, float2* yC,
+                                  long long planes, int nz, int ny, int nx, hipStream_t s) {
+    if (yT == nullptr) return hipErrorInvalidValue;
+    return cs_task(yT, nullptr, (uint2*)tb, base, yC, planes, nz, ny, nx, s);
 }
 
 hipError_t launch_copy2(const float2* a, float2* b, float2* c, long long n, hipStream_t s) {

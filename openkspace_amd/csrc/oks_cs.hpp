@@ -21,6 +21,8 @@ struct CsColArgs {
     int map;                    // LDS slot map of the lines (lds_slot: 0 identity, 1 XOR, 2 padding)
     int pipe;                   // persistent prefetching form (compile-time lengths, 16-column strips)
     int coop;                   // workgroup-cooperative transforms (coded radices, 16-column strips)
+    int reg;                    // register four-step transforms (k_cs_cols_reg; sampling in task order)
+    const uint2* tb;            // reg: [nz, nx, N1] task sampling words and offsets (k_cs_taskbits)
     FftPlan p, p_fwd, p_inv;    // n = ny
 };
 
@@ -43,6 +45,13 @@ struct CsUpdArgs {
 size_t cs_cols_lds(int sx, int n, int map);
 size_t cs_upd_lds(int n, int map);
 hipError_t launch_cs_cols(const CsColArgs& a, hipStream_t s);
+// Register four-step column pass: available lengths, and the factor N1 of
+// N = N1 N2 that sizes its task sampling table (N1 words per column).
+bool cs_reg_ok(int n);
+int cs_reg_n1(int n);
+hipError_t launch_cs_taskbits(const unsigned char* mT, uint2* tb, int nz, int ny, int nx, hipStream_t s);
+hipError_t launch_cs_compact_task(const float2* yT, const uint2* tb, const long long* base, float2* yC,
+                                  long long planes, int nz, int ny, int nx, hipStream_t s);
 hipError_t launch_cs_update(const CsUpdArgs& a, hipStream_t s);
 hipError_t launch_rss_crop(const float2* x, int nc, int nz, int ny, int nx, int z0, int tz, int y0, int ty, int x0,
                            int tx, float* out, hipStream_t s);

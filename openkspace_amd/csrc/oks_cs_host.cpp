@@ -59,10 +59,11 @@ static long long cs_chunk(const oks_ctx* ctx, long long planes) {
     return ctx->cs_chunk > 0 ? std::min<long long>(ctx->cs_chunk, planes) : planes;
 }
 
-// 8-byte slots of CsProblem::bits.
+// 8-byte slots of CsProblem::bits: column offsets, mask words, and the
+// register column pass's task sampling table (N1 entries per column).
 static size_t cs_bits_slots(long long mask_planes, int ny, int nx) {
     const size_t cols = (size_t)mask_planes * nx;
-    return cols + 1 + cols * (size_t)((ny + 63) / 64);
+    return cols + 1 + cols * (size_t)((ny + 63) / 64) + (cs_reg_ok(ny) ? cols * (size_t)cs_reg_n1(ny) : 0);
 }
 
 static void cs_iterations(oks_ctx* ctx, const CsProblem& P, const std::vector<float>& alpha) {
@@ -72,14 +73,25 @@ static void cs_iterations(oks_ctx* ctx, const CsProblem& P, const std::vector<fl
     const long long cols = (long long)P.mask_planes * P.nx;
     long long* base = P.bits;
     unsigned long long* words = (unsigned long long*)(P.bits + cols + 1);
+    uint2* tb = (uint2*)(P.bits + cols + 1 + cols * kw);
+    const bool reg = ctx->cs_reg && cs_reg_ok(P.ny);
     // mask words, per-column offsets and the compact measured data, once per
-    // call (T is free until the first row pass and holds the column-major y)
+    // call (T is free until the first row pass and holds the column-major y);
+    // the register column pass takes the sampling bits and the measured data
+    // in its task order
     OKS_HIP(launch_transpose_u8(P.mask, P.mT, P.mask_planes, P.ny, P.nx, ctx->stream));
     OKS_HIP(launch_cs_maskbits(P.mT, words, base, cols, P.ny, kw, ctx->stream));
     OKS_HIP(launch_cs_scan(base, cols, ctx->stream));
+    if (reg) OKS_HIP(launch_cs_taskbits(P.mT, tb, P.mask_planes, P.ny, P.nx, ctx->stream));
     OKS_HIP(launch_transpose_c64(P.y, P.T, P.planes, P.ny, P.nx, ctx->stream));
-    OKS_HIP(launch_cs_compact(P.T, words, base, P.yC, P.planes, P.mask_planes, P.ny, P.nx, kw, ctx->stream));
+    if (reg) {
+        OKS_HIP(launch_cs_compact_task(P.T, tb, base, P.yC, P.planes, P.mask_planes, P.ny, P.nx, ctx->stream));
+    } else {
+        OKS_HIP(launch_cs_compact(P.T, words, base, P.yC, P.planes, P.mask_planes, P.ny, P.nx, kw, ctx->stream));
+    }
     CsColArgs ca{};
+    ca.reg = reg ? 1 : 0;
+    ca.tb = tb;
     ca.yC = P.yC;
     ca.mB = words;
     ca.base = base;
@@ -133,7 +145,7 @@ static void cs_iterations(oks_ctx* ctx, const CsProblem& P, const std::vector<fl
 static void run_fista(oks_ctx* ctx, const CsProblem& P) {
     TuneScope tuned(ctx, "cs", {P.ny, P.nx});
     const float sx_ = 1.0f / std::sqrt((float)P.nx), sy_ = 1.0f / std::sqrt((float)P.ny);
-    if (cs_cols_lds(cs_sx(ctx, P.ny), P.ny, ctx->cs_cols_map) > 160 * 1024 ||
+    if ((!(ctx->cs_reg && cs_reg_ok(P.ny)) && cs_cols_lds(cs_sx(ctx, P.ny), P.ny, ctx->cs_cols_map) > 160 * 1024) ||
         cs_upd_lds(P.nx, ctx->cs_upd_map) > 160 * 1024)
         fail(OKS_E_UNSUPPORTED, "image too large for the CS passes");
     // plans and attributes exist before any capture (no allocation inside a graph)
@@ -171,7 +183,7 @@ static void run_fista(oks_ctx* ctx, const CsProblem& P) {
     uint32_t lb;
     std::memcpy(&lb, &P.lambda, 4);
     key.i[5] = lb;
-    key.i[6] = ((long long)ctx->cs_coop << 52) | ((long long)ctx->cs_pipe << 48) | ((long long)ctx->cs_upd_map << 40) | ((long long)ctx->cs_cols_map << 32) |
+    key.i[6] = ((long long)ctx->cs_reg << 56) | ((long long)ctx->cs_coop << 52) | ((long long)ctx->cs_pipe << 48) | ((long long)ctx->cs_upd_map << 40) | ((long long)ctx->cs_cols_map << 32) |
                ((long long)ctx->cs_sx << 16) | ctx->cs_threads;
     key.i[7] = ctx->cs_chunk;
     hipGraphExec_t exec = ctx->graph_for("cs", key);

@@ -196,23 +196,46 @@ def test_cs_chunked_planes_match(ctx, iters):
 
 @pytest.mark.parametrize("ny,nx", [(256, 256), (320, 320), (384, 384), (368, 640), (640, 368), (160, 96)])
 def test_launch_forms_bit_identical(ny, nx):
-    # The persistent prefetching column pass, the LDS slot maps and the strip
-    # widths only move time: every form gives the default's bits.
+    # Two column-pass families: the register four-step transforms (default)
+    # and the LDS Stockham transforms (cs_reg = 0).  Within a family the
+    # persistent prefetching column pass, the LDS slot maps, the strip widths
+    # and the workgroup sizes only move time: every form gives the family
+    # default's bits.
     from openkspace_amd._lib import Context
     case = synth.make_case(nc=3, nz=2, ny=ny, nx=nx, kind="cs", seed=ny + nx, ellipses=4)
     d = abi.CsDesc.default(iters=7)
-    forms = [{}, {"cs_coop": 1}, {"cs_pipe": 1}, {"cs_pipe": 1, "cs_cols_map": 1}, {"cs_cols_map": 2, "cs_upd_map": 1},
-             {"cs_upd_map": 2}, {"cs_sx": 8}, {"cs_threads": 1024}]
-    imgs = []
-    for opts in forms:
-        c = Context(0)
-        try:
-            c.set_option("tuning", 0)
-            for k, v in opts.items():
-                c.set_option(k, v)
-            got, _, _ = recon_dev(c, "oks_recon_cs_dev", d, case)
-            imgs.append(got)
-        finally:
-            c.close()
-    for opts, img in zip(forms[1:], imgs[1:]):
-        assert np.array_equal(img, imgs[0]), opts
+    families = [
+        [{}, {"cs_pipe": 1}, {"cs_upd_map": 1}, {"cs_upd_map": 2}, {"cs_sx": 8}, {"cs_threads": 1024}],
+        [{"cs_reg": 0}, {"cs_reg": 0, "cs_coop": 1}, {"cs_reg": 0, "cs_pipe": 1},
+         {"cs_reg": 0, "cs_pipe": 1, "cs_cols_map": 1}, {"cs_reg": 0, "cs_cols_map": 2, "cs_upd_map": 1},
+         {"cs_reg": 0, "cs_upd_map": 2}, {"cs_reg": 0, "cs_sx": 8}, {"cs_reg": 0, "cs_threads": 1024}],
+    ]
+    for forms in families:
+        imgs = []
+        for opts in forms:
+            c = Context(0)
+            try:
+                c.set_option("tuning", 0)
+                for k, v in opts.items():
+                    c.set_option(k, v)
+                got, _, _ = recon_dev(c, "oks_recon_cs_dev", d, case)
+                imgs.append(got)
+            finally:
+                c.close()
+        for opts, img in zip(forms[1:], imgs[1:]):
+            assert np.array_equal(img, imgs[0]), opts
+
+
+@pytest.mark.parametrize("ny,nx", [(320, 320), (368, 640), (64, 48)])
+def test_lds_column_pass_vs_oracle(ctx, ny, nx):
+    # the LDS Stockham column pass (cs_reg = 0) keeps its own parity
+    case = synth.make_case(nc=1, nz=1, ny=ny, nx=nx, kind="cs", seed=ny * 3 + nx, ellipses=6)
+    kzf, mask = case["kspace"][0, 0], case["mask"][0]
+    ctx.set_option("cs_reg", 0)
+    try:
+        got = gpu_fista(ctx, kzf, mask, 30, 0.01)[0]
+    finally:
+        ctx.set_option("cs_reg", 1)
+    _, r32 = O.fista_cs(kzf, mask, 30, 0.01, "f32")
+    _, r64 = O.fista_cs(kzf, mask, 30, 0.01, "f64")
+    assert_parity(got, r32, r64, TOL, f"fista lds {ny}x{nx}", iterative=True)
