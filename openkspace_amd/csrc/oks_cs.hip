@@ -344,12 +344,11 @@ __global__ void __launch_bounds__(kCsPipeThreads) k_cs_cols_pipe(CsColArgs a) {
 template <int N>
 constexpr int cs_reg_threads() { return RegGeom<N, 16>::NT; }
 
-// PIPE: persistent, workgroup b walks the strips b, b + gridDim.x, ...; the
-// next strip's phase-A samples and sampling words are fetched into registers
-// right after this strip's phase A has consumed them, so their latency hides
-// behind phases B and C (the barriers order LDS only).  Otherwise one strip
-// per workgroup.  Both forms compute the same bits.
-template <int N, bool PIPE>
+// One strip per workgroup.  The sampling word comes first, then the strip's
+// phase-A samples, then (once the word is in) the measured samples of phase
+// B, so every global load of the strip is in flight before phase A's
+// transform starts.
+template <int N>
 __global__ void __launch_bounds__(cs_reg_threads<N>()) k_cs_cols_reg(CsColArgs a) {
     using G = RegGeom<N, 16>;
     constexpr int N1 = G::N1, N2 = G::N2, SX = G::SX, N2P = G::N2P, CS = G::CS, NT = G::NT;
@@ -360,110 +359,75 @@ __global__ void __launch_bounds__(cs_reg_threads<N>()) k_cs_cols_reg(CsColArgs a
     const int t = threadIdx.x;
     for (int i = t; i < N; i += NT) tw[i] = a.p.tw[i];
     const int strips = (a.nx + SX - 1) / SX;
-    const long long total = a.planes * strips;
-    const int col0 = t % SX, grp0 = t / SX;
+    const long long plane = blockIdx.x / strips;
+    const int xs = (int)(blockIdx.x - plane * strips) * SX;
+    const int col = t % SX, grp = t / SX;
+    const bool cin = xs + col < a.nx;
+    const int z = (int)((a.plane0 + plane) % a.nz);
+    const long long coil = (a.plane0 + plane) / a.nz;
+    const long long zx = (long long)z * a.nx + (cin ? xs + col : 0);
+    float2* Lc = L + col * CS;
+    float2* T = a.T + (size_t)plane * N * a.nx + xs + col;
+    // the sampling word of task (col, k1 = grp)
+    const uint2 tb = (grp < N1 && cin) ? a.tb[zx * N1 + grp] : make_uint2(0u, 0u);
+    // A's samples x[N2 n1 + n2], n1 < N1, of task (col, n2 = grp)
     float2 va[N1];
-    uint2 tb = make_uint2(0u, 0u);
-    // strip st: x[N2 n1 + n2] for n1 < N1 of task (col, n2 = grp), and the
-    // sampling word of task (col, k1 = grp).  The row offsets come from an
-    // opaque copy of nx, so that the persistent form forms them again in
-    // every strip instead of holding them across the loop.
-    auto fetch = [&](long long st, int col, int grp) {
-        const long long plane = st / strips;
-        const int xs = (int)(st - plane * strips) * SX;
-        const bool cin = xs + col < a.nx;
-        const long long zx = (long long)((a.plane0 + plane) % a.nz) * a.nx + (cin ? xs + col : 0);
-        tb = (grp < N1 && cin) ? a.tb[zx * N1 + grp] : make_uint2(0u, 0u);
-        const float2* T = a.T + (size_t)plane * N * a.nx + xs + col;
-        int nxo = a.nx;
-        asm volatile("" : "+s"(nxo));
-        if (grp < N2) {
+    if (grp < N2) {
 #pragma unroll
-            for (int n1 = 0; n1 < N1; ++n1) {
-                const int y = wrap_n(N2 * n1 + grp + half, N);
-                va[n1] = cin ? T[(unsigned)(y * nxo)] : make_float2(0.f, 0.f);
-            }
+        for (int n1 = 0; n1 < N1; ++n1) {
+            const int y = wrap_n(N2 * n1 + grp + half, N);
+            va[n1] = cin ? T[(unsigned)(y * a.nx)] : make_float2(0.f, 0.f);
         }
-    };
-    auto strip = [&](long long st, int col, int grp) {
-        float2* Lc = L + col * CS;
-        const long long plane = st / strips;
-        const int xs = (int)(st - plane * strips) * SX;
-        const bool cin = xs + col < a.nx;
-        const int z = (int)((a.plane0 + plane) % a.nz);
-        const long long coil = (a.plane0 + plane) / a.nz;
-        const long long zx = (long long)z * a.nx + (cin ? xs + col : 0);
-        const uint2 tbc = tb;
-        // A: N1-point DFT, output k1 times W_N^{-n2 k1} into row k1
-        // (n2 k1 <= (N1 - 1)(N2 - 1) < N)
-        if (grp < N2) {
-            const int n2 = grp;
-            rdft_emit<N1, false>(va, [&](int k1, float2 v) {
-                Lc[k1 * N2P + n2] = k1 == 0 ? v : cmul(v, twid<false>(tw, n2 * k1));
-            });
+    }
+    // B's measured samples of the task's sampled slots, in slot order
+    float2 yv[N2];
+    if (grp < N1) {
+        const float2* YC = a.yC + coil * a.base[(long long)a.nz * a.nx] + a.base[zx] + tb.y;
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) {
+            const int r = __builtin_popcount(tb.x & ((1u << k2) - 1u));
+            yv[k2] = ((tb.x >> k2) & 1u) ? YC[r] : make_float2(0.f, 0.f);
         }
-        if constexpr (PIPE) {
-            __builtin_amdgcn_sched_barrier(0);
-            if (st + gridDim.x < total) fetch(st + gridDim.x, col, grp);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        lds_barrier();
-        // B: N2-point DFT, masked residual on slots k1 + N1 k2, N2-point
-        // inverse DFT, twiddles W_N^{m2 k1}
-        if (grp < N1) {
-            const int k1 = grp;
-            float2* row = Lc + k1 * N2P;
-            float2 v[N2];
-#pragma unroll
-            for (int j = 0; j < N2; ++j) v[j] = row[j];
-            // the measured samples of this task's sampled slots, in slot order
-            float2 yv[N2];
-            const float2* YC = a.yC + coil * a.base[(long long)a.nz * a.nx] + a.base[zx] + tbc.y;
-#pragma unroll
-            for (int k2 = 0; k2 < N2; ++k2) {
-                const int r = __builtin_popcount(tbc.x & ((1u << k2) - 1u));
-                yv[k2] = ((tbc.x >> k2) & 1u) ? YC[r] : make_float2(0.f, 0.f);
-            }
-            rdft<N2, false>(v);
-#pragma unroll
-            for (int k2 = 0; k2 < N2; ++k2)
-                v[k2] = ((tbc.x >> k2) & 1u) ? csub(cscale(v[k2], a.scale), yv[k2]) : make_float2(0.f, 0.f);
-            rdft<N2, true>(v);
-#pragma unroll
-            for (int m2 = 1; m2 < N2; ++m2) v[m2] = cmul(v[m2], twid<true>(tw, m2 * k1));
-#pragma unroll
-            for (int j = 0; j < N2; ++j) row[j] = v[j];
-        }
-        lds_barrier();
-        // C: N1-point inverse DFT of entry m2 of every row, into T
-        if (grp < N2 && cin) {
-            const int m2 = grp;
-            float2* T = a.T + (size_t)plane * N * a.nx + xs + col;
-            int nxo = a.nx;
-            asm volatile("" : "+s"(nxo));
-            float2 v[N1];
-#pragma unroll
-            for (int k1 = 0; k1 < N1; ++k1) v[k1] = Lc[k1 * N2P + m2];
-            rdft_emit<N1, true>(v, [&](int m1, float2 r) {
-                const int yo = wrap_n(m2 + N2 * m1 + half, N);
-                T[(unsigned)(yo * nxo)] = cscale(r, a.scale);
-            });
-        }
-    };
-    long long st = blockIdx.x;
-    if (st < total) fetch(st, col0, grp0);
+    }
     lds_barrier();  // the twiddle table
-    if constexpr (PIPE) {
-        for (; st < total; st += gridDim.x) {
-            // task index and column, opaque per strip: every address derived
-            // from them is formed again in each strip
-            int grp = grp0, col = col0;
-            asm volatile("" : "+v"(grp), "+v"(col));
-            strip(st, col, grp);
-            lds_barrier();  // C's reads before the next strip's A writes
-        }
-    } else {
-        if (st < total) strip(st, col0, grp0);
+    // A: N1-point DFT, output k1 times W_N^{-n2 k1} into row k1
+    // (n2 k1 <= (N1 - 1)(N2 - 1) < N)
+    if (grp < N2) {
+        const int n2 = grp;
+        rdft_emit<N1, false>(va, [&](int k1, float2 v) {
+            Lc[k1 * N2P + n2] = k1 == 0 ? v : cmul(v, twid<false>(tw, n2 * k1));
+        });
+    }
+    lds_barrier();
+    // B: N2-point DFT, masked residual on slots k1 + N1 k2, N2-point inverse
+    // DFT, twiddles W_N^{m2 k1}
+    if (grp < N1) {
+        const int k1 = grp;
+        float2* row = Lc + k1 * N2P;
+        float2 v[N2];
+#pragma unroll
+        for (int j = 0; j < N2; ++j) v[j] = row[j];
+        rdft<N2, false>(v);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2)
+            v[k2] = ((tb.x >> k2) & 1u) ? csub(cscale(v[k2], a.scale), yv[k2]) : make_float2(0.f, 0.f);
+        rdft<N2, true>(v);
+#pragma unroll
+        for (int m2 = 1; m2 < N2; ++m2) v[m2] = cmul(v[m2], twid<true>(tw, m2 * k1));
+#pragma unroll
+        for (int j = 0; j < N2; ++j) row[j] = v[j];
+    }
+    lds_barrier();
+    // C: N1-point inverse DFT of entry m2 of every row, into T
+    if (grp < N2 && cin) {
+        const int m2 = grp;
+        float2 v[N1];
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) v[k1] = Lc[k1 * N2P + m2];
+        rdft_emit<N1, true>(v, [&](int m1, float2 r) {
+            const int yo = wrap_n(m2 + N2 * m1 + half, N);
+            T[(unsigned)(yo * a.nx)] = cscale(r, a.scale);
+        });
     }
 }
 
@@ -838,29 +802,16 @@ static hipError_t go_cs_cols_coop(const CsColArgs& a, hipStream_t s) {
     }
 }
 
-template <int N, bool PIPE>
-static hipError_t go_cs_cols_reg_form(const CsColArgs& a, hipStream_t s) {
-    using G = RegGeom<N, 16>;
-    const void* k = (const void*)k_cs_cols_reg<N, PIPE>;
-    OKS_ATTR2(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)G::lds_bytes));
-    const long long strips = a.planes * ((a.nx + G::SX - 1) / G::SX);
-    long long grid = strips;
-    if (PIPE) {
-        int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, G::NT, G::lds_bytes) != hipSuccess || per_cu < 1)
-            per_cu = 1;
-        grid = std::min<long long>(strips, 256LL * per_cu);
-    }
-    if (grid < 1 || grid > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_cs_cols_reg<N, PIPE>), dim3((unsigned)grid), dim3(G::NT), G::lds_bytes, s, a);
-    return hipGetLastError();
-}
-
 template <int N>
 static hipError_t go_cs_cols_reg(const CsColArgs& a, hipStream_t s) {
+    using G = RegGeom<N, 16>;
     if (!a.tb) return hipErrorInvalidValue;
-    // (the persistent form of 640 would spill: it runs one strip per workgroup)
-    return a.pipe && N != 640 ? go_cs_cols_reg_form<N, true>(a, s) : go_cs_cols_reg_form<N, false>(a, s);
+    OKS_ATTR2(hipFuncSetAttribute((const void*)k_cs_cols_reg<N>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)G::lds_bytes));
+    const long long grid = a.planes * ((a.nx + G::SX - 1) / G::SX);
+    if (grid < 1 || grid > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_cs_cols_reg<N>, dim3((unsigned)grid), dim3(G::NT), G::lds_bytes, s, a);
+    return hipGetLastError();
 }
 
 template <int N>

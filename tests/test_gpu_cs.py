@@ -205,7 +205,8 @@ def test_launch_forms_bit_identical(ny, nx):
     case = synth.make_case(nc=3, nz=2, ny=ny, nx=nx, kind="cs", seed=ny + nx, ellipses=4)
     d = abi.CsDesc.default(iters=7)
     families = [
-        [{}, {"cs_pipe": 1}, {"cs_upd_map": 1}, {"cs_upd_map": 2}, {"cs_sx": 8}, {"cs_threads": 1024}],
+        [{}, {"cs_pipe": 1}, {"cs_coop": 1}, {"cs_upd_map": 1}, {"cs_upd_map": 2}, {"cs_sx": 8},
+         {"cs_threads": 1024}],
         [{"cs_reg": 0}, {"cs_reg": 0, "cs_coop": 1}, {"cs_reg": 0, "cs_pipe": 1},
          {"cs_reg": 0, "cs_pipe": 1, "cs_cols_map": 1}, {"cs_reg": 0, "cs_cols_map": 2, "cs_upd_map": 1},
          {"cs_reg": 0, "cs_upd_map": 2}, {"cs_reg": 0, "cs_sx": 8}, {"cs_reg": 0, "cs_threads": 1024}],
