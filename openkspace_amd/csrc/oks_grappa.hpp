@@ -32,6 +32,25 @@ struct ApplyArgs {
     long long wz;  // weight stride between slices (0: one kernel for every slice)
 };
 
+// Geometry of the grouped synthesis (oks_grappa_synth.hip): coils rounded up
+// to ncp (16 or 32), the (c', a) columns nc kky rounded up to ca (a multiple
+// of 32), passes of 3 target rows over R - 1.
+struct SynthGeom {
+    int nc, r, kky, kkx, ncp, ca, passes;
+};
+
+struct SynthArgs {
+    float2* grid;        // [nc, nz, ny, nx]; the missing rows are written
+    const uint2* gs;     // the grid split into bf16 hi/lo (its sampled rows are the sources)
+    const uint4* wp;     // weights in the B-operand layout (launch_grappa_pack_w)
+    long long wpz;       // packed-weight stride between slices (0: one kernel for every slice)
+    const int2* groups;  // [nz][ny] {ky0, target-row bits} (launch_grappa_groups)
+    const int* gcnt;     // [nz]
+    const int* skip;     // [nz] calibration failed -> leave the slice undersampled, or null
+    int nz, ny, nx, n_kx;
+    SynthGeom g;
+};
+
 struct Zgemm64Args {
     const double2* A;
     const double2* B;
@@ -69,6 +88,14 @@ hipError_t launch_split_bf16(const float2* in, uint2* out, long long n, hipStrea
 hipError_t launch_grappa_fills(const GrappaFillArgs& a, hipStream_t s);
 hipError_t launch_calib_gemm(const CalibArgs& a, hipStream_t s);
 hipError_t launch_apply_gemm(const ApplyArgs& a, hipStream_t s);
+SynthGeom synth_geom(int nc, int r, int kky, int kkx);
+size_t synth_lds(const SynthGeom& g);
+bool synth_ok(const SynthGeom& g, int ny);
+size_t synth_wpack_entries(const SynthGeom& g, int nzw);  // uint4 entries of the packed weights
+hipError_t launch_grappa_groups(const int2* list, const int* cnt, int2* groups, int* gcnt, int nz, int ny, int r,
+                                hipStream_t s);
+hipError_t launch_grappa_pack_w(const float2* w, uint4* wp, int nzw, long long wz, const SynthGeom& g, hipStream_t s);
+hipError_t launch_grappa_synth(const SynthArgs& a, hipStream_t s);
 hipError_t launch_zgemm64(const Zgemm64Args& g, int batch, hipStream_t s);
 hipError_t launch_normal64(const Normal64Args& a, int nz, hipStream_t s);
 hipError_t launch_potrf64_diag(const Potrf64Args& a, int nz, hipStream_t s);

@@ -167,22 +167,48 @@ static void grappa_calibrate_batch(oks_ctx* ctx, const float2* grid, const uint2
     if (launches) *launches = cnt + 1;
 }
 
-// Fill the missing rows of every slice of `grid` in place.
+// Fill the missing rows of every slice of `grid` in place.  Default: the
+// grouped synthesis (one workgroup per source row group and kx tile, the
+// source tile resident in LDS, weights packed in the MFMA operand layout);
+// option "grappa_synth" = 0, or a geometry it does not take, runs one
+// workgroup per filled row and kx tile instead.
 static void grappa_apply_batch(oks_ctx* ctx, float2* grid, const uint2* gs, int nc, int nz, int ny, int nx, int r,
                                int kky, int kkx, const float2* w, long long wz, const int* skip, const GrGeom& g) {
     if (!gs) gs = split_source(ctx, grid, (size_t)nc * nz * ny * nx, "gr_split");
-    // the weights split once (one copy per slice, or one for every slice when wz = 0)
-    const uint2* ws = split_source(ctx, w, (size_t)(wz ? wz * nz : (long long)(r - 1) * nc * g.n_src), "gr_wsplit");
     int2* list = (int2*)ctx->buf("gr_list", sizeof(int2) * (size_t)nz * (r - 1) * ny);
     int* cnt = (int*)ctx->buf("gr_cnt", sizeof(int) * (size_t)nz * (r - 1));
     unsigned char* sampled = (unsigned char*)ctx->buf("gr_sampled", (size_t)nz * ny);
+    const SynthGeom sg = synth_geom(nc, r, kky, kkx);
+    const bool grouped = ctx->grappa_synth && synth_ok(sg, ny);
+    uint4* wp = nullptr;
+    if (grouped) {
+        // the weights in the B-operand layout (one copy per slice, or one for every slice when wz = 0)
+        const int nzw = wz ? nz : 1;
+        wp = (uint4*)ctx->buf("gr_wpack", sizeof(uint4) * synth_wpack_entries(sg, nzw));
+        OKS_HIP(launch_grappa_pack_w(w, wp, nzw, wz, sg, ctx->stream));
+    }
     GrappaFillArgs fa{grid, list, cnt, sampled, nz, ny, nx, r, kky};
     OKS_HIP(launch_grappa_fills(fa, ctx->stream));
-    ApplyArgs aa{grid, gs, ws, list, cnt, skip, nc, nz, ny, nx, r, kky, kkx, nx - kkx + 1, g.n_src, wz};
+    int2* groups = nullptr;
+    int* gcnt = nullptr;
+    if (grouped) {
+        groups = (int2*)ctx->buf("gr_groups", sizeof(int2) * (size_t)nz * ny);
+        gcnt = (int*)ctx->buf("gr_gcnt", sizeof(int) * (size_t)nz);
+        OKS_HIP(launch_grappa_groups(list, cnt, groups, gcnt, nz, ny, r, ctx->stream));
+    }
     // flops: 8 per complex MAC, nc x n_src MACs per filled point, (nx-kkx+1)
     // points per filled row; the filled rows are counted on the device
     StageTimer timer(ctx, OKS_STAGE_GRAPPA_APPLY, 0.0, cnt, nz * (r - 1), 8.0 * (nx - kkx + 1) * nc * g.n_src);
-    OKS_HIP(launch_apply_gemm(aa, ctx->stream));
+    if (grouped) {
+        SynthArgs sa{grid, gs, wp, wz ? (long long)synth_wpack_entries(sg, 1) : 0LL, groups, gcnt, skip, nz, ny, nx,
+                     nx - kkx + 1, sg};
+        OKS_HIP(launch_grappa_synth(sa, ctx->stream));
+    } else {
+        // the weights split once (one copy per slice, or one for every slice when wz = 0)
+        const uint2* ws = split_source(ctx, w, (size_t)(wz ? wz * nz : (long long)(r - 1) * nc * g.n_src), "gr_wsplit");
+        ApplyArgs aa{grid, gs, ws, list, cnt, skip, nc, nz, ny, nx, r, kky, kkx, nx - kkx + 1, g.n_src, wz};
+        OKS_HIP(launch_apply_gemm(aa, ctx->stream));
+    }
 }
 
 // GrappaRss::reconstruct steps 3-4 (recon/strategy.rs:349-399) on the

@@ -168,3 +168,40 @@ def test_grappa_c2_shape(ctx):
     got, flags, _ = recon_dev(ctx, "oks_recon_grappa_dev", d, case)
     assert st == 0 and flags == rflags and got.shape == (2, 384, 384)
     assert_parity(got, r32, r64, TOL, "C2 grappa")
+
+
+@pytest.mark.parametrize("nc,r,kky,kkx", [(32, 4, 4, 5), (15, 4, 4, 5), (17, 3, 2, 3), (4, 8, 2, 7), (1, 6, 4, 1),
+                                          (9, 5, 4, 5)])
+def test_synthesis_forms(nc, r, kky, kkx):
+    # The grouped synthesis (default: one workgroup per source row group, the
+    # source tile resident in LDS, three target rows per pass) and the
+    # row-per-workgroup form (option grappa_synth = 0) fill the same rows and
+    # leave every other sample bit-equal; both are within the split-BF16 bar
+    # of the FP64 oracle.  Covers both coil paddings (16, 32), several passes
+    # (R = 5, 6, 8) and single- and wide-tap kernels.
+    from openkspace_amd._lib import Context
+    rng = np.random.default_rng(nc * 100 + r * 10 + kkx)
+    ny, nx, nz = 8 * r + 40, 70, 2
+    k = (rng.standard_normal((nc, nz, ny, nx)) + 1j * rng.standard_normal((nc, nz, ny, nx))).astype(np.complex64)
+    rows = (np.arange(ny) % r == 1)
+    k[:, :, ~rows, :] = 0
+    n_src = nc * kky * kkx
+    w = ((rng.standard_normal((r - 1, nc, n_src)) + 1j * rng.standard_normal((r - 1, nc, n_src)))
+         / np.sqrt(n_src)).astype(np.complex64)
+    outs = []
+    for form in (1, 0):
+        c = Context(0)
+        try:
+            c.set_option("grappa_synth", form)
+            outs.append(gpu_synthesize(c, w, k, r, kky, kkx))
+        finally:
+            c.close()
+    grouped, rowwise = outs
+    want = O.grappa_synthesize(w, k, r, kky, kkx, "f64")
+    filled = np.any(want != k, axis=(0, 3))  # [nz, ny] rows the reference writes
+    keep = np.broadcast_to(~filled[None, :, :, None], k.shape)
+    for got in (grouped, rowwise):
+        assert np.array_equal(got[keep], k[keep])
+        assert rel(got, want) < TOL
+    assert filled.any()
+    assert rel(grouped, rowwise) < 1e-4
