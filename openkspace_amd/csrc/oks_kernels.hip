@@ -13,6 +13,7 @@
 
 #include "oks_fft.hpp"
 #include "oks_kernels.hpp"
+#include "oks_regfft.hpp"
 
 namespace oks {
 
@@ -735,6 +736,105 @@ __global__ void __launch_bounds__(512) k_cols(ColArgs a) {
     }
 }
 
+// Register four-step form of k_cols for the RSS epilogue (mode 1, inverse,
+// compile-time n = N1 N2 with reg_fft_ok): one workgroup per (coil group, z,
+// strip of 16 columns), coils ascending.  Per coil, phase A (task (column,
+// n2)) reads x[N2 n1 + n2] from the grid (the ifftshift folded into the rows
+// read), runs the N1-point inverse DFT in registers and writes row k1 of the
+// column's LDS block times W_n^{n2 k1}; phase B (task (column, k1)) runs the
+// N2-point inverse DFT of row k1, whose outputs are the slots k1 + N1 k2, and
+// adds |.|^2 to its per-slot sums in registers.  The next coil's samples are
+// fetched as soon as phase A has consumed the current ones, so they are in
+// flight through phase B.  The sums go out as in k_cols: sqrt(sum) * scale,
+// or with G > 1 the group's partial sums.
+template <int N>
+constexpr int cols_reg_threads() { return RegGeom<N, 16>::NT; }
+
+template <int N>
+__global__ void __launch_bounds__(cols_reg_threads<N>()) k_cols_rss_reg(ColArgs a) {
+    using Gm = RegGeom<N, 16>;
+    constexpr int N1 = Gm::N1, N2 = Gm::N2, SX = Gm::SX, N2P = Gm::N2P, CS = Gm::CS, NT = Gm::NT;
+    extern __shared__ float2 lds[];
+    float2* tw = lds;  // exp(+2 pi i e / N)
+    float2* L = lds + Gm::TW;
+    const int t = threadIdx.x;
+    for (int i = t; i < N; i += NT) tw[i] = a.p.tw[i];
+    const int strips = (a.X + SX - 1) / SX;
+    const int per_g = a.Zo * strips;
+    const int g = blockIdx.x / per_g;
+    const int rem = blockIdx.x - g * per_g;
+    const int zi = rem / strips;
+    const int z = a.z0 + zi;
+    const int xs = (rem - zi * strips) * SX;
+    const int c0 = g * a.C / a.G, c1 = (g + 1) * a.C / a.G;
+    const int half = a.centred ? N / 2 : 0;
+    const int col0 = t % SX, grp0 = t / SX;
+    float2 va[N1];
+    // The row offsets come from an opaque copy of X, and the task indices are
+    // made opaque in every coil, so that the addresses derived from them are
+    // formed again per coil instead of held in registers across the loop.
+    auto fetch = [&](int c, int col, int grp) {
+        const float2* src = a.in + ((size_t)c * a.Z + z) * N * a.X + xs + col;
+        int xo = a.X;
+        asm volatile("" : "+s"(xo));
+        if (grp < N2) {
+            const bool cin = xs + col < a.X;
+#pragma unroll
+            for (int n1 = 0; n1 < N1; ++n1) {
+                const int y = wrap_n(N2 * n1 + grp + half, N);
+                va[n1] = cin ? src[(unsigned)(y * xo)] : make_float2(0.f, 0.f);
+            }
+        }
+    };
+    float acc[N2];
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) acc[k2] = 0.f;
+    if (c0 < c1) fetch(c0, col0, grp0);
+    lds_barrier();  // the twiddle table
+    for (int c = c0; c < c1; ++c) {
+        int grp = grp0, col = col0;
+        asm volatile("" : "+v"(grp), "+v"(col));
+        float2* Lc = L + col * CS;
+        if (grp < N2) {
+            const int n2 = grp;
+            // (n2 k1 <= (N1 - 1)(N2 - 1) < N)
+            rdft_emit<N1, true>(va, [&](int k1, float2 v) {
+                Lc[k1 * N2P + n2] = k1 == 0 ? v : cmul(v, twid<true>(tw, n2 * k1));
+            });
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (c + 1 < c1) fetch(c + 1, col, grp);
+        __builtin_amdgcn_sched_barrier(0);
+        lds_barrier();
+        if (grp < N1) {
+            const float2* row = Lc + grp * N2P;
+            float2 v[N2];
+#pragma unroll
+            for (int j = 0; j < N2; ++j) v[j] = row[j];
+            rdft<N2, true>(v);
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) acc[k2] += v[k2].x * v[k2].x + v[k2].y * v[k2].y;
+        }
+        lds_barrier();  // B's reads before the next coil's A writes
+    }
+    const int col = col0, grp = grp0;
+    if (grp < N1 && xs + col < a.X) {
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) {
+            // slot k1 + N1 k2 holds the centred output row yo with
+            // slot = (y0 + yo + half) mod N
+            int yo = grp + N1 * k2 - a.y0 - half;
+            yo = ((yo % N) + N) % N;
+            if (yo < a.ty) {
+                if (a.G == 1)
+                    a.out_r[((size_t)zi * a.ty + yo) * a.X + xs + col] = sqrtf(acc[k2]) * a.scale;
+                else
+                    a.part[(((size_t)g * a.Zo + zi) * a.ty + yo) * a.X + xs + col] = acc[k2];
+            }
+        }
+    }
+}
+
 // out = sqrt(sum_g part[g]) * scale, groups summed in ascending order.
 __global__ void k_rss_finalize(const float* __restrict__ part, int G, long long M, float* __restrict__ out,
                                float scale) {
@@ -1021,8 +1121,37 @@ static hipError_t go_cols(const ColArgs& a, hipStream_t s) {
     return go_cols_k<F, false>(a, lds, grid, s);
 }
 
+template <int N>
+static hipError_t go_cols_reg(const ColArgs& a, hipStream_t s) {
+    using Gm = RegGeom<N, 16>;
+    OKS_ATTR(hipFuncSetAttribute((const void*)k_cols_rss_reg<N>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)Gm::lds_bytes));
+    const long long grid = (long long)a.G * a.Zo * ((a.X + Gm::SX - 1) / Gm::SX);
+    if (grid < 1 || grid > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_cols_rss_reg<N>, dim3((unsigned)grid), dim3(Gm::NT), Gm::lds_bytes, s, a);
+    return hipGetLastError();
+}
+
+bool cols_reg_ok(int n, int mode, bool inverse) {
+    if (mode != 1 || !inverse) return false;
+    switch (n) {
+        case 64: case 128: case 160: case 256: case 320: case 368: case 384: case 512: case 640: return true;
+        default: return false;
+    }
+}
+
 template <bool INV>
 static hipError_t go_cols_dir(const ColArgs& a, hipStream_t s) {
+    if (a.reg && cols_reg_ok(a.p.n, a.mode, INV)) {
+        switch (a.p.n) {
+#define OKS_CASE(N) \
+    case N: return go_cols_reg<N>(a, s);
+            OKS_CASE(64) OKS_CASE(128) OKS_CASE(160) OKS_CASE(256) OKS_CASE(320) OKS_CASE(368) OKS_CASE(384)
+            OKS_CASE(512) OKS_CASE(640)
+#undef OKS_CASE
+            default: break;
+        }
+    }
     switch (a.p.n) {
 #define OKS_CASE(N) \
     case N: return go_cols<FixedFft<N, INV>>(a, s);
@@ -1033,7 +1162,8 @@ static hipError_t go_cols_dir(const ColArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_cols(const ColArgs& a, bool inverse, hipStream_t s) {
-    if (cols_lds(a.sx, a.p.n, a.ty, a.mode) > 160 * 1024) return hipErrorInvalidValue;
+    if (!(a.reg && cols_reg_ok(a.p.n, a.mode, inverse)) && cols_lds(a.sx, a.p.n, a.ty, a.mode) > 160 * 1024)
+        return hipErrorInvalidValue;
     if (a.sx < 1 || (a.sx & (a.sx - 1)) != 0) return hipErrorInvalidValue;  // strip index maps use shifts
     if (a.G < 1 || a.G > a.C) return hipErrorInvalidValue;
     if (a.mode == 1 && a.G > 1 && !a.part) return hipErrorInvalidValue;

@@ -71,6 +71,7 @@ struct ColArgs {
     float scale;
     int pipe;           // prefetch the next coil's strip when a workgroup loops over coils (512 threads)
     int threads;        // workgroup size: 0 = 512, or 256
+    int reg;            // RSS epilogue with the register four-step transforms where cols_reg_ok
     FftPlan p;
 };
 
@@ -81,6 +82,9 @@ hipError_t launch_rows_prewhiten_ifft(const RowPwArgs& a, hipStream_t s);
 hipError_t launch_rows(const RowArgs& a, bool inverse, hipStream_t s);
 size_t cols_lds(int sx, int n, int ty, int mode);
 hipError_t launch_cols(const ColArgs& a, bool inverse, hipStream_t s);
+// The column pass runs k_cols_rss_reg (ColArgs::reg set): RSS epilogue,
+// inverse transform, a length with a register four-step factorisation.
+bool cols_reg_ok(int n, int mode, bool inverse);
 size_t rows_lds(int n);
 hipError_t launch_rss_finalize(const float* part, int G, long long M, float* out, float scale, hipStream_t s);
 hipError_t launch_prewhiten(const float2* in, float2* d, const float2* W, int nc, long long M, hipStream_t s);

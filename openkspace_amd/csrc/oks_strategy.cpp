@@ -62,7 +62,9 @@ void col_pass(oks_ctx* ctx, const float2* in, float2* out_c, float* out_r, int C
     a.sx = pick_sx(n, ty, mode, X);
     if (ctx->cols_sx > 0) a.sx = std::min(a.sx, ctx->cols_sx);
     a.threads = ctx->cols_threads;
-    if (cols_lds(a.sx, n, ty, mode) > 160 * 1024) fail(OKS_E_UNSUPPORTED, "transform length too large for LDS");
+    a.reg = ctx->cols_reg;
+    if (!(a.reg && cols_reg_ok(n, mode, inverse)) && cols_lds(a.sx, n, ty, mode) > 160 * 1024)
+        fail(OKS_E_UNSUPPORTED, "transform length too large for LDS");
     a.G = std::max(1, std::min(G, C));
     const long long M = (long long)Zo * ty * X;
     if (mode == 1 && a.G > 1) a.part = (float*)ctx->buf(slot ? slot : "rss_part", sizeof(float) * a.G * M);

@@ -214,3 +214,25 @@ def test_ifft_rss_column_pass_shapes(ctx):
     for name, bad in (("cols_sx", 3), ("cols_sx", 32), ("cols_threads", 128)):
         with pytest.raises(Exception):
             ctx.set_option(name, bad)
+
+
+@pytest.mark.parametrize("ny", [64, 160, 256, 320, 368, 384, 512, 640])
+def test_rss_column_pass_register_form(ctx, ny):
+    # The RSS column pass with register four-step transforms (default for
+    # these lengths) and the LDS Stockham form (option cols_reg = 0) both meet
+    # the bar and agree to FP32 rounding; 7 coils over the RSS coil groups.
+    case = _case(nc=7, nz=3, ny=ny, nx=48, os_x=2)
+    d = abi.IfftRssDesc.default(remove_oversampling=0)
+    st, ref32, rflags, _ = O.recon("ifft_rss", d, case, "f32")
+    _, ref64, _, _ = O.recon("ifft_rss", d, case, "f64")
+    outs = []
+    try:
+        for form in (1, 0):
+            ctx.set_option("cols_reg", form)
+            got, flags, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", d, case)
+            assert flags == rflags and got.shape == ref32.shape
+            assert_parity(got, ref32, ref64, TOL32, f"ny={ny} cols_reg={form}")
+            outs.append(got)
+    finally:
+        ctx.set_option("cols_reg", 1)
+    assert rel(outs[0], outs[1]) < 2e-6
