@@ -207,9 +207,10 @@ CS_BYTES_PER_PIXEL_ITER = 5 * 8 + 1
 # The stage whose launches each multi-slice workload's roofline is taken on.
 WORKLOAD_STAGE = {"c1": "ifft_rss", "c1os": "ifft_rss", "c4": "ifft_rss", "c2": "grappa_apply",
                   "c3": "cs_iterations"}
-STAGE_KERNELS = {"ifft_rss": "IFFT + crop + RSS tail: k_rows_pw* + k_cols (+ k_rss_finalize)",
-                 "grappa_apply": "GRAPPA synthesis GEMM k_zgemm_sbf16<ApplyLoader> (split-bf16 MFMA)",
-                 "cs_iterations": "FISTA iteration pair k_cs_cols + k_cs_update, one HIP graph per call"}
+STAGE_KERNELS = {"ifft_rss": "IFFT + crop + RSS tail: k_rows_pw* + k_cols_rss_reg (+ k_rss_finalize)",
+                 "grappa_apply": "GRAPPA synthesis GEMM k_grappa_synth (split-bf16 MFMA, one workgroup per "
+                                 "source row group and kx tile)",
+                 "cs_iterations": "FISTA iteration pair k_cs_cols_reg + k_cs_update, one HIP graph per call"}
 
 # Stages a roofline can be taken on: (oks_stage id, bound, unit, peak, scale).
 STAGES = {
@@ -737,7 +738,7 @@ def run_served(a):
     cs_gbs = step_cs_bytes / step_s / 1e9
     out["roofline"] = {"bound": "hbm", "achieved": cs_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": cs_gbs / HBM_PEAK_GBS, "traffic": step_cs_bytes * tr if tr else None,
-                       "kernel": "CS iteration kernels (k_cs_cols + k_cs_update), whole step",
+                       "kernel": "CS iteration kernels (k_cs_cols_reg + k_cs_update), whole step",
                        "basis": "the CS requests' algorithmic bytes per pass of the stream (41 B per "
                                 "pixel-iteration) over ms_per_step; the other strategies' work shares the step, "
                                 "so this is a lower bound on the CS kernels' own rate",
