@@ -43,6 +43,7 @@ sys.path.insert(0, ROOT)
 METRIC = "slices/sec + p50/p99 latency at stated rel-L2 vs FP64; served vs plain GPU"
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
+FP64_PEAK_TFLOPS = 78.6
 
 
 def _args():
@@ -192,8 +193,33 @@ def workload_c4():
     )
 
 
+def workload_c4e():
+    """C4 with a ky count divisible by R = 3 (258 instead of 256), so SenseRss
+    does not fall back: kz IFFT, ESPIRiT maps of every kz partition (kernel 6,
+    threshold 0.02, 30 iterations, 32 coils, recon/espirit.rs:52-282), the
+    aliased 2-D IFFTs and the per-pixel FP64 R x R unfold with g-factor
+    (recon/strategy.rs:542-733).  No oracle leg: the FP32 restatement's
+    ESPIRiT takes minutes per kz partition on the host; GPU parity of this
+    composition is tested at reduced sizes."""
+    from openkspace_amd import abi, synth
+    case = synth.make_config("c4_sense3d", ny=258)
+    nc, nz, ny, nx = case["kspace"].shape
+    desc = abi.SenseDesc.default(map_source=abi.MAPS_ESPIRIT, espirit_kernel=6, espirit_threshold=0.02,
+                                 espirit_iters=30, ridge=1e-4, compute_gfactor=1)
+    algo_bytes = nc * nz * ny * nx * 8 + nz * ny * nx * 4
+    return dict(
+        name="c4e_sense3d",
+        config={"workload": "c4e_sense3d", "volumes": 1, "coils": nc, "kx": nx, "ky": ny, "kz": nz, "R": 3,
+                "acs_lines": 32, "strategy": "SenseRss", "maps": "ESPIRiT (kernel 6, threshold 0.02, 30 it)",
+                "ridge": 1e-4, "gfactor": True, "path_taken": "maps + unfold (Ny % R == 0)"},
+        case=case, desc=desc, fn="oks_recon_sense_dev", oracle=None, slices=nz,
+        bound="hbm", algo=algo_bytes, kernel="SenseRss launch sequence (kz IFFT, ESPIRiT, 2-D IFFT, unfold)",
+        cpu_unit=None, dtype="c64/f32 (ESPIRiT and the R x R solves fp64)",
+    )
+
+
 WORKLOADS = {"c1": workload_c1, "c2": workload_c2, "c3": workload_c3, "c4": workload_c4, "c1h": workload_c1h,
-             "c1os": workload_c1os}
+             "c1os": workload_c1os, "c4e": workload_c4e}
 
 # HBM bytes the dominant kernels move per algorithmic byte, measured with the
 # FETCH_SIZE / WRITE_SIZE counters scaled by a copy kernel of known size
@@ -206,17 +232,20 @@ CS_BYTES_PER_PIXEL_ITER = 5 * 8 + 1
 
 # The stage whose launches each multi-slice workload's roofline is taken on.
 WORKLOAD_STAGE = {"c1": "ifft_rss", "c1os": "ifft_rss", "c4": "ifft_rss", "c2": "grappa_apply",
-                  "c3": "cs_iterations"}
+                  "c3": "cs_iterations", "c4e": "espirit"}
 STAGE_KERNELS = {"ifft_rss": "IFFT + crop + RSS tail: k_rows_pw* + k_cols_rss_reg (+ k_rss_finalize)",
                  "grappa_apply": "GRAPPA synthesis GEMM k_grappa_synth (split-bf16 MFMA, one workgroup per "
                                  "source row group and kx tile)",
-                 "cs_iterations": "FISTA iteration pair k_cs_cols_reg + k_cs_update, one HIP graph per call"}
+                 "cs_iterations": "FISTA iteration pair k_cs_cols_reg + k_cs_update, one HIP graph per call",
+                 "espirit": "ESPIRiT maps (k_espirit_gram, k_espirit_eig, k_espirit_q, k_espirit_voxel): FP64 "
+                            "vector arithmetic, against the FP64 peak"}
 
 # Stages a roofline can be taken on: (oks_stage id, bound, unit, peak, scale).
 STAGES = {
     "ifft_rss": (1, "hbm", "GB/s", HBM_PEAK_GBS, 1e9),
     "grappa_apply": (3, "mfma", "TFLOP/s", BF16_PEAK_TFLOPS, 1e12),
     "cs_iterations": (4, "hbm", "GB/s", HBM_PEAK_GBS, 1e9),
+    "espirit": (6, "mfma", "TFLOP/s", FP64_PEAK_TFLOPS, 1e12),
 }
 
 
@@ -762,7 +791,7 @@ def run_served(a):
         rl[f"{key[1]} {synth.SERVE_SHAPES[key[0]][0]}x{synth.SERVE_SHAPES[key[0]][1]}"] = {
             "served_vs_alone": float(np.linalg.norm(s_img - p_img) / np.linalg.norm(p_img)),
             "batch": int(c.batch)}
-    if not a.no_check:
+    if not a.no_check and wl["oracle"]:
         import oracle as O
         O.set_threads(min(16, os.cpu_count() or 1))
         for key, i in check_idx.items():
@@ -927,13 +956,15 @@ def main():
         wl["config"].setdefault("ctx_options", {})[name] = int(val)
     dc = DevCase(ctx, wl["case"])
     ctx.synchronize()
+    print(f"{wl['name']}: inputs resident", file=sys.stderr, flush=True)
 
     def step():
         dc.run(wl["fn"], wl["desc"])
 
-    for _ in range(a.warmup):
+    for i in range(a.warmup):
         step()
-    ctx.synchronize()
+        ctx.synchronize()
+        print(f"{wl['name']}: warm-up step {i + 1} of {a.warmup} done", file=sys.stderr, flush=True)
     evs = [(Event(ctx), Event(ctx)) for _ in range(a.steps)]
     t0 = time.perf_counter()
     for s, e in evs:
@@ -985,7 +1016,7 @@ def main():
                                "kernel": STAGE_KERNELS[stage], "launches": n, "avg_launch_ms": ms / n,
                                "algorithmic_units_per_launch": units / n}
             out["roofline_whole_call"] = whole
-    if not a.no_check:
+    if not a.no_check and wl["oracle"]:
         import oracle as O
         O.set_threads(min(16, os.cpu_count() or 1))
         got, _ = dc.result()
@@ -994,7 +1025,7 @@ def main():
         out["rel_l2"] = {"vs_fp32_oracle": float(np.linalg.norm(got - r32) / np.linalg.norm(r32)),
                          "vs_fp64_oracle": float(np.linalg.norm(got - r64) / np.linalg.norm(r64)),
                          "fp32_oracle_vs_fp64": float(np.linalg.norm(r32 - r64) / np.linalg.norm(r64))}
-    if not a.no_cpu:
+    if not a.no_cpu and wl["oracle"]:
         out["cpu_baseline"] = cpu_baseline(wl, a.cpu_seconds)
         out["cpu_baseline_all_cores"] = cpu_baseline(wl, a.cpu_seconds, a.cpu_threads)
     ctx.close()

@@ -246,7 +246,9 @@ enum oks_stage {
     OKS_STAGE_GRAPPA_CALIBRATE = 2,  /* Gram, rhs, FP64 ridge solve; flops of recon/grappa.rs:156-308 */
     OKS_STAGE_GRAPPA_APPLY = 3,      /* synthesis GEMM; flops 8 x filled points x nc x n_src          */
     OKS_STAGE_CS_ITERATIONS = 4,     /* FISTA iterations; bytes 5*8 + 1 per pixel, plane, iteration   */
-    OKS_STAGE_PLACE = 5              /* readout placement; bytes: samples read + placed samples written */
+    OKS_STAGE_PLACE = 5,             /* readout placement; bytes: samples read + placed samples written */
+    OKS_STAGE_ESPIRIT = 6            /* ESPIRiT maps; FP64 flops of the calibration Gram and the per-voxel
+                                      * Gram and power iteration (the eigenpair search is not counted) */
 };
 int oks_ctx_stage_time(oks_ctx* ctx, double* ms, double* units, int64_t* launches);
 

@@ -160,6 +160,7 @@ STRATEGY_IFFT_RSS, STRATEGY_GRAPPA, STRATEGY_SENSE, STRATEGY_CS = 0, 1, 2, 3
 SERVER_MAX_SLOTS = 16
 STAGE_NONE, STAGE_IFFT_RSS, STAGE_GRAPPA_CALIBRATE, STAGE_GRAPPA_APPLY, STAGE_CS_ITERATIONS, STAGE_PLACE = \
     0, 1, 2, 3, 4, 5
+STAGE_ESPIRIT = 6
 SERVER_MAX_BATCH = 64
 
 # descriptor class -> (strategy id, reference strategy name, oracle name)

@@ -394,7 +394,7 @@ int oks_ctx_set_option(oks_ctx* ctx, const char* name, int64_t value) {
         if (n == "graphs") {
             ctx->use_graphs = value != 0;
         } else if (n == "time_stage") {
-            if (value < OKS_STAGE_NONE || value > OKS_STAGE_PLACE) fail(OKS_E_INVALID_ARG, "unknown stage");
+            if (value < OKS_STAGE_NONE || value > OKS_STAGE_ESPIRIT) fail(OKS_E_INVALID_ARG, "unknown stage");
             OKS_HIP(hipStreamSynchronize(ctx->stream));
             ctx->timed_collect();
             ctx->timed_stage = (int)value;

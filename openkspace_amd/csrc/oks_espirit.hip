@@ -3,11 +3,12 @@
 //
 //   k_espirit_gram  : A^H A of the sliding kw x kw x nc calibration windows,
 //                     gathered straight from the ACS block (:80-112).
-//   k_espirit_eig   : the min(n, 256) leading eigenpairs by power iteration
-//                     with the reference's seeds, Rayleigh-quotient early exit
-//                     and Hotelling deflation (top_hermitian_eigpairs,
-//                     :291-360), then the sigma >= threshold sigma_max cut
-//                     (:119-127).  One workgroup per slice.
+//   k_espirit_eig   : the leading eigenpairs (at most min(n, 256)) by power
+//                     iteration with the reference's seeds, Rayleigh-quotient
+//                     early exit and Hotelling deflation (top_hermitian_eigpairs,
+//                     :291-360), up to the first pair below the sigma >=
+//                     threshold sigma_max cut (:119-127), which ends the kept
+//                     kernels.  One workgroup per slice.
 //   k_espirit_q     : the per-voxel Gram G = sum_k m_k m_k^H of the kernel
 //                     images m_k = centred IFFT of the zero-padded conjugate
 //                     kernel (:141-190) is a trigonometric polynomial in the
@@ -147,6 +148,10 @@ __global__ void __launch_bounds__(1024) k_espirit_eig(EspEigArgs a) {
     double2* w = vw + n;
     const int num = min(n, kEspMaxKept);
     int count = 0;
+    // Only the pairs before the first one with sigma < threshold sigma_max
+    // are kept (recon/espirit.rs:119-127): the search stops at that pair,
+    // which leaves the kept kernels and every map the same.
+    double cut = 0.0;
     for (int k = 0; k < num; ++k) {
         // seed theta = 0.13 + 0.37 i + 0.91 k (recon/espirit.rs:306)
         double s0 = 0.0;
@@ -196,6 +201,12 @@ __global__ void __launch_bounds__(1024) k_espirit_eig(EspEigArgs a) {
             prev = lambda;
         }
         if (lambda <= 0.0) break;
+        if (k == 0) cut = fmax(a.threshold * sqrt(fmax(lambda, 0.0)), 0.0);
+        if (sqrt(fmax(lambda, 0.0)) < cut) {
+            if (t == 0) lam[count] = lambda;
+            ++count;
+            break;
+        }
         // Hotelling deflation M -= lambda v v^H
         for (size_t e = t; e < (size_t)n * n; e += blockDim.x) {
             const int i = (int)(e / n), j = (int)(e - (size_t)i * n);

@@ -49,6 +49,12 @@ static void espirit_batch(oks_ctx* ctx, const EspAcs& acs, int nc, int nz, int n
     int* kept = (int*)ctx->buf("esp_kept", sizeof(int) * zc);
     double2* Q = (double2*)ctx->buf("esp_Q", sizeof(double2) * (size_t)nc * nc * D * D * zc);
     const double s = std::sqrt((double)ny * nx) / ((double)kw * kw) / ((double)ny * nx);
+    // FP64 flops, 8 per complex MAC: the calibration Gram (n_rows n (n + 1) / 2
+    // per slice) and, per voxel, the closed-form Gram ((2 kw - 1)^2 nc^2) and
+    // the power iteration (iters nc^2); the eigenpair search is not counted
+    const double n_rows = (double)(acs.ay - kw + 1) * (acs.ax - kw + 1);
+    StageTimer timer(ctx, OKS_STAGE_ESPIRIT,
+                     8.0 * nz * (n_rows * n * (n + 1.0) / 2.0 + (double)ny * nx * ((double)D * D + iters) * nc * nc));
     for (int z0 = 0; z0 < nz; z0 += zc) {
         const int nzc = std::min(zc, nz - z0);
         EspAcs sa = acs;
