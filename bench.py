@@ -791,7 +791,7 @@ def run_served(a):
         rl[f"{key[1]} {synth.SERVE_SHAPES[key[0]][0]}x{synth.SERVE_SHAPES[key[0]][1]}"] = {
             "served_vs_alone": float(np.linalg.norm(s_img - p_img) / np.linalg.norm(p_img)),
             "batch": int(c.batch)}
-    if not a.no_check and wl["oracle"]:
+    if not a.no_check:
         import oracle as O
         O.set_threads(min(16, os.cpu_count() or 1))
         for key, i in check_idx.items():
