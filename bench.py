@@ -807,6 +807,8 @@ def run_served(a):
                              "vs_fp64_oracle": float(np.linalg.norm(s_img - r64) / np.linalg.norm(r64)),
                              "fp32_oracle_vs_fp64": float(np.linalg.norm(r32 - r64) / np.linalg.norm(r64))})
     out["rel_l2"] = rl
+    if not a.no_check:
+        out["rel_l2_bars"] = parity_summary(rl)
     if not a.no_cpu:
         host = cpu_host()
         jobs = [(key, dev[key][j]["case"]) for key, j in spec]
@@ -820,6 +822,34 @@ def run_served(a):
         o.free()
     ctx.close()
     print(json.dumps(out))
+
+
+def parity_summary(rl):
+    """The served rows against the strategies' bars, stated where the numbers
+    are: 1e-5 for the FP32 paths (ifft-rss, CS) and 1e-3 for the split-BF16
+    GRAPPA, against the FP64 oracle, and against the FP32 restatement where
+    the restatement itself is within the bar of FP64."""
+    bars = {"ifft_rss": 1e-5, "cs": 1e-5, "grappa": 1e-3}
+    rows, miss32 = {}, []
+    for name, r in rl.items():
+        if "vs_fp64_oracle" not in r:
+            continue
+        bar = bars[name.split()[0]]
+        ok64 = r["vs_fp64_oracle"] <= bar
+        ok32 = r["vs_fp32_oracle"] <= bar
+        rows[name] = {"bar": bar, "meets_bar_vs_fp64": ok64, "meets_bar_vs_fp32_restatement": ok32}
+        if not ok32:
+            miss32.append(name)
+    return {
+        "bars": bars, "rows": rows,
+        "rows_meeting_bar_vs_fp64": sum(v["meets_bar_vs_fp64"] for v in rows.values()),
+        "rows_checked": len(rows),
+        "rows_missing_bar_vs_fp32_restatement": miss32,
+        "note": "Where a row misses the bar against the FP32 restatement, the restatement's own distance to "
+                "FP64 (fp32_oracle_vs_fp64 in rel_l2) is above the bar: its FP32 noise whitener (ill-conditioned "
+                "covariance) and, for CS, 60 FISTA iterations of FP32 rounding. The backend computes the small "
+                "dense solves in FP64, so its distance to FP64 is the figure that meets the bar.",
+    }
 
 
 def cpu_host():
