@@ -236,3 +236,25 @@ def test_rss_column_pass_register_form(ctx, ny):
     finally:
         ctx.set_option("cols_reg", 1)
     assert rel(outs[0], outs[1]) < 2e-6
+
+
+@pytest.mark.parametrize("three_d,mode", [(True, abi.FFT_2D), (False, abi.FFT_3D), (True, abi.FFT_3D),
+                                          (False, abi.FFT_2D)])
+def test_forced_fft_mode(ctx, three_d, mode):
+    # FftMode::TwoD / ThreeD override the header's 3-D flag
+    # (recon/strategy.rs:168-172): a 3-D acquisition forced to 2-D is
+    # transformed slice by slice, a multi-slice one forced to 3-D gets the kz
+    # transform; both against the oracle, and different from Auto where the
+    # forced mode differs from the header's
+    case = synth.make_case(nc=4, nz=6, ny=32, nx=40, os_x=2, kind="full", three_d=three_d, seed=12, ellipses=4)
+    d = abi.IfftRssDesc.default(fft_mode=mode, partial_fourier=0)
+    st, ref32, rflags, _ = O.recon("ifft_rss", d, case, "f32")
+    _, ref64, _, _ = O.recon("ifft_rss", d, case, "f64")
+    got, flags, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", d, case)
+    assert st == 0 and flags == rflags and got.shape == ref32.shape
+    assert_parity(got, ref32, ref64, TOL32, f"fft_mode={mode} 3d={three_d}")
+    auto, _, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", abi.IfftRssDesc.default(partial_fourier=0), case)
+    if (mode == abi.FFT_3D) != three_d:
+        assert rel(got, auto) > 1e-3
+    else:
+        assert np.array_equal(got, auto)
