@@ -406,9 +406,11 @@ int oks_ctx_set_option(oks_ctx* ctx, const char* name, int64_t value) {
         } else if (n == "rows_pipe" || n == "cols_pipe") {
             if (value != 0 && value != 1) fail(OKS_E_INVALID_ARG, n + " must be 0 or 1");
             (n == "rows_pipe" ? ctx->rows_pipe : ctx->cols_pipe) = (int)value;
-        } else if (n == "cs_reg" || n == "grappa_synth" || n == "cols_reg") {
+        } else if (n == "cs_reg" || n == "grappa_synth" || n == "cols_reg" || n == "esp_rowt") {
             if (value != 0 && value != 1) fail(OKS_E_INVALID_ARG, n + " must be 0 or 1");
-            (n == "cs_reg" ? ctx->cs_reg : n == "cols_reg" ? ctx->cols_reg : ctx->grappa_synth) = (int)value;
+            (n == "cs_reg" ? ctx->cs_reg : n == "cols_reg" ? ctx->cols_reg : n == "esp_rowt" ? ctx->esp_rowt
+                                                                                            : ctx->grappa_synth) =
+                (int)value;
         } else if (n == "calib_cache_mb") {
             if (value < 0) fail(OKS_E_INVALID_ARG, "calib_cache_mb must be >= 0");
             OKS_HIP(hipStreamSynchronize(ctx->stream));
@@ -441,8 +443,11 @@ int oks_ctx_get_option(oks_ctx* ctx, const char* name, int64_t* value) {
         if (!value) fail(OKS_E_INVALID_ARG, "null value");
         if (n == "tuning") {
             *value = ctx->use_tuning ? 1 : 0;
-        } else if (n == "cs_reg" || n == "grappa_synth" || n == "cols_reg") {
-            *value = n == "cs_reg" ? ctx->cs_reg : n == "cols_reg" ? ctx->cols_reg : ctx->grappa_synth;
+        } else if (n == "cs_reg" || n == "grappa_synth" || n == "cols_reg" || n == "esp_rowt") {
+            *value = n == "cs_reg" ? ctx->cs_reg
+                     : n == "cols_reg" ? ctx->cols_reg
+                     : n == "esp_rowt" ? ctx->esp_rowt
+                                       : ctx->grappa_synth;
         } else if (n == "tuned_calls") {
             *value = ctx->tuned_calls;
         } else if (n == "calib_hits" || n == "calib_misses") {

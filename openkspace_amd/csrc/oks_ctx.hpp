@@ -79,6 +79,7 @@ struct oks_ctx {
     int cs_reg = 1;                                // CS column pass: register four-step transforms where available
     int grappa_synth = 1;                          // GRAPPA synthesis: grouped kernel where the geometry allows
     int cols_reg = 1;                              // RSS column passes: register four-step transforms where available
+    int esp_rowt = 1;                              // ESPIRiT per-voxel stage from the row-contracted Gram
     int rows_pipe = 1;                             // ifft-rss row pass: pipelined persistent kernel
     int rows_waves = 0;                            // one-row-per-workgroup row passes: wavefronts (0 = automatic)
     int cols_sx = 0;                               // column passes: columns per workgroup cap (0 = automatic)

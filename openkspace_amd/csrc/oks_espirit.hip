@@ -347,6 +347,200 @@ __global__ void __launch_bounds__(128) k_espirit_voxel(EspVoxArgs a) {
     }
 }
 
+// Row contraction of Q over the y taps, for the Hermitian half of the Gram:
+//   T[z][y][e][ix] = sum_iy Q[z][c1][c2][iy][ix] e^{2 pi i (iy - (kw - 1)) (y + fy) / ny},
+// e = c1 (c1 + 1) / 2 + c2 (c1 >= c2).  One thread per (e, ix) of slice
+// blockIdx.y, looping over the rows; the row phases come from a table of
+// e^{2 pi i m / ny} in LDS.
+__global__ void __launch_bounds__(256) k_espirit_rowt(EspRowTArgs a) {
+    extern __shared__ double2 cy_tab[];  // ny entries
+    const int z = blockIdx.y, nc = a.nc, kw = a.kw, D = 2 * kw - 1, ny = a.ny;
+    for (int m = threadIdx.x; m < ny; m += blockDim.x) cy_tab[m] = cis_mod(m, ny);
+    __syncthreads();
+    const int E = nc * (nc + 1) / 2;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= E * D) return;
+    const int e = idx / D, ix = idx - e * D;
+    int c1 = 0;
+    while ((c1 + 1) * (c1 + 2) / 2 <= e) ++c1;
+    const int c2 = e - c1 * (c1 + 1) / 2;
+    const double2* q = a.Q + (((size_t)z * nc + c1) * nc + c2) * D * D + ix;
+    double2 qv[15];
+#pragma unroll
+    for (int iy = 0; iy < 15; ++iy) qv[iy] = iy < D ? q[(size_t)iy * D] : make_double2(0.0, 0.0);
+    const int fy = ny / 2;
+    double2* out = a.T + ((size_t)z * ny * E + e) * D + ix;
+    for (int y = 0; y < ny; ++y) {
+        double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int iy = 0; iy < 15; ++iy) {
+            if (iy < D) {
+                int m = (int)(((long long)(iy - (kw - 1)) * (y + fy)) % ny);
+                if (m < 0) m += ny;
+                acc = zadd(acc, zm(qv[iy], cy_tab[m]));
+            }
+        }
+        out[(size_t)y * E * D] = acc;
+    }
+}
+
+// Per-voxel maps from the row-contracted Gram: one workgroup of 8 wavefronts
+// per (row y, slice z), the row's T block, the table e^{2 pi i m / nx} and the
+// first kernel contracted over the y taps (U[c][dx]) staged in LDS;
+// wavefront w takes the voxels x = w, w + 8, ...  Lane (c, h) = (l % 32,
+// l / 32) holds G[c][16 h + j], j < 16, in registers:
+//   G[c1][c2] = sum_ix T[c1][c2][ix] e^{2 pi i (ix - (kw - 1)) (x + fx) / nx}  (c1 >= c2),
+//   G[c2][c1] = conj(G[c1][c2]).
+// Then, as k_espirit_voxel: power iteration from m_0 with the reference's
+// guards, the largest-magnitude coil rotated to real-positive, times
+// sqrt(lambda) (recon/espirit.rs:199-272).
+constexpr int kEspVox2Threads = 512;
+
+__global__ void __launch_bounds__(kEspVox2Threads) k_espirit_voxel2(EspVox2Args a) {
+    extern __shared__ double2 sm[];
+    const int nc = a.nc, kw = a.kw, D = 2 * kw - 1, kk = kw * kw, ny = a.ny, nx = a.nx;
+    const int E = nc * (nc + 1) / 2;
+    double2* Tr = sm;                // E * D
+    double2* cx_tab = Tr + E * D;    // nx
+    double2* U = cx_tab + nx;        // nc * kw
+    double2* ey = U + nc * kw;       // kw
+    double2* vbuf = ey + kw;         // 8 * 32
+    const int y = blockIdx.x, z = blockIdx.y, t = threadIdx.x;
+    const long long plane = (long long)ny * nx;
+    float2* outrow = a.maps + (size_t)z * plane + (size_t)y * nx;
+    const int K = a.kept[z];
+    if (K == 0) {
+        for (int i = t; i < nc * nx; i += blockDim.x)
+            outrow[(size_t)(i / nx) * a.map_cs + i % nx] = make_float2(0.f, 0.f);
+        return;
+    }
+    const int fy = ny / 2, fx = nx / 2, cy = (ny + 1) / 2, cx = (nx + 1) / 2;
+    const int y0 = (ny - kw) / 2, x0 = (nx - kw) / 2;
+    const double2* Tg = a.T + ((size_t)z * ny + y) * E * D;
+    for (int i = t; i < E * D; i += blockDim.x) Tr[i] = Tg[i];
+    for (int m = t; m < nx; m += blockDim.x) cx_tab[m] = cis_mod(m, nx);
+    if (t < kw) ey[t] = cis_mod((long long)(y0 + t - cy) * (y + fy), ny);
+    __syncthreads();
+    const double2* v0 = a.vec + (size_t)z * kEspMaxKept * a.n;
+    for (int i = t; i < nc * kw; i += blockDim.x) {
+        const int c = i / kw, dx = i - c * kw;
+        double2 r = make_double2(0.0, 0.0);
+        for (int dy = 0; dy < kw; ++dy) r = zadd(r, zm(zc(v0[c * kk + dy * kw + dx]), ey[dy]));
+        U[i] = r;
+    }
+    __syncthreads();
+    const int w = t >> 6, lane = t & 63, c = lane & 31, h = lane >> 5;
+    double2* vb = vbuf + w * 32;
+    const bool row = c < nc;
+    for (int x = w; x < nx; x += kEspVox2Threads / 64) {
+        // G[c][16 h + j]
+        double2 G[16];
+        int mx0 = (int)(((long long)(-(kw - 1)) * (x + fx)) % nx);
+        if (mx0 < 0) mx0 += nx;
+        const int step = (x + fx) % nx;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int c2 = 16 * h + j;
+            double2 g = make_double2(0.0, 0.0);
+            if (row && c2 < nc) {
+                const int hi = c > c2 ? c : c2, lo = c > c2 ? c2 : c;
+                const double2* te = Tr + (hi * (hi + 1) / 2 + lo) * D;
+                int m = mx0;
+                for (int ix = 0; ix < D; ++ix) {
+                    g = zadd(g, zm(te[ix], cx_tab[m]));
+                    m += step;
+                    if (m >= nx) m -= nx;
+                }
+                if (c2 > c) g = zc(g);
+            }
+            G[j] = g;
+        }
+        // v = m_0 at this voxel: s sum_dx U[c][dx] e^{2 pi i (x0 + dx - cx) (x + fx) / nx}
+        double2 vc = make_double2(0.0, 0.0);
+        if (row && h == 0) {
+            for (int dx = 0; dx < kw; ++dx) {
+                int m = (int)(((long long)(x0 + dx - cx) * (x + fx)) % nx);
+                if (m < 0) m += nx;
+                vc = zadd(vc, zm(U[c * kw + dx], cx_tab[m]));
+            }
+            vc = make_double2(vc.x * a.s, vc.y * a.s);
+        }
+        double nrm0 = wave_sum((row && h == 0) ? vc.x * vc.x + vc.y * vc.y : 0.0);
+        if (nrm0 < 1e-30) {
+            if (lane == 0) vc = make_double2(1.0, 0.0);
+            nrm0 = 1.0;
+        }
+        const double i0 = 1.0 / sqrt(nrm0);
+        vc = make_double2(vc.x * i0, vc.y * i0);
+        if (row && h == 0) vb[c] = vc;
+        wave_fence();
+        double lambda = 0.0;
+        for (int it = 0; it < a.iters; ++it) {
+            double2 p = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (16 * h + j < nc) p = zadd(p, zm(G[j], vb[16 * h + j]));
+            const double2 wc = make_double2(p.x + __shfl_xor(p.x, 32), p.y + __shfl_xor(p.y, 32));
+            const double nrm = sqrt(wave_sum((row && h == 0) ? wc.x * wc.x + wc.y * wc.y : 0.0));
+            if (nrm < 1e-30) break;
+            const double inv = 1.0 / nrm;
+            vc = make_double2(wc.x * inv, wc.y * inv);
+            wave_fence();
+            if (row && h == 0) vb[c] = vc;
+            wave_fence();
+            lambda = nrm;
+        }
+        // rotate the largest-magnitude coil (first of equals) to real-positive
+        double mg = (row && h == 0) ? hypot(vc.x, vc.y) : -1.0;
+        int idx = lane;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double mo = __shfl_xor(mg, o);
+            const int io = __shfl_xor(idx, o);
+            if (mo > mg || (mo == mg && io < idx)) {
+                mg = mo;
+                idx = io;
+            }
+        }
+        const double2 r = make_double2(__shfl(vc.x, idx), __shfl(vc.y, idx));
+        const double sc = sqrt(lambda);
+        if (row && h == 0) {
+            double2 o = vc;
+            if (mg > 0.0) o = zm(vc, make_double2(r.x / mg, -r.y / mg));
+            outrow[(size_t)c * a.map_cs + x] = make_float2((float)(o.x * sc), (float)(o.y * sc));
+        }
+        wave_fence();  // this voxel's reads of vb before the next voxel's writes
+    }
+}
+
+size_t espirit_vox2_lds(int nc, int kw, int nx) {
+    const int D = 2 * kw - 1;
+    return sizeof(double2) * ((size_t)nc * (nc + 1) / 2 * D + nx + (size_t)nc * kw + kw + 8 * 32);
+}
+
+hipError_t launch_espirit_rowt(const EspRowTArgs& a, int nz, hipStream_t s) {
+    const int D = 2 * a.kw - 1;
+    if (a.nc < 1 || a.nc > 32 || a.kw < 1 || a.kw > 8 || a.ny < 1) return hipErrorInvalidValue;
+    const long long total = (long long)a.nc * (a.nc + 1) / 2 * D;
+    const size_t lds = sizeof(double2) * (size_t)a.ny;
+    hipError_t e = hipFuncSetAttribute((const void*)k_espirit_rowt, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_espirit_rowt, dim3((unsigned)((total + 255) / 256), (unsigned)nz), dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_espirit_voxel2(const EspVox2Args& a, hipStream_t s) {
+    if (a.nc < 1 || a.nc > 32 || a.kw < 1 || a.kw > 8) return hipErrorInvalidValue;
+    const size_t lds = espirit_vox2_lds(a.nc, a.kw, a.nx);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    hipError_t e = hipFuncSetAttribute((const void*)k_espirit_voxel2, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_espirit_voxel2, dim3((unsigned)a.ny, (unsigned)a.nz), dim3(kEspVox2Threads), lds, s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_espirit_gram(const EspGramArgs& a, int nz, hipStream_t s) {
     if (a.n < 1 || a.n > kEspMaxN || nz < 1) return hipErrorInvalidValue;
     const unsigned t = (unsigned)((a.n + 31) / 32);

@@ -49,6 +49,26 @@ struct EspVoxArgs {
     double s;        // sqrt(ny nx) / kw^2 / (ny nx)
 };
 
+// Row-contracted form of the per-voxel stage (k_espirit_rowt + k_espirit_voxel2).
+struct EspRowTArgs {
+    const double2* Q;  // [nz][nc][nc][D][D]
+    double2* T;        // [nz][ny][nc (nc + 1) / 2][D]
+    int nc, kw, ny;
+};
+
+struct EspVox2Args {
+    const double2* T;
+    const double2* vec;
+    const int* kept;
+    float2* maps;      // [nc][nz][ny][nx] (coil stride map_cs)
+    long long map_cs;
+    int nc, kw, n, nz, ny, nx, iters;
+    double s;
+};
+
+size_t espirit_vox2_lds(int nc, int kw, int nx);  // bytes of LDS per workgroup (<= 160 KiB to run)
+hipError_t launch_espirit_rowt(const EspRowTArgs& a, int nz, hipStream_t s);
+hipError_t launch_espirit_voxel2(const EspVox2Args& a, hipStream_t s);
 hipError_t launch_espirit_gram(const EspGramArgs& a, int nz, hipStream_t s);
 hipError_t launch_espirit_eig(const EspEigArgs& a, int nz, hipStream_t s);
 hipError_t launch_espirit_q(const EspQArgs& a, int nz, hipStream_t s);

@@ -62,8 +62,24 @@ static void espirit_batch(oks_ctx* ctx, const EspAcs& acs, int nc, int nz, int n
         OKS_HIP(launch_espirit_gram(EspGramArgs{sa, nc, kw, n, M}, nzc, ctx->stream));
         OKS_HIP(launch_espirit_eig(EspEigArgs{M, lam, vec, kept, n, iters, (double)thr}, nzc, ctx->stream));
         OKS_HIP(launch_espirit_q(EspQArgs{vec, kept, Q, nc, kw, n, s * s}, nzc, ctx->stream));
-        EspVoxArgs va{Q, vec, kept, maps + (size_t)z0 * ny * nx, map_cs, nc, kw, n, nzc, ny, nx, iters, s};
-        OKS_HIP(launch_espirit_voxel(va, ctx->stream));
+        if (ctx->esp_rowt && espirit_vox2_lds(nc, kw, nx) <= 160 * 1024) {
+            // the Gram's Hermitian half contracted over the y taps per row, in
+            // sub-chunks of slices whose rows stay within 1 GiB
+            const size_t per_slice = sizeof(double2) * (size_t)ny * nc * (nc + 1) / 2 * D;
+            const int tc = std::max(1, std::min(nzc, (int)((size_t(1) << 30) / per_slice)));
+            double2* T = (double2*)ctx->buf("esp_T", per_slice * tc);
+            for (int t0 = 0; t0 < nzc; t0 += tc) {
+                const int ntc = std::min(tc, nzc - t0);
+                OKS_HIP(launch_espirit_rowt(EspRowTArgs{Q + (size_t)t0 * nc * nc * D * D, T, nc, kw, ny}, ntc,
+                                            ctx->stream));
+                EspVox2Args va{T, vec + (size_t)t0 * kEspMaxKept * n, kept + t0, maps + (size_t)(z0 + t0) * ny * nx,
+                               map_cs, nc, kw, n, ntc, ny, nx, iters, s};
+                OKS_HIP(launch_espirit_voxel2(va, ctx->stream));
+            }
+        } else {
+            EspVoxArgs va{Q, vec, kept, maps + (size_t)z0 * ny * nx, map_cs, nc, kw, n, nzc, ny, nx, iters, s};
+            OKS_HIP(launch_espirit_voxel(va, ctx->stream));
+        }
     }
 }
 

@@ -340,3 +340,22 @@ def test_espirit_c4_kernel6_32_coils(ctx):
         O.set_threads(1)
     assert kept == k64
     assert_parity(got, r32, r64, TOL, f"espirit k=6 nc=32 ({kept} kernels)")
+
+
+@pytest.mark.parametrize("nc,kernel,ny,nx", [(5, 5, 48, 40), (32, 6, 47, 39), (17, 3, 33, 64)])
+def test_espirit_voxel_forms(ctx, nc, kernel, ny, nx):
+    # The per-voxel stage from the row-contracted Hermitian Gram (default)
+    # and from the full trigonometric polynomial per voxel (option esp_rowt
+    # = 0) give the same maps to FP64 rounding; odd sizes included.
+    case = synth.make_case(nc=nc, nz=1, ny=ny, nx=nx, kind="full", seed=nc + ny, ellipses=5)
+    acs = case["kspace"][:, 0, ny // 2 - 8:ny // 2 + 8, :]
+    outs = []
+    try:
+        for form in (1, 0):
+            ctx.set_option("esp_rowt", form)
+            outs.append(gpu_espirit(ctx, acs, ny, nx, kernel, 0.02, 30))
+    finally:
+        ctx.set_option("esp_rowt", 1)
+    (a, ka), (b, kb) = outs
+    assert ka == kb and ka > 0
+    assert rel(a, b) < 1e-9
