@@ -234,6 +234,16 @@ int guarded(oks_ctx* ctx, F f) {
 
 }  // namespace
 
+// Switches between two forms of one computation whose results differ by
+// rounding only (not launch options: the tuning table never sets them).
+static int* algorithm_option(oks_ctx* c, const std::string& n) {
+    if (n == "cs_reg") return &c->cs_reg;
+    if (n == "cols_reg") return &c->cols_reg;
+    if (n == "grappa_synth") return &c->grappa_synth;
+    if (n == "esp_rowt") return &c->esp_rowt;
+    return nullptr;
+}
+
 // Launch-shape options a tuning table may set: none changes the computed bits.
 static const char* const kLaunchOptions[] = {"cs_sx",        "cs_threads", "cols_sx",   "cols_threads",
                                              "rows_waves",   "cs_cols_map", "cs_upd_map", "cs_pipe",
@@ -406,11 +416,9 @@ int oks_ctx_set_option(oks_ctx* ctx, const char* name, int64_t value) {
         } else if (n == "rows_pipe" || n == "cols_pipe") {
             if (value != 0 && value != 1) fail(OKS_E_INVALID_ARG, n + " must be 0 or 1");
             (n == "rows_pipe" ? ctx->rows_pipe : ctx->cols_pipe) = (int)value;
-        } else if (n == "cs_reg" || n == "grappa_synth" || n == "cols_reg" || n == "esp_rowt") {
+        } else if (int* sw = algorithm_option(ctx, n)) {
             if (value != 0 && value != 1) fail(OKS_E_INVALID_ARG, n + " must be 0 or 1");
-            (n == "cs_reg" ? ctx->cs_reg : n == "cols_reg" ? ctx->cols_reg : n == "esp_rowt" ? ctx->esp_rowt
-                                                                                            : ctx->grappa_synth) =
-                (int)value;
+            *sw = (int)value;
         } else if (n == "calib_cache_mb") {
             if (value < 0) fail(OKS_E_INVALID_ARG, "calib_cache_mb must be >= 0");
             OKS_HIP(hipStreamSynchronize(ctx->stream));
@@ -443,11 +451,8 @@ int oks_ctx_get_option(oks_ctx* ctx, const char* name, int64_t* value) {
         if (!value) fail(OKS_E_INVALID_ARG, "null value");
         if (n == "tuning") {
             *value = ctx->use_tuning ? 1 : 0;
-        } else if (n == "cs_reg" || n == "grappa_synth" || n == "cols_reg" || n == "esp_rowt") {
-            *value = n == "cs_reg" ? ctx->cs_reg
-                     : n == "cols_reg" ? ctx->cols_reg
-                     : n == "esp_rowt" ? ctx->esp_rowt
-                                       : ctx->grappa_synth;
+        } else if (int* sw = algorithm_option(ctx, n)) {
+            *value = *sw;
         } else if (n == "tuned_calls") {
             *value = ctx->tuned_calls;
         } else if (n == "calib_hits" || n == "calib_misses") {
