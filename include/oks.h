@@ -208,7 +208,12 @@ void* oks_ctx_stream(oks_ctx* ctx);
  * the one-row-per-workgroup row passes; "cols_sx" (0 = automatic, or a power
  * of two up to 16) caps the columns per workgroup of the column passes and
  * "cols_threads" (0 = 512, or 256) sets their workgroup size (prefetching
- * needs 512). */
+ * needs 512).  Four switches (0/1, default 1) choose between two forms of
+ * one computation whose FP32 rounding differs, both inside the strategies'
+ * tolerances; 0 selects the earlier LDS-staged form: "cs_reg" (CS column
+ * pass as register four-step transforms), "cols_reg" (IFFT + RSS column
+ * pass likewise), "grappa_synth" (GRAPPA synthesis grouped by source row)
+ * and "esp_rowt" (ESPIRiT per-voxel stage from the row-contracted Gram). */
 int oks_ctx_set_option(oks_ctx* ctx, const char* name, int64_t value);
 /* Launch shapes from the autotuner (python -m openkspace_amd.autotune, no
  * counterpart in the reference, whose FftPlanner plans per call,
