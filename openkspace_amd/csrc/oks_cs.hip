@@ -341,16 +341,16 @@ __global__ void __launch_bounds__(kCsPipeThreads) k_cs_cols_pipe(CsColArgs a) {
 // task order (k_cs_taskbits, k_cs_compact_task): tb[(z nx + x) N1 + k1] =
 // {bit k2 set when slot k1 + N1 k2 is sampled, sampled slots of the column's
 // tasks before k1}.
-template <int N>
-constexpr int cs_reg_threads() { return RegGeom<N, 16>::NT; }
+template <int N, int SX>
+constexpr int cs_reg_threads() { return RegGeom<N, SX>::NT; }
 
 // One strip per workgroup.  The sampling word comes first, then the strip's
 // phase-A samples, then (once the word is in) the measured samples of phase
 // B, so every global load of the strip is in flight before phase A's
 // transform starts.
-template <int N>
-__global__ void __launch_bounds__(cs_reg_threads<N>()) k_cs_cols_reg(CsColArgs a) {
-    using G = RegGeom<N, 16>;
+template <int N, int SX_>
+__global__ void __launch_bounds__((cs_reg_threads<N, SX_>())) k_cs_cols_reg(CsColArgs a) {
+    using G = RegGeom<N, SX_>;
     constexpr int N1 = G::N1, N2 = G::N2, SX = G::SX, N2P = G::N2P, CS = G::CS, NT = G::NT;
     constexpr int half = N / 2;
     extern __shared__ float2 lds[];
@@ -802,16 +802,22 @@ static hipError_t go_cs_cols_coop(const CsColArgs& a, hipStream_t s) {
     }
 }
 
-template <int N>
-static hipError_t go_cs_cols_reg(const CsColArgs& a, hipStream_t s) {
-    using G = RegGeom<N, 16>;
+template <int N, int SX>
+static hipError_t go_cs_cols_reg_sx(const CsColArgs& a, hipStream_t s) {
+    using G = RegGeom<N, SX>;
     if (!a.tb) return hipErrorInvalidValue;
-    OKS_ATTR2(hipFuncSetAttribute((const void*)k_cs_cols_reg<N>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    OKS_ATTR2(hipFuncSetAttribute((const void*)k_cs_cols_reg<N, SX>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)G::lds_bytes));
     const long long grid = a.planes * ((a.nx + G::SX - 1) / G::SX);
     if (grid < 1 || grid > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_cs_cols_reg<N>, dim3((unsigned)grid), dim3(G::NT), G::lds_bytes, s, a);
+    hipLaunchKernelGGL((k_cs_cols_reg<N, SX>), dim3((unsigned)grid), dim3(G::NT), G::lds_bytes, s, a);
     return hipGetLastError();
+}
+
+// Strip width: 8 columns when asked for 8 or fewer, else 16 (the same bits).
+template <int N>
+static hipError_t go_cs_cols_reg(const CsColArgs& a, hipStream_t s) {
+    return a.sx <= 8 ? go_cs_cols_reg_sx<N, 8>(a, s) : go_cs_cols_reg_sx<N, 16>(a, s);
 }
 
 template <int N>
