@@ -583,104 +583,134 @@ __global__ void __launch_bounds__(256) k_normal64(Normal64Args a) {
 }
 
 // Cholesky of the diagonal block G[k0:k0+kn, k0:k0+kn] (lower) and its
-// explicit inverse (recon/prewhiten.rs:148-210, in FP64).  1024 threads;
-// thread (ti, tc) keeps the 2 x 2 elements (ti + 32 a, tc + 32 b) of the
-// block in registers.  Step j: the owners of column j publish it to LDS (two
-// buffers alternate, so one barrier per step), every thread updates its
-// elements with j < c <= i.  The inverse is built by doubling: four
-// wavefronts invert the 16 x 16 diagonal blocks by forward substitution, then
+// explicit inverse (recon/prewhiten.rs:148-210, in FP64).  1024 threads; the
+// block sits in LDS and is factored 16 columns at a time:
+//   (a) wavefront 0 factors the 16 x 16 diagonal sub-block S, lane i holding
+//       row i in registers: each pivot reaches the lanes by a lane read, each
+//       scaled column through LDS, so the 16 steps need no workgroup barrier;
+//   (b) the panel below it, x S^H = a, one thread per row;
+//   (c) the trailing lower triangle, A -= panel panel^H, all threads.
+// The inverse is built by doubling: four wavefronts invert the 16 x 16
+// diagonal blocks by forward substitution, then
 //   [[A, 0], [B, C]]^-1 = [[A^-1, 0], [-C^-1 B A^-1, C^-1]]
 // joins them into 32 x 32 and 64 x 64 blocks, with B A^-1 parked in the
 // (zero) upper triangle of the inverse.  A non-PD pivot marks the slice
 // failed (the reference leaves it undersampled).
 constexpr int kPotrfThreads = 1024;
 
+__device__ __forceinline__ double lane_read(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __global__ void __launch_bounds__(kPotrfThreads) k_potrf64_diag(Potrf64Args a) {
     __shared__ double2 L[64][65];
     __shared__ double2 I[64][65];
-    __shared__ double2 colb[2][64];
-    __shared__ double2 piv[2];  // (l_jj, 1 / l_jj) of the column in colb[j & 1]
     __shared__ double dinv[64];
     __shared__ int badf;
     const int z = blockIdx.x;
     if (a.skip[z]) return;
     const int n = a.n, k0 = a.k0, kn = a.kn, t = threadIdx.x;
-    const int ti = t >> 5, tc = t & 31;
+    const int w = t >> 6, lane = t & 63;
     const double2* G = a.G + (size_t)z * n * n;
     double2* Lg = a.L + (size_t)z * n * n;
     double2* D = a.D + ((size_t)z * a.nblk + a.kb) * 64 * 64;
-    double2 r[2][2];
-#pragma unroll
-    for (int ia = 0; ia < 2; ++ia)
-#pragma unroll
-        for (int ib = 0; ib < 2; ++ib) {
-            const int i = ti + 32 * ia, c = tc + 32 * ib;
-            r[ia][ib] = (i < kn && c <= i) ? G[(size_t)(k0 + i) * n + k0 + c] : make_double2(0.0, 0.0);
-        }
     for (int e = t; e < 64 * 64; e += kPotrfThreads) {
-        L[e / 64][e % 64] = make_double2(0.0, 0.0);
-        I[e / 64][e % 64] = make_double2(0.0, 0.0);
+        const int i = e / 64, c = e % 64;
+        L[i][c] = (i < kn && c <= i) ? G[(size_t)(k0 + i) * n + k0 + c] : make_double2(0.0, 0.0);
+        I[i][c] = make_double2(0.0, 0.0);
     }
     if (t < 64) dinv[t] = 0.0;
-    // The pivot of each column (square root and reciprocal of its diagonal
-    // entry) is formed once, by the thread that holds that entry, when it
-    // hands the column over, rather than by every thread after the barrier.
-    auto pivot = [&](double d, int slot) {
-        if (!(d > 0.0) || !isfinite(d)) badf = 1;
-        const double ljj = sqrt(fmax(d, 1e-300));
-        piv[slot] = make_double2(ljj, 1.0 / ljj);
-    };
     if (t == 0) badf = 0;
-    if (tc == 0)
-#pragma unroll
-        for (int ia = 0; ia < 2; ++ia) colb[0][ti + 32 * ia] = r[ia][0];
     __syncthreads();
-    if (t == 0 && kn > 0) pivot(r[0][0].x, 0);
-    __syncthreads();
-    for (int j = 0; j < kn; ++j) {
-        const double2* col = colb[j & 1];
-        const double ljj = piv[j & 1].x, inv = piv[j & 1].y;
-        // scaled column j: L[i][j] = col[i] / ljj (i > j), L[j][j] = ljj
-        if (t < 64) {
-            if (t > j && t < kn) L[t][j] = make_double2(col[t].x * inv, col[t].y * inv);
-            if (t == j) {
-                L[j][j] = make_double2(ljj, 0.0);
-                dinv[j] = inv;
-            }
-        }
-        const double s2 = inv * inv;
+    for (int c0 = 0; c0 < kn; c0 += 16) {
+        const int cn = min(16, kn - c0);
+        if (w == 0) {
+            // lanes past the sub-block's rows and entries above the diagonal
+            // carry values that are never written back
+            double2 r[16];
 #pragma unroll
-        for (int ia = 0; ia < 2; ++ia) {
-            const int i = ti + 32 * ia;
-            if (i <= j || i >= kn) continue;
-            const double2 li = col[i];
+            for (int c = 0; c < 16; ++c)
+                r[c] = (lane < cn && c <= lane) ? L[c0 + lane][c0 + c] : make_double2(0.0, 0.0);
+            double2* col = &I[0][0];  // the inverse's block is free until the factorization ends
+            bool bad = false;
 #pragma unroll
-            for (int ib = 0; ib < 2; ++ib) {
-                const int c = tc + 32 * ib;
-                if (c > j && c <= i) {
-                    const double2 p = zmul(li, zconj(col[c]));
-                    r[ia][ib].x -= p.x * s2;
-                    r[ia][ib].y -= p.y * s2;
+            for (int j = 0; j < 16; ++j) {
+                if (j < cn) {
+                    // pivot of column j from the lane that holds row j:
+                    // 1/sqrt(d) by the hardware estimate and two Newton steps
+                    const double d = lane_read(r[j].x, j);
+                    bad = bad || !(d > 0.0) || !isfinite(d);
+                    const double dd = fmax(d, 1e-300);
+                    double y = __builtin_amdgcn_rsq(dd);
+                    y = y * fma(-0.5 * dd, y * y, 1.5);
+                    y = y * fma(-0.5 * dd, y * y, 1.5);
+                    r[j] = lane == j ? make_double2(dd * y, 0.0) : make_double2(r[j].x * y, r[j].y * y);
+                    if (lane == 0) dinv[c0 + j] = y;
+                    col[lane] = r[j];
+                    wave_lds_sync();
+                    // r[c] -= l_ij conj(l_cj), j < c (only c <= i is kept)
+#pragma unroll
+                    for (int c = j + 1; c < 16; ++c) {
+                        const double2 p = zmul(r[j], zconj(col[c]));
+                        r[c].x -= p.x;
+                        r[c].y -= p.y;
+                    }
+                    wave_lds_sync();
                 }
             }
+            if (lane < cn) {
+#pragma unroll
+                for (int c = 0; c < 16; ++c)
+                    if (c <= lane) L[c0 + lane][c0 + c] = r[c];
+            }
+            if (bad && lane == 0) badf = 1;
         }
-        const int jn = j + 1;
-        if (jn < kn && tc == (jn & 31)) {
-            const int jb = jn >> 5;
+        __syncthreads();
+        const int r0 = c0 + cn;  // cn == 16 whenever rows remain below
+        if (r0 >= kn) break;
+        if (t < kn - r0) {
+            // row i of the panel: x_j = (a_j - sum_{k<j} x_k conj(S_jk)) / S_jj
+            const int i = r0 + t;
 #pragma unroll
-            for (int ib = 0; ib < 2; ++ib)
-                if (ib == jb) {
+            for (int j = 0; j < 16; ++j) {
+                double2 acc = L[i][c0 + j];
 #pragma unroll
-                    for (int ia = 0; ia < 2; ++ia) colb[jn & 1][ti + 32 * ia] = r[ia][ib];
-                    if (ti == tc) pivot(r[ib][ib].x, jn & 1);  // row jn: ia == jb
+                for (int k = 0; k < j; ++k) {
+                    const double2 p = zmul(L[i][c0 + k], zconj(L[c0 + j][c0 + k]));
+                    acc.x -= p.x;
+                    acc.y -= p.y;
                 }
+                const double iv = dinv[c0 + j];
+                L[i][c0 + j] = make_double2(acc.x * iv, acc.y * iv);
+            }
+        }
+        __syncthreads();
+        const int m = kn - r0;
+        for (int e = t; e < m * m; e += kPotrfThreads) {
+            const int i = r0 + e / m, c = r0 + e % m;
+            if (c > i) continue;
+            double2 acc = L[i][c];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const double2 p = zmul(L[i][c0 + k], zconj(L[c][c0 + k]));
+                acc.x -= p.x;
+                acc.y -= p.y;
+            }
+            L[i][c] = acc;
         }
         __syncthreads();
     }
     const int bad = badf;
     // Level 0: wavefront b inverts the 16 x 16 block at 16 b; four lanes own
     // one column and split its dot products.
-    const int w = t >> 6, lane = t & 63;
     if (w < 4) {
         const int b0 = 16 * w, c = b0 + (lane >> 2), part = lane & 3;
         const int end = min(b0 + 16, kn);
