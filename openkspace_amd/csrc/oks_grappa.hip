@@ -778,6 +778,30 @@ __global__ void __launch_bounds__(kPotrfThreads) k_potrf64_diag(Potrf64Args a) {
     }
 }
 
+// Last step of the backward substitution, every diagonal block at once:
+// X[z][k0 + i][c] = sum_{r >= i} conj(D_k[r][i]) Y[z][k0 + r][c] (D_k = the
+// inverse of diagonal block k, lower, zero past the block's kn rows).  One
+// thread per output, columns fastest (a wavefront shares its D entries).
+__global__ void __launch_bounds__(256) k_blockdiag_h(const double2* __restrict__ D, const double2* __restrict__ Y,
+                                                     double2* __restrict__ X, int n, int m, int nblk, long long total) {
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= total) return;
+    const long long nm = (long long)n * m;
+    const int z = (int)(e / nm);
+    const long long rem = e - z * nm;
+    const int row = (int)(rem / m), c = (int)(rem - (long long)row * m);
+    const int kb = row >> 6, i = row & 63, k0 = kb * 64, kn = min(64, n - k0);
+    const double2* Dk = D + ((size_t)z * nblk + kb) * 64 * 64;
+    const double2* Yz = Y + (size_t)z * nm + (size_t)k0 * m + c;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int r = i; r < kn; ++r) {
+        const double2 d = Dk[r * 64 + i], y = Yz[(size_t)r * m];
+        acc.x = fma(d.x, y.x, fma(d.y, y.y, acc.x));
+        acc.y = fma(d.x, y.y, fma(-d.y, y.x, acc.y));
+    }
+    X[e] = acc;
+}
+
 // weights[z][d-1][ch][j] = X[z][j][(d-1) nc + ch]  (recon/grappa.rs:286-299)
 __global__ void k_weights_out(const double2* X, float2* w, int nz, int nc, int r, int n_src) {
     const int n_tgt = nc * (r - 1);
@@ -853,6 +877,15 @@ hipError_t launch_normal64(const Normal64Args& a, int nz, hipStream_t s) {
 
 hipError_t launch_potrf64_diag(const Potrf64Args& a, int nz, hipStream_t s) {
     hipLaunchKernelGGL(k_potrf64_diag, dim3(nz), dim3(kPotrfThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_blockdiag_h(const double2* D, const double2* Y, double2* X, int nz, int n, int m, int nblk,
+                              hipStream_t s) {
+    const long long total = (long long)nz * n * m;
+    if (total < 1 || (total + 255) / 256 > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_blockdiag_h, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, D, Y, X, n, m, nblk,
+                       total);
     return hipGetLastError();
 }
 

@@ -99,6 +99,9 @@ hipError_t launch_grappa_synth(const SynthArgs& a, hipStream_t s);
 hipError_t launch_zgemm64(const Zgemm64Args& g, int batch, hipStream_t s);
 hipError_t launch_normal64(const Normal64Args& a, int nz, hipStream_t s);
 hipError_t launch_potrf64_diag(const Potrf64Args& a, int nz, hipStream_t s);
+// X_k = D_k^H Y_k for every diagonal block k of every slice (Y, X [nz][n][m]).
+hipError_t launch_blockdiag_h(const double2* D, const double2* Y, double2* X, int nz, int n, int m, int nblk,
+                              hipStream_t s);
 hipError_t launch_weights_out(const double2* X, float2* w, int nz, int nc, int r, int n_src, hipStream_t s);
 
 }  // namespace oks

@@ -115,6 +115,10 @@ static void grappa_calibrate_batch(oks_ctx* ctx, const float2* grid, const uint2
     double2* X = (double2*)ctx->buf("gr_X64", sizeof(double2) * nz * nm);
     const int nblk = (n + 63) / 64;
     double2* D = (double2*)ctx->buf("gr_D64", sizeof(double2) * (size_t)nz * nblk * 64 * 64);
+    // M_k = L[k0:k0+kn, 0:k0]^H D_k^H at rows [0, k0), columns [k0, k0+kn):
+    // the backward substitution's update Y[0:k0] -= L_k^H (D_k^H Y_k) as one
+    // product, formed off the critical path during the forward sweep
+    double2* M = (double2*)ctx->buf("gr_M64", sizeof(double2) * nz * nn);
     OKS_HIP(hipMemsetAsync(fail, 0, sizeof(int) * nz, ctx->stream));
     Normal64Args na{G32, R32, G, R, n, m, nz, splits, ridge};
     OKS_HIP(launch_normal64(na, nz, ctx->stream));
@@ -148,21 +152,21 @@ static void grappa_calibrate_batch(oks_ctx* ctx, const float2* grid, const uint2
         if (rn > 0)
             zg(ctx, nz, L + (size_t)r0 * n + k0, snn, n, 0, Y + (size_t)k0 * m, snm, m, 0, R + (size_t)r0 * m, snm, m,
                rn, m, kn, -1.0, 1.0, fail, 0, s2);
-        cnt += 5;
+        if (k0 > 0)
+            zg(ctx, nz, L + (size_t)k0 * n, snn, n, 2, Dk, sD, 64, 2, M + k0, snn, n, k0, kn, kn, 1.0, 0.0, fail, 0, s2);
+        cnt += k0 > 0 ? 6 : 5;
     }
     OKS_HIP(hipEventRecord(ctx->aux_join, s2));
     OKS_HIP(hipStreamWaitEvent(ctx->stream, ctx->aux_join, 0));
-    for (int kb = nblk - 1; kb >= 0; --kb) {
+    // Y[0:k0] -= M_k Y_k from the last block up, then X_k = D_k^H Y_k for
+    // every block at once
+    for (int kb = nblk - 1; kb >= 1; --kb) {
         const int k0 = kb * 64, kn = std::min(64, n - k0);
-        const double2* Dk = D + (size_t)kb * 64 * 64;
-        // X_k = Dk^H Y_k ; Y[0:k0] -= L[k0:k0+kn, 0:k0]^H X_k
-        zg(ctx, nz, Dk, sD, 64, 2, Y + (size_t)k0 * m, snm, m, 0, X + (size_t)k0 * m, snm, m, kn, m, kn, 1.0, 0.0,
-           fail);
-        if (k0 > 0)
-            zg(ctx, nz, L + (size_t)k0 * n, snn, n, 2, X + (size_t)k0 * m, snm, m, 0, Y, snm, m, k0, m, kn, -1.0, 1.0,
-               fail);
-        cnt += 2;
+        zg(ctx, nz, M + k0, snn, n, 0, Y + (size_t)k0 * m, snm, m, 0, Y, snm, m, k0, m, kn, -1.0, 1.0, fail);
+        cnt += 1;
     }
+    OKS_HIP(launch_blockdiag_h(D, Y, X, nz, n, m, nblk, ctx->stream));
+    cnt += 1;
     OKS_HIP(launch_weights_out(X, w, nz, nc, r, n, ctx->stream));
     if (launches) *launches = cnt + 1;
 }
