@@ -205,14 +205,17 @@ void* oks_ctx_stream(oks_ctx* ctx);
  * before it (the same linear map, FP32 rounding in another order);
  * "cols_pipe" (0/1, default 1) lets column passes prefetch the next coil;
  * "rows_waves" (0 = automatic, or 8..16) sets the wavefronts per workgroup of
- * the one-row-per-workgroup row passes; "cols_sx" (0 = automatic, or a power
+ * the one-row-per-workgroup row passes; "rows_wgs" (0 = as many as fit, 1,
+ * 2) the persistent workgroups per CU of the pipelined row pass; "cols_sx"
+ * (0 = automatic, or a power
  * of two up to 16) caps the columns per workgroup of the column passes and
  * "cols_threads" (0 = 512, or 256) sets their workgroup size (prefetching
  * needs 512).  Four switches (0/1, default 1) choose between two forms of
  * one computation whose FP32 rounding differs, both inside the strategies'
  * tolerances; 0 selects the earlier LDS-staged form: "cs_reg" (CS column
- * pass as register four-step transforms), "cols_reg" (IFFT + RSS column
- * pass likewise), "grappa_synth" (GRAPPA synthesis grouped by source row)
+ * pass as register four-step transforms), "cols_reg" (the inverse column
+ * passes likewise: the IFFT + RSS pass, and the complex-output passes at
+ * 64 ... 512 points), "grappa_synth" (GRAPPA synthesis grouped by source row)
  * and "esp_rowt" (ESPIRiT per-voxel stage from the row-contracted Gram). */
 int oks_ctx_set_option(oks_ctx* ctx, const char* name, int64_t value);
 /* Launch shapes from the autotuner (python -m openkspace_amd.autotune, no

@@ -247,7 +247,7 @@ static int* algorithm_option(oks_ctx* c, const std::string& n) {
 // Launch-shape options a tuning table may set: none changes the computed bits.
 static const char* const kLaunchOptions[] = {"cs_sx",        "cs_threads", "cols_sx",   "cols_threads",
                                              "rows_waves",   "cs_cols_map", "cs_upd_map", "cs_pipe",
-                                             "cs_coop"};
+                                             "cs_coop",      "rows_wgs"};
 
 int* oks::launch_option(oks_ctx* c, const std::string& n) {
     if (n == "cs_sx") return &c->cs_sx;
@@ -259,6 +259,7 @@ int* oks::launch_option(oks_ctx* c, const std::string& n) {
     if (n == "cs_upd_map") return &c->cs_upd_map;
     if (n == "cs_pipe") return &c->cs_pipe;
     if (n == "cs_coop") return &c->cs_coop;
+    if (n == "rows_wgs") return &c->rows_wgs;
     return nullptr;
 }
 
@@ -285,6 +286,9 @@ bool oks::launch_option_valid(const std::string& n, int64_t v, std::string* why)
     } else if (n == "rows_waves") {
         ok = v == 0 || (v >= 8 && v <= 16);
         *why = "rows_waves must be 0 or 8..16";
+    } else if (n == "rows_wgs") {
+        ok = v >= 0 && v <= 2;
+        *why = "rows_wgs must be 0 (as many as fit), 1 or 2";
     } else if (n == "cs_pipe" || n == "cs_coop") {
         ok = v == 0 || v == 1;
         *why = n + " must be 0 or 1";

@@ -82,6 +82,7 @@ struct oks_ctx {
     int esp_rowt = 1;                              // ESPIRiT per-voxel stage from the row-contracted Gram
     int rows_pipe = 1;                             // ifft-rss row pass: pipelined persistent kernel
     int rows_waves = 0;                            // one-row-per-workgroup row passes: wavefronts (0 = automatic)
+    int rows_wgs = 0;                              // pipelined row pass: workgroups per CU (0 = as many as fit)
     int cols_sx = 0;                               // column passes: columns per workgroup cap (0 = automatic)
     int cols_threads = 0;                          // column passes: workgroup size (0 = 512, or 256)
     int cols_pipe = 1;                             // column passes: prefetch the next coil

@@ -176,10 +176,14 @@ __global__ void __launch_bounds__(kWhitenThreads) k_whitener(WhitenArgs wa) {
             for (int i = 0; i < nc; ++i) {
                 const double2 v = i < lane ? make_double2(0.0, 0.0) : Li[i * nc + lane];
                 W[i * nc + lane] = make_float2((float)v.x, (float)v.y);
+                if (wa.Wt) wa.Wt[lane * nc + i] = make_float2((float)v.x, (float)v.y);
             }
         }
     } else {
-        for (int e = lane; e < nc * nc; e += 64) W[e] = make_float2(e / nc == e % nc ? 1.0f : 0.0f, 0.0f);
+        for (int e = lane; e < nc * nc; e += 64) {
+            W[e] = make_float2(e / nc == e % nc ? 1.0f : 0.0f, 0.0f);
+            if (wa.Wt) wa.Wt[e] = W[e];
+        }
     }
     if (lane == 0) {
         if (wa.per_request)
@@ -282,6 +286,39 @@ __device__ __forceinline__ int tri_bound(int g, int P, int nc, bool tri) {
     return s;
 }
 
+// W^T is read through the constant address space: the compiler then knows
+// that nothing in the kernel writes it, which wavefront-uniform scalar loads
+// require.
+typedef __attribute__((address_space(4))) const float const_float;
+
+__device__ __forceinline__ float2 ld_w(const const_float* Wt, int e) { return make_float2(Wt[2 * e], Wt[2 * e + 1]); }
+
+// Outputs ib .. ib + K - 1 of the whitening y = W s at one column `src` of the
+// transformed coils in LDS (A[j N + src] = coil j), j ascending as in
+// recon/prewhiten.rs:128-141: the columns below the block in full, then the
+// block's own lower triangle (j <= i) unrolled.  Wt = W^T in global memory;
+// ib is wavefront-uniform, so its reads are scalar loads.
+template <int K>
+__device__ __forceinline__ void white_block(const const_float* Wt, int nc, int ib, const float2* A, int N,
+                                            int src, float2* dst, long long out_plane, float scale) {
+    float2 acc[K];
+#pragma unroll
+    for (int u = 0; u < K; ++u) acc[u] = make_float2(0.f, 0.f);
+    for (int j = 0; j < ib; ++j) {
+        const float2 f = A[j * N + src];
+#pragma unroll
+        for (int u = 0; u < K; ++u) acc[u] = cadd(acc[u], cmul(ld_w(Wt, j * nc + ib + u), f));
+    }
+#pragma unroll
+    for (int jj = 0; jj < K; ++jj) {
+        const float2 f = A[(ib + jj) * N + src];
+#pragma unroll
+        for (int u = jj; u < K; ++u) acc[u] = cadd(acc[u], cmul(ld_w(Wt, (ib + jj) * nc + ib + u), f));
+    }
+#pragma unroll
+    for (int u = 0; u < K; ++u) dst[(size_t)(ib + u) * out_plane] = cscale(acc[u], scale);
+}
+
 template <int NCMAX, int N>
 __global__ void __launch_bounds__(kRowsPipeThreads) k_rows_pw_pipe(RowPwArgs a) {
     using F = FixedFft<N, true>;
@@ -297,6 +334,7 @@ __global__ void __launch_bounds__(kRowsPipeThreads) k_rows_pw_pipe(RowPwArgs a) 
     const float2* tw = stage_twiddles<F>(lds, a.p_enc);
     float2* Wt = lds + tw_slots<F>();            // nc * nc, Wt[j nc + i] = W[i nc + j]
     float2* A = Wt + (white ? nc * nc : 0);      // nc * N
+    int* red = (int*)(A + nc * N);               // 2 x 16: per-wavefront nonzero votes, by row parity
     if (white)
         for (int e = threadIdx.x; e < nc * nc; e += T) {
             const int i = e / nc, j = e - i * nc;
@@ -316,11 +354,23 @@ __global__ void __launch_bounds__(kRowsPipeThreads) k_rows_pw_pipe(RowPwArgs a) 
     };
     int P = 1;
     while (P < 8 && 2 * P <= nc && 2 * P * tx <= T) P *= 2;
-    const int span = T / P, q = (int)threadIdx.x / span, xt = (int)threadIdx.x - q * span;
+    // span is a multiple of 64, so the group q is the same across a wavefront
+    const int span = T / P, q = __builtin_amdgcn_readfirstlane((int)threadIdx.x / span);
+    const int xt = (int)threadIdx.x - q * span;
     const int i0 = tri_bound(q, P, nc, white), i1 = tri_bound(q + 1, P, nc, white);
-    long long row = blockIdx.x;
-    if (row < a.rows) prefetch(row);
-    for (; row < a.rows; row += gridDim.x) {
+    // Each workgroup walks a contiguous range of rows.  A row of every coil
+    // is a separate stream (the coil planes lie far apart), so a workgroup
+    // keeps its 2 nc streams (read and write) on the same pages for many rows;
+    // a stride through the rows would take new address translations for
+    // every coil at every row, and those misses, not the arithmetic, bounded
+    // the pass (measured: ~2800 first-level translation misses per CU on a
+    // 32 x 160 x 256 x 256 volume, 2.99 ms).
+    const long long rbeg = (long long)blockIdx.x * a.rows / gridDim.x;
+    const long long rend = (long long)(blockIdx.x + 1) * a.rows / gridDim.x;
+    long long row = rbeg;
+    if (row < rend) prefetch(row);
+    for (int par = 0; row < rend; ++row, par ^= 1) {
+        bool any = false;
 #pragma unroll
         for (int k = 0; k < KP; ++k) {
             const int e = (int)threadIdx.x + k * T;
@@ -328,6 +378,7 @@ __global__ void __launch_bounds__(kRowsPipeThreads) k_rows_pw_pipe(RowPwArgs a) 
                 const int c = e / N, x = e - c * N;
                 const int pos = x + half < N ? x + half : x + half - N;  // ifftshift
                 A[c * N + pos] = pf[k];
+                any |= (pf[k].x != 0.f) | (pf[k].y != 0.f);
             }
         }
 #pragma unroll 4
@@ -336,17 +387,51 @@ __global__ void __launch_bounds__(kRowsPipeThreads) k_rows_pw_pipe(RowPwArgs a) 
             if (e < total) {
                 const int c = e / N, x = e - c * N;
                 const int pos = x + half < N ? x + half : x + half - N;
-                A[c * N + pos] = a.in[(size_t)c * a.in_plane + (size_t)row * N + x];
+                const float2 v = a.in[(size_t)c * a.in_plane + (size_t)row * N + x];
+                A[c * N + pos] = v;
+                any |= (v.x != 0.f) | (v.y != 0.f);
             }
         }
+        // A row whose samples are all zero transforms to zero (W s = 0
+        // exactly when s = 0): it is flagged and nothing of it is written.
+        // The votes are double-buffered by row parity: a wavefront can reach
+        // the next row's vote before a slower one has read this row's, but
+        // not the row after (one barrier per row).
+        if (a.rowflag) {
+            const bool vote = __ballot(any) != 0ull;  // every lane takes part
+            if (lane == 0) red[par * 16 + w] = vote;
+        }
         lds_barrier();
-        if (row + gridDim.x < a.rows) prefetch(row + gridDim.x);
+        if (row + 1 < rend) prefetch(row + 1);
+        if (a.rowflag) {
+            int live = 0;
+#pragma unroll
+            for (int v = 0; v < T / 64; ++v) live |= red[par * 16 + v];
+            if (threadIdx.x == 0) a.rowflag[row] = (uint8_t)(live != 0);
+            if (!live) continue;
+        }
         for (int c = w; c < nc; c += T / 64) CFft<N, true>::run(A + c * N, tw, lane);
         lds_barrier();
         for (int xo = xt; xo < tx; xo += span) {
             const int src = (x0 + xo + half) % N;
             float2* dst = a.out + (size_t)row * tx + xo;
-            if (white) {
+            if (white && a.Wt) {
+                // W^T from global memory at wavefront-uniform addresses
+                // (scalar loads into SGPRs, FMA operands straight from them),
+                // blocks of up to U outputs with their exact triangle.  The
+                // constant address space tells the compiler that nothing in
+                // the kernel writes W^T, which scalar loads require.
+                const const_float* WtC = (const const_float*)a.Wt;
+                for (int ib = i0; ib < i1; ib += U) {
+                    switch (min(U, i1 - ib)) {
+#define OKS_WB(K) \
+    case K: white_block<K>(WtC, nc, ib, A, N, src, dst, a.out_plane, a.scale); break;
+                        OKS_WB(8) OKS_WB(7) OKS_WB(6) OKS_WB(5) OKS_WB(4) OKS_WB(3) OKS_WB(2) OKS_WB(1)
+#undef OKS_WB
+                        default: break;
+                    }
+                }
+            } else if (white) {
                 for (int ib = i0; ib < i1; ib += U) {
                     const int ie = min(ib + U, i1);
                     float2 acc[U];
@@ -649,6 +734,30 @@ __global__ void __launch_bounds__(512) k_cols(ColArgs a) {
     const int c0 = g * a.C / a.G, c1 = (g + 1) * a.C / a.G;
     const int half = a.centred ? n / 2 : 0;
     const int ty = a.ty, y0 = a.y0;
+    // zero-row flags: element (y, column) is read only where live(y, column).
+    // When the strip lies inside one flag column (rf_div a multiple of SX),
+    // its n line flags are staged in LDS once.
+    const bool fl_strip = a.rflag && a.rf_div % SX == 0;
+    uint8_t* fl = (uint8_t*)(acc + (a.mode == 1 ? (size_t)SX * ty : 0));  // n bytes
+    auto live = [&](int y, int col) {
+        if (!a.rflag) return true;
+        if (fl_strip) return fl[y] != 0;
+        return a.rflag[(size_t)y * a.rf_stride + (xs + col) / a.rf_div] != 0;
+    };
+    auto col_live = [&](int col) { return !a.sflag || a.sflag[(xs + col) / a.rf_div] != 0; };
+    if (a.sflag) {
+        bool any = false;
+        if (a.rf_div % SX == 0)
+            any = col_live(0);
+        else
+            for (int col = 0; col < SX; ++col) any |= xs + col < a.X && col_live(col);
+        if (!any) return;  // the same for every thread, before any barrier
+    }
+    if (fl_strip) {
+        const size_t xo = (size_t)(xs / a.rf_div);
+        for (int y = threadIdx.x; y < n; y += blockDim.x) fl[y] = a.rflag[(size_t)y * a.rf_stride + xo];
+        __syncthreads();
+    }
     if (a.mode == 1)
         for (int i = threadIdx.x; i < SX * ty; i += blockDim.x) acc[i] = 0.f;
     // PIPE (fixed lengths, several coils per workgroup): coil c + 1's strip is
@@ -664,7 +773,7 @@ __global__ void __launch_bounds__(512) k_cols(ColArgs a) {
             const int i = (int)threadIdx.x + k * 512;
             if (i < n * SX) {
                 const int y = i >> lsx, col = i & (SX - 1);
-                pf[k] = xs + col < a.X ? src[(size_t)y * a.X + xs + col] : make_float2(0.f, 0.f);
+                pf[k] = xs + col < a.X && live(y, col) ? src[(size_t)y * a.X + xs + col] : make_float2(0.f, 0.f);
             }
         }
     };
@@ -688,7 +797,7 @@ __global__ void __launch_bounds__(512) k_cols(ColArgs a) {
             for (int i = threadIdx.x; i < n * SX; i += blockDim.x) {
                 const int y = i >> lsx, col = i & (SX - 1);
                 float2 v = make_float2(0.f, 0.f);
-                if (xs + col < a.X) v = src[(size_t)y * a.X + xs + col];
+                if (xs + col < a.X && live(y, col)) v = src[(size_t)y * a.X + xs + col];
                 A[(size_t)col * ls + wrap_n(y + half, n)] = v;
             }
             __syncthreads();
@@ -712,7 +821,7 @@ __global__ void __launch_bounds__(512) k_cols(ColArgs a) {
             float2* dst = a.out + ((size_t)c * a.Zo + zi) * ty * a.X;
             for (int i = threadIdx.x; i < ty * SX; i += blockDim.x) {
                 const int yo = i >> lsx, col = i & (SX - 1);
-                if (xs + col < a.X)
+                if (xs + col < a.X && col_live(col))
                     dst[(size_t)yo * a.X + xs + col] = cscale(A[(size_t)col * ls + wrap_n(y0 + yo + half, n)], a.scale);
             }
             lds_barrier();
@@ -769,6 +878,18 @@ __global__ void __launch_bounds__(cols_reg_threads<N>()) k_cols_rss_reg(ColArgs 
     const int c0 = g * a.C / a.G, c1 = (g + 1) * a.C / a.G;
     const int half = a.centred ? N / 2 : 0;
     const int col0 = t % SX, grp0 = t / SX;
+    // zero-row flags (the same for every coil and plane): bit n1 of `lv` is
+    // set where row N2 n1 + grp (ifftshifted) of this column is read
+    unsigned long long lv = ~0ull;
+    if (a.rflag && grp0 < N2 && xs + col0 < a.X) {
+        lv = 0;
+        const size_t xo = (size_t)((xs + col0) / a.rf_div);
+        for (int n1 = 0; n1 < N1; ++n1) {
+            const int y = wrap_n(N2 * n1 + grp0 + half, N);
+            if (a.rflag[(size_t)y * a.rf_stride + xo]) lv |= 1ull << n1;
+        }
+    }
+    static_assert(N1 <= 64, "row flags are held as one 64-bit word");
     float2 va[N1];
     // The row offsets come from an opaque copy of X, and the task indices are
     // made opaque in every coil, so that the addresses derived from them are
@@ -782,7 +903,7 @@ __global__ void __launch_bounds__(cols_reg_threads<N>()) k_cols_rss_reg(ColArgs 
 #pragma unroll
             for (int n1 = 0; n1 < N1; ++n1) {
                 const int y = wrap_n(N2 * n1 + grp + half, N);
-                va[n1] = cin ? src[(unsigned)(y * xo)] : make_float2(0.f, 0.f);
+                va[n1] = cin && ((lv >> n1) & 1ull) ? src[(unsigned)(y * xo)] : make_float2(0.f, 0.f);
             }
         }
     };
@@ -831,6 +952,87 @@ __global__ void __launch_bounds__(cols_reg_threads<N>()) k_cols_rss_reg(ColArgs 
                 else
                     a.part[(((size_t)g * a.Zo + zi) * a.ty + yo) * a.X + xs + col] = acc[k2];
             }
+        }
+    }
+}
+
+// Register four-step form of k_cols mode 0 (complex output, inverse,
+// compile-time n = N1 N2): one workgroup per (coil c, output slice zi, strip
+// of 16 columns).  Phase A (task (column, n2)) reads x[N2 n1 + n2] of the
+// column (the ifftshift folded into the rows read, zero-row flags as in
+// k_cols), runs the N1-point inverse DFT in registers and writes row k1 of
+// the column's LDS block times W_n^{n2 k1}; phase B (task (column, k1)) runs
+// the N2-point inverse DFT of row k1, whose outputs are the slots
+// k1 + N1 k2, and stores those in the centred window [y0, y0 + ty) times
+// scale.  Every read of the strip precedes the barrier and every write
+// follows it, so the pass may run in place.
+template <int N>
+__global__ void __launch_bounds__(cols_reg_threads<N>()) k_cols_reg_c(ColArgs a) {
+    using Gm = RegGeom<N, 16>;
+    constexpr int N1 = Gm::N1, N2 = Gm::N2, SX = Gm::SX, N2P = Gm::N2P, CS = Gm::CS, NT = Gm::NT;
+    static_assert(N1 <= 64, "row flags are held as one 64-bit word");
+    extern __shared__ float2 lds[];
+    float2* tw = lds;  // exp(+2 pi i e / N)
+    float2* L = lds + Gm::TW;
+    const int t = threadIdx.x;
+    const int strips = (a.X + SX - 1) / SX;
+    const int per_c = a.Zo * strips;
+    const int c = blockIdx.x / per_c;
+    const int rem = blockIdx.x - c * per_c;
+    const int zi = rem / strips;
+    const int z = a.z0 + zi;
+    const int xs = (rem - zi * strips) * SX;
+    auto col_live = [&](int col) { return !a.sflag || a.sflag[(xs + col) / a.rf_div] != 0; };
+    if (a.sflag) {
+        bool any = false;
+        for (int col = 0; col < SX; ++col) any |= xs + col < a.X && col_live(col);
+        if (!any) return;  // the same for every thread, before any barrier
+    }
+    for (int i = t; i < N; i += NT) tw[i] = a.p.tw[i];
+    const int half = a.centred ? N / 2 : 0;
+    const int col = t % SX, grp = t / SX;
+    const bool cin = xs + col < a.X;
+    float2 va[N1];
+    if (grp < N2) {
+        unsigned long long lv = ~0ull;
+        if (a.rflag && cin) {
+            lv = 0;
+            const size_t xo = (size_t)((xs + col) / a.rf_div);
+            for (int n1 = 0; n1 < N1; ++n1) {
+                const int y = wrap_n(N2 * n1 + grp + half, N);
+                if (a.rflag[(size_t)y * a.rf_stride + xo]) lv |= 1ull << n1;
+            }
+        }
+        const float2* src = a.in + ((size_t)c * a.Z + z) * N * a.X + xs + col;
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) {
+            const int y = wrap_n(N2 * n1 + grp + half, N);
+            va[n1] = cin && ((lv >> n1) & 1ull) ? src[(size_t)y * a.X] : make_float2(0.f, 0.f);
+        }
+    }
+    lds_barrier();  // the twiddle table
+    float2* Lc = L + col * CS;
+    if (grp < N2) {
+        const int n2 = grp;
+        rdft_emit<N1, true>(va, [&](int k1, float2 v) {
+            Lc[k1 * N2P + n2] = k1 == 0 ? v : cmul(v, twid<true>(tw, n2 * k1));
+        });
+    }
+    lds_barrier();
+    if (grp < N1 && cin && col_live(col)) {
+        const float2* row = Lc + grp * N2P;
+        float2 v[N2];
+#pragma unroll
+        for (int j = 0; j < N2; ++j) v[j] = row[j];
+        rdft<N2, true>(v);
+        float2* dst = a.out + ((size_t)c * a.Zo + zi) * a.ty * a.X + xs + col;
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) {
+            // slot k1 + N1 k2 holds the centred output row yo with
+            // slot = (y0 + yo + half) mod N
+            int yo = grp + N1 * k2 - a.y0 - half;
+            yo = ((yo % N) + N) % N;
+            if (yo < a.ty) dst[(size_t)yo * a.X] = cscale(v[k2], a.scale);
         }
     }
 }
@@ -894,12 +1096,28 @@ __global__ void k_rss(const float2* __restrict__ in, int C, long long M, float* 
     }
 }
 
+// yflag[y] = OR over z of rowflag[z * ny + y].
+__global__ void __launch_bounds__(256) k_or_rows(const uint8_t* __restrict__ rowflag, int nz, int ny,
+                                                 uint8_t* __restrict__ yflag) {
+    // workgroup: 64 consecutive y, its 4 wavefronts splitting z
+    __shared__ uint8_t part[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int y = blockIdx.x * 64 + lane;
+    uint8_t f = 0;
+    if (y < ny)
+        for (int z = w; z < nz; z += 4) f |= rowflag[(size_t)z * ny + y];
+    part[w][lane] = f;
+    __syncthreads();
+    if (w == 0 && y < ny) yflag[y] = part[0][lane] | part[1][lane] | part[2][lane] | part[3][lane];
+}
+
 // ------------------------------------------------------------- launchers
-hipError_t launch_whitener(const float2* noise, int nc, int ns, float2* W, int* status, hipStream_t s) {
+hipError_t launch_whitener(const float2* noise, int nc, int ns, float2* W, int* status, hipStream_t s, float2* Wt) {
     WhitenArgs a{};
     a.noise[0] = noise;
     a.ns[0] = ns;
     a.W = W;
+    a.Wt = Wt;
     a.status = status;
     a.nc = nc;
     a.per_request = 0;
@@ -908,6 +1126,7 @@ hipError_t launch_whitener(const float2* noise, int nc, int ns, float2* W, int* 
 
 hipError_t launch_whitener_batch(const WhitenArgs& a, int B, hipStream_t s) {
     if (a.nc < 1 || a.nc > 32 || B < 1 || B > OKS_MAX_BATCH) return hipErrorInvalidValue;
+    if (a.Wt && B != 1) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_whitener, dim3((unsigned)B), dim3(kWhitenThreads), 0, s, a);
     return hipGetLastError();
 }
@@ -955,15 +1174,15 @@ static bool pipe_size(int n) {
         default: return false;
     }
 }
-static bool rows_pipe_used(bool pipe, int os_mode, int nx) { return pipe && !os_mode && pipe_size(nx); }
+bool rows_pipe_used(bool pipe, int os_mode, int nx) { return pipe && !os_mode && pipe_size(nx); }
 static bool rows_pipe_used(const RowPwArgs& a) { return rows_pipe_used(a.pipe != 0, a.os_mode, a.nx); }
 
 size_t rows_prewhiten_lds(int nc, int nx, int os_mode, int rec, bool white, bool pipe) {
     if (os_mode) return os_rows_lds(nc, nx, rec);
     if (!is_fixed(nx)) return ((size_t)nc * nx + (size_t)8 * nx) * sizeof(float2);
     // pipelined kernel: twiddles, W^T when whitening, coils
-    if (rows_pipe_used(pipe, os_mode, nx))
-        return (tw_lds_slots(nx) + (white ? (size_t)nc * nc : 0) + (size_t)nc * nx) * sizeof(float2);
+    if (rows_pipe_used(pipe, os_mode, nx))  // + 32 ints of nonzero votes
+        return (tw_lds_slots(nx) + (white ? (size_t)nc * nc : 0) + (size_t)nc * nx + 16) * sizeof(float2);
     return (tw_lds_slots(nx) + (size_t)nc * nx) * sizeof(float2);
 }
 
@@ -1014,7 +1233,8 @@ template <int NCMAX, int N>
 static hipError_t go_rows_pw_pipe(const RowPwArgs& a, size_t lds, hipStream_t s) {
     OKS_ATTR(hipFuncSetAttribute((const void*)k_rows_pw_pipe<NCMAX, N>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)lds));
-    const long long per_cu = std::max<long long>(1, std::min<long long>(2, (160 * 1024) / (long long)lds));
+    long long per_cu = std::max<long long>(1, std::min<long long>(2, (160 * 1024) / (long long)lds));
+    if (a.wgs > 0) per_cu = std::min<long long>(per_cu, a.wgs);
     const long long grid = std::min<long long>(a.rows, 256 * per_cu);
     hipLaunchKernelGGL((k_rows_pw_pipe<NCMAX, N>), dim3((unsigned)grid), dim3(kRowsPipeThreads), lds, s, a);
     return hipGetLastError();
@@ -1097,8 +1317,9 @@ hipError_t launch_rows(const RowArgs& a, bool inverse, hipStream_t s) {
 }
 
 size_t cols_lds(int sx, int n, int ty, int mode) {
+    // + n bytes of zero-row flags
     return (tw_lds_slots(n) + (size_t)(is_fixed(n) ? 1 : 2) * sx * (n + 1)) * sizeof(float2) +
-           (mode == 1 ? (size_t)sx * ty * sizeof(float) : 0);
+           (mode == 1 ? (size_t)sx * ty * sizeof(float) : 0) + (((size_t)n + 7) & ~(size_t)7);
 }
 
 static int cols_threads(const ColArgs& a) { return a.threads == 256 ? 256 : 512; }
@@ -1132,6 +1353,25 @@ static hipError_t go_cols_reg(const ColArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <int N>
+static hipError_t go_cols_reg_c(const ColArgs& a, hipStream_t s) {
+    using Gm = RegGeom<N, 16>;
+    OKS_ATTR(hipFuncSetAttribute((const void*)k_cols_reg_c<N>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)Gm::lds_bytes));
+    const long long grid = (long long)a.C * a.Zo * ((a.X + Gm::SX - 1) / Gm::SX);
+    if (grid < 1 || grid > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_cols_reg_c<N>, dim3((unsigned)grid), dim3(Gm::NT), Gm::lds_bytes, s, a);
+    return hipGetLastError();
+}
+
+bool cols_reg_c_ok(int n, int mode, bool inverse) {
+    if (mode != 0 || !inverse) return false;
+    switch (n) {
+        case 64: case 128: case 160: case 256: case 320: case 384: case 512: return true;
+        default: return false;
+    }
+}
+
 bool cols_reg_ok(int n, int mode, bool inverse) {
     if (mode != 1 || !inverse) return false;
     switch (n) {
@@ -1142,6 +1382,15 @@ bool cols_reg_ok(int n, int mode, bool inverse) {
 
 template <bool INV>
 static hipError_t go_cols_dir(const ColArgs& a, hipStream_t s) {
+    if (a.reg_c && a.mode == 0 && INV) {
+        switch (a.p.n) {
+#define OKS_CASE(N) \
+    case N: return go_cols_reg_c<N>(a, s);
+            OKS_CASE(64) OKS_CASE(128) OKS_CASE(160) OKS_CASE(256) OKS_CASE(320) OKS_CASE(384) OKS_CASE(512)
+#undef OKS_CASE
+            default: break;
+        }
+    }
     if (a.reg && cols_reg_ok(a.p.n, a.mode, INV)) {
         switch (a.p.n) {
 #define OKS_CASE(N) \
@@ -1162,12 +1411,15 @@ static hipError_t go_cols_dir(const ColArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_cols(const ColArgs& a, bool inverse, hipStream_t s) {
-    if (!(a.reg && cols_reg_ok(a.p.n, a.mode, inverse)) && cols_lds(a.sx, a.p.n, a.ty, a.mode) > 160 * 1024)
+    if (!(a.reg && cols_reg_ok(a.p.n, a.mode, inverse)) && !(a.reg_c && cols_reg_c_ok(a.p.n, a.mode, inverse)) &&
+        cols_lds(a.sx, a.p.n, a.ty, a.mode) > 160 * 1024)
         return hipErrorInvalidValue;
     if (a.sx < 1 || (a.sx & (a.sx - 1)) != 0) return hipErrorInvalidValue;  // strip index maps use shifts
     if (a.G < 1 || a.G > a.C) return hipErrorInvalidValue;
     if (a.mode == 1 && a.G > 1 && !a.part) return hipErrorInvalidValue;
     if (a.threads != 0 && a.threads != 256 && a.threads != 512) return hipErrorInvalidValue;
+    if ((a.rflag || a.sflag) && a.rf_div < 1) return hipErrorInvalidValue;
+    if (a.sflag && a.mode != 0) return hipErrorInvalidValue;  // skipped strips write nothing
     return inverse ? go_cols_dir<true>(a, s) : go_cols_dir<false>(a, s);
 }
 
@@ -1230,6 +1482,12 @@ hipError_t launch_prewhiten(const float2* in, float2* d, const float2* W, int nc
     else if (nc <= 16) hipLaunchKernelGGL((k_prewhiten<16, 2>), dim3(grid), dim3(256), 0, s, in, d, W, nc, M);
     else if (nc <= 32) hipLaunchKernelGGL((k_prewhiten<32, 1>), dim3(grid), dim3(256), 0, s, in, d, W, nc, M);
     else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_or_rows(const uint8_t* rowflag, int nz, int ny, uint8_t* yflag, hipStream_t s) {
+    if (nz < 1 || ny < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_or_rows, dim3((unsigned)((ny + 63) / 64)), dim3(256), 0, s, rowflag, nz, ny, yflag);
     return hipGetLastError();
 }
 

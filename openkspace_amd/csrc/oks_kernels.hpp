@@ -18,6 +18,7 @@ struct WhitenArgs {
     const float2* noise[OKS_MAX_BATCH];  // [nc, ns[b]] per request, or nullptr
     int ns[OKS_MAX_BATCH];
     float2* W;          // [B, nc, nc]
+    float2* Wt;         // optional, B = 1 only: W transposed, Wt[j nc + i] = W[i nc + j]
     int* status;        // per_request ? [B] (0/1 written) : [1] (sticky, raised only)
     int nc;
     int per_request;
@@ -28,6 +29,7 @@ struct RowPwArgs {
     const float2* in;   // first row of the chunk, coil 0; coil c at in + c * in_plane
     float2* out;        // [nc, rows, tx] with coil stride out_plane
     const float2* W;    // [nc, nc] lower triangular, or nullptr
+    const float2* Wt;   // optional: W transposed in global memory, read with scalar loads (pipelined kernel)
     int nc;
     long long rows;     // rows in this launch
     long long in_plane, out_plane;
@@ -36,8 +38,13 @@ struct RowPwArgs {
     int os_mode, rec;   // reference oversampling removal to `rec` samples
     int pipe;           // fixed lengths: pipelined persistent kernel (1) or one row per workgroup (0)
     int waves;          // one row per workgroup: wavefronts per workgroup (0 = automatic, 8..16)
+    int wgs;            // pipelined: persistent workgroups per CU (0 = as many as fit, at most 2)
     FftPlan p_enc;      // n = nx, inverse
     FftPlan p_rec;      // n = rec, forward (os_mode)
+    // optional [rows], pipelined kernel only (rows_pipe_used): 1 where any
+    // input sample of the row (any coil) is nonzero, else 0; a zero row's
+    // output is then not written (its transform is zero)
+    uint8_t* rowflag;
 };
 
 // Generic centred transform along the contiguous axis.
@@ -72,12 +79,25 @@ struct ColArgs {
     int pipe;           // prefetch the next coil's strip when a workgroup loops over coils (512 threads)
     int threads;        // workgroup size: 0 = 512, or 256
     int reg;            // RSS epilogue with the register four-step transforms where cols_reg_ok
+    int reg_c;          // mode 0, inverse: register four-step transforms for n in {64 .. 512}
     FftPlan p;
+    // Optional zero-row flags, the same for every plane: input element (line
+    // index i along n, column x) is zero, and not read, where
+    // rflag[i * rf_stride + x / rf_div] == 0.  With sflag (mode 0), columns x
+    // with sflag[x / rf_div] == 0 are neither read nor written; a workgroup
+    // whose strip holds no other column exits at once.
+    const uint8_t* rflag;
+    const uint8_t* sflag;
+    long long rf_stride;
+    int rf_div;
 };
 
-hipError_t launch_whitener(const float2* noise, int nc, int ns, float2* W, int* status, hipStream_t s);
+hipError_t launch_whitener(const float2* noise, int nc, int ns, float2* W, int* status, hipStream_t s,
+                           float2* Wt = nullptr);
 hipError_t launch_whitener_batch(const WhitenArgs& a, int B, hipStream_t s);
 size_t rows_prewhiten_lds(int nc, int nx, int os_mode, int rec, bool white, bool pipe);
+// Whether launch_rows_prewhiten_ifft runs the pipelined persistent kernel.
+bool rows_pipe_used(bool pipe, int os_mode, int nx);
 hipError_t launch_rows_prewhiten_ifft(const RowPwArgs& a, hipStream_t s);
 hipError_t launch_rows(const RowArgs& a, bool inverse, hipStream_t s);
 size_t cols_lds(int sx, int n, int ty, int mode);
@@ -85,11 +105,15 @@ hipError_t launch_cols(const ColArgs& a, bool inverse, hipStream_t s);
 // The column pass runs k_cols_rss_reg (ColArgs::reg set): RSS epilogue,
 // inverse transform, a length with a register four-step factorisation.
 bool cols_reg_ok(int n, int mode, bool inverse);
+// The column pass runs k_cols_reg_c (ColArgs::reg_c set): mode 0, inverse.
+bool cols_reg_c_ok(int n, int mode, bool inverse);
 size_t rows_lds(int n);
 hipError_t launch_rss_finalize(const float* part, int G, long long M, float* out, float scale, hipStream_t s);
 hipError_t launch_prewhiten(const float2* in, float2* d, const float2* W, int nc, long long M, hipStream_t s);
 hipError_t launch_prep_os(const RowPwArgs& a, const uint8_t* acq, hipStream_t s);
 hipError_t launch_rss(const float2* in, int C, long long M, float* out, float scale, hipStream_t s);
+// yflag[y] = OR over z of rowflag[z * ny + y]
+hipError_t launch_or_rows(const uint8_t* rowflag, int nz, int ny, uint8_t* yflag, hipStream_t s);
 hipError_t launch_os_remove(const float2* in, float2* out, long long rows, int enc, int rec, const FftPlan& p_enc,
                             const FftPlan& p_rec, hipStream_t s);
 

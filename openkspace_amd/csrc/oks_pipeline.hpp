@@ -12,9 +12,21 @@
 
 namespace oks {
 
+// The noise whitener W [nc, nc] (nullptr when skipped), followed in the same
+// buffer by its transpose.
 const float2* whitener(oks_ctx* ctx, const float2* noise, int64_t nc, int64_t nn, uint32_t* flags, const char* slot);
+// The transpose that follows a W returned by whitener().
+inline const float2* whitener_t(const float2* W, int64_t nc) { return W ? W + nc * nc : nullptr; }
+// Zero-row flags of a column pass (ColArgs::rflag / sflag): none by default.
+struct ColFlags {
+    const uint8_t* rflag = nullptr;
+    const uint8_t* sflag = nullptr;
+    long long rf_stride = 0;
+    int rf_div = 1;
+};
 void col_pass(oks_ctx* ctx, const float2* in, float2* out_c, float* out_r, int C, int Z, int z0, int Zo, int n,
-              int X, int mode, int y0, int ty, float scale, bool inverse, bool centred, int G, const char* slot);
+              int X, int mode, int y0, int ty, float scale, bool inverse, bool centred, int G, const char* slot,
+              const ColFlags& fl = ColFlags{});
 void row_pass(oks_ctx* ctx, const float2* in, float2* out, long long rows, int n, int tx, int x0, float scale,
               bool inverse, bool centred, const uint8_t* mask, long long mask_rows);
 void fft_axis(oks_ctx* ctx, float2* d, const int64_t* shape, int ndim, int axis, bool inverse, float scale,

@@ -29,8 +29,9 @@ const float2* whitener(oks_ctx* ctx, const float2* noise, int64_t nc, int64_t nn
         return nullptr;
     }
     if (nc > 32) fail(OKS_E_UNSUPPORTED, "pre-whitening supports at most 32 channels");
-    float2* W = (float2*)ctx->buf(slot, sizeof(float2) * nc * nc);
-    OKS_HIP(launch_whitener(noise, (int)nc, (int)nn, W, ctx->d_status + 0, ctx->stream));
+    // W, then W^T (whitener_t): the pipelined row pass reads the transpose
+    float2* W = (float2*)ctx->buf(slot, sizeof(float2) * 2 * nc * nc);
+    OKS_HIP(launch_whitener(noise, (int)nc, (int)nn, W, ctx->d_status + 0, ctx->stream, W + nc * nc));
     return W;
 }
 
@@ -42,8 +43,13 @@ static int pick_sx(int n, int ty, int mode, int X) {
 }
 
 void col_pass(oks_ctx* ctx, const float2* in, float2* out_c, float* out_r, int C, int Z, int z0, int Zo, int n,
-              int X, int mode, int y0, int ty, float scale, bool inverse, bool centred, int G, const char* slot) {
+              int X, int mode, int y0, int ty, float scale, bool inverse, bool centred, int G, const char* slot,
+              const ColFlags& fl) {
     ColArgs a{};
+    a.rflag = fl.rflag;
+    a.sflag = fl.sflag;
+    a.rf_stride = fl.rf_stride;
+    a.rf_div = fl.rf_div;
     a.in = in;
     a.out = out_c;
     a.out_r = out_r;
@@ -63,7 +69,9 @@ void col_pass(oks_ctx* ctx, const float2* in, float2* out_c, float* out_r, int C
     if (ctx->cols_sx > 0) a.sx = std::min(a.sx, ctx->cols_sx);
     a.threads = ctx->cols_threads;
     a.reg = ctx->cols_reg;
-    if (!(a.reg && cols_reg_ok(n, mode, inverse)) && cols_lds(a.sx, n, ty, mode) > 160 * 1024)
+    a.reg_c = ctx->cols_reg;
+    if (!(a.reg && cols_reg_ok(n, mode, inverse)) && !(a.reg_c && cols_reg_c_ok(n, mode, inverse)) &&
+        cols_lds(a.sx, n, ty, mode) > 160 * 1024)
         fail(OKS_E_UNSUPPORTED, "transform length too large for LDS");
     a.G = std::max(1, std::min(G, C));
     const long long M = (long long)Zo * ty * X;
@@ -159,6 +167,7 @@ float2* prep_grid(oks_ctx* ctx, const oks_kspace* k, const float2* W, int os_mod
         a.in = (const float2*)k->kspace;
         a.out = g;
         a.W = W;
+        a.Wt = whitener_t(W, k->nc);
         a.nc = (int)k->nc;
         a.rows = rows;
         a.in_plane = (long long)plane;
@@ -167,6 +176,7 @@ float2* prep_grid(oks_ctx* ctx, const oks_kspace* k, const float2* W, int os_mod
         a.rec = rec;
         a.os_mode = 1;
         a.waves = ctx->rows_waves;
+        a.wgs = ctx->rows_wgs;
         a.p_enc = ctx->plan((int)k->nx);
         a.p_rec = ctx->plan(rec);
         OKS_HIP(launch_prep_os(a, acq, ctx->stream));
@@ -213,7 +223,9 @@ void ifft_rss_grid(oks_ctx* ctx, const float2* k, int64_t nc, int64_t nz, int64_
     RowPwArgs a{};
     a.pipe = ctx->rows_pipe;
     a.waves = ctx->rows_waves;
+    a.wgs = ctx->rows_wgs;
     a.W = W;
+    a.Wt = whitener_t(W, nc);
     a.nc = (int)nc;
     a.nx = (int)nx;
     a.tx = tx;
@@ -256,20 +268,45 @@ void ifft_rss_grid(oks_ctx* ctx, const float2* k, int64_t nc, int64_t nz, int64_
                      (int)w[1].off, ty, 1.0f / (float)ny, true, true, G, "rss_part");
         }
     } else {
+        // 3-D: kx (with the whitening and x crop), then kz in place, then ky
+        // with the RSS and the y/z windows.  The row pass flags the (z, y)
+        // rows whose samples are all zero (the ky lines an undersampled
+        // acquisition never filled) and writes nothing for them; the kz pass
+        // reads only flagged rows and skips the ky lines that are zero for
+        // every z, which the ky pass then does not read either.  The
+        // transforms of zero lines are zero, so the result is unchanged.
+        // Only the pipelined row kernel flags rows (the one-row-per-workgroup
+        // kernels have no LDS to spare for the vote at 15 x 640 samples).
         const int64_t rows = nz * ny;
         float2* tmp = (float2*)ctx->buf("rss_tmp", sizeof(float2) * nc * rows * tx);
+        uint8_t* rowflag = rows_pipe_used(ctx->rows_pipe != 0, os_mode, (int)nx)
+                               ? (uint8_t*)ctx->buf("rss_rowflag", (size_t)rows + (size_t)ny)
+                               : nullptr;
+        uint8_t* yflag = rowflag ? rowflag + rows : nullptr;
         a.in = k;
         a.out = tmp;
         a.rows = rows;
         a.out_plane = rows * tx;
+        a.rowflag = rowflag;
         OKS_HIP(launch_rows_prewhiten_ifft(a, ctx->stream));
-        float2* tmp2 = (float2*)ctx->buf("rss_tmp2", sizeof(float2) * nc * nz * ty * tx);
-        col_pass(ctx, tmp, tmp2, nullptr, 1, (int)(nc * nz), 0, (int)(nc * nz), (int)ny, tx, 0, (int)w[1].off, ty,
-                 1.0f / (float)ny, true, true, 1, nullptr);
-        const int strips = (ty * tx + 15) / 16;
-        const int G = rss_groups((int)nc, strips);
-        col_pass(ctx, tmp2, nullptr, out, (int)nc, 1, 0, 1, (int)nz, ty * tx, 1, (int)w[0].off, (int)w[0].len,
-                 (nz > 1) ? 1.0f / (float)nz : 1.0f, true, true, G, "rss_part");
+        if (rowflag) OKS_HIP(launch_or_rows(rowflag, (int)nz, (int)ny, yflag, ctx->stream));
+        if (nz > 1) {
+            ColFlags fz;
+            fz.rflag = rowflag;
+            fz.sflag = yflag;
+            fz.rf_stride = ny;
+            fz.rf_div = tx;
+            col_pass(ctx, tmp, tmp, nullptr, 1, (int)nc, 0, (int)nc, (int)nz, (int)(ny * tx), 0, 0, (int)nz,
+                     1.0f / (float)nz, true, true, 1, nullptr, fz);
+        }
+        ColFlags fy;
+        fy.rflag = yflag;
+        fy.rf_stride = 1;
+        fy.rf_div = tx;
+        const int strips = (tx + 15) / 16;
+        const int G = rss_groups((int)nc, (int)w[0].len * strips);
+        col_pass(ctx, tmp, nullptr, out, (int)nc, (int)nz, (int)w[0].off, (int)w[0].len, (int)ny, tx, 1,
+                 (int)w[1].off, ty, 1.0f / (float)ny, true, true, G, "rss_part", fy);
     }
 }
 
@@ -294,7 +331,9 @@ static void homodyne_dev(oks_ctx* ctx, const oks_kspace* k, const float2* W, int
         RowPwArgs a{};
         a.pipe = ctx->rows_pipe;
         a.waves = ctx->rows_waves;
+        a.wgs = ctx->rows_wgs;
         a.W = W;
+        a.Wt = whitener_t(W, nc);
         a.nc = (int)nc;
         a.nx = (int)nx;
         a.tx = tx;

@@ -258,3 +258,46 @@ def test_forced_fft_mode(ctx, three_d, mode):
         assert rel(got, auto) > 1e-3
     else:
         assert np.array_equal(got, auto)
+
+
+@pytest.mark.parametrize("rows_pipe", [1, 0])
+@pytest.mark.parametrize("nc,nx,prewhiten", [(20, 64, 1), (5, 40, 1), (6, 128, 0)])
+def test_ifft_rss_3d_zero_rows(ctx, rows_pipe, nc, nx, prewhiten):
+    # 3-D tail with zero (z, y) rows: whole ky lines never acquired (as in a
+    # ky-undersampled volume), single rows zero in some partitions only, and
+    # a line zero in every partition but one.  The pipelined row pass flags
+    # all-zero rows and the kz/ky passes skip them; the image equals the
+    # oracle's.  nc = 20 and 5 take the 32- and 16-channel kernels; nx = 40
+    # (runtime transform length) and rows_pipe 0 run the one-row-per-workgroup
+    # kernel, which flags nothing, so nothing is skipped.
+    case = synth.make_case(nc=nc, nz=6, ny=32, nx=nx, os_x=1, kind="full", three_d=True, seed=11, ellipses=4)
+    k, m = case["kspace"], case["mask"]
+    rng = np.random.default_rng(3)
+    dead_y = [y for y in range(32) if y % 3 == 1 and not 12 <= y < 20]
+    k[:, :, dead_y, :] = 0
+    m[:, dead_y, :] = 0
+    for z, y in zip(rng.integers(0, 6, 10), rng.integers(0, 32, 10)):
+        k[:, z, y, :] = 0
+        m[z, y, :] = 0
+    k[:, 1:, 2, :] = 0
+    m[1:, 2, :] = 0
+    k[3, 2, 5, 7] = 0.5 + 0.25j  # one nonzero sample in one coil keeps the row live
+    m[2, 5, 7] = 1
+    d = abi.IfftRssDesc.default(remove_oversampling=0, prewhiten=prewhiten)
+    st, ref32, rflags, _ = O.recon("ifft_rss", d, case, "f32")
+    _, ref64, _, _ = O.recon("ifft_rss", d, case, "f64")
+    ctx.set_option("rows_pipe", rows_pipe)
+    try:
+        got, flags, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", d, case)
+    finally:
+        ctx.set_option("rows_pipe", 1)
+    assert st == 0 and flags == rflags and got.shape == ref32.shape
+    assert_parity(got, ref32, ref64, TOL32, f"3-D zero rows nc={nc} nx={nx} pipe={rows_pipe}")
+
+
+def test_ifft_rss_3d_all_zero(ctx):
+    case = synth.make_case(nc=4, nz=4, ny=16, nx=32, os_x=1, kind="full", three_d=True, seed=2, ellipses=2)
+    case["kspace"][:] = 0
+    d = abi.IfftRssDesc.default(remove_oversampling=0, prewhiten=0, partial_fourier=0)
+    got, _, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", d, case)
+    assert got.shape == (4, 16, 32) and not np.any(got)
