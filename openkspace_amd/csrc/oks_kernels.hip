@@ -234,51 +234,8 @@ __device__ __forceinline__ void load_whitened(const RowPwArgs& a, float2* A, int
     }
 }
 
-template <int NCMAX, class F>
-__global__ void __launch_bounds__(rows_pw_bound(F::n)) k_rows_pw(RowPwArgs a) {
-    extern __shared__ float2 lds[];
-    const int n = F::n ? F::n : a.nx;
-    const int nc = a.nc;
-    const int WPB = blockDim.x >> 6;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const float2* tw = stage_twiddles<F>(lds, a.p_enc);
-    float2* A = lds + tw_slots<F>();                          // nc * n
-    float2* B = F::scratch ? A + (size_t)nc * n : nullptr;    // WPB * n
-    const size_t row = blockIdx.x;
-    load_whitened<NCMAX>(a, A, n, row);
-    __syncthreads();
-    const int tx = a.tx, x0 = a.x0, half = n / 2;
-    for (int c = w; c < nc; c += WPB) {
-        float2* res = F::run(A + (size_t)c * n, B ? B + (size_t)w * n : nullptr, a.p_enc, tw, lane);
-        float2* dstp = a.out + (size_t)c * a.out_plane + row * tx;
-        for (int x = lane; x < tx; x += 64) dstp[x] = cscale(res[wrap_n(x0 + x + half, n)], a.scale);
-        wave_sync();
-    }
-}
-
-// Pipelined row pass for the fixed transform lengths.  Persistent workgroups
-// walk rows blockIdx.x, blockIdx.x + gridDim.x, ...; while the coils of one
-// row are transformed in LDS, the raw samples of the next row are already on
-// their way into registers.  The whitening y = W s (recon/prewhiten.rs:128-141)
-// is applied after the kx IFFT, to the kept columns only: both are linear
-// maps on different axes, so they commute.  In the output phase P thread
-// groups share each kept column, group g taking the coil rows [b_g, b_g+1)
-// chosen so the lower-triangular multiply counts balance; a thread forms 8
-// output rows at a time from the transformed coils in LDS and W^T in LDS.
-constexpr int kRowsPipeThreads = 1024;
-
-// Lengths the pipelined kernel is built for.  368 and 640 are left to the
-// one-row-per-workgroup kernel: their transforms need more registers than a
-// 1024-thread workgroup leaves, and on 640-point rows with 15 coils the
-// pipelined form measured 20 % slower.
-#define OKS_PIPE_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(160) X(256) X(320) X(384) X(512)
-
-// Elements per thread of the next row that are prefetched into registers.
-constexpr int rows_pipe_prefetch(int ncmax, int n, int kmax) {
-    const int cap = (ncmax == 32 && n == 512) ? 8 : kmax;
-    return kmax < cap ? kmax : cap;
-}
-
+// Output rows [g nc / P, (g + 1) nc / P) of thread group g of P, or with
+// tri the split that balances the lower-triangular multiply counts.
 __device__ __forceinline__ int tri_bound(int g, int P, int nc, bool tri) {
     if (!tri) return g * nc / P;
     int s = 0;  // smallest s with s (s + 1) P >= g nc (nc + 1)
@@ -292,6 +249,15 @@ __device__ __forceinline__ int tri_bound(int g, int P, int nc, bool tri) {
 typedef __attribute__((address_space(4))) const float const_float;
 
 __device__ __forceinline__ float2 ld_w(const const_float* Wt, int e) { return make_float2(Wt[2 * e], Wt[2 * e + 1]); }
+
+// acc + w f as four fused multiply-adds
+__device__ __forceinline__ float2 cfma(float2 w, float2 f, float2 acc) {
+    acc.x = fmaf(w.x, f.x, acc.x);
+    acc.x = fmaf(-w.y, f.y, acc.x);
+    acc.y = fmaf(w.x, f.y, acc.y);
+    acc.y = fmaf(w.y, f.x, acc.y);
+    return acc;
+}
 
 // Outputs ib .. ib + K - 1 of the whitening y = W s at one column `src` of the
 // transformed coils in LDS (A[j N + src] = coil j), j ascending as in
@@ -307,17 +273,64 @@ __device__ __forceinline__ void white_block(const const_float* Wt, int nc, int i
     for (int j = 0; j < ib; ++j) {
         const float2 f = A[j * N + src];
 #pragma unroll
-        for (int u = 0; u < K; ++u) acc[u] = cadd(acc[u], cmul(ld_w(Wt, j * nc + ib + u), f));
+        for (int u = 0; u < K; ++u) acc[u] = cfma(ld_w(Wt, j * nc + ib + u), f, acc[u]);
     }
 #pragma unroll
     for (int jj = 0; jj < K; ++jj) {
         const float2 f = A[(ib + jj) * N + src];
 #pragma unroll
-        for (int u = jj; u < K; ++u) acc[u] = cadd(acc[u], cmul(ld_w(Wt, (ib + jj) * nc + ib + u), f));
+        for (int u = jj; u < K; ++u) acc[u] = cfma(ld_w(Wt, (ib + jj) * nc + ib + u), f, acc[u]);
     }
 #pragma unroll
     for (int u = 0; u < K; ++u) dst[(size_t)(ib + u) * out_plane] = cscale(acc[u], scale);
 }
+
+template <int NCMAX, class F>
+__global__ void __launch_bounds__(rows_pw_bound(F::n)) k_rows_pw(RowPwArgs a) {
+    extern __shared__ float2 lds[];
+    const int n = F::n ? F::n : a.nx;
+    const int nc = a.nc;
+    const int WPB = blockDim.x >> 6;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const float2* tw = stage_twiddles<F>(lds, a.p_enc);
+    float2* A = lds + tw_slots<F>();                          // nc * n
+    float2* B = F::scratch ? A + (size_t)nc * n : nullptr;    // WPB * n
+    const size_t row = blockIdx.x;
+    const int tx = a.tx, x0 = a.x0, half = n / 2;
+    load_whitened<NCMAX>(a, A, n, row);
+    __syncthreads();
+    for (int c = w; c < nc; c += WPB) {
+        float2* res = F::run(A + (size_t)c * n, B ? B + (size_t)w * n : nullptr, a.p_enc, tw, lane);
+        float2* dstp = a.out + (size_t)c * a.out_plane + row * tx;
+        for (int x = lane; x < tx; x += 64) dstp[x] = cscale(res[wrap_n(x0 + x + half, n)], a.scale);
+        wave_sync();
+    }
+}
+
+// Pipelined row pass for the fixed transform lengths.  Persistent workgroups
+// walk contiguous row ranges; while the coils of one row are transformed in
+// LDS, the raw samples of the next row are already on their way into
+// registers.  The whitening y = W s (recon/prewhiten.rs:128-141) is applied
+// after the kx IFFT, to the kept columns only: both are linear maps on
+// different axes, so they commute.  In the output phase P thread groups share
+// each kept column, group g taking the coil rows [b_g, b_g+1) chosen so the
+// lower-triangular multiply counts balance; a thread forms up to 8 output
+// rows at a time from the transformed coils in LDS and W^T (scalar loads from
+// global memory when RowPwArgs::Wt is set, else a copy in LDS).
+constexpr int kRowsPipeThreads = 1024;
+
+// Lengths the pipelined kernel is built for.  368 and 640 are left to the
+// one-row-per-workgroup kernel: their transforms need more registers than a
+// 1024-thread workgroup leaves, and on 640-point rows with 15 coils the
+// pipelined form measured 20 % slower.
+#define OKS_PIPE_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(160) X(256) X(320) X(384) X(512)
+
+// Elements per thread of the next row that are prefetched into registers.
+constexpr int rows_pipe_prefetch(int ncmax, int n, int kmax) {
+    const int cap = (ncmax == 32 && n == 512) ? 8 : kmax;
+    return kmax < cap ? kmax : cap;
+}
+
 
 template <int NCMAX, int N>
 __global__ void __launch_bounds__(kRowsPipeThreads) k_rows_pw_pipe(RowPwArgs a) {
