@@ -188,7 +188,9 @@ def workload_c4():
                 "acs_lines": 32, "strategy": "SenseRss", "maps": "ESPIRiT (kernel 6, threshold 0.02, 30 it)",
                 "ridge": 1e-4, "gfactor": True, "path_taken": "fallback 3-D IFFT + RSS (Ny % R != 0)"},
         case=case, desc=desc, fn="oks_recon_sense_dev", oracle="sense", slices=nz,
-        bound="hbm", algo=algo_bytes, kernel="SenseRss launch sequence (kz IFFT, ky/kx IFFT, RSS)",
+        bound="hbm", algo=algo_bytes,
+        kernel="SenseRss launch sequence (whitened kx IFFT flagging zero rows, kz IFFT of the live ky lines, "
+               "ky IFFT + RSS)",
         cpu_unit=None, dtype="c64/f32",
     )
 
@@ -224,8 +226,12 @@ WORKLOADS = {"c1": workload_c1, "c2": workload_c2, "c3": workload_c3, "c4": work
 # HBM bytes the dominant kernels move per algorithmic byte, measured with the
 # FETCH_SIZE / WRITE_SIZE counters scaled by a copy kernel of known size
 # (profiles/hbm_traffic_counters.md): CS 57.5 B per pixel-iteration (33
-# algorithmic), ifft-rss 60.8 MB per 15 x 368 x 640 slice (28.73 MB algorithmic).
-TRAFFIC_PER_ALGO_BYTE = {"c1": 60.8e6 / 28.73e6, "c3": 57.5 / 41.0, "served": 57.5 / 41.0}
+# algorithmic), ifft-rss 60.8 MB per 15 x 368 x 640 slice (28.73 MB algorithmic),
+# the 3-D tail of a 32 x 160 x 256 x 256 volume with R = 3 ky lines 7.175 GB
+# (2.726 GB algorithmic; the row pass reads every row, the later passes the
+# live ky lines only).
+TRAFFIC_PER_ALGO_BYTE = {"c1": 60.8e6 / 28.73e6, "c3": 57.5 / 41.0, "served": 57.5 / 41.0,
+                         "c4": 7.175e9 / 2.7263e9}
 # FISTA's algorithmic bytes per coil-pixel and iteration: read z, x, the
 # measured k-space and the mask, write x and z (recon/cs.rs:99-136)
 CS_BYTES_PER_PIXEL_ITER = 5 * 8 + 1
@@ -233,7 +239,8 @@ CS_BYTES_PER_PIXEL_ITER = 5 * 8 + 1
 # The stage whose launches each multi-slice workload's roofline is taken on.
 WORKLOAD_STAGE = {"c1": "ifft_rss", "c1os": "ifft_rss", "c4": "ifft_rss", "c2": "grappa_apply",
                   "c3": "cs_iterations", "c4e": "espirit"}
-STAGE_KERNELS = {"ifft_rss": "IFFT + crop + RSS tail: k_rows_pw* + k_cols_rss_reg (+ k_rss_finalize)",
+STAGE_KERNELS = {"ifft_rss": "IFFT + crop + RSS tail: k_rows_pw* (+ k_cols_reg_c along kz for 3-D) + "
+                             "k_cols_rss_reg (+ k_rss_finalize)",
                  "grappa_apply": "GRAPPA synthesis GEMM k_grappa_synth (split-bf16 MFMA, one workgroup per "
                                  "source row group and kx tile)",
                  "cs_iterations": "FISTA iteration pair k_cs_cols_reg + k_cs_update, one HIP graph per call",

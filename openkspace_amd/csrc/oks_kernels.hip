@@ -1240,13 +1240,30 @@ static hipError_t go_rows_pw(const RowPwArgs& a, size_t lds, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Persistent grid: as many workgroups as fit on the 256 CUs at once (LDS and
-// the 2048-thread limit), never more than there are rows.
+// Persistent grid: as many workgroups as fit on the 256 CUs at once (LDS,
+// registers and the 2048-thread limit, by HIP's occupancy calculator), never
+// more than there are rows.
 template <int NCMAX, int N>
 static hipError_t go_rows_pw_pipe(const RowPwArgs& a, size_t lds, hipStream_t s) {
     OKS_ATTR(hipFuncSetAttribute((const void*)k_rows_pw_pipe<NCMAX, N>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)lds));
     long long per_cu = std::max<long long>(1, std::min<long long>(2, (160 * 1024) / (long long)lds));
+    static std::mutex mu;
+    static std::map<size_t, int> memo;  // by LDS bytes
+    int fit = 0;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        const auto it = memo.find(lds);
+        if (it != memo.end()) {
+            fit = it->second;
+        } else {
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, (const void*)k_rows_pw_pipe<NCMAX, N>,
+                                                             kRowsPipeThreads, lds) != hipSuccess)
+                fit = 0;
+            memo[lds] = fit;
+        }
+    }
+    if (fit > 0) per_cu = std::min<long long>(per_cu, fit);
     if (a.wgs > 0) per_cu = std::min<long long>(per_cu, a.wgs);
     const long long grid = std::min<long long>(a.rows, 256 * per_cu);
     hipLaunchKernelGGL((k_rows_pw_pipe<NCMAX, N>), dim3((unsigned)grid), dim3(kRowsPipeThreads), lds, s, a);
