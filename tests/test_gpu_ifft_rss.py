@@ -301,3 +301,23 @@ def test_ifft_rss_3d_all_zero(ctx):
     d = abi.IfftRssDesc.default(remove_oversampling=0, prewhiten=0, partial_fourier=0)
     got, _, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", d, case)
     assert got.shape == (4, 16, 32) and not np.any(got)
+
+
+@pytest.mark.parametrize("n", [64, 128, 160, 256, 320, 384, 512])
+@pytest.mark.parametrize("inner", [20, 48])
+def test_strided_inverse_register_form(ctx, n, inner):
+    # the inverse transform along a middle axis runs the register four-step
+    # complex-output column pass at these lengths; inner = 20 leaves the last
+    # 16-column strip ragged.  The LDS Stockham form (cols_reg = 0) meets the
+    # same bar.
+    rng = np.random.default_rng(n + inner)
+    x = (rng.standard_normal((3, n, inner)) + 1j * rng.standard_normal((3, n, inner))).astype(np.complex64)
+    want = O.ifft_centered(x, (1,), "f64")
+    got = fft_centered(ctx, x, (1,), inverse=True, norm=1)
+    assert rel(got, want) < 2e-6
+    ctx.set_option("cols_reg", 0)
+    try:
+        lds = fft_centered(ctx, x, (1,), inverse=True, norm=1)
+    finally:
+        ctx.set_option("cols_reg", 1)
+    assert rel(lds, want) < 2e-6
