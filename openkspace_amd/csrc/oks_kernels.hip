@@ -1008,7 +1008,8 @@ __global__ void __launch_bounds__(cols_reg_threads<N>()) k_cols_reg_c(ColArgs a)
     float2 va[N1];
     if (grp < N2) {
         unsigned long long lv = ~0ull;
-        if (a.rflag && cin) {
+        // a column whose every row is live (sflag bit 1) needs no row flags
+        if (a.rflag && cin && !(a.sflag && (a.sflag[(xs + col) / a.rf_div] & 2))) {
             lv = 0;
             const size_t xo = (size_t)((xs + col) / a.rf_div);
             for (int n1 = 0; n1 < N1; ++n1) {
@@ -1109,19 +1110,31 @@ __global__ void k_rss(const float2* __restrict__ in, int C, long long M, float* 
     }
 }
 
-// yflag[y] = OR over z of rowflag[z * ny + y].
+// yflag[y]: bit 0 = OR over z of rowflag[z * ny + y] (some row of ky line y
+// is live), bit 1 = AND (every row is).
 __global__ void __launch_bounds__(256) k_or_rows(const uint8_t* __restrict__ rowflag, int nz, int ny,
                                                  uint8_t* __restrict__ yflag) {
     // workgroup: 64 consecutive y, its 4 wavefronts splitting z
     __shared__ uint8_t part[4][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int y = blockIdx.x * 64 + lane;
-    uint8_t f = 0;
+    uint8_t any = 0, all = 1;
     if (y < ny)
-        for (int z = w; z < nz; z += 4) f |= rowflag[(size_t)z * ny + y];
-    part[w][lane] = f;
+        for (int z = w; z < nz; z += 4) {
+            const uint8_t f = rowflag[(size_t)z * ny + y];
+            any |= f;
+            all &= f;
+        }
+    part[w][lane] = (uint8_t)(any | (all << 1));
     __syncthreads();
-    if (w == 0 && y < ny) yflag[y] = part[0][lane] | part[1][lane] | part[2][lane] | part[3][lane];
+    if (w == 0 && y < ny) {
+        uint8_t o = 0, n = 2;
+        for (int v = 0; v < 4; ++v) {
+            o |= part[v][lane] & 1;
+            n &= part[v][lane];
+        }
+        yflag[y] = (uint8_t)(o | n);
+    }
 }
 
 // ------------------------------------------------------------- launchers

@@ -85,7 +85,8 @@ struct ColArgs {
     // index i along n, column x) is zero, and not read, where
     // rflag[i * rf_stride + x / rf_div] == 0.  With sflag (mode 0), columns x
     // with sflag[x / rf_div] == 0 are neither read nor written; a workgroup
-    // whose strip holds no other column exits at once.
+    // whose strip holds no other column exits at once; bit 1 of sflag marks
+    // a column whose rows are all live (rflag need not be read).
     const uint8_t* rflag;
     const uint8_t* sflag;
     long long rf_stride;
@@ -112,7 +113,7 @@ hipError_t launch_rss_finalize(const float* part, int G, long long M, float* out
 hipError_t launch_prewhiten(const float2* in, float2* d, const float2* W, int nc, long long M, hipStream_t s);
 hipError_t launch_prep_os(const RowPwArgs& a, const uint8_t* acq, hipStream_t s);
 hipError_t launch_rss(const float2* in, int C, long long M, float* out, float scale, hipStream_t s);
-// yflag[y] = OR over z of rowflag[z * ny + y]
+// yflag[y] = OR over z of rowflag[z * ny + y] | (AND over z) << 1
 hipError_t launch_or_rows(const uint8_t* rowflag, int nz, int ny, uint8_t* yflag, hipStream_t s);
 hipError_t launch_os_remove(const float2* in, float2* out, long long rows, int enc, int rec, const FftPlan& p_enc,
                             const FftPlan& p_rec, hipStream_t s);
