@@ -205,6 +205,15 @@ __global__ void __launch_bounds__(kWhitenThreads) k_whitener(WhitenArgs wa) {
 // 128 registers a 1024-thread workgroup leaves.
 constexpr int rows_pw_bound(int n) { return (n == 0 || n == 368) ? 512 : 1024; }
 
+// acc + w f as four fused multiply-adds
+__device__ __forceinline__ float2 cfma(float2 w, float2 f, float2 acc) {
+    acc.x = fmaf(w.x, f.x, acc.x);
+    acc.x = fmaf(-w.y, f.y, acc.x);
+    acc.y = fmaf(w.x, f.y, acc.y);
+    acc.y = fmaf(w.y, f.x, acc.y);
+    return acc;
+}
+
 template <int NCMAX>
 __device__ __forceinline__ void load_whitened(const RowPwArgs& a, float2* A, int n, size_t row) {
     const int nc = a.nc;
@@ -222,7 +231,7 @@ __device__ __forceinline__ void load_whitened(const RowPwArgs& a, float2* A, int
                 if (i < nc) {
                     float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
-                    for (int j = 0; j <= i; ++j) acc = cadd(acc, cmul(a.W[i * nc + j], s[j]));
+                    for (int j = 0; j <= i; ++j) acc = cfma(a.W[i * nc + j], s[j], acc);
                     A[(size_t)i * n + pos] = acc;
                 }
             }
@@ -250,14 +259,6 @@ typedef __attribute__((address_space(4))) const float const_float;
 
 __device__ __forceinline__ float2 ld_w(const const_float* Wt, int e) { return make_float2(Wt[2 * e], Wt[2 * e + 1]); }
 
-// acc + w f as four fused multiply-adds
-__device__ __forceinline__ float2 cfma(float2 w, float2 f, float2 acc) {
-    acc.x = fmaf(w.x, f.x, acc.x);
-    acc.x = fmaf(-w.y, f.y, acc.x);
-    acc.y = fmaf(w.x, f.y, acc.y);
-    acc.y = fmaf(w.y, f.x, acc.y);
-    return acc;
-}
 
 // Outputs ib .. ib + K - 1 of the whitening y = W s at one column `src` of the
 // transformed coils in LDS (A[j N + src] = coil j), j ascending as in
