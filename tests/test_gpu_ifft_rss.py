@@ -321,3 +321,26 @@ def test_strided_inverse_register_form(ctx, n, inner):
     finally:
         ctx.set_option("cols_reg", 1)
     assert rel(lds, want) < 2e-6
+
+
+@pytest.mark.parametrize("nz", [6, 64])
+def test_ifft_rss_3d_zero_rows_ragged_crop(ctx, nz):
+    # the x crop leaves 100 columns, so column strips of 16 straddle ky lines
+    # in the kz pass (flags read per column) and the last strip of the ky
+    # pass is ragged; nz = 64 runs the register kz pass, nz = 6 the LDS one
+    case = synth.make_case(nc=6, nz=nz, ny=32, nx=128, os_x=1, kind="full", three_d=True, seed=nz, ellipses=4)
+    case["hdr"]["recon_x"] = 100
+    k, m = case["kspace"], case["mask"]
+    dead_y = [y for y in range(32) if y % 3 != 0 and not 12 <= y < 20]
+    k[:, :, dead_y, :] = 0
+    m[:, dead_y, :] = 0
+    rng = np.random.default_rng(nz)
+    for z, y in zip(rng.integers(0, nz, 8), rng.integers(0, 32, 8)):
+        k[:, z, y, :] = 0
+        m[z, y, :] = 0
+    d = abi.IfftRssDesc.default(remove_oversampling=0)
+    st, ref32, rflags, _ = O.recon("ifft_rss", d, case, "f32")
+    _, ref64, _, _ = O.recon("ifft_rss", d, case, "f64")
+    got, flags, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", d, case)
+    assert st == 0 and flags == rflags and got.shape == ref32.shape == (nz, 32, 100)
+    assert_parity(got, ref32, ref64, TOL32, f"3-D zero rows, crop to 100 columns, nz={nz}")
