@@ -277,7 +277,9 @@ int oks_event_destroy(oks_ctx* ctx, void* ev);
  * read_kspace_with_mask and read_calibration produce.  The plain form takes
  * host buffers and blocks until the image is on the host; the _dev form takes
  * device kspace/noise/image (mask stays on the host), is asynchronous on the
- * ctx stream, and leaves device-raised flags for oks_ctx_take_flags. */
+ * ctx stream, and leaves device-raised flags for oks_ctx_take_flags.
+ * Channel counts (the reference has no limit): ifft-rss and CS take up to 64
+ * channels, GRAPPA and SENSE up to 32; more fail with OKS_E_UNSUPPORTED. */
 int oks_recon_ifft_rss(oks_ctx* ctx, const oks_ifft_rss_desc* d, const oks_kspace* k, oks_image* img);
 int oks_recon_ifft_rss_dev(oks_ctx* ctx, const oks_ifft_rss_desc* d, const oks_kspace* k, oks_image* img);
     /* IfftRss::reconstruct, recon/strategy.rs:113-223 */
@@ -322,7 +324,8 @@ int oks_crop_dev(oks_ctx* ctx, const float* in, int64_t nz, int64_t ny, int64_t 
  * enc <= rec, enc % rec != 0 or a ratio above 8). */
 int oks_os_remove_dev(oks_ctx* ctx, const oks_c32* in, int64_t rows, int64_t enc, int64_t rec, oks_c32* out);
 /* NoisePrewhitener::from_noise_acqs (recon/prewhiten.rs:48-111): W = chol(Psi)^-1
- * [nc, nc] lower, FP64 inside; W = I and a device flag when not PD. */
+ * [nc, nc] lower, FP64 inside; W = I and a device flag when not PD.  nc <= 64
+ * here and in oks_prewhiten_apply_dev (OKS_E_INVALID_ARG above). */
 int oks_prewhiten_calibrate_dev(oks_ctx* ctx, const oks_c32* noise, int32_t nc, int64_t ns, oks_c32* W);
 /* NoisePrewhitener::apply (recon/prewhiten.rs:118-142) on [nc, ns] in place. */
 int oks_prewhiten_apply_dev(oks_ctx* ctx, const oks_c32* W, oks_c32* data, int32_t nc, int64_t ns);

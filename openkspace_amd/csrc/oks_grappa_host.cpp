@@ -276,6 +276,7 @@ static void grappa_volume(oks_ctx* ctx, float2* grid, const oks_kspace* k, const
 
 void recon_grappa_dev(oks_ctx* ctx, const oks_grappa_desc* d, const oks_kspace* k, oks_image* img) {
     check_kspace(k);
+    if (k->nc > kFusedChannels) fail(OKS_E_UNSUPPORTED, "GRAPPA: at most 32 channels");
     if (!d || !img || !img->data) fail(OKS_E_INVALID_ARG, "null descriptor or image");
     uint32_t flags = 0;
     const float2* W = d->prewhiten ? whitener(ctx, (const float2*)k->noise, k->nc, k->n_noise, &flags, "W") : nullptr;

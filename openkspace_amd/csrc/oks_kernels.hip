@@ -30,12 +30,13 @@ namespace oks {
 // W = L^-1, all in double; W is stored as complex64.  Where the reference
 // returns None (fewer samples than channels, or a covariance that is not
 // positive definite: recon/prewhiten.rs:49-66,160-164) W = I and the
-// request's status is raised.  512 threads, nc <= 32.
+// request's status is raised.  512 threads, nc <= NCM (32 or 64).
 //
 //   covariance : lower-triangle pairs (i, j) x S sample ranges spread over
 //                the workgroup, four independent FP64 accumulators per item;
-//                noise staged 256 samples at a time at a row stride of 257
-//                samples, so that the rows start in different LDS banks;
+//                noise staged CH samples at a time (256 for 32 channels, 128
+//                for 64) at a row stride of CH + 1 samples, so that the rows
+//                start in different LDS banks;
 //   Cholesky   : wavefront 0, lane j for row j, left-looking by columns
 //                (recon/prewhiten.rs:148-179): step i forms every lane's
 //                s_j = Psi_ji - sum_{k<i} L_jk conj(L_ik) at once, the
@@ -44,12 +45,18 @@ namespace oks {
 //   inverse    : wavefront 0, lane c forward-substitutes column c of L^-1.
 constexpr int kWhitenThreads = 512;
 
+template <int NCM>
 __global__ void __launch_bounds__(kWhitenThreads) k_whitener(WhitenArgs wa) {
-    constexpr int kStride = 257;
+    constexpr int CH = NCM <= 32 ? 256 : 128;
+    constexpr int kStride = CH + 1;
     constexpr int T = kWhitenThreads;
-    __shared__ double2 L[32 * 32];  // Psi's lower triangle, then the factor, [i nc + k]
-    __shared__ __align__(16) float2 chunk[32 * kStride];
-    __shared__ double dinv[32];
+    // covariance items per thread: every lower-triangle pair at least once
+    constexpr int KI = (NCM * (NCM + 1) / 2 + T - 1) / T;
+    __shared__ double2 L[NCM * NCM];  // Psi's lower triangle, then the factor, [i nc + k]
+    __shared__ __align__(16) float2 chunk[NCM * kStride];
+    __shared__ double dinv[NCM];
+    static_assert(NCM <= 64, "one lane per channel in the factorization");
+    static_assert(sizeof(float2) * NCM * kStride >= sizeof(double2) * NCM * NCM, "L^-1 reuses the staging buffer");
     const int t = threadIdx.x, b = blockIdx.x, nc = wa.nc;
     const float2* __restrict__ noise = wa.noise[b];
     const int ns = (noise && wa.ns[b] >= nc) ? wa.ns[b] : 0;
@@ -59,10 +66,10 @@ __global__ void __launch_bounds__(kWhitenThreads) k_whitener(WhitenArgs wa) {
     const int npairs = nc * (nc + 1) / 2;
     const int S = npairs >= T ? 1 : min(8, T / npairs);
     const int items = npairs * S;
-    int pi[2], pj[2], ps[2];
-    double acc[2][4][2] = {};
+    int pi[KI], pj[KI], ps[KI];
+    double acc[KI][4][2] = {};
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < KI; ++k) {
         const int it = t + k * T;
         const int p = it % npairs;
         int i = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
@@ -72,16 +79,16 @@ __global__ void __launch_bounds__(kWhitenThreads) k_whitener(WhitenArgs wa) {
         pj[k] = p - i * (i + 1) / 2;
         ps[k] = it / npairs;
     }
-    for (int s0 = 0; s0 < ns; s0 += 256) {
-        const int cs = min(256, ns - s0);
+    for (int s0 = 0; s0 < ns; s0 += CH) {
+        const int cs = min(CH, ns - s0);
         __syncthreads();
-        for (int e = t; e < nc * 256; e += T) {
-            const int c = e >> 8, q = e & 255;
+        for (int e = t; e < nc * CH; e += T) {
+            const int c = e / CH, q = e % CH;
             chunk[c * kStride + q] = q < cs ? noise[(size_t)c * ns + s0 + q] : make_float2(0.f, 0.f);
         }
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
+        for (int k = 0; k < KI; ++k) {
             if (t + k * T < items) {
                 const float2* x = chunk + pi[k] * kStride;
                 const float2* y = chunk + pj[k] * kStride;
@@ -107,7 +114,7 @@ __global__ void __launch_bounds__(kWhitenThreads) k_whitener(WhitenArgs wa) {
     double2* part = reinterpret_cast<double2*>(chunk);
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < KI; ++k) {
         const int it = t + k * T;
         if (it < items)
             part[it] = make_double2((acc[k][0][0] + acc[k][1][0]) + (acc[k][2][0] + acc[k][3][0]),
@@ -1152,9 +1159,12 @@ hipError_t launch_whitener(const float2* noise, int nc, int ns, float2* W, int* 
 }
 
 hipError_t launch_whitener_batch(const WhitenArgs& a, int B, hipStream_t s) {
-    if (a.nc < 1 || a.nc > 32 || B < 1 || B > OKS_MAX_BATCH) return hipErrorInvalidValue;
+    if (a.nc < 1 || a.nc > 64 || B < 1 || B > OKS_MAX_BATCH) return hipErrorInvalidValue;
     if (a.Wt && B != 1) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_whitener, dim3((unsigned)B), dim3(kWhitenThreads), 0, s, a);
+    if (a.nc <= 32)
+        hipLaunchKernelGGL(k_whitener<32>, dim3((unsigned)B), dim3(kWhitenThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_whitener<64>, dim3((unsigned)B), dim3(kWhitenThreads), 0, s, a);
     return hipGetLastError();
 }
 
@@ -1525,6 +1535,7 @@ hipError_t launch_prewhiten(const float2* in, float2* d, const float2* W, int nc
     else if (nc <= 8) hipLaunchKernelGGL((k_prewhiten<8, 4>), dim3(grid), dim3(256), 0, s, in, d, W, nc, M);
     else if (nc <= 16) hipLaunchKernelGGL((k_prewhiten<16, 2>), dim3(grid), dim3(256), 0, s, in, d, W, nc, M);
     else if (nc <= 32) hipLaunchKernelGGL((k_prewhiten<32, 1>), dim3(grid), dim3(256), 0, s, in, d, W, nc, M);
+    else if (nc <= 64) hipLaunchKernelGGL((k_prewhiten<64, 1>), dim3(grid), dim3(256), 0, s, in, d, W, nc, M);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

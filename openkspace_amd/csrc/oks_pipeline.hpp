@@ -17,6 +17,13 @@ namespace oks {
 const float2* whitener(oks_ctx* ctx, const float2* noise, int64_t nc, int64_t nn, uint32_t* flags, const char* slot);
 // The transpose that follows a W returned by whitener().
 inline const float2* whitener_t(const float2* W, int64_t nc) { return W ? W + nc * nc : nullptr; }
+
+// Channels that the fused row kernels hold per sample (the coil vector in
+// registers, a row of every channel in LDS).  Grids with more channels (up to
+// kMaxChannels) are whitened by a pass of their own and then transformed in
+// channel groups of at most this many.
+constexpr int64_t kFusedChannels = 32;
+constexpr int64_t kMaxChannels = 64;
 // Zero-row flags of a column pass (ColArgs::rflag / sflag): none by default.
 struct ColFlags {
     const uint8_t* rflag = nullptr;

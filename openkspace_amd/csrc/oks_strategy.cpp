@@ -28,7 +28,7 @@ const float2* whitener(oks_ctx* ctx, const float2* noise, int64_t nc, int64_t nn
         *flags |= OKS_F_PREWHITEN_SKIPPED;
         return nullptr;
     }
-    if (nc > 32) fail(OKS_E_UNSUPPORTED, "pre-whitening supports at most 32 channels");
+    if (nc > kMaxChannels) fail(OKS_E_UNSUPPORTED, "pre-whitening supports at most 64 channels");
     // W, then W^T (whitener_t): the pipelined row pass reads the transpose
     float2* W = (float2*)ctx->buf(slot, sizeof(float2) * 2 * nc * nc);
     OKS_HIP(launch_whitener(noise, (int)nc, (int)nn, W, ctx->d_status + 0, ctx->stream, W + nc * nc));
@@ -118,7 +118,7 @@ void fft_axis(oks_ctx* ctx, float2* d, const int64_t* shape, int ndim, int axis,
 void check_kspace(const oks_kspace* k) {
     if (!k || !k->kspace || !k->mask) fail(OKS_E_INVALID_ARG, "null k-space or mask");
     if (k->nc <= 0 || k->nz <= 0 || k->ny <= 0 || k->nx <= 0) fail(OKS_E_INVALID_ARG, "empty k-space");
-    if (k->nc > 32) fail(OKS_E_UNSUPPORTED, "at most 32 channels");
+    if (k->nc > kMaxChannels) fail(OKS_E_UNSUPPORTED, "at most 64 channels");
     if (k->ny > 4096 || k->nx > 4096 || k->nz > 65535) fail(OKS_E_UNSUPPORTED, "grid too large");
 }
 
@@ -163,12 +163,18 @@ float2* prep_grid(oks_ctx* ctx, const oks_kspace* k, const float2* W, int os_mod
         for (int64_t row = 0; row < rows; ++row) ctx->h_acq[row] = row_any(k->mask + row * k->nx, k->nx);
         uint8_t* acq = (uint8_t*)ctx->buf("prep_acq", rows);
         OKS_HIP(hipMemcpyAsync(acq, ctx->h_acq.data(), rows, hipMemcpyHostToDevice, ctx->stream));
+        const float2* src = (const float2*)k->kspace;
+        if (W && k->nc > kFusedChannels) {
+            // the whitening mixes every channel: a pass of its own, then the
+            // per-channel chain in groups
+            float2* wk = (float2*)ctx->buf("prep_white", sizeof(float2) * total);
+            OKS_HIP(launch_prewhiten(src, wk, W, (int)k->nc, (long long)plane, ctx->stream));
+            src = wk;
+            W = nullptr;
+        }
         RowPwArgs a{};
-        a.in = (const float2*)k->kspace;
-        a.out = g;
         a.W = W;
         a.Wt = whitener_t(W, k->nc);
-        a.nc = (int)k->nc;
         a.rows = rows;
         a.in_plane = (long long)plane;
         a.out_plane = (long long)plane;
@@ -179,7 +185,12 @@ float2* prep_grid(oks_ctx* ctx, const oks_kspace* k, const float2* W, int os_mod
         a.wgs = ctx->rows_wgs;
         a.p_enc = ctx->plan((int)k->nx);
         a.p_rec = ctx->plan(rec);
-        OKS_HIP(launch_prep_os(a, acq, ctx->stream));
+        for (int64_t c0 = 0; c0 < k->nc; c0 += kFusedChannels) {
+            a.nc = (int)std::min(kFusedChannels, k->nc - c0);
+            a.in = src + c0 * (int64_t)plane;
+            a.out = g + c0 * (int64_t)plane;
+            OKS_HIP(launch_prep_os(a, acq, ctx->stream));
+        }
     } else if (W) {
         OKS_HIP(launch_prewhiten((const float2*)k->kspace, g, W, (int)k->nc, (long long)plane, ctx->stream));
     } else {
@@ -215,6 +226,17 @@ static int rss_groups(int nc, long long wgs_per_group) {
 void ifft_rss_grid(oks_ctx* ctx, const float2* k, int64_t nc, int64_t nz, int64_t ny, int64_t nx, const float2* W,
                    int os_mode, int rec, bool three_d, const Win w[3], float* out) {
     TuneScope tuned(ctx, "ifft_rss", {nc, ny, nx, (int64_t)w[2].len, three_d ? nz : 1});
+    // More channels than the fused row pass holds: whiten first, then run the
+    // row pass per channel group (without whitening the channels are
+    // independent there).
+    const bool grouped = nc > kFusedChannels;
+    if (grouped && W) {
+        const int64_t plane = nz * ny * nx;
+        float2* wk = (float2*)ctx->buf("rss_white", sizeof(float2) * nc * plane);
+        OKS_HIP(launch_prewhiten(k, wk, W, (int)nc, (long long)plane, ctx->stream));
+        k = wk;
+        W = nullptr;
+    }
     const int tx = (int)w[2].len, ty = (int)w[1].len;
     // algorithmic bytes: the k-space slices read once, the image written once
     const int64_t zread = three_d ? nz : w[0].len;
@@ -236,8 +258,19 @@ void ifft_rss_grid(oks_ctx* ctx, const float2* k, int64_t nc, int64_t nz, int64_
     a.in_plane = nz * ny * nx;
     a.p_enc = ctx->plan((int)nx);
     if (os_mode) a.p_rec = ctx->plan(rec);
-    if (rows_prewhiten_lds((int)nc, (int)nx, os_mode, rec, W != nullptr, ctx->rows_pipe != 0) > 160 * 1024)
+    if (rows_prewhiten_lds((int)std::min(nc, kFusedChannels), (int)nx, os_mode, rec, W != nullptr,
+                           ctx->rows_pipe != 0) > 160 * 1024)
         fail(OKS_E_UNSUPPORTED, "readout too long for the fused row pass");
+    // the row pass over every channel group of a.in/a.out
+    auto rows = [&](const RowPwArgs& r) {
+        for (int64_t c0 = 0; c0 < nc; c0 += kFusedChannels) {
+            RowPwArgs g = r;
+            g.nc = (int)std::min(kFusedChannels, nc - c0);
+            g.in = r.in + c0 * r.in_plane;
+            g.out = r.out + c0 * r.out_plane;
+            OKS_HIP(launch_rows_prewhiten_ifft(g, ctx->stream));
+        }
+    };
     if (!three_d) {
         // slices actually written: the z window of the crop
         const int64_t zlo = w[0].off, zn = w[0].len;
@@ -262,7 +295,7 @@ void ifft_rss_grid(oks_ctx* ctx, const float2* k, int64_t nc, int64_t nz, int64_
             a.out = tmp;
             a.rows = cz * ny;
             a.out_plane = cz * ny * tx;
-            OKS_HIP(launch_rows_prewhiten_ifft(a, ctx->stream));
+            rows(a);
             const int G = rss_groups((int)nc, cz * strips);
             col_pass(ctx, tmp, nullptr, out + z * ty * tx, (int)nc, (int)cz, 0, (int)cz, (int)ny, tx, 1,
                      (int)w[1].off, ty, 1.0f / (float)ny, true, true, G, "rss_part");
@@ -276,19 +309,20 @@ void ifft_rss_grid(oks_ctx* ctx, const float2* k, int64_t nc, int64_t nz, int64_
         // every z, which the ky pass then does not read either.  The
         // transforms of zero lines are zero, so the result is unchanged.
         // Only the pipelined row kernel flags rows (the one-row-per-workgroup
-        // kernels have no LDS to spare for the vote at 15 x 640 samples).
-        const int64_t rows = nz * ny;
-        float2* tmp = (float2*)ctx->buf("rss_tmp", sizeof(float2) * nc * rows * tx);
-        uint8_t* rowflag = rows_pipe_used(ctx->rows_pipe != 0, os_mode, (int)nx)
-                               ? (uint8_t*)ctx->buf("rss_rowflag", (size_t)rows + (size_t)ny)
+        // kernels have no LDS to spare for the vote at 15 x 640 samples), and
+        // only in one channel group (each group's launch writes every flag).
+        const int64_t nrows = nz * ny;
+        float2* tmp = (float2*)ctx->buf("rss_tmp", sizeof(float2) * nc * nrows * tx);
+        uint8_t* rowflag = !grouped && rows_pipe_used(ctx->rows_pipe != 0, os_mode, (int)nx)
+                               ? (uint8_t*)ctx->buf("rss_rowflag", (size_t)nrows + (size_t)ny)
                                : nullptr;
-        uint8_t* yflag = rowflag ? rowflag + rows : nullptr;
+        uint8_t* yflag = rowflag ? rowflag + nrows : nullptr;
         a.in = k;
         a.out = tmp;
-        a.rows = rows;
-        a.out_plane = rows * tx;
+        a.rows = nrows;
+        a.out_plane = nrows * tx;
         a.rowflag = rowflag;
-        OKS_HIP(launch_rows_prewhiten_ifft(a, ctx->stream));
+        rows(a);
         if (rowflag) OKS_HIP(launch_or_rows(rowflag, (int)nz, (int)ny, yflag, ctx->stream));
         if (nz > 1) {
             ColFlags fz;
@@ -324,7 +358,7 @@ static void homodyne_dev(oks_ctx* ctx, const oks_kspace* k, const float2* W, int
     const int sx = ny > 384 ? 4 : 8;
     if (homodyne_cols_lds((int)ny, sx, ty) > 160 * 1024) fail(OKS_E_UNSUPPORTED, "ky length too large for LDS");
     float2* T = (float2*)ctx->buf("hd_rows", sizeof(float2) * nc * zn * ny * tx);
-    if (nx % 2 == 0 &&
+    if (nx % 2 == 0 && nc <= kFusedChannels &&
         rows_prewhiten_lds((int)nc, (int)nx, os_mode, rec, W != nullptr, ctx->rows_pipe != 0) <= 160 * 1024) {
         // even kx: NumPy's shift pair is the ifft-rss row pass's, which fuses
         // the reader closure (whitening, oversampling removal) and the x crop
@@ -474,7 +508,7 @@ int oks_rss_dev(oks_ctx* ctx, const oks_c32* in, int64_t nc, int64_t inner, floa
 
 int oks_prewhiten_calibrate_dev(oks_ctx* ctx, const oks_c32* noise, int32_t nc, int64_t ns, oks_c32* W) {
     return guarded(ctx, [&] {
-        if (!noise || !W || nc < 1 || nc > 32 || ns < 1) fail(OKS_E_INVALID_ARG, "bad whitener arguments");
+        if (!noise || !W || nc < 1 || nc > kMaxChannels || ns < 1) fail(OKS_E_INVALID_ARG, "bad whitener arguments");
         if (ns < nc) fail(OKS_E_INCONSISTENT, "fewer noise samples than channels");
         OKS_HIP(launch_whitener((const float2*)noise, nc, (int)ns, (float2*)W, ctx->d_status + 0, ctx->stream));
     });
@@ -482,7 +516,7 @@ int oks_prewhiten_calibrate_dev(oks_ctx* ctx, const oks_c32* noise, int32_t nc, 
 
 int oks_prewhiten_apply_dev(oks_ctx* ctx, const oks_c32* W, oks_c32* data, int32_t nc, int64_t ns) {
     return guarded(ctx, [&] {
-        if (!W || !data || nc < 1 || nc > 32 || ns < 0) fail(OKS_E_INVALID_ARG, "bad whitening arguments");
+        if (!W || !data || nc < 1 || nc > kMaxChannels || ns < 0) fail(OKS_E_INVALID_ARG, "bad whitening arguments");
         OKS_HIP(launch_prewhiten((const float2*)data, (float2*)data, (const float2*)W, nc, ns, ctx->stream));
     });
 }

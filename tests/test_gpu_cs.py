@@ -123,6 +123,19 @@ def test_cs_strategy_small(ctx, prewhiten):
     assert_parity(got, r32, r64, TOL, f"cs pw={prewhiten}", iterative=True)
 
 
+@pytest.mark.parametrize("nc,ny,os_x", [(40, 64, 1), (64, 64, 2), (48, 33, 2)])
+def test_cs_more_than_32_channels(ctx, nc, ny, os_x):
+    # CS solves every channel's planes on their own; above 32 channels only
+    # the whitening (and the odd-ky fallback's row pass) change form
+    case = _cs_case(nc=nc, ny=ny, nx=32 * os_x, os_x=os_x, seed=nc)
+    d = abi.CsDesc.default(iters=10, prewhiten=1)
+    st, r32, rflags, _ = O.recon("cs", d, case, "f32")
+    _, r64, _, _ = O.recon("cs", d, case, "f64")
+    got, flags, _ = recon_dev(ctx, "oks_recon_cs_dev", d, case)
+    assert st == 0 and got.shape == r32.shape and flags == rflags
+    assert_parity(got, r32, r64, TOL, f"cs nc={nc} ny={ny} os={os_x}", iterative=True)
+
+
 @pytest.mark.parametrize("three_d,os_x", [(False, 2), (True, 1), (True, 2)])
 def test_cs_fallback_fused(ctx, three_d, os_x):
     # odd ny: whitened, (oversampling-removed,) kz-transformed IFFT + RSS in one pass

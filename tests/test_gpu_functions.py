@@ -43,7 +43,7 @@ def test_rss(ctx, nc, inner):
     assert_parity(got, O.rss(x, "f32"), O.rss(x, "f64"), TOL, f"rss nc={nc}")
 
 
-@pytest.mark.parametrize("nc,ns", [(4, 256), (15, 256), (32, 300)])
+@pytest.mark.parametrize("nc,ns", [(4, 256), (15, 256), (32, 300), (33, 256), (48, 256), (64, 1000)])
 def test_prewhiten_calibrate_and_apply(ctx, nc, ns):
     rng = np.random.default_rng(100 + nc)
     A = _cplx(rng, nc, nc) / np.sqrt(2 * nc)
@@ -124,17 +124,18 @@ def test_os_remove_none_cases(ctx, enc, rec):
     assert O.os_remove(lines, rec, "f32") is None
 
 
-@pytest.mark.parametrize("fn,desc", [("oks_recon_ifft_rss_dev", abi.IfftRssDesc.default()),
-                                     ("oks_recon_grappa_dev", abi.GrappaDesc.default()),
-                                     ("oks_recon_sense_dev", abi.SenseDesc.default()),
-                                     ("oks_recon_cs_dev", abi.CsDesc.default(iters=2))])
-def test_more_than_32_channels_refused(ctx, fn, desc):
-    # A documented limit of this backend (the reference has none): grids of
-    # more than 32 channels are refused with OKS_E_UNSUPPORTED before any work.
+@pytest.mark.parametrize("fn,desc,limit", [("oks_recon_ifft_rss_dev", abi.IfftRssDesc.default(), 64),
+                                           ("oks_recon_grappa_dev", abi.GrappaDesc.default(), 32),
+                                           ("oks_recon_sense_dev", abi.SenseDesc.default(), 32),
+                                           ("oks_recon_cs_dev", abi.CsDesc.default(iters=2), 64)])
+def test_channel_limit_refused(ctx, fn, desc, limit):
+    # A documented limit of this backend (the reference has none): ifft-rss
+    # and CS take up to 64 channels, GRAPPA and SENSE up to 32; larger grids
+    # are refused with OKS_E_UNSUPPORTED before any work.
     from openkspace_amd import synth
     from openkspace_amd._lib import OksError
     from tests.gpu_util import recon_dev
-    case = synth.make_case(nc=33, nz=1, ny=32, nx=32, kind="full", seed=3, ellipses=3)
+    case = synth.make_case(nc=limit + 1, nz=1, ny=32, nx=32, kind="full", seed=3, ellipses=3)
     with pytest.raises(OksError) as e:
         recon_dev(ctx, fn, desc, case)
-    assert e.value.code == abi.OKS_E_UNSUPPORTED and "32" in str(e.value)
+    assert e.value.code == abi.OKS_E_UNSUPPORTED and str(limit) in str(e.value)

@@ -68,6 +68,13 @@ def test_homodyne_oversampled_readout_and_crop(ctx, remove_os):
     _check(ctx, case, d, f"homodyne os={remove_os}")
 
 
+@pytest.mark.parametrize("remove_os", [0, 1])
+def test_homodyne_more_than_32_channels(ctx, remove_os):
+    case = _pf_case(40, 2, 64, 64, lo=12, os_x=2, seed=40)
+    d = abi.IfftRssDesc.default(remove_oversampling=remove_os, prewhiten=1)
+    _check(ctx, case, d, f"homodyne nc=40 os={remove_os}")
+
+
 def test_homodyne_c1_geometry(ctx):
     # 15 coils, 368 ky (6/8 partial Fourier), 640 readout with 2x oversampling
     case = synth.make_config("c1_ifft_rss", nz=4)

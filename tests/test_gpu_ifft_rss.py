@@ -344,3 +344,17 @@ def test_ifft_rss_3d_zero_rows_ragged_crop(ctx, nz):
     got, flags, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", d, case)
     assert st == 0 and flags == rflags and got.shape == ref32.shape == (nz, 32, 100)
     assert_parity(got, ref32, ref64, TOL32, f"3-D zero rows, crop to 100 columns, nz={nz}")
+
+
+@pytest.mark.parametrize("nc", [33, 48, 64])
+@pytest.mark.parametrize("three_d,remove_os", [(False, 0), (False, 1), (True, 0), (True, 1)])
+def test_ifft_rss_more_than_32_channels(ctx, nc, three_d, remove_os):
+    # Above the 32 channels the fused row pass holds, the whitening runs as a
+    # pass of its own and the row pass per group of at most 32 channels.
+    case = _case(nc=nc, nz=4 if three_d else 3, ny=40, nx=64, three_d=three_d, seed=nc)
+    d = abi.IfftRssDesc.default(remove_oversampling=remove_os, prewhiten=1)
+    st, ref32, rflags, _ = O.recon("ifft_rss", d, case, "f32")
+    _, ref64, _, _ = O.recon("ifft_rss", d, case, "f64")
+    got, flags, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", d, case)
+    assert st == 0 and got.shape == ref32.shape and flags == rflags
+    assert_parity(got, ref32, ref64, TOL32, f"nc={nc} 3d={three_d} os={remove_os}")
