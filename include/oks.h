@@ -278,8 +278,8 @@ int oks_event_destroy(oks_ctx* ctx, void* ev);
  * host buffers and blocks until the image is on the host; the _dev form takes
  * device kspace/noise/image (mask stays on the host), is asynchronous on the
  * ctx stream, and leaves device-raised flags for oks_ctx_take_flags.
- * Channel counts (the reference has no limit): ifft-rss and CS take up to 64
- * channels, GRAPPA and SENSE up to 32; more fail with OKS_E_UNSUPPORTED. */
+ * Channel counts (the reference has no limit): ifft-rss, GRAPPA and CS take
+ * up to 64 channels, SENSE up to 32; more fail with OKS_E_UNSUPPORTED. */
 int oks_recon_ifft_rss(oks_ctx* ctx, const oks_ifft_rss_desc* d, const oks_kspace* k, oks_image* img);
 int oks_recon_ifft_rss_dev(oks_ctx* ctx, const oks_ifft_rss_desc* d, const oks_kspace* k, oks_image* img);
     /* IfftRss::reconstruct, recon/strategy.rs:113-223 */

@@ -380,10 +380,11 @@ struct ApplyLoader {
         long long wz;
     };
     Args a;
-    int z, d, ty, ky0, m0;
+    int z, d, ty, ky0, m0, n0;
     bool act;
     const uint2* base;
     const int* src_of;
+    // grid x = kx tiles x target-channel tiles of 32 (one tile up to 32 channels)
     __device__ ApplyLoader(const Args& x) : a(x) {
         const Blk b = xcd_block();
         const int zd = b.z;
@@ -397,7 +398,9 @@ struct ApplyLoader {
             ty = e.x;
             ky0 = e.y;
         }
-        m0 = b.x * 64;
+        const int mt = (a.n_kx + 63) / 64, ct = b.x / mt;
+        m0 = (b.x - ct * mt) * 64;
+        n0 = ct * 32;
         base = a.gs + ((size_t)z * a.ny + ky0) * a.nx + m0;
     }
     __device__ bool active() const { return act; }
@@ -433,7 +436,7 @@ struct ApplyLoader {
     __device__ void fetch_b(int k0, uint2 (&v)[8]) const {
         const int t = threadIdx.x;
         if (t >= BN * 4) return;
-        const int n = t >> 2, k8 = (t & 3) * 8;
+        const int n = n0 + (t >> 2), k8 = (t & 3) * 8;
         const uint2* wd = a.ws + (size_t)z * a.wz + (size_t)(d - 1) * a.nc * a.n_src + (size_t)n * a.n_src;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
@@ -448,6 +451,7 @@ struct ApplyLoader {
     }
     __device__ void store(int m, int n, float2 v) const {
         m += m0;
+        n += n0;
         if (m < a.n_kx && n < a.nc)
             a.grid[(((size_t)n * a.nz + z) * a.ny + ty) * a.nx + m + a.kkx / 2] = v;
     }
@@ -847,10 +851,10 @@ hipError_t launch_calib_gemm(const CalibArgs& c, hipStream_t s) {
 }
 
 hipError_t launch_apply_gemm(const ApplyArgs& c, hipStream_t s) {
-    if (c.nc > 32) return hipErrorInvalidValue;
+    if (c.nc < 1 || c.nc > 64 || c.n_src > ApplyLoader::kTab) return hipErrorInvalidValue;
     ApplyLoader::Args a{c.grid, c.gs, c.ws, c.list, c.cnt, c.skip, c.nc, c.nz, c.ny, c.nx, c.r, c.kky, c.kkx, c.n_kx, c.n_src,
                         c.wz};
-    dim3 grid((c.n_kx + 63) / 64, c.ny, c.nz * (c.r - 1));
+    dim3 grid((c.n_kx + 63) / 64 * ((c.nc + 31) / 32), c.ny, c.nz * (c.r - 1));
     hipLaunchKernelGGL((k_zgemm_sbf16<ApplyLoader, 32>), grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }

@@ -276,7 +276,6 @@ static void grappa_volume(oks_ctx* ctx, float2* grid, const oks_kspace* k, const
 
 void recon_grappa_dev(oks_ctx* ctx, const oks_grappa_desc* d, const oks_kspace* k, oks_image* img) {
     check_kspace(k);
-    if (k->nc > kFusedChannels) fail(OKS_E_UNSUPPORTED, "GRAPPA: at most 32 channels");
     if (!d || !img || !img->data) fail(OKS_E_INVALID_ARG, "null descriptor or image");
     uint32_t flags = 0;
     const float2* W = d->prewhiten ? whitener(ctx, (const float2*)k->noise, k->nc, k->n_noise, &flags, "W") : nullptr;
@@ -316,7 +315,7 @@ int oks_recon_grappa_dev(oks_ctx* ctx, const oks_grappa_desc* d, const oks_kspac
 int oks_grappa_calibrate_dev(oks_ctx* ctx, const oks_c32* acs, int64_t nc, int64_t ny_acs, int64_t nx, int64_t r,
                              int64_t kky, int64_t kkx, float ridge, oks_c32* weights) {
     return guarded(ctx, [&] {
-        if (!acs || !weights || nc < 1 || nc > 32) fail(OKS_E_INVALID_ARG, "bad calibration arguments");
+        if (!acs || !weights || nc < 1 || nc > kMaxChannels) fail(OKS_E_INVALID_ARG, "bad calibration arguments");
         GrGeom g{};
         std::string why;
         const int st = grappa_geom(nc, ny_acs, nx, r, kky, kkx, &g, &why);
@@ -335,7 +334,7 @@ int oks_grappa_calibrate_dev(oks_ctx* ctx, const oks_c32* acs, int64_t nc, int64
 int oks_grappa_synthesize_dev(oks_ctx* ctx, const oks_c32* weights, oks_c32* kspace, int64_t nc, int64_t nz,
                               int64_t ny, int64_t nx, int64_t r, int64_t kky, int64_t kkx) {
     return guarded(ctx, [&] {
-        if (!weights || !kspace || nc < 1 || nc > 32 || r < 2 || kky < 2 || kkx < 1 || nx < kkx)
+        if (!weights || !kspace || nc < 1 || nc > kMaxChannels || r < 2 || kky < 2 || kkx < 1 || nx < kkx)
             fail(OKS_E_INVALID_ARG, "bad synthesis arguments");
         GrGeom g{};
         g.n_src = (int)(nc * kky * kkx);

@@ -125,13 +125,13 @@ def test_os_remove_none_cases(ctx, enc, rec):
 
 
 @pytest.mark.parametrize("fn,desc,limit", [("oks_recon_ifft_rss_dev", abi.IfftRssDesc.default(), 64),
-                                           ("oks_recon_grappa_dev", abi.GrappaDesc.default(), 32),
+                                           ("oks_recon_grappa_dev", abi.GrappaDesc.default(), 64),
                                            ("oks_recon_sense_dev", abi.SenseDesc.default(), 32),
                                            ("oks_recon_cs_dev", abi.CsDesc.default(iters=2), 64)])
 def test_channel_limit_refused(ctx, fn, desc, limit):
-    # A documented limit of this backend (the reference has none): ifft-rss
-    # and CS take up to 64 channels, GRAPPA and SENSE up to 32; larger grids
-    # are refused with OKS_E_UNSUPPORTED before any work.
+    # A documented limit of this backend (the reference has none): ifft-rss,
+    # GRAPPA and CS take up to 64 channels, SENSE up to 32; larger grids are
+    # refused with OKS_E_UNSUPPORTED before any work.
     from openkspace_amd import synth
     from openkspace_amd._lib import OksError
     from tests.gpu_util import recon_dev

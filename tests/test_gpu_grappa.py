@@ -114,6 +114,19 @@ def test_grappa_strategy_small(ctx, r, acs):
     assert_parity(got, r32, r64, TOL, f"grappa R={r}")
 
 
+@pytest.mark.parametrize("nc,r,kky,kkx,acs", [(40, 2, 4, 5, 16), (64, 3, 2, 3, 18)])
+def test_grappa_more_than_32_channels(ctx, nc, r, kky, kkx, acs):
+    # above 32 channels the fill runs as the per-row MFMA product with the
+    # target channels in tiles of 32 (the grouped synthesis holds 32)
+    case = _gcase(nc=nc, r=r, acs=acs, seed=nc)
+    d = abi.GrappaDesc.default(prewhiten=1, kernel_ky=kky, kernel_kx=kkx)
+    st, r32, rflags, _ = O.recon("grappa", d, case, "f32")
+    _, r64, _, _ = O.recon("grappa", d, case, "f64")
+    got, flags, _ = recon_dev(ctx, "oks_recon_grappa_dev", d, case)
+    assert st == 0 and got.shape == r32.shape and flags == rflags
+    assert_parity(got, r32, r64, TOL, f"grappa nc={nc} R={r}")
+
+
 def test_grappa_no_pattern_and_bad_config(ctx):
     case = _gcase(kind="full")
     d = abi.GrappaDesc.default()
