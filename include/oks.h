@@ -471,7 +471,9 @@ int oks_server_create(const oks_server_config* cfg, oks_server** out);
 void oks_server_destroy(oks_server* s);              /* finishes running batches, drops queued ones */
 const char* oks_server_last_error(oks_server* s);
 double oks_server_now(const oks_server* s);          /* server clock, seconds since create */
-/* OKS_E_OOM when the request alone exceeds the budget, OKS_E_BUSY when full. */
+/* OKS_E_OOM when the request alone exceeds the budget, OKS_E_BUSY when full,
+ * OKS_E_INVALID_ARG above 64 channels.  Requests of more than 32 channels
+ * are never batched: each runs alone on the plain path. */
 int oks_server_submit(oks_server* s, const oks_request* r);
 /* Up to `max` completions; waits up to timeout_ms (<0 forever) for the first. */
 int oks_server_poll(oks_server* s, oks_completion* out, int32_t max, int32_t timeout_ms, int32_t* n);

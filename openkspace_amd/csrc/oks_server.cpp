@@ -497,9 +497,9 @@ int oks_server_submit(oks_server* s, const oks_request* r) {
         return OKS_E_INVALID_ARG;
     }
     const oks_kspace& k = r->k;
-    if (!k.kspace || !k.mask || k.nc < 1 || k.nz < 1 || k.ny < 1 || k.nx < 1 || k.nc > 32) {
+    if (!k.kspace || !k.mask || k.nc < 1 || k.nz < 1 || k.ny < 1 || k.nx < 1 || k.nc > oks::kMaxChannels) {
         std::lock_guard<std::mutex> lk(s->mu);
-        s->err = "request k-space must be non-empty with at most 32 channels";
+        s->err = "request k-space must be non-empty with at most 64 channels";
         return OKS_E_INVALID_ARG;
     }
     Req* q = new (std::nothrow) Req();
@@ -522,7 +522,10 @@ int oks_server_submit(oks_server* s, const oks_request* r) {
     // no z crop, and no per-pixel failure state shared by a batch (SENSE).
     bool three_d = desc_fft_mode(r->strategy, &q->desc) == OKS_FFT_3D ||
                    (desc_fft_mode(r->strategy, &q->desc) == OKS_FFT_AUTO && k.hdr.is_3d_encoding);
-    q->groupable = k.nz == 1 && !three_d && r->strategy != OKS_STRATEGY_SENSE && !(crop && k.hdr.recon_z >= 2);
+    // the batched kernels hold at most 32 channels; larger requests run alone
+    // on the plain path
+    q->groupable = k.nz == 1 && !three_d && r->strategy != OKS_STRATEGY_SENSE && !(crop && k.hdr.recon_z >= 2) &&
+                   k.nc <= oks::kFusedChannels;
     q->footprint = footprint(*q);
     std::lock_guard<std::mutex> lk(s->mu);
     if (q->footprint > s->cfg.hbm_budget) {

@@ -255,3 +255,41 @@ def test_partial_fourier_requests_served(ctx):
     finally:
         for sl in slices:
             sl.free()
+
+
+def test_more_than_32_channel_requests_run_alone(ctx):
+    # the batched kernels hold at most 32 channels: larger requests (up to
+    # 64) run alone on the plain path beside batches of small ones; above 64
+    # a request is refused at submission
+    specs = [(abi.IfftRssDesc.default(), dict(nc=40, ny=48, nx=64, os_x=2, kind="full")),
+             (abi.CsDesc.default(iters=10), dict(nc=36, ny=48, nx=48, kind="cs")),
+             (abi.GrappaDesc.default(kernel_ky=2, kernel_kx=3), dict(nc=34, ny=64, nx=48, kind="grappa", r=2, acs=16)),
+             (abi.IfftRssDesc.default(), dict(nc=8, ny=48, nx=64, os_x=2, kind="full"))]
+    slices = [Slice(ctx, synth.make_case(nz=1, seed=700 + i, ellipses=5, **kw), desc, i)
+              for i, (desc, kw) in enumerate(specs)]
+    try:
+        mixed = slices + slices
+        with Server(slots=2, max_batch=8) as s:
+            t0 = s.now() + 0.05
+            for j, sl in enumerate(mixed):
+                s.submit(Server.request(sl.desc, sl.ks, sl.out.addr, None, 100 + j, t0))
+            got = s.collect(len(mixed))
+            big = slices[0]
+            ks = abi.KSpace(big.k.addr, big.mask.ctypes.data, 65, 1, 48, 64, None, 0, abi.Header(**big.case["hdr"]))
+            with pytest.raises(OksError) as e:
+                s.submit(Server.request(big.desc, ks, big.out.addr, None, 999, 0.0))
+            assert e.value.code == abi.OKS_E_INVALID_ARG and "64" in str(e.value)
+        for j, sl in enumerate(mixed):
+            c = got[100 + j]
+            nc = sl.case["kspace"].shape[0]
+            assert c.status == 0
+            if nc > 32:
+                assert c.batch == 1
+            st, r32, fl, _ = O.recon(sl.oracle, sl.desc, sl.case, "f32")
+            _, r64, _, _ = O.recon(sl.oracle, sl.desc, sl.case, "f64")
+            assert st == 0 and c.flags == fl
+            assert_parity(sl.image(c), r32, r64, TOL[sl.oracle], f"served {sl.oracle} nc={nc}",
+                          iterative=sl.oracle == "cs")
+    finally:
+        for sl in slices:
+            sl.free()
