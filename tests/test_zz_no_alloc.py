@@ -79,6 +79,17 @@ def test_homodyne_and_3d(ctx):
     _oracle(vol, abi.IfftRssDesc.default(partial_fourier=0), got, flags, "noalloc 3-D")
 
 
+@pytest.mark.parametrize("three_d", [False, True])
+def test_more_than_32_channels(ctx, three_d):
+    # above 32 channels the whitened copy of the grid and the channel-grouped
+    # row pass take their workspaces from the static scratch too
+    case = synth.make_case(nc=40, nz=4, ny=40, nx=64, os_x=2, kind="full", seed=47, ellipses=5, three_d=three_d)
+    d = abi.IfftRssDesc.default(partial_fourier=0)
+    got, want, flags = _both(ctx, case, d)
+    assert np.array_equal(got, want)
+    _oracle(case, d, got, flags, f"noalloc nc=40 3d={three_d}")
+
+
 def test_too_large_is_refused(ctx):
     # a 3-D volume keeps the whole row-pass intermediate: 8 x 64 x 128 x 128 x 8 B = 67 MB
     vol = synth.make_case(nc=8, nz=64, ny=128, nx=128, kind="full", seed=45, ellipses=4, three_d=True)
