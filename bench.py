@@ -102,6 +102,22 @@ def workload_c1os():
     return w
 
 
+def workload_c1w():
+    """C1's geometry at 64 channels: above the 32 channels the fused row pass
+    holds, the whitening runs as a pass of its own and the row pass in two
+    groups of 32 channels."""
+    from openkspace_amd import abi, synth
+    case = synth.make_config("c1_ifft_rss", nc=64)
+    nc, nz, ny, nx = case["kspace"].shape
+    tx = nx // 2
+    w = workload_c1()
+    w.update(name="c1w_ifft_rss", case=case, algo=nz * (nc * ny * nx * 8 + ny * tx * 4),
+             kernel="ifft_rss pass sequence (whitener + whitening pass + grouped row passes + column pass)")
+    w["config"] = dict(w["config"], workload="c1w_ifft_rss", coils=nc, noise_samples=case["noise"].shape[1])
+    w["desc"] = abi.IfftRssDesc.default(remove_oversampling=0)
+    return w
+
+
 def workload_c1h():
     """C1 acquired with 6/8 partial Fourier along ky (rows below 92 of 368
     missing): ifft-rss dispatches to the homodyne reconstruction."""
@@ -221,7 +237,7 @@ def workload_c4e():
 
 
 WORKLOADS = {"c1": workload_c1, "c2": workload_c2, "c3": workload_c3, "c4": workload_c4, "c1h": workload_c1h,
-             "c1os": workload_c1os, "c4e": workload_c4e}
+             "c1os": workload_c1os, "c1w": workload_c1w, "c4e": workload_c4e}
 
 # HBM bytes the dominant kernels move per algorithmic byte, measured with the
 # FETCH_SIZE / WRITE_SIZE counters scaled by a copy kernel of known size
@@ -237,7 +253,7 @@ TRAFFIC_PER_ALGO_BYTE = {"c1": 60.8e6 / 28.73e6, "c3": 57.5 / 41.0, "served": 57
 CS_BYTES_PER_PIXEL_ITER = 5 * 8 + 1
 
 # The stage whose launches each multi-slice workload's roofline is taken on.
-WORKLOAD_STAGE = {"c1": "ifft_rss", "c1os": "ifft_rss", "c4": "ifft_rss", "c2": "grappa_apply",
+WORKLOAD_STAGE = {"c1": "ifft_rss", "c1os": "ifft_rss", "c1w": "ifft_rss", "c4": "ifft_rss", "c2": "grappa_apply",
                   "c3": "cs_iterations", "c4e": "espirit"}
 STAGE_KERNELS = {"ifft_rss": "IFFT + crop + RSS tail: k_rows_pw* (+ k_cols_reg_c along kz for 3-D) + "
                              "k_cols_rss_reg (+ k_rss_finalize)",

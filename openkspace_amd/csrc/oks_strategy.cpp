@@ -151,6 +151,16 @@ void resolve_os(const oks_kspace* k, int enabled, int* os_mode, int* rec, uint32
     }
 }
 
+// Channels per row-pass launch when a grid has more than the fused row pass
+// holds: the largest group whose row of every channel fits in LDS (at most
+// kFusedChannels), then spread evenly over the groups.
+static int64_t channel_group(int64_t nc, int nx, int os_mode, int rec, bool pipe) {
+    int64_t g = std::min(nc, kFusedChannels);
+    while (g > 1 && rows_prewhiten_lds((int)g, nx, os_mode, rec, false, pipe) > 160 * 1024) --g;
+    const int64_t groups = (nc + g - 1) / g;
+    return (nc + groups - 1) / groups;
+}
+
 // The reader closure (whitening, oversampling removal; recon/strategy.rs:153-161)
 // applied to the raw grid, into workspace `slot` [nc, nz, ny, nx].
 float2* prep_grid(oks_ctx* ctx, const oks_kspace* k, const float2* W, int os_mode, int rec, const char* slot) {
@@ -185,8 +195,9 @@ float2* prep_grid(oks_ctx* ctx, const oks_kspace* k, const float2* W, int os_mod
         a.wgs = ctx->rows_wgs;
         a.p_enc = ctx->plan((int)k->nx);
         a.p_rec = ctx->plan(rec);
-        for (int64_t c0 = 0; c0 < k->nc; c0 += kFusedChannels) {
-            a.nc = (int)std::min(kFusedChannels, k->nc - c0);
+        const int64_t gsz = W ? k->nc : channel_group(k->nc, (int)k->nx, 1, rec, false);
+        for (int64_t c0 = 0; c0 < k->nc; c0 += gsz) {
+            a.nc = (int)std::min(gsz, k->nc - c0);
             a.in = src + c0 * (int64_t)plane;
             a.out = g + c0 * (int64_t)plane;
             OKS_HIP(launch_prep_os(a, acq, ctx->stream));
@@ -258,14 +269,14 @@ void ifft_rss_grid(oks_ctx* ctx, const float2* k, int64_t nc, int64_t nz, int64_
     a.in_plane = nz * ny * nx;
     a.p_enc = ctx->plan((int)nx);
     if (os_mode) a.p_rec = ctx->plan(rec);
-    if (rows_prewhiten_lds((int)std::min(nc, kFusedChannels), (int)nx, os_mode, rec, W != nullptr,
-                           ctx->rows_pipe != 0) > 160 * 1024)
+    const int64_t gsz = grouped ? channel_group(nc, (int)nx, os_mode, rec, ctx->rows_pipe != 0) : nc;
+    if (rows_prewhiten_lds((int)gsz, (int)nx, os_mode, rec, W != nullptr, ctx->rows_pipe != 0) > 160 * 1024)
         fail(OKS_E_UNSUPPORTED, "readout too long for the fused row pass");
     // the row pass over every channel group of a.in/a.out
     auto rows = [&](const RowPwArgs& r) {
-        for (int64_t c0 = 0; c0 < nc; c0 += kFusedChannels) {
+        for (int64_t c0 = 0; c0 < nc; c0 += gsz) {
             RowPwArgs g = r;
-            g.nc = (int)std::min(kFusedChannels, nc - c0);
+            g.nc = (int)std::min(gsz, nc - c0);
             g.in = r.in + c0 * r.in_plane;
             g.out = r.out + c0 * r.out_plane;
             OKS_HIP(launch_rows_prewhiten_ifft(g, ctx->stream));

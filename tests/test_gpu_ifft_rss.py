@@ -358,3 +358,16 @@ def test_ifft_rss_more_than_32_channels(ctx, nc, three_d, remove_os):
     got, flags, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", d, case)
     assert st == 0 and got.shape == ref32.shape and flags == rflags
     assert_parity(got, ref32, ref64, TOL32, f"nc={nc} 3d={three_d} os={remove_os}")
+
+
+@pytest.mark.parametrize("nc,remove_os", [(64, 0), (64, 1), (33, 0)])
+def test_ifft_rss_many_channels_long_readout(ctx, nc, remove_os):
+    # a row of 32 channels x 640 samples does not fit in LDS: the channel
+    # groups shrink until one does (22 + 21 + 21 at 64 channels)
+    case = _case(nc=nc, nz=2, ny=32, nx=640, seed=nc + 640)
+    d = abi.IfftRssDesc.default(remove_oversampling=remove_os, prewhiten=1)
+    st, ref32, rflags, _ = O.recon("ifft_rss", d, case, "f32")
+    _, ref64, _, _ = O.recon("ifft_rss", d, case, "f64")
+    got, flags, _ = recon_dev(ctx, "oks_recon_ifft_rss_dev", d, case)
+    assert st == 0 and got.shape == ref32.shape and flags == rflags
+    assert_parity(got, ref32, ref64, TOL32, f"nc={nc} nx=640 os={remove_os}")
